@@ -1,0 +1,231 @@
+/*
+ * qmb.h -- C ABI of the MI355X exact-diagonalisation engine (libqmb.so).
+ *
+ * The reference (jeohjee/QMBCuda) has no FFI: its boundary is a C++ class API
+ * plus one free function. Each entry point below names the reference code it
+ * replaces (paths relative to the reference repository root, Q/ = QMBCuda/).
+ * The C++ facade in include/qmb/ (the reference's class API) calls only these
+ * functions.
+ *
+ * Rules of the ABI
+ *  - plain pointers and sizes; no C++ types cross it; no exception escapes;
+ *  - every function returns a qmb_status; qmb_last_error() gives the text of the
+ *    last failure on the calling thread;
+ *  - opaque handles own their device memory and are released by *_destroy;
+ *    host arrays always belong to the caller;
+ *  - a handle is used by one host thread at a time; every handle made from a
+ *    context runs on that context's HIP stream;
+ *  - states are uint64 (at most 64 sites); Lanczos runs in FP64.
+ *
+ * Conventions kept from the reference (bit-exact):
+ *  - site s <-> bit s, spin up = bit set (Q/exact_diagonalization/utils.cu:137,157);
+ *  - group_el[g*n_sites + s] = bit that site s moves to under element g
+ *    (utils.cu:52-59; Q/symmetries/SymmetryClass.cu:55-99); element 0 is the identity;
+ *  - chars[(a*G + g)*2 + {0,1}] = (re, im) of the character of irrep a at element g
+ *    (Q/symmetries/Groups.cu:89-112);
+ *  - operator types 0 = Sz, 1 = Sp, 2 = Sm (Q/quantum_operators/SingleParticleOperators.h:4);
+ *  - term tables are the conjugated tables of BuildOperatorVecs
+ *    (Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:306-336);
+ *  - rows are orbit representatives (minimum integer of the orbit) in ascending order;
+ *    ELL slot 0 is the diagonal; off-diagonal slots follow term order, unmerged
+ *    (utils.cu:83-184).
+ * New in this engine: uint64 states, an optional spin-flip bit per group element
+ * (image XOR all-ones), the compact layout (only orbits compatible with the irrep).
+ */
+#ifndef QMB_H
+#define QMB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    QMB_OK = 0,
+    QMB_ERR_INVALID = 1,       /* bad argument or inconsistent input */
+    QMB_ERR_HIP = 2,           /* a HIP runtime call failed */
+    QMB_ERR_OOM = 3,           /* device or host allocation failed */
+    QMB_ERR_NOT_CONVERGED = 4, /* Lanczos stopped at max_iter before its tolerance */
+    QMB_ERR_NO_DEVICE = 5,     /* no usable GPU */
+    QMB_ERR_UNSUPPORTED = 6    /* valid request this build does not implement */
+} qmb_status;
+
+enum { QMB_OP_SZ = 0, QMB_OP_SP = 1, QMB_OP_SM = 2 };
+enum { QMB_LAYOUT_REFERENCE = 0, QMB_LAYOUT_COMPACT = 1 };
+enum { QMB_REAL = 0, QMB_COMPLEX = 1 };
+
+typedef struct qmb_ctx qmb_ctx;
+typedef struct qmb_group qmb_group;
+typedef struct qmb_terms qmb_terms;
+typedef struct qmb_basis qmb_basis;
+typedef struct qmb_ell qmb_ell;
+
+/* ---- errors, version -------------------------------------------------------
+ * Replaces CUDA_CHECK / CUBLAS_CHECK / CUSOLVER_CHECK exceptions
+ * (Q/cublas_utils.h:45-62, Q/cusolver_utils.h:47-54). */
+const char *qmb_last_error(void);
+int qmb_version(void);
+
+/* ---- context: device + one HIP stream ---------------------------------------
+ * The reference runs everything on the default device and stream with a
+ * cudaDeviceSynchronize after each kernel (Q/exact_diagonalization/lanczos.cu:83,214). */
+qmb_status qmb_ctx_create(int device, qmb_ctx **out);
+qmb_status qmb_ctx_destroy(qmb_ctx *ctx);
+qmb_status qmb_ctx_synchronize(qmb_ctx *ctx);
+/* Device count without creating a context (0 when no GPU). */
+int qmb_device_count(void);
+
+/* ---- symmetry group ---------------------------------------------------------
+ * Replaces SymmetryClass<T>::GetGroupElemArr + the character table copy of the
+ * driver (Q/symmetries/SymmetryClass.cu:26-71; Q/exact_diagonalization/
+ * HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:45-61). flip may be NULL (no spin
+ * flip). The element list must be closed under inversion and G <= 128. */
+qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el,
+                            const uint8_t *flip, int n_irreps, const double *chars,
+                            qmb_group **out);
+qmb_status qmb_group_destroy(qmb_group *g);
+/* Inverse-element table (inv[g] = index of g^-1) and 1 if the fast torus
+ * (cyclic-translation) path was detected, 0 for the generic permutation path. */
+qmb_status qmb_group_info(const qmb_group *g, int32_t *inv, int *torus_path);
+
+/* ---- term table -------------------------------------------------------------
+ * The flat, conjugated Hamiltonian tables of BuildOperatorVecs
+ * (HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:306-336): op_type/site are
+ * [n_terms*max_ops] (site -1 pads), scalar [n_terms*2] (re, im), n_ops[n_terms].
+ * Every term must conserve Sz (as many Sp as Sm), or QMB_ERR_INVALID. */
+qmb_status qmb_terms_create(int n_sites, int n_terms, int max_ops, const int32_t *op_type,
+                            const int32_t *site, const double *scalar,
+                            const int32_t *n_ops, qmb_terms **out);
+qmb_status qmb_terms_destroy(qmb_terms *t);
+
+/* ---- symmetry-reduced basis (on device) -------------------------------------
+ * Replaces BuildBasisStates + BuildSRStates (HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:
+ * 277-296, 341-400) and the kernels SetUpSRS, SolveSRS_min, SetUpSRS_from_indices,
+ * ComputeSRSNorms, ComputeOrbitIndices (Q/exact_diagonalization/utils.cu:39-80,
+ * 716-805). Rows = orbit representatives in ascending order. REFERENCE layout keeps
+ * every orbit (null orbits of the irrep become empty rows, like the reference);
+ * COMPACT keeps only the orbits compatible with the irrep. */
+typedef struct {
+    int64_t sector_dim; /* C(n_sites, n_up) */
+    int64_t n_rows;     /* rows of the matrix */
+    int64_t n_null;     /* rows that are null for the irrep (REFERENCE layout) */
+    int32_t n_sites, n_up, irrep, layout;
+    int64_t device_bytes;
+} qmb_basis_info;
+
+qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep,
+                           int layout, qmb_basis **out);
+qmb_status qmb_basis_destroy(qmb_basis *b);
+qmb_status qmb_basis_get_info(const qmb_basis *b, qmb_basis_info *info);
+/* Host copies of the representatives, stabiliser sizes and null flags (any may be NULL). */
+qmb_status qmb_basis_export(const qmb_basis *b, uint64_t *reps, uint16_t *stab, uint8_t *is_null);
+/* Row lookup of arbitrary sector states (replaces the rank -> (orbit, g) table
+ * orbit_indices, utils.cu:66-80): row[i] = row of the state's orbit or -1 if that
+ * orbit is not a row; elem[i] = smallest g with image_g(rep) = state. */
+qmb_status qmb_basis_lookup(const qmb_basis *b, int64_t n, const uint64_t *states,
+                            int64_t *row, int32_t *elem);
+
+/* ---- ELL Hamiltonian (on device) --------------------------------------------
+ * Replaces BuildHamiltonianXXZForAbelianGroup (utils.cu:83-184) with its launch
+ * (HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:127-167). max_terms = the reference's
+ * row width W; <= 0 means the reference rule n_terms*n_up/n_sites + 1 (:71-72).
+ * A row that needs more than W slots is an error (the reference overflows). */
+typedef struct {
+    int64_t n_rows;
+    int64_t nnz;          /* stored entries excluding padding (sum of row lengths) */
+    int64_t stored;       /* entries including chunk padding */
+    int32_t max_row_len;
+    int32_t max_terms;    /* W of the reference layout */
+    int32_t value_type;   /* QMB_REAL or QMB_COMPLEX */
+    int32_t chunk_rows;   /* rows per SELL chunk (64 = one wavefront) */
+    int64_t device_bytes;
+    double spmv_bytes;    /* algorithmic bytes of one SpMV */
+} qmb_ell_info;
+
+qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t,
+                         int value_type, int max_terms, qmb_ell **out);
+qmb_status qmb_ell_destroy(qmb_ell *e);
+qmb_status qmb_ell_get_info(const qmb_ell *e, qmb_ell_info *info);
+/* Reference row-major layout (ind[n_rows*W], val[n_rows*W] or [n_rows*W*2] for
+ * complex, track[n_rows]), exactly as the reference kernel leaves its buffers. */
+qmb_status qmb_ell_export_reference(const qmb_ell *e, uint32_t *ind, double *val, int16_t *track);
+
+/* ---- SpMV -------------------------------------------------------------------
+ * Replaces SparseMatDenseVec_prod_EEL (utils.cu:601-624). Vectors are n_rows
+ * doubles (REAL) or n_rows interleaved (re, im) pairs (COMPLEX). */
+qmb_status qmb_spmv_host(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y);
+qmb_status qmb_spmv_device(qmb_ctx *ctx, const qmb_ell *e, const double *x_dev, double *y_dev);
+/* Runs reps back-to-back device SpMVs on internal vectors and returns the mean
+ * kernel time in ms (HIP events on the context stream). */
+qmb_status qmb_spmv_benchmark(qmb_ctx *ctx, const qmb_ell *e, int reps, double *ms_per_spmv);
+
+/* ---- Lanczos ----------------------------------------------------------------
+ * Replaces LanczosEigVals_CUDA (Q/exact_diagonalization/lanczos.cu:128-317) and
+ * LanczosEigenStateFromSeed (:31-104) with an FP64 device loop: the same
+ * three-term recurrence, a convergence test on the Ritz residual, optional
+ * full reorthogonalisation (needed for n_eigs > 1), host FP64 tridiagonal solve. */
+enum { QMB_START_RANDOM = 0, QMB_START_USER = 1 };
+
+typedef struct {
+    int32_t max_iter;    /* upper bound on iterations */
+    int32_t n_eigs;      /* lowest eigenvalues wanted (1..8) */
+    int32_t check_every; /* iterations between convergence checks (0 -> 10) */
+    int32_t reorth;      /* 0 plain recurrence, 1 full reorthogonalisation */
+    int32_t fixed_iter;  /* 1: run exactly max_iter steps, no test (reference semantics) */
+    int32_t start;       /* QMB_START_RANDOM or QMB_START_USER */
+    int32_t want_eigvec; /* 1: rebuild the ground-state vector (second pass) */
+    int32_t profile;     /* 1: time every SpMV with HIP events */
+    double tol;          /* eigenvalue tolerance, relative to max(1,|E|) (0 -> 1e-13) */
+    uint64_t seed;       /* start-vector seed (reference default 2212456) */
+    const double *start_vec; /* host start vector when start == QMB_START_USER */
+} qmb_lanczos_opts;
+
+typedef struct {
+    int32_t iters;       /* Lanczos steps taken */
+    int32_t converged;   /* 1 if the tolerance was met */
+    int32_t breakdown;   /* 1 if an invariant subspace was hit (exact result) */
+    int32_t n_spmv;
+    double residual[8];  /* Ritz residuals of the returned eigenvalues */
+    double eig_residual; /* ||H x - E x|| of the rebuilt vector (want_eigvec) */
+    double ms_total;     /* wall time of the call */
+    double ms_spmv;      /* summed SpMV kernel time (profile) */
+    double ms_host;      /* host tridiagonal work */
+} qmb_lanczos_report;
+
+void qmb_lanczos_default_opts(qmb_lanczos_opts *o);
+
+/* evals[n_eigs]; krylov (optional, [max_iter]) = Krylov coefficients of the ground
+ * state (lanczos.cu:268-279); eigvec (optional, host, n_rows values) = ground
+ * state when want_eigvec. */
+qmb_status qmb_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts *opts,
+                       double *evals, double *krylov, double *eigvec,
+                       qmb_lanczos_report *report);
+
+/* Drop-in for the reference's free function LanczosEigVals_CUDA
+ * (Q/exact_diagonalization/lanczos.h:21-32): fixed NIter steps on a
+ * reference-layout ELL given as HOST arrays (index_mat[nobv*max_terms],
+ * vals_mat[nobv*max_terms] (x2 if complex), track_vec[nobv]); E_tmp gets the
+ * lowest eigenvalue; eig_vec_Krylov_out (optional) the Krylov coefficients; with
+ * checkEigState the function value is ||Hx - Ex||/nobv like the reference's,
+ * else 0. */
+double qmb_lanczos_eigvals_ref(double *E_tmp, int NStates, int NIter, int nobv, int max_terms,
+                               long long seed, const uint32_t *index_mat, const double *vals_mat,
+                               const int16_t *track_vec, double *eig_vec_Krylov_out,
+                               int exportKrylovVec, int checkEigState, int is_complex,
+                               qmb_status *status);
+
+/* ---- batched Lanczos over many sectors (new; no reference counterpart) -------
+ * The reference solves irreps one after another (HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:127-229).
+ * This solves n independent ELLs together: one launch per phase covers every
+ * instance (a work table of row chunks), the iteration chunk is captured once as
+ * a HIP graph and replayed, converged instances drop out. evals[n*n_eigs],
+ * reports[n] (may be NULL). */
+qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
+                               const qmb_lanczos_opts *opts, double *evals,
+                               qmb_lanczos_report *reports);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,264 @@
+"""Python bindings of libqmb, the MI355X exact-diagonalisation engine.
+
+The product is the C ABI in include/qmb.h (hand-written gfx950 HIP kernels behind
+it) and the C++ facade in include/qmb/. This module is a thin ctypes layer used
+by the tests and the benchmark; it never computes anything on the CPU in place
+of the library, and it raises if libqmb.so is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (COMPLEX, LAYOUT_COMPACT, LAYOUT_REFERENCE, REAL, START_RANDOM, START_USER, BasisInfo,
+                   EllInfo, LanczosOpts, LanczosReport)
+from . import models
+
+__all__ = ["QMBError", "Context", "Group", "Terms", "Basis", "Ell", "lanczos", "lanczos_batched", "models",
+           "LAYOUT_REFERENCE", "LAYOUT_COMPACT", "REAL", "COMPLEX", "device_count", "lib"]
+
+
+class QMBError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        self.status = status
+        super().__init__(f"{_abi.STATUS_NAMES.get(status, status)}: {message}")
+
+
+def lib():
+    return _abi.load()
+
+
+def _check(status: int):
+    if status != 0:
+        raise QMBError(status, lib().qmb_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def device_count() -> int:
+    return int(lib().qmb_device_count())
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().qmb_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        _check(lib().qmb_ctx_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qmb_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Group:
+    def __init__(self, group_el, chars, flip=None):
+        ge = np.ascontiguousarray(group_el, np.uint32)
+        self.G, self.n_sites = ge.shape
+        ch = np.ascontiguousarray(chars, np.complex128)
+        self.n_irreps = ch.shape[0]
+        self.chars = ch
+        self._ge = ge
+        self._ch = np.ascontiguousarray(ch.view(np.float64))
+        self._fl = None if flip is None else np.ascontiguousarray(flip, np.uint8)
+        h = C.c_void_p()
+        _check(lib().qmb_group_create(self.n_sites, self.G, _ptr(self._ge), _ptr(self._fl), self.n_irreps,
+                                      _ptr(self._ch), C.byref(h)))
+        self.h = h
+
+    def info(self):
+        inv = np.zeros(self.G, np.int32)
+        torus = C.c_int(0)
+        _check(lib().qmb_group_info(self.h, _ptr(inv), C.byref(torus)))
+        return inv, bool(torus.value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().qmb_group_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Terms:
+    def __init__(self, n_sites, table):
+        op, site, scalar, n_ops, max_ops = table
+        self._op = np.ascontiguousarray(op, np.int32)
+        self._site = np.ascontiguousarray(site, np.int32)
+        self._sc = np.ascontiguousarray(np.asarray(scalar, np.complex128).view(np.float64))
+        self._nops = np.ascontiguousarray(n_ops, np.int32)
+        self.n_terms = len(self._nops)
+        self.scalars = np.asarray(scalar, np.complex128)
+        h = C.c_void_p()
+        _check(lib().qmb_terms_create(n_sites, self.n_terms, max_ops, _ptr(self._op), _ptr(self._site),
+                                      _ptr(self._sc), _ptr(self._nops), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().qmb_terms_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Basis:
+    def __init__(self, ctx: Context, group: Group, n_up: int, irrep: int, layout=LAYOUT_COMPACT):
+        self.ctx, self.group = ctx, group
+        h = C.c_void_p()
+        _check(lib().qmb_basis_build(ctx.h, group.h, n_up, irrep, layout, C.byref(h)))
+        self.h = h
+        self.irrep, self.layout = irrep, layout
+
+    def info(self) -> BasisInfo:
+        inf = BasisInfo()
+        _check(lib().qmb_basis_get_info(self.h, C.byref(inf)))
+        return inf
+
+    @property
+    def n_rows(self) -> int:
+        return int(self.info().n_rows)
+
+    def export(self):
+        n = self.n_rows
+        reps = np.zeros(n, np.uint64)
+        stab = np.zeros(n, np.uint16)
+        nul = np.zeros(n, np.uint8)
+        _check(lib().qmb_basis_export(self.h, _ptr(reps), _ptr(stab), _ptr(nul)))
+        return reps, stab, nul
+
+    def lookup(self, states):
+        st = np.ascontiguousarray(states, np.uint64)
+        rows = np.zeros(len(st), np.int64)
+        el = np.zeros(len(st), np.int32)
+        _check(lib().qmb_basis_lookup(self.h, len(st), _ptr(st), _ptr(rows), _ptr(el)))
+        return rows, el
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().qmb_basis_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Ell:
+    def __init__(self, ctx: Context, basis: Basis, terms: Terms, value_type=None, max_terms: int = 0):
+        if value_type is None:
+            real = models.is_real_irrep(basis.group.chars, basis.irrep) and np.all(terms.scalars.imag == 0)
+            value_type = REAL if real else COMPLEX
+        self.ctx, self.basis, self.terms = ctx, basis, terms
+        self.value_type = value_type
+        h = C.c_void_p()
+        _check(lib().qmb_ell_build(ctx.h, basis.h, terms.h, value_type, max_terms, C.byref(h)))
+        self.h = h
+
+    @property
+    def is_complex(self) -> bool:
+        return self.value_type == COMPLEX
+
+    def info(self) -> EllInfo:
+        inf = EllInfo()
+        _check(lib().qmb_ell_get_info(self.h, C.byref(inf)))
+        return inf
+
+    def export_reference(self):
+        inf = self.info()
+        n, W = int(inf.n_rows), int(inf.max_terms)
+        ind = np.zeros(n * W, np.uint32)
+        val = np.zeros(n * W * (2 if self.is_complex else 1), np.float64)
+        track = np.zeros(n, np.int16)
+        _check(lib().qmb_ell_export_reference(self.h, _ptr(ind), _ptr(val), _ptr(track)))
+        return ind, (val.view(np.complex128) if self.is_complex else val), track
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.complex128 if self.is_complex else np.float64)
+        y = np.zeros_like(x)
+        _check(lib().qmb_spmv_host(self.ctx.h, self.h, _ptr(x), _ptr(y)))
+        return y
+
+    def spmv_benchmark(self, reps: int = 20) -> float:
+        ms = C.c_double(0.0)
+        _check(lib().qmb_spmv_benchmark(self.ctx.h, self.h, reps, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().qmb_ell_destroy(self.h)
+        except Exception:
+            pass
+
+
+def default_opts(**kw) -> LanczosOpts:
+    o = LanczosOpts()
+    lib().qmb_lanczos_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def lanczos(ell: Ell, n_eigs=1, max_iter=300, tol=1e-13, seed=2212456, reorth=None, check_every=10,
+            fixed_iter=False, start_vec=None, want_eigvec=False, profile=False, allow_unconverged=False):
+    """Lowest eigenvalue(s) of an ELL Hamiltonian on the GPU.
+    Returns (evals, report, krylov, eigvec)."""
+    if reorth is None:
+        reorth = n_eigs > 1
+    o = default_opts(max_iter=max_iter, n_eigs=n_eigs, tol=tol, seed=seed, reorth=int(reorth),
+                     check_every=check_every, fixed_iter=int(fixed_iter), want_eigvec=int(want_eigvec),
+                     profile=int(profile))
+    sv = None
+    if start_vec is not None:
+        sv = np.ascontiguousarray(start_vec, np.complex128 if ell.is_complex else np.float64)
+        o.start = START_USER
+        o.start_vec = sv.ctypes.data
+    ev = np.zeros(n_eigs, np.float64)
+    kr = np.zeros(max_iter, np.float64)
+    n = ell.basis.n_rows
+    vec = np.zeros(n, np.complex128 if ell.is_complex else np.float64) if want_eigvec else None
+    rep = LanczosReport()
+    st = lib().qmb_lanczos(ell.ctx.h, ell.h, C.byref(o), _ptr(ev), _ptr(kr), _ptr(vec), C.byref(rep))
+    if st == 4 and allow_unconverged:
+        pass
+    else:
+        _check(st)
+    return ev, rep, kr[: rep.iters], vec
+
+
+def lanczos_batched(ells, n_eigs=1, max_iter=300, tol=1e-13, seed=2212456, reorth=None, check_every=10):
+    """Solve several ELLs together (one launch per phase for all of them)."""
+    if reorth is None:
+        reorth = n_eigs > 1
+    o = default_opts(max_iter=max_iter, n_eigs=n_eigs, tol=tol, seed=seed, reorth=int(reorth),
+                     check_every=check_every)
+    arr = (C.c_void_p * len(ells))(*[e.h for e in ells])
+    ev = np.zeros(len(ells) * n_eigs, np.float64)
+    reps = (LanczosReport * len(ells))()
+    _check(lib().qmb_lanczos_batched(ells[0].ctx.h, len(ells), arr, C.byref(o), _ptr(ev), reps))
+    return ev.reshape(len(ells), n_eigs), list(reps)
+
+
+def eigvals_ref(E_out, NIter, nobv, max_terms, seed, ind, val, track, is_complex, check=True):
+    """The reference-signature drop-in (qmb_lanczos_eigvals_ref) on host arrays."""
+    E = C.c_double(0.0)
+    kr = np.zeros(NIter, np.float64)
+    st = C.c_int(0)
+    v = np.ascontiguousarray(val.view(np.float64) if is_complex else val, np.float64)
+    r = lib().qmb_lanczos_eigvals_ref(C.byref(E), 1, NIter, nobv, max_terms, seed, _ptr(np.ascontiguousarray(ind)),
+                                      _ptr(v), _ptr(np.ascontiguousarray(track)), _ptr(kr), 1, int(check),
+                                      int(is_complex), C.byref(st))
+    _check(st.value)
+    return E.value, r, kr

@@ -1,0 +1,411 @@
+// Context, group and term-table handles, error reporting, device scans.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace qmb {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+qmb_status fail(qmb_status code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+void ensure_binom(qmb_ctx *ctx) {
+    if (ctx->binom.p) return;
+    std::vector<uint64_t> t(64 * 33, 0);
+    // Pascal's rule, exact in uint64 for n < 64.
+    std::vector<std::vector<uint64_t>> c(65, std::vector<uint64_t>(65, 0));
+    for (int n = 0; n <= 64; ++n) {
+        c[n][0] = 1;
+        for (int k = 1; k <= n; ++k) c[n][k] = c[n - 1][k - 1] + (k <= n - 1 ? c[n - 1][k] : 0);
+    }
+    for (int p = 0; p < 64; ++p)
+        for (int j = 0; j < 33; ++j) t[p * 33 + j] = c[p][j];
+    ctx->binom.alloc(t.size() * sizeof(uint64_t));
+    QMB_HIP(hipMemcpy(ctx->binom.p, t.data(), t.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+}
+
+// ---- exclusive scans --------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <class T>
+__device__ inline T block_exclusive_scan(T v, T *lds, T &total) {
+    // Hillis-Steele over 256 values in LDS; fine for tile-level use.
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        T add = (t >= off) ? lds[t - off] : T(0);
+        __syncthreads();
+        lds[t] += add;
+        __syncthreads();
+    }
+    total = lds[kScanThreads - 1];
+    T incl = lds[t];
+    __syncthreads();
+    return incl - v;
+}
+
+struct PopcountSrc {
+    const uint64_t *w;
+    __device__ int64_t operator()(int64_t i) const { return __popcll(w[i]); }
+};
+
+struct ChunkSizeSrc {
+    const uint16_t *row_len;
+    int64_t n_rows;
+    // value = 64 * (max row length in the chunk)
+    __device__ int64_t operator()(int64_t c) const {
+        int m = 0;
+        const int64_t r0 = c * kChunk;
+        const int64_t r1 = r0 + kChunk < n_rows ? r0 + kChunk : n_rows;
+        for (int64_t r = r0; r < r1; ++r) m = m > row_len[r] ? m : (int)row_len[r];
+        return (int64_t)kChunk * m;
+    }
+};
+
+template <class Src>
+__global__ __launch_bounds__(kScanThreads) void scan_tile_sums(Src src, int64_t n, int64_t *tile_sums) {
+    __shared__ int64_t lds[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int64_t s = 0;
+    for (int i = 0; i < kScanItems; ++i)
+        if (base + i < n) s += src(base + i);
+    int64_t total;
+    block_exclusive_scan<int64_t>(s, lds, total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Single-block exclusive scan of the tile sums (in place); writes the grand total at [n].
+__global__ __launch_bounds__(kScanThreads) void scan_tile_offsets(int64_t *tile_sums, int64_t n) {
+    __shared__ int64_t lds[kScanThreads];
+    int64_t carry = 0;
+    for (int64_t base = 0; base < n; base += kScanThreads) {
+        const int64_t i = base + threadIdx.x;
+        const int64_t v = i < n ? tile_sums[i] : 0;
+        int64_t total;
+        const int64_t ex = block_exclusive_scan<int64_t>(v, lds, total);
+        if (i < n) tile_sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) tile_sums[n] = carry;
+}
+
+template <class Src, class Out>
+__global__ __launch_bounds__(kScanThreads) void scan_write(Src src, int64_t n, const int64_t *tile_off,
+                                                           Out *out) {
+    __shared__ int64_t lds[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int64_t vals[kScanItems];
+    int64_t s = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        vals[i] = (base + i < n) ? src(base + i) : 0;
+        s += vals[i];
+    }
+    int64_t total;
+    int64_t run = block_exclusive_scan<int64_t>(s, lds, total) + tile_off[blockIdx.x];
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = (Out)run;
+        run += vals[i];
+    }
+}
+
+template <class Src, class Out>
+static int64_t run_scan(qmb_ctx *ctx, Src src, int64_t n, Out *out) {
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    DevBuf sums((size_t)(tiles + 1) * sizeof(int64_t));
+    if (tiles > 0) {
+        hipLaunchKernelGGL(scan_tile_sums<Src>, dim3((unsigned)tiles), dim3(kScanThreads), 0, ctx->stream, src, n,
+                           sums.as<int64_t>());
+        QMB_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(scan_tile_offsets, dim3(1), dim3(kScanThreads), 0, ctx->stream, sums.as<int64_t>(), tiles);
+    QMB_HIP(hipGetLastError());
+    if (tiles > 0) {
+        hipLaunchKernelGGL((scan_write<Src, Out>), dim3((unsigned)tiles), dim3(kScanThreads), 0, ctx->stream, src, n,
+                           sums.as<int64_t>(), out);
+        QMB_HIP(hipGetLastError());
+    }
+    int64_t total = 0;
+    QMB_HIP(hipMemcpyAsync(&total, sums.as<int64_t>() + tiles, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    QMB_HIP(hipStreamSynchronize(ctx->stream));
+    return total;
+}
+
+int64_t scan_popcount_u32(qmb_ctx *ctx, const uint64_t *words, uint32_t *prefix, int64_t n) {
+    return run_scan(ctx, PopcountSrc{words}, n, prefix);
+}
+
+// Row-length statistics per chunk: max and sum, one wavefront per chunk.
+__global__ __launch_bounds__(256) void chunk_stats(const uint16_t *row_len, int64_t n_rows, int64_t n_chunks,
+                                                   int64_t *nnz_partial, int *max_partial) {
+    __shared__ int64_t s_sum[4];
+    __shared__ int s_max[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int64_t sum = 0;
+    int mx = 0;
+    for (int64_t c = (int64_t)blockIdx.x * 4 + wv; c < n_chunks; c += (int64_t)gridDim.x * 4) {
+        const int64_t r = c * kChunk + lane;
+        const int l = r < n_rows ? (int)row_len[r] : 0;
+        sum += l;
+        mx = mx > l ? mx : l;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sum += __shfl_down(sum, off, 64);
+        const int o = __shfl_down(mx, off, 64);
+        mx = mx > o ? mx : o;
+    }
+    if (lane == 0) {
+        s_sum[wv] = sum;
+        s_max[wv] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        int m = 0;
+        for (int i = 0; i < 4; ++i) {
+            t += s_sum[i];
+            m = m > s_max[i] ? m : s_max[i];
+        }
+        nnz_partial[blockIdx.x] = t;
+        max_partial[blockIdx.x] = m;
+    }
+}
+
+int64_t scan_chunks_i64(qmb_ctx *ctx, const uint16_t *row_len, int64_t n_rows, int64_t *chunk_ptr,
+                        int64_t n_chunks, int64_t *nnz_out, int *max_len_out) {
+    const int64_t total = run_scan(ctx, ChunkSizeSrc{row_len, n_rows}, n_chunks, chunk_ptr);
+    QMB_HIP(hipMemcpyAsync(chunk_ptr + n_chunks, &total, sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    const int blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n_chunks + 3) / 4));
+    DevBuf np((size_t)blocks * sizeof(int64_t)), mp((size_t)blocks * sizeof(int));
+    hipLaunchKernelGGL(chunk_stats, dim3(blocks), dim3(256), 0, ctx->stream, row_len, n_rows, n_chunks,
+                       np.as<int64_t>(), mp.as<int>());
+    QMB_HIP(hipGetLastError());
+    std::vector<int64_t> hn(blocks);
+    std::vector<int> hm(blocks);
+    QMB_HIP(hipMemcpyAsync(hn.data(), np.p, blocks * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    QMB_HIP(hipMemcpyAsync(hm.data(), mp.p, blocks * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    QMB_HIP(hipStreamSynchronize(ctx->stream));
+    int64_t nnz = 0;
+    int mx = 0;
+    for (int i = 0; i < blocks; ++i) {
+        nnz += hn[i];
+        mx = std::max(mx, hm[i]);
+    }
+    *nnz_out = nnz;
+    *max_len_out = mx;
+    return total;
+}
+
+// ---- group analysis ---------------------------------------------------------
+// Finds a block width B for which every element is a cyclic translation:
+// site p -> ((p/B + a) mod (N/B))*B + ((p mod B) + s) mod B.
+static bool detect_torus(qmb_group &g) {
+    const int N = g.n_sites;
+    for (int B = N; B >= 1; --B) {
+        if (N % B) continue;
+        const int nb = N / B;
+        std::vector<uint64_t> keep(g.G);
+        std::vector<uint8_t> sm(g.G), bg(g.G);
+        bool ok = true;
+        for (int e = 0; e < g.G && ok; ++e) {
+            const uint32_t *p = &g.perm[(size_t)e * N];
+            const int a = (int)(p[0] / B), s = (int)(p[0] % B);
+            for (int q = 0; q < N && ok; ++q) {
+                const int want = (((q / B) + a) % nb) * B + ((q % B) + s) % B;
+                if ((int)p[q] != want) ok = false;
+            }
+            if (!ok) break;
+            uint64_t k = 0;
+            for (int blk = 0; blk < nb; ++blk)
+                for (int j = 0; j < B - s; ++j) k |= 1ULL << (blk * B + j);
+            keep[e] = k;
+            sm[e] = (uint8_t)s;
+            bg[e] = (uint8_t)(a * B);
+        }
+        if (ok) {
+            g.torus = 1;
+            g.block = B;
+            g.keep = keep;
+            g.small = sm;
+            g.big = bg;
+            return true;
+        }
+    }
+    g.torus = 0;
+    g.block = N;
+    g.keep.assign(g.G, 0);
+    g.small.assign(g.G, 0);
+    g.big.assign(g.G, 0);
+    return false;
+}
+
+}  // namespace qmb
+
+using namespace qmb;
+
+extern "C" {
+
+const char *qmb_last_error(void) { return qmb::g_last_error.c_str(); }
+
+int qmb_version(void) { return 10000; }
+
+int qmb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+qmb_status qmb_ctx_create(int device, qmb_ctx **out) {
+    return guarded([&] {
+        QMB_REQUIRE(out != nullptr, "qmb_ctx_create: out is NULL");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+            throw Error{QMB_ERR_NO_DEVICE, "qmb_ctx_create: no HIP device"};
+        QMB_REQUIRE(device >= 0 && device < n, "qmb_ctx_create: device index out of range");
+        QMB_HIP(hipSetDevice(device));
+        auto *c = new qmb_ctx();
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            throw Error{QMB_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+        }
+        try {
+            ensure_binom(c);
+        } catch (...) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+qmb_status qmb_ctx_destroy(qmb_ctx *ctx) {
+    if (!ctx) return QMB_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->binom.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return QMB_OK;
+}
+
+qmb_status qmb_ctx_synchronize(qmb_ctx *ctx) {
+    return guarded([&] {
+        QMB_REQUIRE(ctx, "qmb_ctx_synchronize: NULL context");
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el, const uint8_t *flip,
+                            int n_irreps, const double *chars, qmb_group **out) {
+    return guarded([&] {
+        QMB_REQUIRE(out, "qmb_group_create: out is NULL");
+        *out = nullptr;
+        QMB_REQUIRE(n_sites >= 1 && n_sites <= 64, "qmb_group_create: n_sites must be in [1, 64]");
+        QMB_REQUIRE(n_elems >= 1 && n_elems <= kMaxGroup, "qmb_group_create: group size must be in [1, 128]");
+        QMB_REQUIRE(n_irreps >= 1, "qmb_group_create: need at least one irrep");
+        QMB_REQUIRE(group_el && chars, "qmb_group_create: NULL table");
+        auto g = std::make_unique<qmb_group>();
+        g->n_sites = n_sites;
+        g->G = n_elems;
+        g->n_irreps = n_irreps;
+        g->perm.assign(group_el, group_el + (size_t)n_elems * n_sites);
+        g->flip.assign(n_elems, 0);
+        if (flip)
+            for (int e = 0; e < n_elems; ++e) g->flip[e] = flip[e] ? 1 : 0;
+        g->chars.assign(chars, chars + (size_t)n_irreps * n_elems * 2);
+        for (int e = 0; e < n_elems; ++e) {
+            std::vector<int> seen(n_sites, 0);
+            for (int s = 0; s < n_sites; ++s) {
+                const uint32_t d = g->perm[(size_t)e * n_sites + s];
+                QMB_REQUIRE(d < (uint32_t)n_sites, "qmb_group_create: destination site out of range");
+                QMB_REQUIRE(!seen[d], "qmb_group_create: element is not a permutation");
+                seen[d] = 1;
+            }
+        }
+        for (int s = 0; s < n_sites; ++s)
+            QMB_REQUIRE(g->perm[s] == (uint32_t)s && !g->flip[0], "qmb_group_create: element 0 must be the identity");
+        // inverse table: h with perm_h[perm_g[s]] == s and equal flip
+        g->inv.assign(n_elems, -1);
+        for (int e = 0; e < n_elems; ++e) {
+            for (int h = 0; h < n_elems && g->inv[e] < 0; ++h) {
+                if (g->flip[h] != g->flip[e]) continue;
+                bool ok = true;
+                for (int s = 0; s < n_sites && ok; ++s)
+                    ok = g->perm[(size_t)h * n_sites + g->perm[(size_t)e * n_sites + s]] == (uint32_t)s;
+                if (ok) g->inv[e] = h;
+            }
+            QMB_REQUIRE(g->inv[e] >= 0, "qmb_group_create: element list is not closed under inversion");
+        }
+        detect_torus(*g);
+        *out = g.release();
+    });
+}
+
+qmb_status qmb_group_destroy(qmb_group *g) {
+    delete g;
+    return QMB_OK;
+}
+
+qmb_status qmb_group_info(const qmb_group *g, int32_t *inv, int *torus_path) {
+    return guarded([&] {
+        QMB_REQUIRE(g, "qmb_group_info: NULL group");
+        if (inv) std::copy(g->inv.begin(), g->inv.end(), inv);
+        if (torus_path) *torus_path = g->torus;
+    });
+}
+
+qmb_status qmb_terms_create(int n_sites, int n_terms, int max_ops, const int32_t *op_type, const int32_t *site,
+                            const double *scalar, const int32_t *n_ops, qmb_terms **out) {
+    return guarded([&] {
+        QMB_REQUIRE(out, "qmb_terms_create: out is NULL");
+        *out = nullptr;
+        QMB_REQUIRE(n_sites >= 1 && n_sites <= 64, "qmb_terms_create: n_sites must be in [1, 64]");
+        QMB_REQUIRE(n_terms >= 0 && max_ops >= 0 && max_ops <= 16, "qmb_terms_create: bad table shape");
+        QMB_REQUIRE(n_terms == 0 || (op_type && site && scalar && n_ops), "qmb_terms_create: NULL table");
+        auto t = std::make_unique<qmb_terms>();
+        t->n_sites = n_sites;
+        t->n_terms = n_terms;
+        t->max_ops = max_ops;
+        t->op_type.assign(op_type, op_type + (size_t)n_terms * max_ops);
+        t->site.assign(site, site + (size_t)n_terms * max_ops);
+        t->scalar.assign(scalar, scalar + (size_t)n_terms * 2);
+        t->n_ops.assign(n_ops, n_ops + n_terms);
+        for (int h = 0; h < n_terms; ++h) {
+            QMB_REQUIRE(n_ops[h] >= 0 && n_ops[h] <= max_ops, "qmb_terms_create: n_ops out of range");
+            int up = 0, dn = 0;
+            for (int k = 0; k < n_ops[h]; ++k) {
+                const int ty = op_type[(size_t)h * max_ops + k];
+                const int s = site[(size_t)h * max_ops + k];
+                if (ty != QMB_OP_SZ && ty != QMB_OP_SP && ty != QMB_OP_SM)
+                    throw Error{QMB_ERR_UNSUPPORTED, "qmb_terms_create: only Sz, Sp and Sm operators are supported"};
+                QMB_REQUIRE(s >= 0 && s < n_sites, "qmb_terms_create: operator site out of range");
+                up += ty == QMB_OP_SP;
+                dn += ty == QMB_OP_SM;
+            }
+            QMB_REQUIRE(up == dn, "qmb_terms_create: term " + std::to_string(h) + " does not conserve Sz");
+        }
+        *out = t.release();
+    });
+}
+
+qmb_status qmb_terms_destroy(qmb_terms *t) {
+    delete t;
+    return QMB_OK;
+}
+
+}  // extern "C"

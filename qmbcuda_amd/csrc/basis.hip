@@ -1,0 +1,289 @@
+// Symmetry-reduced basis on device.
+//
+// Replaces the reference's host Sz enumeration plus its five full-sector
+// kernels (BuildBasisStates, SetUpSRS, SolveSRS_min, sort/unique,
+// SetUpSRS_from_indices, ComputeSRSNorms, ComputeOrbitIndices:
+// Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:277-400,
+// Q/exact_diagonalization/utils.cu:39-80,716-805). Nothing of size
+// C(N,k)*G is materialised:
+//   mark    : one thread per 64 consecutive sector ranks; unrank the first,
+//             step with the next-combination trick, keep a state when it is
+//             the minimum of its orbit (and, in the compact layout, when the
+//             irrep's character is 1 on its stabiliser) -> one bitmap word;
+//   scan    : exclusive popcount scan of the words -> row offsets;
+//   compact : rewrite the kept states in ascending order with their
+//             stabiliser size and null flag.
+// The bitmap + offsets also serve as the rank -> row dictionary of the ELL
+// build (replacing the C(N,k)*2 orbit_indices table).
+#include <cmath>
+#include <memory>
+
+#include "device_common.hpp"
+
+qmb::GroupView qmb_basis::view() const {
+    qmb::GroupView v;
+    v.n_sites = group->n_sites;
+    v.G = group->G;
+    v.torus = group->torus;
+    v.block = group->block;
+    v.full = group->n_sites == 64 ? ~0ULL : ((1ULL << group->n_sites) - 1ULL);
+    v.keep = g_keep.as<uint64_t>();
+    v.small = g_small.as<uint8_t>();
+    v.big = g_big.as<uint8_t>();
+    v.flip = g_flip.as<uint8_t>();
+    v.perm = g_perm.as<uint8_t>();
+    v.inv = g_inv.as<uint8_t>();
+    return v;
+}
+
+namespace qmb {
+
+constexpr int kBasisThreads = 256;
+
+__device__ inline bool compatible(uint64_t slo, uint64_t shi, uint64_t tlo, uint64_t thi) {
+    return ((slo & ~tlo) | (shi & ~thi)) == 0;
+}
+
+__global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const uint64_t *binom, int n_up,
+                                                            int64_t dim, int64_t n_words, int compact,
+                                                            uint64_t triv_lo, uint64_t triv_hi,
+                                                            uint64_t *bitmap) {
+    __shared__ GroupLds lds;
+    __shared__ uint8_t perm_lds[kMaxGroup * 64];
+    __shared__ uint64_t bin[64 * kBinJ];
+    stage_group(lds, perm_lds, gv);
+    stage_binom(bin, binom);
+    __syncthreads();
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_words) return;
+    const int64_t r0 = w * 64;
+    const int cnt = (int)((dim - r0) < 64 ? (dim - r0) : 64);
+    uint64_t x = colex_unrank(r0, n_up, gv.n_sites, bin);
+    uint64_t bits = 0;
+    for (int i = 0; i < cnt; ++i) {
+        uint64_t slo, shi;
+        if (is_representative(x, gv, lds, perm_lds, slo, shi)) {
+            if (!compact || compatible(slo, shi, triv_lo, triv_hi)) bits |= 1ULL << i;
+        }
+        if (i + 1 < cnt) x = next_comb(x);
+    }
+    bitmap[w] = bits;
+}
+
+__global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, const uint64_t *binom, int n_up,
+                                                               int64_t dim, int64_t n_words,
+                                                               uint64_t triv_lo, uint64_t triv_hi,
+                                                               const uint64_t *bitmap, const uint32_t *prefix,
+                                                               uint64_t *reps, uint8_t *stab, uint8_t *nullf) {
+    __shared__ GroupLds lds;
+    __shared__ uint8_t perm_lds[kMaxGroup * 64];
+    __shared__ uint64_t bin[64 * kBinJ];
+    stage_group(lds, perm_lds, gv);
+    stage_binom(bin, binom);
+    __syncthreads();
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_words) return;
+    uint64_t bits = bitmap[w];
+    if (!bits) return;
+    int64_t pos = prefix[w];
+    const int64_t r0 = w * 64;
+    const int last = 63 - __builtin_clzll(bits);
+    uint64_t x = colex_unrank(r0, n_up, gv.n_sites, bin);
+    for (int i = 0; i <= last; ++i) {
+        if ((bits >> i) & 1ULL) {
+            uint64_t slo, shi;
+            is_representative(x, gv, lds, perm_lds, slo, shi);
+            reps[pos] = x;
+            stab[pos] = (uint8_t)(__popcll(slo) + __popcll(shi));
+            nullf[pos] = compatible(slo, shi, triv_lo, triv_hi) ? 0 : 1;
+            ++pos;
+        }
+        if (i < last) x = next_comb(x);
+    }
+}
+
+__global__ __launch_bounds__(kBasisThreads) void basis_lookup(GroupView gv, const uint64_t *binom, int64_t n,
+                                                              const uint64_t *states, const uint64_t *bitmap,
+                                                              const uint32_t *prefix, int64_t *rows,
+                                                              int32_t *elems) {
+    __shared__ GroupLds lds;
+    __shared__ uint8_t perm_lds[kMaxGroup * 64];
+    __shared__ uint64_t bin[64 * kBinJ];
+    stage_group(lds, perm_lds, gv);
+    stage_binom(bin, binom);
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int gm;
+    const uint64_t rep = orbit_min(states[i], gv, lds, perm_lds, gm);
+    rows[i] = row_of_rep(rep, bin, bitmap, prefix);
+    elems[i] = gm;
+}
+
+static void upload_group(qmb_basis *b, const qmb_group *g) {
+    const int G = g->G;
+    b->g_keep.alloc(G * sizeof(uint64_t));
+    b->g_small.alloc(G);
+    b->g_big.alloc(G);
+    b->g_flip.alloc(G);
+    b->g_inv.alloc(G);
+    b->g_perm.alloc((size_t)G * g->n_sites);
+    std::vector<uint8_t> inv8(G), perm8((size_t)G * g->n_sites);
+    for (int e = 0; e < G; ++e) inv8[e] = (uint8_t)g->inv[e];
+    for (size_t i = 0; i < perm8.size(); ++i) perm8[i] = (uint8_t)g->perm[i];
+    QMB_HIP(hipMemcpy(b->g_keep.p, g->keep.data(), G * sizeof(uint64_t), hipMemcpyHostToDevice));
+    QMB_HIP(hipMemcpy(b->g_small.p, g->small.data(), G, hipMemcpyHostToDevice));
+    QMB_HIP(hipMemcpy(b->g_big.p, g->big.data(), G, hipMemcpyHostToDevice));
+    QMB_HIP(hipMemcpy(b->g_flip.p, g->flip.data(), G, hipMemcpyHostToDevice));
+    QMB_HIP(hipMemcpy(b->g_inv.p, inv8.data(), G, hipMemcpyHostToDevice));
+    QMB_HIP(hipMemcpy(b->g_perm.p, perm8.data(), perm8.size(), hipMemcpyHostToDevice));
+}
+
+static uint64_t binom_host(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    uint64_t acc = 1;
+    for (int i = 1; i <= k; ++i) {
+        // exact: acc * (n-k+i) / i stays integral at every step
+        unsigned __int128 t = (unsigned __int128)acc * (uint64_t)(n - k + i);
+        acc = (uint64_t)(t / (uint64_t)i);
+    }
+    return acc;
+}
+
+}  // namespace qmb
+
+using namespace qmb;
+
+extern "C" {
+
+qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep, int layout, qmb_basis **out) {
+    return guarded([&] {
+        QMB_REQUIRE(out, "qmb_basis_build: out is NULL");
+        *out = nullptr;
+        QMB_REQUIRE(ctx && g, "qmb_basis_build: NULL context or group");
+        QMB_REQUIRE(n_up >= 0 && n_up <= g->n_sites, "qmb_basis_build: n_up out of range");
+        QMB_REQUIRE(n_up <= 32, "qmb_basis_build: at most 32 up spins (use the flipped sector)");
+        QMB_REQUIRE(irrep >= 0 && irrep < g->n_irreps, "qmb_basis_build: irrep out of range");
+        QMB_REQUIRE(layout == QMB_LAYOUT_REFERENCE || layout == QMB_LAYOUT_COMPACT, "qmb_basis_build: bad layout");
+        for (int e = 0; e < g->G; ++e)
+            QMB_REQUIRE(!g->flip[e] || 2 * n_up == g->n_sites,
+                        "qmb_basis_build: spin flip needs n_up == n_sites/2");
+        const uint64_t dim = binom_host(g->n_sites, n_up);
+        QMB_REQUIRE(dim > 0 && dim <= (1ULL << 38), "qmb_basis_build: sector too large for the rank bitmap");
+        QMB_HIP(hipSetDevice(ctx->device));
+        auto b = std::make_unique<qmb_basis>();
+        b->ctx = ctx;
+        b->group = g;
+        b->n_sites = g->n_sites;
+        b->n_up = n_up;
+        b->irrep = irrep;
+        b->layout = layout;
+        b->sector_dim = (int64_t)dim;
+        b->n_words = (int64_t)((dim + 63) / 64);
+        const double *ch = &g->chars[(size_t)irrep * g->G * 2];
+        for (int e = 0; e < g->G; ++e) {
+            const bool one = std::fabs(ch[2 * e] - 1.0) < 1e-9 && std::fabs(ch[2 * e + 1]) < 1e-9;
+            if (one) {
+                if (e < 64)
+                    b->triv_lo |= 1ULL << e;
+                else
+                    b->triv_hi |= 1ULL << (e - 64);
+            }
+        }
+        upload_group(b.get(), g);
+        const GroupView gv = b->view();
+        b->bitmap.alloc((size_t)b->n_words * sizeof(uint64_t));
+        b->prefix.alloc((size_t)b->n_words * sizeof(uint32_t));
+        const unsigned blocks = (unsigned)((b->n_words + kBasisThreads - 1) / kBasisThreads);
+        hipLaunchKernelGGL(basis_mark, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, gv,
+                           ctx->binom.as<uint64_t>(), n_up, b->sector_dim, b->n_words,
+                           layout == QMB_LAYOUT_COMPACT ? 1 : 0, b->triv_lo, b->triv_hi, b->bitmap.as<uint64_t>());
+        QMB_HIP(hipGetLastError());
+        const int64_t rows = scan_popcount_u32(ctx, b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), b->n_words);
+        QMB_REQUIRE(rows < (int64_t)INT32_MAX, "qmb_basis_build: more than 2^31 rows (int32 columns)");
+        b->n_rows = rows;
+        b->reps.alloc((size_t)std::max<int64_t>(rows, 1) * sizeof(uint64_t));
+        b->stab.alloc((size_t)std::max<int64_t>(rows, 1));
+        b->nullf.alloc((size_t)std::max<int64_t>(rows, 1));
+        if (rows > 0) {
+            hipLaunchKernelGGL(basis_compact, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, gv,
+                               ctx->binom.as<uint64_t>(), n_up, b->sector_dim, b->n_words, b->triv_lo, b->triv_hi,
+                               b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), b->reps.as<uint64_t>(),
+                               b->stab.as<uint8_t>(), b->nullf.as<uint8_t>());
+            QMB_HIP(hipGetLastError());
+        }
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        if (layout == QMB_LAYOUT_REFERENCE && rows > 0) {
+            std::vector<uint8_t> nf(rows);
+            QMB_HIP(hipMemcpy(nf.data(), b->nullf.p, rows, hipMemcpyDeviceToHost));
+            int64_t nn = 0;
+            for (auto v : nf) nn += v;
+            b->n_null = nn;
+        }
+        *out = b.release();
+    });
+}
+
+qmb_status qmb_basis_destroy(qmb_basis *b) {
+    if (b && b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return QMB_OK;
+}
+
+qmb_status qmb_basis_get_info(const qmb_basis *b, qmb_basis_info *info) {
+    return guarded([&] {
+        QMB_REQUIRE(b && info, "qmb_basis_get_info: NULL argument");
+        info->sector_dim = b->sector_dim;
+        info->n_rows = b->n_rows;
+        info->n_null = b->n_null;
+        info->n_sites = b->n_sites;
+        info->n_up = b->n_up;
+        info->irrep = b->irrep;
+        info->layout = b->layout;
+        info->device_bytes = (int64_t)(b->bitmap.bytes + b->prefix.bytes + b->reps.bytes + b->stab.bytes +
+                                       b->nullf.bytes);
+    });
+}
+
+qmb_status qmb_basis_export(const qmb_basis *b, uint64_t *reps, uint16_t *stab, uint8_t *is_null) {
+    return guarded([&] {
+        QMB_REQUIRE(b, "qmb_basis_export: NULL basis");
+        QMB_HIP(hipSetDevice(b->ctx->device));
+        QMB_HIP(hipStreamSynchronize(b->ctx->stream));
+        const int64_t n = b->n_rows;
+        if (n == 0) return;
+        if (reps) QMB_HIP(hipMemcpy(reps, b->reps.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        if (stab) {
+            std::vector<uint8_t> s(n);
+            QMB_HIP(hipMemcpy(s.data(), b->stab.p, n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) stab[i] = s[i];
+        }
+        if (is_null) QMB_HIP(hipMemcpy(is_null, b->nullf.p, n, hipMemcpyDeviceToHost));
+    });
+}
+
+qmb_status qmb_basis_lookup(const qmb_basis *b, int64_t n, const uint64_t *states, int64_t *row, int32_t *elem) {
+    return guarded([&] {
+        QMB_REQUIRE(b && states && row && elem, "qmb_basis_lookup: NULL argument");
+        QMB_REQUIRE(n >= 0, "qmb_basis_lookup: negative count");
+        if (n == 0) return;
+        const uint64_t full = b->n_sites == 64 ? ~0ULL : ((1ULL << b->n_sites) - 1ULL);
+        for (int64_t i = 0; i < n; ++i)
+            QMB_REQUIRE((states[i] & ~full) == 0 && __builtin_popcountll(states[i]) == b->n_up,
+                        "qmb_basis_lookup: state outside the sector");
+        qmb_ctx *ctx = b->ctx;
+        QMB_HIP(hipSetDevice(ctx->device));
+        DevBuf ds(n * sizeof(uint64_t)), dr(n * sizeof(int64_t)), de(n * sizeof(int32_t));
+        QMB_HIP(hipMemcpyAsync(ds.p, states, n * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+        const unsigned blocks = (unsigned)((n + kBasisThreads - 1) / kBasisThreads);
+        hipLaunchKernelGGL(basis_lookup, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, b->view(),
+                           ctx->binom.as<uint64_t>(), n, ds.as<uint64_t>(), b->bitmap.as<uint64_t>(),
+                           b->prefix.as<uint32_t>(), dr.as<int64_t>(), de.as<int32_t>());
+        QMB_HIP(hipGetLastError());
+        QMB_HIP(hipMemcpyAsync(row, dr.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipMemcpyAsync(elem, de.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"

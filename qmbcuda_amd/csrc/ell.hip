@@ -1,0 +1,337 @@
+// On-device ELL Hamiltonian assembly.
+//
+// Replaces BuildHamiltonianXXZForAbelianGroup (Q/exact_diagonalization/
+// utils.cu:83-184). Row semantics are the reference's, entry by entry:
+//   slot 0 = diagonal (column = the row itself), accumulated over terms whose
+//   conjugated operators map the representative onto itself;
+//   every other surviving term, in term order, gets its own slot (no merging):
+//   column = row of the target's orbit, value = (coef * chi(gm)) * (n_t / n_s)
+//   with gm the smallest element taking the orbit representative onto the
+//   target, n = sqrt(G*m)/G the orbit norm (m = stabiliser size).
+// Storage is SELL-64: rows in chunks of one wavefront, each chunk stored
+// slot-major and padded to its own longest row, so an SpMV wave reads every
+// slot with one coalesced load. Two passes: count -> chunk offsets -> fill.
+#include <cmath>
+#include <memory>
+
+#include "device_common.hpp"
+
+namespace qmb {
+
+struct TermsView {
+    int n_terms, max_ops;
+    const int8_t *op;      // [n_terms*max_ops]
+    const int8_t *site;    // [n_terms*max_ops]
+    const double2 *coef;   // [n_terms]
+    const int8_t *n_ops;   // [n_terms]
+};
+
+constexpr int kEllThreads = 256;
+
+template <bool FILL, bool CPLX>
+__global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
+                                                        int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
+                                                        const uint8_t *nullf, const uint64_t *bitmap,
+                                                        const uint32_t *prefix, TermsView tv, const double2 *chi,
+                                                        const double *nrm, int W, uint16_t *row_len, int *err,
+                                                        const int64_t *chunk_ptr, int32_t *col, double *val) {
+    __shared__ GroupLds lds;
+    __shared__ uint8_t perm_lds[kMaxGroup * 64];
+    __shared__ uint64_t bin[64 * kBinJ];
+    __shared__ double2 s_chi[kMaxGroup];
+    __shared__ double s_nrm[kMaxGroup + 1];
+    stage_group(lds, perm_lds, gv);
+    stage_binom(bin, binom);
+    for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
+    for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    const int64_t c = r / kChunk;
+    const int lane = (int)(r % kChunk);
+    int64_t base = 0;
+    int L = 0;
+    if (FILL) {
+        base = chunk_ptr[c];
+        L = (int)((chunk_ptr[c + 1] - base) / kChunk);
+    }
+    int slot = 1;
+    const bool null_row = (layout == QMB_LAYOUT_REFERENCE) && nullf[r];
+    double dr = 0.0, di = 0.0;
+    if (!null_row) {
+        const uint64_t src = reps[r];
+        const double n_src = s_nrm[stab[r]];
+        for (int h = 0; h < tv.n_terms; ++h) {
+            uint64_t tgt = src;
+            const double2 c0 = tv.coef[h];
+            double cr = c0.x, ci = c0.y;
+            bool dead = false;
+            const int nk = tv.n_ops[h];
+            for (int k = 0; k < nk; ++k) {
+                const int s = tv.site[h * tv.max_ops + k];
+                const int op = tv.op[h * tv.max_ops + k];
+                const int spin = (int)((tgt >> s) & 1ULL);
+                if (op == QMB_OP_SM) {
+                    if (!spin) { dead = true; break; }
+                    tgt &= ~(1ULL << s);
+                } else if (op == QMB_OP_SP) {
+                    if (spin) { dead = true; break; }
+                    tgt |= 1ULL << s;
+                } else {
+                    const double f = -1.0 + 2.0 * (double)spin;
+                    cr = cr * 0.5 * f;
+                    ci = ci * 0.5 * f;
+                }
+            }
+            if (dead) continue;
+            if (tgt == src) {
+                dr += cr;
+                di += ci;
+                continue;
+            }
+            int gm;
+            const uint64_t rep = orbit_min(tgt, gv, lds, perm_lds, gm);
+            const int64_t r2 = row_of_rep(rep, bin, bitmap, prefix);
+            if (r2 < 0) continue;
+            if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
+            if (FILL) {
+                if (slot < L) {
+                    const double ratio = s_nrm[stab[r2]] / n_src;
+                    const double2 x = s_chi[gm];
+                    const double pr = cr * x.x - ci * x.y;
+                    const double pi = cr * x.y + ci * x.x;
+                    const int64_t pos = base + (int64_t)slot * kChunk + lane;
+                    col[pos] = (int32_t)r2;
+                    if (CPLX) {
+                        val[2 * pos] = pr * ratio;
+                        val[2 * pos + 1] = pi * ratio;
+                    } else {
+                        val[pos] = pr * ratio;
+                    }
+                }
+            }
+            ++slot;
+        }
+    }
+    const int len = null_row ? 0 : slot;
+    if (!FILL) {
+        if (len > W) atomicMax(err, 1);
+        row_len[r] = (uint16_t)(len < 65535 ? len : 65535);
+        return;
+    }
+    // diagonal slot (also written for empty rows, like the reference's ind[id*W] = id)
+    if (L > 0) {
+        const int64_t pos0 = base + lane;
+        col[pos0] = (int32_t)r;
+        if (CPLX) {
+            val[2 * pos0] = null_row ? 0.0 : dr;
+            val[2 * pos0 + 1] = null_row ? 0.0 : di;
+        } else {
+            val[pos0] = null_row ? 0.0 : dr;
+        }
+    }
+    for (int s = (len > 1 ? len : 1); s < L; ++s) {
+        const int64_t pos = base + (int64_t)s * kChunk + lane;
+        col[pos] = (int32_t)r;
+        if (CPLX) {
+            val[2 * pos] = 0.0;
+            val[2 * pos + 1] = 0.0;
+        } else {
+            val[pos] = 0.0;
+        }
+    }
+}
+
+// Padding entries of the rows past n_rows in the last chunk.
+__global__ void ell_pad_tail(int64_t n_rows, int64_t n_chunks, const int64_t *chunk_ptr, int32_t *col, double *val,
+                             int cplx) {
+    const int lane = threadIdx.x;
+    const int64_t c = n_chunks - 1;
+    const int64_t r = c * kChunk + lane;
+    if (r < n_rows) return;
+    const int64_t base = chunk_ptr[c];
+    const int L = (int)((chunk_ptr[c + 1] - base) / kChunk);
+    for (int s = 0; s < L; ++s) {
+        const int64_t pos = base + (int64_t)s * kChunk + lane;
+        col[pos] = 0;
+        if (cplx) {
+            val[2 * pos] = 0.0;
+            val[2 * pos + 1] = 0.0;
+        } else {
+            val[pos] = 0.0;
+        }
+    }
+}
+
+}  // namespace qmb
+
+using namespace qmb;
+
+double qmb_ell::spmv_bytes() const {
+    const double sv = value_type == QMB_COMPLEX ? 16.0 : 8.0;
+    return (double)nnz * (4.0 + sv) + (double)n_rows * (2.0 * sv) + (double)n_rows * 2.0;
+}
+
+int64_t qmb_ell::device_bytes() const {
+    return (int64_t)(chunk_ptr.bytes + row_len.bytes + col.bytes + val.bytes);
+}
+
+extern "C" {
+
+qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, int value_type, int max_terms,
+                         qmb_ell **out) {
+    return guarded([&] {
+        QMB_REQUIRE(out, "qmb_ell_build: out is NULL");
+        *out = nullptr;
+        QMB_REQUIRE(ctx && b && t, "qmb_ell_build: NULL argument");
+        QMB_REQUIRE(b->ctx == ctx, "qmb_ell_build: basis belongs to another context");
+        QMB_REQUIRE(t->n_sites == b->n_sites, "qmb_ell_build: term table and basis disagree on n_sites");
+        QMB_REQUIRE(value_type == QMB_REAL || value_type == QMB_COMPLEX, "qmb_ell_build: bad value type");
+        const qmb_group *g = b->group;
+        const double *ch = &g->chars[(size_t)b->irrep * g->G * 2];
+        if (value_type == QMB_REAL) {
+            for (int e = 0; e < g->G; ++e)
+                QMB_REQUIRE(ch[2 * e + 1] == 0.0, "qmb_ell_build: REAL values need a real character row");
+            for (int h = 0; h < t->n_terms; ++h)
+                QMB_REQUIRE(t->scalar[2 * h + 1] == 0.0, "qmb_ell_build: REAL values need real term scalars");
+        }
+        int W = max_terms;
+        if (W <= 0) W = t->n_terms * b->n_up / b->n_sites + 1;  // reference rule
+        QMB_REQUIRE(W <= 65535, "qmb_ell_build: max_terms too large");
+        QMB_HIP(hipSetDevice(ctx->device));
+        auto e = std::make_unique<qmb_ell>();
+        e->ctx = ctx;
+        e->basis = b;
+        e->value_type = value_type;
+        e->max_terms = W;
+        e->n_rows = b->n_rows;
+        e->n_chunks = (b->n_rows + kChunk - 1) / kChunk;
+        // term table and irrep data on device
+        const int nt = t->n_terms, mo = t->max_ops;
+        std::vector<int8_t> op8((size_t)std::max(1, nt * mo)), site8((size_t)std::max(1, nt * mo)),
+            nops8((size_t)std::max(1, nt));
+        std::vector<double> coef((size_t)std::max(1, nt) * 2, 0.0);
+        for (int i = 0; i < nt * mo; ++i) {
+            op8[i] = (int8_t)t->op_type[i];
+            site8[i] = (int8_t)(t->site[i] < 0 ? 0 : t->site[i]);
+        }
+        for (int h = 0; h < nt; ++h) {
+            nops8[h] = (int8_t)t->n_ops[h];
+            coef[2 * h] = t->scalar[2 * h];
+            coef[2 * h + 1] = t->scalar[2 * h + 1];
+        }
+        std::vector<double> nrm(g->G + 1, 0.0);
+        for (int m = 1; m <= g->G; ++m) nrm[m] = std::sqrt((double)g->G * (double)m) / (double)g->G;
+        DevBuf d_op(op8.size()), d_site(site8.size()), d_nops(nops8.size()), d_coef(coef.size() * sizeof(double)),
+            d_chi((size_t)g->G * 2 * sizeof(double)), d_nrm(nrm.size() * sizeof(double)), d_err(sizeof(int));
+        QMB_HIP(hipMemcpy(d_op.p, op8.data(), op8.size(), hipMemcpyHostToDevice));
+        QMB_HIP(hipMemcpy(d_site.p, site8.data(), site8.size(), hipMemcpyHostToDevice));
+        QMB_HIP(hipMemcpy(d_nops.p, nops8.data(), nops8.size(), hipMemcpyHostToDevice));
+        QMB_HIP(hipMemcpy(d_coef.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
+        QMB_HIP(hipMemcpy(d_chi.p, ch, (size_t)g->G * 2 * sizeof(double), hipMemcpyHostToDevice));
+        QMB_HIP(hipMemcpy(d_nrm.p, nrm.data(), nrm.size() * sizeof(double), hipMemcpyHostToDevice));
+        QMB_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), ctx->stream));
+        TermsView tv{nt, mo, d_op.as<int8_t>(), d_site.as<int8_t>(), d_coef.as<double2>(), d_nops.as<int8_t>()};
+        const GroupView gv = b->view();
+        const int64_t n = b->n_rows;
+        e->row_len.alloc((size_t)std::max<int64_t>(n, 1) * sizeof(uint16_t));
+        e->chunk_ptr.alloc((size_t)(e->n_chunks + 1) * sizeof(int64_t));
+        const unsigned blocks = (unsigned)std::max<int64_t>(1, (n + kEllThreads - 1) / kEllThreads);
+        const bool cplx = value_type == QMB_COMPLEX;
+        if (n > 0) {
+            auto kcount = cplx ? ell_rows<false, true> : ell_rows<false, false>;
+            hipLaunchKernelGGL(kcount, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
+                               ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
+                               b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
+                               b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
+                               e->row_len.as<uint16_t>(), d_err.as<int>(), nullptr, nullptr, nullptr);
+            QMB_HIP(hipGetLastError());
+        }
+        int herr = 0;
+        QMB_HIP(hipMemcpyAsync(&herr, d_err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        QMB_REQUIRE(herr == 0, "qmb_ell_build: a row needs more than max_terms = " + std::to_string(W) + " slots");
+        int max_len = 0;
+        e->stored = n > 0 ? scan_chunks_i64(ctx, e->row_len.as<uint16_t>(), n, e->chunk_ptr.as<int64_t>(),
+                                            e->n_chunks, &e->nnz, &max_len)
+                          : 0;
+        e->max_row_len = max_len;
+        e->col.alloc((size_t)std::max<int64_t>(e->stored, 1) * sizeof(int32_t));
+        e->val.alloc((size_t)std::max<int64_t>(e->stored, 1) * (cplx ? 16 : 8));
+        if (n > 0) {
+            auto kfill = cplx ? ell_rows<true, true> : ell_rows<true, false>;
+            hipLaunchKernelGGL(kfill, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
+                               ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
+                               b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
+                               b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
+                               e->row_len.as<uint16_t>(), d_err.as<int>(), e->chunk_ptr.as<int64_t>(),
+                               e->col.as<int32_t>(), e->val.as<double>());
+            QMB_HIP(hipGetLastError());
+            if (n % kChunk) {
+                hipLaunchKernelGGL(ell_pad_tail, dim3(1), dim3(kChunk), 0, ctx->stream, n, e->n_chunks,
+                                   e->chunk_ptr.as<int64_t>(), e->col.as<int32_t>(), e->val.as<double>(),
+                                   cplx ? 1 : 0);
+                QMB_HIP(hipGetLastError());
+            }
+        }
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        *out = e.release();
+    });
+}
+
+qmb_status qmb_ell_destroy(qmb_ell *e) {
+    if (e && e->ctx) (void)hipStreamSynchronize(e->ctx->stream);
+    delete e;
+    return QMB_OK;
+}
+
+qmb_status qmb_ell_get_info(const qmb_ell *e, qmb_ell_info *info) {
+    return guarded([&] {
+        QMB_REQUIRE(e && info, "qmb_ell_get_info: NULL argument");
+        info->n_rows = e->n_rows;
+        info->nnz = e->nnz;
+        info->stored = e->stored;
+        info->max_row_len = e->max_row_len;
+        info->max_terms = e->max_terms;
+        info->value_type = e->value_type;
+        info->chunk_rows = kChunk;
+        info->device_bytes = e->device_bytes();
+        info->spmv_bytes = e->spmv_bytes();
+    });
+}
+
+qmb_status qmb_ell_export_reference(const qmb_ell *e, uint32_t *ind, double *val, int16_t *track) {
+    return guarded([&] {
+        QMB_REQUIRE(e && ind && val && track, "qmb_ell_export_reference: NULL argument");
+        QMB_HIP(hipSetDevice(e->ctx->device));
+        QMB_HIP(hipStreamSynchronize(e->ctx->stream));
+        const int64_t n = e->n_rows;
+        const int W = e->max_terms;
+        const int vs = e->value_type == QMB_COMPLEX ? 2 : 1;
+        std::fill(ind, ind + (size_t)n * W, 0u);
+        std::fill(val, val + (size_t)n * W * vs, 0.0);
+        std::fill(track, track + n, (int16_t)0);
+        if (n == 0) return;
+        std::vector<int64_t> cp(e->n_chunks + 1);
+        std::vector<uint16_t> rl(n);
+        std::vector<int32_t> cl(e->stored);
+        std::vector<double> vl((size_t)e->stored * vs);
+        QMB_HIP(hipMemcpy(cp.data(), e->chunk_ptr.p, cp.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+        QMB_HIP(hipMemcpy(rl.data(), e->row_len.p, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+        QMB_HIP(hipMemcpy(cl.data(), e->col.p, cl.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        QMB_HIP(hipMemcpy(vl.data(), e->val.p, vl.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t r = 0; r < n; ++r) {
+            const int64_t c = r / kChunk, lane = r % kChunk;
+            ind[(size_t)r * W] = (uint32_t)r;
+            const int len = rl[r];
+            track[r] = (int16_t)len;
+            for (int s = 0; s < len; ++s) {
+                const int64_t pos = cp[c] + (int64_t)s * kChunk + lane;
+                ind[(size_t)r * W + s] = (uint32_t)cl[pos];
+                for (int q = 0; q < vs; ++q) val[((size_t)r * W + s) * vs + q] = vl[(size_t)pos * vs + q];
+            }
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,837 @@
+// ELL SpMV and the FP64 Lanczos loop.
+//
+// Replaces SparseMatDenseVec_prod_EEL (Q/exact_diagonalization/utils.cu:601-624)
+// and LanczosEigVals_CUDA / LanczosEigenStateFromSeed (Q/exact_diagonalization/
+// lanczos.cu:31-317). The recurrence is the reference's, in FP64:
+//   b[0] = 0; phi_0 = start / ||start||;
+//   y = H phi_n; a[n] = Re<phi_n, y>; w = (-a[n]) phi_n + (y + (-b[n]) phi_{n-1});
+//   b[n+1] = ||w||; phi_{n+1} = w * (1/b[n+1]).
+// Per step there are two kernels and no host round trip:
+//   K1 spmv_sell<LZ>: y = H (w * 1/b) gathered on the fly, phi_n = w/b written
+//      out, a[n] reduced in the same pass (block partials + last-block sum);
+//   K2 lz_update: w = (-a) phi_n + (y - b phi_{n-1}), ||w||^2 reduced the same
+//      way -> b[n+1] and 1/b[n+1] stay on device.
+// The host reads a[], b[] every check_every steps and solves the tridiagonal
+// problem in FP64 (lanczos_host.cpp). Reductions are deterministic: fixed
+// block partial slots, summed in a fixed order by whichever block finishes last.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "internal.hpp"
+
+namespace qmb {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxBlocks = 4096;
+
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Sum over the block, valid in thread 0.
+__device__ inline double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) lds[wv] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kWaves; ++i) t += lds[i];
+    __syncthreads();
+    return t;
+}
+
+// Writes the block's partial and returns true in the block that finished last.
+__device__ inline bool publish_partial(double v_thread0, double *partials, unsigned *counter) {
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = v_thread0;
+        __threadfence();
+        const unsigned prev = atomicAdd(counter, 1u);
+        is_last = (prev == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last != 0;
+}
+
+// Deterministic sum of partials[0..n) by one block (fixed order).
+__device__ inline double sum_partials(const double *partials, int n, double *lds) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += __builtin_nontemporal_load(partials + i);
+    return block_sum(s, lds);
+}
+
+// ---- K1: SELL-64 SpMV --------------------------------------------------------
+// One wavefront per 64-row chunk; slot-major storage makes every slot one
+// coalesced 256 B (col) + 512 B/1 KB (val) read; the x gathers are the only
+// random accesses. LZ fuses the Lanczos normalisation and the a[n] dot.
+template <bool CPLX, bool LZ>
+__global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_chunks,
+                                                      const int64_t *__restrict__ chunk_ptr,
+                                                      const uint16_t *__restrict__ row_len,
+                                                      const int32_t *__restrict__ col,
+                                                      const double *__restrict__ val,
+                                                      const double *__restrict__ x, double *__restrict__ y,
+                                                      const double *__restrict__ invb, double *__restrict__ vout,
+                                                      double *__restrict__ partials, unsigned *__restrict__ counter,
+                                                      double *__restrict__ alpha_out) {
+    __shared__ double lds[kWaves];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const double s = LZ ? *invb : 1.0;
+    double dot = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * kWaves + wv; c < n_chunks; c += (int64_t)gridDim.x * kWaves) {
+        const int64_t r = c * 64 + lane;
+        const int64_t base = chunk_ptr[c];
+        const int L = (int)((chunk_ptr[c + 1] - base) >> 6);
+        const int len = r < n_rows ? (int)row_len[r] : 0;
+        const int64_t p0 = base + lane;
+        if (!CPLX) {
+            double acc = 0.0;
+            int k = 0;
+            for (; k + 4 <= L; k += 4) {
+                int32_t j[4];
+                double v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (k + u < len) {
+                        const int64_t p = p0 + (int64_t)(k + u) * 64;
+                        j[u] = __builtin_nontemporal_load(col + p);
+                        v[u] = __builtin_nontemporal_load(val + p);
+                    }
+                }
+                double g[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (k + u < len) g[u] = LZ ? x[j[u]] * s : x[j[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (k + u < len) acc += v[u] * g[u];
+            }
+            for (; k < L; ++k) {
+                if (k < len) {
+                    const int64_t p = p0 + (int64_t)k * 64;
+                    const int32_t jj = __builtin_nontemporal_load(col + p);
+                    const double vv = __builtin_nontemporal_load(val + p);
+                    const double gg = LZ ? x[jj] * s : x[jj];
+                    acc += vv * gg;
+                }
+            }
+            if (r < n_rows) {
+                y[r] = acc;
+                if (LZ) {
+                    const double vr = x[r] * s;
+                    vout[r] = vr;
+                    dot += vr * acc;
+                }
+            }
+        } else {
+            const double2 *val2 = reinterpret_cast<const double2 *>(val);
+            const double2 *x2 = reinterpret_cast<const double2 *>(x);
+            double ar = 0.0, ai = 0.0;
+            int k = 0;
+            for (; k + 4 <= L; k += 4) {
+                int32_t j[4];
+                double2 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (k + u < len) {
+                        const int64_t p = p0 + (int64_t)(k + u) * 64;
+                        j[u] = __builtin_nontemporal_load(col + p);
+                        v[u] = val2[p];
+                    }
+                }
+                double2 g[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (k + u < len) {
+                        g[u] = x2[j[u]];
+                        if (LZ) {
+                            g[u].x = g[u].x * s;
+                            g[u].y = g[u].y * s;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (k + u < len) {
+                        ar += v[u].x * g[u].x - v[u].y * g[u].y;
+                        ai += v[u].x * g[u].y + v[u].y * g[u].x;
+                    }
+                }
+            }
+            for (; k < L; ++k) {
+                if (k < len) {
+                    const int64_t p = p0 + (int64_t)k * 64;
+                    const int32_t jj = __builtin_nontemporal_load(col + p);
+                    const double2 vv = val2[p];
+                    double2 gg = x2[jj];
+                    if (LZ) {
+                        gg.x = gg.x * s;
+                        gg.y = gg.y * s;
+                    }
+                    ar += vv.x * gg.x - vv.y * gg.y;
+                    ai += vv.x * gg.y + vv.y * gg.x;
+                }
+            }
+            if (r < n_rows) {
+                reinterpret_cast<double2 *>(y)[r] = make_double2(ar, ai);
+                if (LZ) {
+                    double2 vr = x2[r];
+                    vr.x = vr.x * s;
+                    vr.y = vr.y * s;
+                    reinterpret_cast<double2 *>(vout)[r] = vr;
+                    dot += vr.x * ar + vr.y * ai;  // Re(conj(v) y)
+                }
+            }
+        }
+    }
+    if (LZ) {
+        const double t = block_sum(dot, lds);
+        if (publish_partial(t, partials, counter)) {
+            const double a = sum_partials(partials, gridDim.x, lds);
+            if (threadIdx.x == 0) {
+                *alpha_out = a;
+                *counter = 0u;
+            }
+        }
+    }
+}
+
+// ---- K2: three-term update + ||w||^2 ----------------------------------------
+__global__ __launch_bounds__(kThreads) void lz_update(int64_t cnt, const double *__restrict__ y,
+                                                      const double *__restrict__ v,
+                                                      const double *__restrict__ vprev, double *__restrict__ w,
+                                                      const double *__restrict__ alpha_p,
+                                                      const double *__restrict__ beta_p,
+                                                      double *__restrict__ partials, unsigned *__restrict__ counter,
+                                                      double *__restrict__ beta_next, double *__restrict__ invb_next) {
+    __shared__ double lds[kWaves];
+    const double a = *alpha_p, b = *beta_p;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+        const double t = y[i] + (-b) * vprev[i];
+        const double wi = (-a) * v[i] + t;
+        w[i] = wi;
+        acc += wi * wi;
+    }
+    const double tb = block_sum(acc, lds);
+    if (publish_partial(tb, partials, counter)) {
+        const double s = sum_partials(partials, gridDim.x, lds);
+        if (threadIdx.x == 0) {
+            const double nb = sqrt(s);
+            *beta_next = nb;
+            *invb_next = nb > 0.0 ? 1.0 / nb : 0.0;
+            *counter = 0u;
+        }
+    }
+}
+
+// ---- start vector, norms, eigenvector helpers -------------------------------
+// Counter-based SplitMix64 (Steele, Lea, Flood 2014) -> uniform (0, 1].
+__device__ inline double splitmix_uniform(uint64_t seed, uint64_t c) {
+    uint64_t z = seed + (c + 1ULL) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return (double)((z >> 11) + 1ULL) * (1.0 / 9007199254740992.0);
+}
+
+__global__ __launch_bounds__(kThreads) void lz_random(int64_t cnt, uint64_t seed, double *w) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+        w[i] = splitmix_uniform(seed, (uint64_t)i);
+}
+
+// out_norm = ||a - e*b|| (b may be NULL -> ||a||); out_inv = 1/out_norm.
+__global__ __launch_bounds__(kThreads) void vec_norm(int64_t cnt, const double *__restrict__ a,
+                                                     const double *__restrict__ b, double e,
+                                                     double *__restrict__ partials, unsigned *__restrict__ counter,
+                                                     double *__restrict__ out_norm, double *__restrict__ out_inv) {
+    __shared__ double lds[kWaves];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+        const double d = b ? a[i] - e * b[i] : a[i];
+        acc += d * d;
+    }
+    const double t = block_sum(acc, lds);
+    if (publish_partial(t, partials, counter)) {
+        const double s = sum_partials(partials, gridDim.x, lds);
+        if (threadIdx.x == 0) {
+            const double nr = sqrt(s);
+            if (out_norm) *out_norm = nr;
+            if (out_inv) *out_inv = nr > 0.0 ? 1.0 / nr : 0.0;
+            *counter = 0u;
+        }
+    }
+}
+
+// x = x + c * v (the reference's saxpy_functor2 accumulation of the eigenvector).
+__global__ __launch_bounds__(kThreads) void vec_axpy(int64_t cnt, double c, const double *__restrict__ v,
+                                                     double *__restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = x[i] + c * v[i];
+}
+
+// ---- full reorthogonalisation (classical Gram-Schmidt, applied twice) -------
+// dots: coef[j] = <V_j, w> for j < nv; grid (blocks, nv).
+template <bool CPLX>
+__global__ __launch_bounds__(kThreads) void reorth_dots(int64_t n, int64_t stride, const double *__restrict__ V,
+                                                        const double *__restrict__ w, double *__restrict__ part,
+                                                        int nb) {
+    __shared__ double lds[kWaves];
+    const int j = blockIdx.y;
+    const double *vj = V + (size_t)j * stride;
+    double sr = 0.0, si = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (CPLX) {
+            const double vr = vj[2 * i], vi = vj[2 * i + 1], wr = w[2 * i], wi = w[2 * i + 1];
+            sr += vr * wr + vi * wi;
+            si += vr * wi - vi * wr;
+        } else {
+            sr += vj[i] * w[i];
+        }
+    }
+    const double tr = block_sum(sr, lds);
+    const double ti = CPLX ? block_sum(si, lds) : 0.0;
+    if (threadIdx.x == 0) {
+        part[((size_t)j * nb + blockIdx.x) * 2] = tr;
+        part[((size_t)j * nb + blockIdx.x) * 2 + 1] = ti;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void reorth_reduce(const double *__restrict__ part, int nb,
+                                                          double *__restrict__ coef) {
+    __shared__ double lds[kWaves];
+    const int j = blockIdx.x;
+    double sr = 0.0, si = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        sr += part[((size_t)j * nb + i) * 2];
+        si += part[((size_t)j * nb + i) * 2 + 1];
+    }
+    const double tr = block_sum(sr, lds);
+    const double ti = block_sum(si, lds);
+    if (threadIdx.x == 0) {
+        coef[2 * j] = tr;
+        coef[2 * j + 1] = ti;
+    }
+}
+
+// w -= sum_j coef_j V_j; ||w|| -> beta_next, invb_next.
+template <bool CPLX>
+__global__ __launch_bounds__(kThreads) void reorth_apply(int64_t n, int64_t stride, int nv,
+                                                         const double *__restrict__ V,
+                                                         const double *__restrict__ coef, double *__restrict__ w,
+                                                         double *__restrict__ partials, unsigned *__restrict__ counter,
+                                                         double *__restrict__ beta_next,
+                                                         double *__restrict__ invb_next) {
+    __shared__ double lds[kWaves];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (CPLX) {
+            double wr = w[2 * i], wi = w[2 * i + 1];
+            for (int j = 0; j < nv; ++j) {
+                const double cr = coef[2 * j], ci = coef[2 * j + 1];
+                const double vr = V[(size_t)j * stride + 2 * i], vi = V[(size_t)j * stride + 2 * i + 1];
+                wr -= cr * vr - ci * vi;
+                wi -= cr * vi + ci * vr;
+            }
+            w[2 * i] = wr;
+            w[2 * i + 1] = wi;
+            acc += wr * wr + wi * wi;
+        } else {
+            double wr = w[i];
+            for (int j = 0; j < nv; ++j) wr -= coef[2 * j] * V[(size_t)j * stride + i];
+            w[i] = wr;
+            acc += wr * wr;
+        }
+    }
+    const double t = block_sum(acc, lds);
+    if (publish_partial(t, partials, counter)) {
+        const double s = sum_partials(partials, gridDim.x, lds);
+        if (threadIdx.x == 0) {
+            const double nb = sqrt(s);
+            *beta_next = nb;
+            *invb_next = nb > 0.0 ? 1.0 / nb : 0.0;
+            *counter = 0u;
+        }
+    }
+}
+
+// x = sum_j z_j V_j (eigenvector from a stored basis).
+__global__ __launch_bounds__(kThreads) void basis_combine(int64_t cnt, int64_t stride, int nv,
+                                                          const double *__restrict__ V, const double *__restrict__ z,
+                                                          double *__restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < nv; ++j) s = s + z[j] * V[(size_t)j * stride + i];
+        x[i] = s;
+    }
+}
+
+// ---- launch helpers -----------------------------------------------------------
+static unsigned grid_for(int64_t work_items, int per_block) {
+    const int64_t b = (work_items + per_block - 1) / per_block;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, kMaxBlocks));
+}
+
+struct LzWork {
+    DevBuf partials, counter, scal;  // scal: alpha[M], beta[M+1], invb[M+1], misc[8]
+    int M = 0;
+    double *alpha() const { return scal.as<double>(); }
+    double *beta() const { return scal.as<double>() + M; }
+    double *invb() const { return scal.as<double>() + 2 * M + 1; }
+    double *misc() const { return scal.as<double>() + 3 * M + 2; }
+    void init(qmb_ctx *ctx, int max_iter) {
+        M = max_iter;
+        partials.alloc(kMaxBlocks * sizeof(double) * 2);
+        counter.alloc(sizeof(unsigned) * 4);
+        scal.alloc((size_t)(3 * M + 2 + 8) * sizeof(double));
+        QMB_HIP(hipMemsetAsync(counter.p, 0, sizeof(unsigned) * 4, ctx->stream));
+        QMB_HIP(hipMemsetAsync(scal.p, 0, (size_t)(3 * M + 2 + 8) * sizeof(double), ctx->stream));
+    }
+};
+
+void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, bool lz, const double *invb,
+                 double *vout, LzWork *wk, double *alpha_out) {
+    const unsigned blocks = grid_for(e->n_chunks, kWaves);
+    const bool cplx = e->value_type == QMB_COMPLEX;
+    auto k = cplx ? (lz ? spmv_sell<true, true> : spmv_sell<true, false>)
+                  : (lz ? spmv_sell<false, true> : spmv_sell<false, false>);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, ctx->stream, e->n_rows, e->n_chunks,
+                       e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
+                       e->val.as<double>(), x, y, invb, vout, wk ? wk->partials.as<double>() : nullptr,
+                       wk ? wk->counter.as<unsigned>() : nullptr, alpha_out);
+    QMB_HIP(hipGetLastError());
+}
+
+static void launch_norm(qmb_ctx *ctx, int64_t cnt, const double *a, const double *b, double e, LzWork &wk,
+                        double *out_norm, double *out_inv) {
+    hipLaunchKernelGGL(vec_norm, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt, a, b, e,
+                       wk.partials.as<double>(), wk.counter.as<unsigned>() + 1, out_norm, out_inv);
+    QMB_HIP(hipGetLastError());
+}
+
+}  // namespace qmb
+
+using namespace qmb;
+
+namespace {
+
+struct Timer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+// The Lanczos engine shared by qmb_lanczos and the reference drop-in.
+qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o, double *evals, double *krylov,
+                       double *eigvec, qmb_lanczos_report *rep) {
+    Timer wall;
+    const bool cplx = e->value_type == QMB_COMPLEX;
+    const int64_t n = e->n_rows;
+    const int64_t cnt = cplx ? 2 * n : n;
+    const int M = o.max_iter;
+    const int k_eigs = o.n_eigs;
+    const bool reorth = o.reorth != 0;
+    const int check = o.fixed_iter ? M : (o.check_every > 0 ? o.check_every : 10);
+    const double tol = o.tol > 0 ? o.tol : 1e-13;
+    QMB_HIP(hipSetDevice(ctx->device));
+    LzWork wk;
+    wk.init(ctx, M);
+    DevBuf P(cnt * sizeof(double)), Y(cnt * sizeof(double)), Z(cnt * sizeof(double));
+    DevBuf V0, V1, VB;
+    if (reorth) {
+        VB.alloc((size_t)M * cnt * sizeof(double));
+    } else {
+        V0.alloc(cnt * sizeof(double));
+        V1.alloc(cnt * sizeof(double));
+    }
+    QMB_HIP(hipMemsetAsync(Z.p, 0, cnt * sizeof(double), ctx->stream));
+    DevBuf rpart, rcoef;
+    const unsigned rblocks = grid_for(n, kThreads * 4);
+    if (reorth) {
+        rpart.alloc((size_t)M * rblocks * 2 * sizeof(double));
+        rcoef.alloc((size_t)M * 2 * sizeof(double));
+    }
+    auto vcur = [&](int it) -> double * {
+        if (reorth) return VB.as<double>() + (size_t)it * cnt;
+        return (it & 1) ? V1.as<double>() : V0.as<double>();
+    };
+    auto vprev = [&](int it) -> const double * {
+        if (it == 0) return Z.as<double>();
+        return vcur(it - 1);
+    };
+    // start vector into P, 1/||P|| into invb[0]; beta[0] stays 0 (T's b[0])
+    auto init_start = [&]() {
+        if (o.start == QMB_START_USER) {
+            QMB_REQUIRE(o.start_vec != nullptr, "qmb_lanczos: start == USER needs start_vec");
+            QMB_HIP(hipMemcpyAsync(P.p, o.start_vec, cnt * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        } else {
+            hipLaunchKernelGGL(lz_random, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt,
+                               o.seed, P.as<double>());
+            QMB_HIP(hipGetLastError());
+        }
+        launch_norm(ctx, cnt, P.as<double>(), nullptr, 0.0, wk, wk.misc(), wk.invb());
+    };
+    const unsigned ublocks = grid_for(cnt, kThreads * 4);
+    auto k1 = [&](int it) {
+        launch_spmv(ctx, e, P.as<double>(), Y.as<double>(), true, wk.invb() + it, vcur(it), &wk, wk.alpha() + it);
+    };
+    auto k2 = [&](int it) {
+        hipLaunchKernelGGL(lz_update, dim3(ublocks), dim3(kThreads), 0, ctx->stream, cnt, Y.as<double>(), vcur(it),
+                           vprev(it), P.as<double>(), wk.alpha() + it, wk.beta() + it, wk.partials.as<double>(),
+                           wk.counter.as<unsigned>() + 2, wk.beta() + it + 1, wk.invb() + it + 1);
+        QMB_HIP(hipGetLastError());
+        if (reorth) {
+            const int nv = it + 1;
+            for (int pass = 0; pass < 2; ++pass) {
+                auto kd = cplx ? reorth_dots<true> : reorth_dots<false>;
+                hipLaunchKernelGGL(kd, dim3(rblocks, nv), dim3(kThreads), 0, ctx->stream, n, (int64_t)cnt,
+                                   VB.as<double>(), P.as<double>(), rpart.as<double>(), (int)rblocks);
+                QMB_HIP(hipGetLastError());
+                hipLaunchKernelGGL(reorth_reduce, dim3(nv), dim3(kThreads), 0, ctx->stream, rpart.as<double>(),
+                                   (int)rblocks, rcoef.as<double>());
+                QMB_HIP(hipGetLastError());
+                auto ka = cplx ? reorth_apply<true> : reorth_apply<false>;
+                hipLaunchKernelGGL(ka, dim3(rblocks), dim3(kThreads), 0, ctx->stream, n, (int64_t)cnt, nv,
+                                   VB.as<double>(), rcoef.as<double>(), P.as<double>(), wk.partials.as<double>(),
+                                   wk.counter.as<unsigned>() + 3, wk.beta() + it + 1, wk.invb() + it + 1);
+                QMB_HIP(hipGetLastError());
+            }
+        }
+    };
+
+    // profiling events (one pair per SpMV in a check window)
+    std::vector<hipEvent_t> ev;
+    if (o.profile) {
+        ev.resize(2 * (size_t)check);
+        for (auto &x : ev) QMB_HIP(hipEventCreate(&x));
+    }
+    double ms_spmv = 0.0, ms_host = 0.0;
+    int n_spmv = 0;
+
+    init_start();
+    std::vector<double> ha(M), hb(M + 1);
+    std::vector<double> w(k_eigs + 1), zl(k_eigs + 1), z0(M);
+    int it = 0;
+    int m_eff = 0;
+    bool converged = false, breakdown = false;
+    while (it < M) {
+        const int win0 = it;
+        const int win1 = std::min(M, it + check);
+        for (; it < win1; ++it) {
+            if (o.profile) QMB_HIP(hipEventRecord(ev[2 * (it - win0)], ctx->stream));
+            k1(it);
+            if (o.profile) QMB_HIP(hipEventRecord(ev[2 * (it - win0) + 1], ctx->stream));
+            k2(it);
+            ++n_spmv;
+        }
+        QMB_HIP(hipMemcpyAsync(ha.data(), wk.alpha(), it * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipMemcpyAsync(hb.data(), wk.beta(), (it + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        if (o.profile) {
+            for (int q = 0; q < it - win0; ++q) {
+                float t = 0.f;
+                QMB_HIP(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
+                ms_spmv += t;
+            }
+        }
+        Timer th;
+        // breakdown: an (almost) zero b[j] ends the Krylov space at j
+        double tsc = 0.0;
+        for (int j = 0; j < it; ++j) tsc = std::max(tsc, std::fabs(ha[j]) + hb[j] + hb[j + 1]);
+        m_eff = it;
+        for (int j = 1; j <= it; ++j) {
+            if (!(hb[j] > 1e-13 * std::max(tsc, 1e-300))) {
+                m_eff = j;
+                breakdown = true;
+                break;
+            }
+        }
+        const int kk = std::min(k_eigs + 1, m_eff);
+        tridiag_lowest(m_eff, ha.data(), hb.data(), kk, w.data(), zl.data(), z0.data());
+        ms_host += th.ms();
+        if (breakdown) {
+            converged = m_eff >= k_eigs;
+            break;
+        }
+        if (o.fixed_iter) break;
+        if (m_eff >= k_eigs) {
+            bool ok = true;
+            const double sq = std::sqrt(tol);
+            for (int j = 0; j < k_eigs && ok; ++j) {
+                const double sc = std::max(1.0, std::fabs(w[j]));
+                const double r = std::fabs(hb[m_eff] * zl[j]);
+                double gap = 1.0;
+                if (j + 1 < kk) gap = w[j + 1] - w[j];
+                if (j > 0) gap = std::min(gap, w[j] - w[j - 1]);
+                ok = r <= sq * sc && r * r / std::max(gap, sq) <= tol * sc;
+            }
+            if (ok) {
+                converged = true;
+                break;
+            }
+        }
+    }
+    if (o.fixed_iter) converged = true;
+    const int m = m_eff;
+    const int kk = std::min(k_eigs, m);
+    for (int j = 0; j < k_eigs; ++j) evals[j] = j < kk ? w[j] : std::nan("");
+    if (krylov) {
+        for (int j = 0; j < M; ++j) krylov[j] = j < m ? z0[j] : 0.0;
+    }
+    double eig_res = 0.0;
+    if (o.want_eigvec) {
+        // rebuild x = sum_j z0[j] phi_j (the reference's second pass, lanczos.cu:31-104)
+        DevBuf X(cnt * sizeof(double));
+        if (reorth) {
+            DevBuf dz(m * sizeof(double));
+            QMB_HIP(hipMemcpyAsync(dz.p, z0.data(), m * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(basis_combine, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt,
+                               (int64_t)cnt, m, VB.as<double>(), dz.as<double>(), X.as<double>());
+            QMB_HIP(hipGetLastError());
+        } else {
+            QMB_HIP(hipMemsetAsync(X.p, 0, cnt * sizeof(double), ctx->stream));
+            QMB_HIP(hipMemsetAsync(Z.p, 0, cnt * sizeof(double), ctx->stream));
+            init_start();
+            for (int j = 0; j < m; ++j) {
+                launch_spmv(ctx, e, P.as<double>(), Y.as<double>(), true, wk.invb() + j, vcur(j), &wk,
+                            wk.alpha() + j);
+                hipLaunchKernelGGL(vec_axpy, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt,
+                                   z0[j], vcur(j), X.as<double>());
+                QMB_HIP(hipGetLastError());
+                hipLaunchKernelGGL(lz_update, dim3(ublocks), dim3(kThreads), 0, ctx->stream, cnt, Y.as<double>(),
+                                   vcur(j), vprev(j), P.as<double>(), wk.alpha() + j, wk.beta() + j,
+                                   wk.partials.as<double>(), wk.counter.as<unsigned>() + 2, wk.beta() + j + 1,
+                                   wk.invb() + j + 1);
+                QMB_HIP(hipGetLastError());
+            }
+        }
+        launch_spmv(ctx, e, X.as<double>(), Y.as<double>(), false, nullptr, nullptr, nullptr, nullptr);
+        launch_norm(ctx, cnt, Y.as<double>(), X.as<double>(), evals[0], wk, wk.misc() + 1, wk.misc() + 2);
+        QMB_HIP(hipMemcpyAsync(&eig_res, wk.misc() + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        if (eigvec)
+            QMB_HIP(hipMemcpyAsync(eigvec, X.p, cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        n_spmv += m + 1;
+    }
+    for (auto &x : ev) (void)hipEventDestroy(x);
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->iters = m;
+        rep->converged = converged ? 1 : 0;
+        rep->breakdown = breakdown ? 1 : 0;
+        rep->n_spmv = n_spmv;
+        for (int j = 0; j < std::min(k_eigs, 8); ++j)
+            rep->residual[j] = j < kk ? std::fabs(hb[m] * zl[j]) : std::nan("");
+        rep->eig_residual = eig_res;
+        rep->ms_total = wall.ms();
+        rep->ms_spmv = ms_spmv;
+        rep->ms_host = ms_host;
+    }
+    return converged ? QMB_OK : QMB_ERR_NOT_CONVERGED;
+}
+
+// Reference-layout host arrays -> SELL-64 device ELL.
+qmb_ell *ell_from_reference(qmb_ctx *ctx, int64_t n, int W, const uint32_t *ind, const double *val,
+                            const int16_t *track, bool cplx) {
+    auto e = std::make_unique<qmb_ell>();
+    e->ctx = ctx;
+    e->value_type = cplx ? QMB_COMPLEX : QMB_REAL;
+    e->max_terms = W;
+    e->n_rows = n;
+    e->n_chunks = (n + kChunk - 1) / kChunk;
+    const int vs = cplx ? 2 : 1;
+    std::vector<int64_t> cp(e->n_chunks + 1, 0);
+    std::vector<uint16_t> rl(std::max<int64_t>(n, 1), 0);
+    int64_t nnz = 0;
+    int mx = 0;
+    for (int64_t c = 0; c < e->n_chunks; ++c) {
+        int L = 0;
+        for (int64_t r = c * kChunk; r < std::min<int64_t>(n, (c + 1) * kChunk); ++r) {
+            QMB_REQUIRE(track[r] >= 0 && track[r] <= W, "reference ELL: track out of range");
+            rl[r] = (uint16_t)track[r];
+            L = std::max(L, (int)track[r]);
+            nnz += track[r];
+        }
+        mx = std::max(mx, L);
+        cp[c + 1] = cp[c] + (int64_t)kChunk * L;
+    }
+    e->stored = cp[e->n_chunks];
+    e->nnz = nnz;
+    e->max_row_len = mx;
+    std::vector<int32_t> cl(std::max<int64_t>(e->stored, 1), 0);
+    std::vector<double> vl((size_t)std::max<int64_t>(e->stored, 1) * vs, 0.0);
+    for (int64_t r = 0; r < n; ++r) {
+        const int64_t c = r / kChunk, lane = r % kChunk;
+        const int L = (int)((cp[c + 1] - cp[c]) / kChunk);
+        for (int s = 0; s < L; ++s) {
+            const int64_t pos = cp[c] + (int64_t)s * kChunk + lane;
+            if (s < track[r]) {
+                const uint32_t j = ind[(size_t)r * W + s];
+                QMB_REQUIRE((int64_t)j < n, "reference ELL: column out of range");
+                cl[pos] = (int32_t)j;
+                for (int q = 0; q < vs; ++q) vl[(size_t)pos * vs + q] = val[((size_t)r * W + s) * vs + q];
+            } else {
+                cl[pos] = (int32_t)r;
+            }
+        }
+    }
+    e->chunk_ptr.alloc(cp.size() * sizeof(int64_t));
+    e->row_len.alloc(rl.size() * sizeof(uint16_t));
+    e->col.alloc(cl.size() * sizeof(int32_t));
+    e->val.alloc(vl.size() * sizeof(double));
+    QMB_HIP(hipMemcpy(e->chunk_ptr.p, cp.data(), cp.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    QMB_HIP(hipMemcpy(e->row_len.p, rl.data(), rl.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    QMB_HIP(hipMemcpy(e->col.p, cl.data(), cl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    QMB_HIP(hipMemcpy(e->val.p, vl.data(), vl.size() * sizeof(double), hipMemcpyHostToDevice));
+    return e.release();
+}
+
+qmb_ctx *default_ctx() {
+    static std::once_flag once;
+    static qmb_ctx *ctx = nullptr;
+    static qmb_status st = QMB_OK;
+    std::call_once(once, [] { st = qmb_ctx_create(0, &ctx); });
+    if (st != QMB_OK) throw Error{st, std::string("default context: ") + qmb_last_error()};
+    return ctx;
+}
+
+void check_opts(const qmb_lanczos_opts &o) {
+    QMB_REQUIRE(o.max_iter >= 1 && o.max_iter <= 100000, "qmb_lanczos: max_iter must be in [1, 100000]");
+    QMB_REQUIRE(o.n_eigs >= 1 && o.n_eigs <= 8, "qmb_lanczos: n_eigs must be in [1, 8]");
+    QMB_REQUIRE(o.n_eigs == 1 || o.reorth, "qmb_lanczos: n_eigs > 1 needs reorth = 1");
+    QMB_REQUIRE(o.start == QMB_START_RANDOM || o.start == QMB_START_USER, "qmb_lanczos: bad start mode");
+    QMB_REQUIRE(!(o.profile && o.reorth), "qmb_lanczos: profile with reorth is not supported");
+}
+
+}  // namespace
+
+extern "C" {
+
+void qmb_lanczos_default_opts(qmb_lanczos_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->max_iter = 300;
+    o->n_eigs = 1;
+    o->check_every = 10;
+    o->tol = 1e-13;
+    o->seed = 2212456ULL;
+}
+
+qmb_status qmb_spmv_device(qmb_ctx *ctx, const qmb_ell *e, const double *x_dev, double *y_dev) {
+    return guarded([&] {
+        QMB_REQUIRE(ctx && e && x_dev && y_dev, "qmb_spmv_device: NULL argument");
+        QMB_HIP(hipSetDevice(ctx->device));
+        if (e->n_rows == 0) return;
+        launch_spmv(ctx, e, x_dev, y_dev, false, nullptr, nullptr, nullptr, nullptr);
+    });
+}
+
+qmb_status qmb_spmv_host(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y) {
+    return guarded([&] {
+        QMB_REQUIRE(ctx && e && x && y, "qmb_spmv_host: NULL argument");
+        QMB_HIP(hipSetDevice(ctx->device));
+        const int64_t cnt = e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1);
+        if (cnt == 0) return;
+        DevBuf dx(cnt * sizeof(double)), dy(cnt * sizeof(double));
+        QMB_HIP(hipMemcpyAsync(dx.p, x, cnt * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        launch_spmv(ctx, e, dx.as<double>(), dy.as<double>(), false, nullptr, nullptr, nullptr, nullptr);
+        QMB_HIP(hipMemcpyAsync(y, dy.p, cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+qmb_status qmb_spmv_benchmark(qmb_ctx *ctx, const qmb_ell *e, int reps, double *ms_per_spmv) {
+    return guarded([&] {
+        QMB_REQUIRE(ctx && e && ms_per_spmv && reps >= 1, "qmb_spmv_benchmark: bad argument");
+        QMB_HIP(hipSetDevice(ctx->device));
+        const int64_t cnt = std::max<int64_t>(1, e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1));
+        DevBuf dx(cnt * sizeof(double)), dy(cnt * sizeof(double));
+        hipLaunchKernelGGL(lz_random, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt,
+                           (uint64_t)7, dx.as<double>());
+        QMB_HIP(hipGetLastError());
+        launch_spmv(ctx, e, dx.as<double>(), dy.as<double>(), false, nullptr, nullptr, nullptr, nullptr);
+        hipEvent_t a, b;
+        QMB_HIP(hipEventCreate(&a));
+        QMB_HIP(hipEventCreate(&b));
+        QMB_HIP(hipEventRecord(a, ctx->stream));
+        for (int i = 0; i < reps; ++i)
+            launch_spmv(ctx, e, dx.as<double>(), dy.as<double>(), false, nullptr, nullptr, nullptr, nullptr);
+        QMB_HIP(hipEventRecord(b, ctx->stream));
+        QMB_HIP(hipEventSynchronize(b));
+        float t = 0.f;
+        QMB_HIP(hipEventElapsedTime(&t, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        *ms_per_spmv = (double)t / reps;
+    });
+}
+
+qmb_status qmb_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts *opts, double *evals, double *krylov,
+                       double *eigvec, qmb_lanczos_report *report) {
+    qmb_status result = QMB_OK;
+    qmb_status st = guarded([&] {
+        QMB_REQUIRE(ctx && e && opts && evals, "qmb_lanczos: NULL argument");
+        QMB_REQUIRE(e->ctx == ctx, "qmb_lanczos: ELL belongs to another context");
+        QMB_REQUIRE(e->n_rows > 0, "qmb_lanczos: empty matrix");
+        check_opts(*opts);
+        qmb_lanczos_opts o = *opts;
+        if (o.max_iter > e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1) + 1 && !o.fixed_iter)
+            o.max_iter = (int)std::max<int64_t>(1, e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1) + 1);
+        result = run_lanczos(ctx, e, o, evals, krylov, eigvec, report);
+        if (result == QMB_ERR_NOT_CONVERGED) set_error("qmb_lanczos: tolerance not met within max_iter");
+    });
+    return st != QMB_OK ? st : result;
+}
+
+qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const qmb_lanczos_opts *opts,
+                               double *evals, qmb_lanczos_report *reports) {
+    return guarded([&] {
+        QMB_REQUIRE(ctx && ells && opts && evals && n >= 1, "qmb_lanczos_batched: bad argument");
+        throw Error{QMB_ERR_UNSUPPORTED, "qmb_lanczos_batched: not built yet"};
+    });
+}
+
+double qmb_lanczos_eigvals_ref(double *E_tmp, int NStates, int NIter, int nobv, int max_terms, long long seed,
+                               const uint32_t *index_mat, const double *vals_mat, const int16_t *track_vec,
+                               double *eig_vec_Krylov_out, int exportKrylovVec, int checkEigState, int is_complex,
+                               qmb_status *status) {
+    double ret = 0.0;
+    qmb_status st = guarded([&] {
+        QMB_REQUIRE(E_tmp && index_mat && vals_mat && track_vec, "qmb_lanczos_eigvals_ref: NULL argument");
+        QMB_REQUIRE(NStates == 1, "qmb_lanczos_eigvals_ref: NStates must be 1 (as in the reference)");
+        QMB_REQUIRE(NIter >= 1 && nobv >= 1 && max_terms >= 1, "qmb_lanczos_eigvals_ref: bad sizes");
+        qmb_ctx *ctx = default_ctx();
+        QMB_HIP(hipSetDevice(ctx->device));
+        std::unique_ptr<qmb_ell> e(
+            ell_from_reference(ctx, nobv, max_terms, index_mat, vals_mat, track_vec, is_complex != 0));
+        qmb_lanczos_opts o;
+        qmb_lanczos_default_opts(&o);
+        o.max_iter = NIter;
+        o.fixed_iter = 1;
+        o.seed = (uint64_t)seed;
+        o.want_eigvec = checkEigState ? 1 : 0;
+        std::vector<double> kr(NIter);
+        qmb_lanczos_report rep;
+        double ev = 0.0;
+        qmb_status r = run_lanczos(ctx, e.get(), o, &ev, kr.data(), nullptr, &rep);
+        if (r != QMB_OK && r != QMB_ERR_NOT_CONVERGED) throw Error{r, qmb_last_error()};
+        *E_tmp = ev;
+        if (exportKrylovVec && eig_vec_Krylov_out)
+            for (int i = 0; i < NIter; ++i) eig_vec_Krylov_out[i] = kr[i];
+        if (checkEigState) ret = rep.eig_residual / (double)nobv;
+    });
+    if (status) *status = st;
+    return st == QMB_OK ? ret : std::nan("");
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+"""Host-side model tables for the Python bindings: lattice Heisenberg XXZ term
+tables and translation (+ spin-flip) group tables in the reference's
+conventions. The C++ facade in include/qmb/ is the drop-in class API; this
+module gives Python callers (tests, benchmark) the same tables.
+
+Conventions (reference paths relative to its root, Q/ = QMBCuda/):
+  site (i, j, alpha) -> (i*N2 + j)*M + alpha   (Q/lattice_models/ArbitrarySpinLattice.cu:19-36)
+  term order: unit cell row-major, then Jxy rows (J * Sp_i Sm_j), then Jz rows
+  (Jz * Sz_i Sz_j)                               (Q/lattice_models/Heisenberg.cu:22-79)
+  table = conjugated op types, product of conjugated scalars
+                                                 (HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:306-336)
+  element e = jx*N2 + jy translates by (jx, jy); group_el[e][dest(s)] = s
+                                                 (Q/symmetries/Groups.cu:49-113, SymmetryClass.cu:26-71)
+  chi_a(e) = exp(+2 pi i (nx*jx/N1 + ny*jy/N2)), a = nx*N2 + ny   (Groups.cu:89-112)
+Translations are computed with integer cell arithmetic (exact for the
+Bravais lattices the reference's coordinate maps describe).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+OP_SZ, OP_SP, OP_SM = 0, 1, 2
+
+
+@dataclass
+class Lattice:
+    N1: int
+    N2: int = 1
+    M: int = 1
+
+    @property
+    def n_sites(self) -> int:
+        return self.N1 * self.N2 * self.M
+
+    def site(self, i: int, j: int, a: int = 0) -> int:
+        return (i * self.N2 + j) * self.M + a
+
+
+@dataclass
+class XXZInfo:
+    """HeisenbergInfo (Q/lattice_models/Heisenberg.h:10-30): rows (di, dj, a, b)."""
+
+    Jxy_terms: list = field(default_factory=list)
+    Jxy_couplings: list = field(default_factory=list)
+    Jz_terms: list = field(default_factory=list)
+    Jz_couplings: list = field(default_factory=list)
+    intra_c_bool: bool = True
+    inter_c_bool: bool = True
+
+
+def chain(n: int, J1: float = 1.0, delta: float = 1.0, J2: float = 0.0):
+    """Periodic J1-J2 XXZ chain, textbook H = sum_d J_d sum_r (Sx Sx + Sy Sy + delta Sz Sz):
+    Jxy rows at +-d with J_d/2, one Jz row at +d with J_d*delta."""
+    info = XXZInfo()
+    for d, J in ((1, J1), (2, J2)):
+        if J == 0.0:
+            continue
+        info.Jxy_terms += [[d, 0, 0, 0], [-d, 0, 0, 0]]
+        info.Jxy_couplings += [J / 2.0, J / 2.0]
+        info.Jz_terms.append([d, 0, 0, 0])
+        info.Jz_couplings.append(J * delta)
+    return Lattice(n, 1, 1), info
+
+
+def square(n1: int, n2: int, J: float = 1.0, delta: float = 1.0):
+    """Periodic nearest-neighbour square-lattice XXZ model (textbook convention)."""
+    info = XXZInfo(
+        Jxy_terms=[[1, 0, 0, 0], [-1, 0, 0, 0], [0, 1, 0, 0], [0, -1, 0, 0]],
+        Jxy_couplings=[J / 2.0] * 4,
+        Jz_terms=[[1, 0, 0, 0], [0, 1, 0, 0]],
+        Jz_couplings=[J * delta] * 2,
+    )
+    return Lattice(n1, n2, 1), info
+
+
+def term_table(lat: Lattice, info: XXZInfo):
+    """Conjugated flat term table of the Heisenberg Hamiltonian.
+    Returns (op_type[H*2], site[H*2], scalar[H] complex, n_ops[H], max_ops=2)."""
+    ops, sites, scal = [], [], []
+    if info.intra_c_bool or info.inter_c_bool:
+        rows = [info.Jxy_terms, info.Jz_terms]
+        for uc in range(lat.N1 * lat.N2):
+            j1, j2 = uc // lat.N2, uc % lat.N2
+            for kind in range(2):
+                for ji, r in enumerate(rows[kind]):
+                    src = uc * lat.M + r[3]
+                    i1, i2 = j1 + r[0], j2 + r[1]
+                    if not info.intra_c_bool and (i1 < 0 or i1 >= lat.N1 or i2 < 0 or i2 >= lat.N2):
+                        continue
+                    if not info.inter_c_bool:
+                        continue  # the reference's condition (Heisenberg.cu:57) is always true here
+                    i1 %= lat.N1
+                    i2 %= lat.N2
+                    tgt = r[2] + lat.M * (lat.N2 * i1 + i2)
+                    if kind == 0:
+                        ops += [OP_SM, OP_SP]  # conj(Sp_tgt), conj(Sm_src)
+                        sites += [tgt, src]
+                        scal.append(np.conj(complex(info.Jxy_couplings[ji])))
+                    else:
+                        ops += [OP_SZ, OP_SZ]
+                        sites += [tgt, src]
+                        scal.append(complex(info.Jz_couplings[ji]))
+    n = len(scal)
+    return (np.array(ops, np.int32), np.array(sites, np.int32), np.array(scal, np.complex128),
+            np.full(n, 2, np.int32), 2)
+
+
+def _root_of_unity(num: int, den: int) -> complex:
+    num %= den
+    if (4 * num) % den == 0:
+        return (1 + 0j, 1j, -1 + 0j, -1j)[(4 * num) // den]
+    ang = 2.0 * math.pi * num / den
+    return complex(math.cos(ang), math.sin(ang))
+
+
+def translation_group(lat: Lattice, spin_flip: bool = False):
+    """Translation group (optionally times spin inversion) in the reference's
+    element/irrep order. With spin_flip, element f*G + e and irrep p*G + a
+    (chi multiplied by (-1)^(p*f)). Returns (group_el[G', LS] uint32,
+    flip[G'] uint8, chars[G', G'] complex128)."""
+    N1, N2, M = lat.N1, lat.N2, lat.M
+    G, LS = N1 * N2, lat.n_sites
+    ge = np.zeros((G, LS), np.uint32)
+    for e in range(G):
+        jx, jy = e // N2, e % N2
+        for i in range(N1):
+            for j in range(N2):
+                for a in range(M):
+                    s = lat.site(i, j, a)
+                    dest = lat.site((i + jx) % N1, (j + jy) % N2, a)
+                    ge[e, dest] = s
+    ch = np.zeros((G, G), np.complex128)
+    for a in range(G):
+        nx, ny = a // N2, a % N2
+        for e in range(G):
+            jx, jy = e // N2, e % N2
+            ch[a, e] = _root_of_unity(((nx * jx) % N1) * N2 + ((ny * jy) % N2) * N1, N1 * N2)
+    if not spin_flip:
+        return ge, np.zeros(G, np.uint8), ch
+    ge2 = np.concatenate([ge, ge])
+    fl = np.concatenate([np.zeros(G, np.uint8), np.ones(G, np.uint8)])
+    ch2 = np.zeros((2 * G, 2 * G), np.complex128)
+    for p in range(2):
+        for f in range(2):
+            ch2[p * G:(p + 1) * G, f * G:(f + 1) * G] = ch * (-1.0 if (p and f) else 1.0)
+    return ge2, fl, ch2
+
+
+def momentum_irrep(lat: Lattice, kx: int, ky: int = 0, flip_parity: int | None = None) -> int:
+    """Irrep index of momentum (2 pi kx/N1, 2 pi ky/N2) and, with spin flip, parity (0 even, 1 odd)."""
+    a = (kx % lat.N1) * lat.N2 + (ky % lat.N2)
+    if flip_parity is not None:
+        a += flip_parity * lat.N1 * lat.N2
+    return a
+
+
+def is_real_irrep(chars, irrep: int) -> bool:
+    return bool(np.all(np.asarray(chars)[irrep].imag == 0.0))

@@ -1,0 +1,88 @@
+"""Shared builders for the tests: model cases in the oracle's restatement and in
+the library's own host tables, plus independent brute-force checks."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle
+from oracle import model as OM
+
+
+def chain_case(n, J1=1.0, delta=1.0, J2=0.0, flip=False):
+    """Oracle-side tables of a periodic J1-J2 XXZ chain (textbook convention)."""
+    geom, info = OM.chain_xxz_info(n, J1, delta, J2)
+    terms = OM.heisenberg_operator(geom, info)
+    table = OM.conjugated_term_table(terms)
+    ge, ch = OM.translation_group(geom)
+    fl = None
+    if flip:
+        ge, fl, ch = OM.with_spin_flip(ge, ch)
+    return dict(n_sites=n, terms=terms, table=table, group_el=ge, chars=ch, flip=fl)
+
+
+def square_case(n1, n2, J=1.0, delta=1.0, flip=False):
+    geom, info = OM.square_xxz_info(n1, n2, J, delta)
+    terms = OM.heisenberg_operator(geom, info)
+    table = OM.conjugated_term_table(terms)
+    ge, ch = OM.translation_group(geom)
+    fl = None
+    if flip:
+        ge, fl, ch = OM.with_spin_flip(ge, ch)
+    return dict(n_sites=n1 * n2, terms=terms, table=table, group_el=ge, chars=ch, flip=fl)
+
+
+def is_real_row(chars, a):
+    return bool(np.all(np.asarray(chars)[a].imag == 0.0))
+
+
+def oracle_orbits(case, n_up):
+    return oracle.Orbits(case["n_sites"], n_up, case["group_el"], case["chars"], flip=case["flip"])
+
+
+def oracle_ell(case, orb, alpha, max_terms=0, is_complex=None):
+    if is_complex is None:
+        is_complex = not (is_real_row(case["chars"], alpha) and np.all(case["table"][2].imag == 0))
+    W = max_terms or OM.max_terms_rule(len(case["table"][3]), orb.n_up, orb.n_sites)
+    return oracle.Ell(orb, case["table"], alpha, W, is_complex)
+
+
+def brute_orbits(n_sites, n_up, group_el, flip=None):
+    """Independent orbit enumeration by explicit images (small N only).
+    Returns {rep: (stabiliser element list, orbit set)}."""
+    states = [int(s) for s in OM.sector_basis(n_sites, n_up)]
+    mask = (1 << n_sites) - 1
+    G = group_el.shape[0]
+
+    def image(s, g):
+        out = 0
+        for site in range(n_sites):
+            if (s >> site) & 1:
+                out |= 1 << int(group_el[g][site])
+        if flip is not None and flip[g]:
+            out ^= mask
+        return out
+
+    seen = set()
+    orbits = {}
+    for s in states:
+        if s in seen:
+            continue
+        imgs = [image(s, g) for g in range(G)]
+        orb = set(imgs)
+        seen |= orb
+        rep = min(orb)
+        stab = [g for g in range(G) if image(rep, g) == rep]
+        orbits[rep] = (stab, orb)
+    return orbits
+
+
+def dense_spectrum(case, n_up):
+    H = OM.dense_sector_hamiltonian(case["n_sites"], n_up, case["terms"])
+    assert np.allclose(H, H.conj().T, atol=1e-13)
+    return np.linalg.eigvalsh(H)
+
+
+def ell_dense_nonnull(ell, orb, alpha):
+    E = ell.dense()
+    keep = ~orb.null_mask(alpha)
+    return E[np.ix_(keep, keep)]

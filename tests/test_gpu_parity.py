@@ -1,0 +1,272 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Integer results (representatives, stabilisers, null flags, lookups, ELL
+columns and track) are compared bit for bit; ELL values and SpMV results too,
+because both sides use the same FP64 operation sequence (no FMA contraction).
+Eigenvalues must agree within 1e-10 relative with the oracle's converged
+Lanczos and with dense diagonalisation where it fits.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import qmbcuda_amd as q
+from oracle import model as OM
+from tests.helpers import chain_case, dense_spectrum, ell_dense_nonnull, oracle_ell, oracle_orbits, square_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gpu_objects(ctx, case, n_up, alpha, layout, max_terms=0, value_type=None):
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    t = q.Terms(case["n_sites"], case["table"])
+    b = q.Basis(ctx, g, n_up, alpha, layout)
+    e = q.Ell(ctx, b, t, value_type=value_type, max_terms=max_terms)
+    return g, t, b, e
+
+
+CASES = [
+    ("chain12_flip", lambda: chain_case(12, 1.0, 1.0, 0.3, flip=True), 6, [0, 5, 12, 17]),
+    ("chain16", lambda: chain_case(16), 8, [0, 3, 8]),
+    ("chain13_odd", lambda: chain_case(13, 1.0, 0.7, 0.2), 6, [0, 2]),
+    ("square4x4_flip", lambda: square_case(4, 4, 1.0, 1.0, flip=True), 8, [0, 5, 16 + 10]),
+    ("square4x3", lambda: square_case(4, 3, 1.0, 0.8), 6, [1, 7]),
+]
+
+
+@pytest.mark.parametrize("label,mk,n_up,alphas", CASES)
+def test_basis_reference_layout_bit_exact(ctx, label, mk, n_up, alphas):
+    case = mk()
+    orb = oracle_orbits(case, n_up)
+    for a in alphas:
+        say(f"basis {label} irrep={a}")
+        g = q.Group(case["group_el"], case["chars"], case["flip"])
+        b = q.Basis(ctx, g, n_up, a, q.LAYOUT_REFERENCE)
+        reps, stab, nul = b.export()
+        assert b.n_rows == orb.n_orb
+        assert np.array_equal(reps, orb.reps)
+        assert np.array_equal(stab.astype(np.int32), orb.stab)
+        assert np.array_equal(nul.astype(bool), orb.null_mask(a))
+        bc = q.Basis(ctx, g, n_up, a, q.LAYOUT_COMPACT)
+        creps, cstab, cnul = bc.export()
+        assert np.array_equal(creps, orb.reps[~orb.null_mask(a)])
+        assert not cnul.any()
+
+
+def test_lookup_matches_orbit_index_map(ctx):
+    case = chain_case(12, 1.0, 1.0, 0.3, flip=True)
+    orb = oracle_orbits(case, 6)
+    say("lookup chain12_flip")
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    b = q.Basis(ctx, g, 6, 0, q.LAYOUT_REFERENCE)
+    rows, el = b.lookup(orb.bas)
+    assert np.array_equal(rows, orb.orbit_idx[0::2])
+    assert np.array_equal(el, orb.orbit_idx[1::2])
+
+
+@pytest.mark.parametrize("label,mk,n_up,alphas", CASES)
+def test_ell_reference_layout_bit_exact(ctx, label, mk, n_up, alphas):
+    case = mk()
+    orb = oracle_orbits(case, n_up)
+    for a in alphas:
+        say(f"ell {label} irrep={a}")
+        ref = oracle_ell(case, orb, a)
+        _, _, b, e = gpu_objects(ctx, case, n_up, a, q.LAYOUT_REFERENCE, max_terms=ref.W,
+                                 value_type=q.COMPLEX if ref.is_complex else q.REAL)
+        ind, val, track = e.export_reference()
+        assert np.array_equal(track, ref.track)
+        assert np.array_equal(ind, ref.ind)
+        assert np.array_equal(val.view(np.float64), ref.val)  # bit-exact values
+        inf = e.info()
+        assert inf.nnz == int(ref.track.sum())
+        assert inf.max_row_len == int(ref.track.max())
+        # SpMV: same slot order, same rounding -> identical results
+        rng = np.random.default_rng(a + 1)
+        x = rng.normal(size=orb.n_orb)
+        if ref.is_complex:
+            x = x + 1j * rng.normal(size=orb.n_orb)
+        assert np.array_equal(e.spmv(x), ref.spmv(x))
+
+
+def test_compact_layout_is_reference_without_null_rows(ctx):
+    case = chain_case(12, 1.0, 1.0, 0.3, flip=True)
+    orb = oracle_orbits(case, 6)
+    for a in (5, 17):
+        say(f"compact chain12_flip irrep={a}")
+        ref = oracle_ell(case, orb, a)
+        E_ref = ell_dense_nonnull(ref, orb, a)
+        _, _, b, e = gpu_objects(ctx, case, 6, a, q.LAYOUT_COMPACT, max_terms=ref.W)
+        ind, val, track = e.export_reference()
+        n = b.n_rows
+        E = OM.ell_to_dense(n, ref.W, ind, val, track)
+        assert np.array_equal(E, E_ref)
+
+
+def test_max_terms_overflow_is_an_error(ctx):
+    case = chain_case(10)
+    say("overflow chain10")
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    t = q.Terms(10, case["table"])
+    b = q.Basis(ctx, g, 5, 0, q.LAYOUT_COMPACT)
+    with pytest.raises(q.QMBError) as ei:
+        q.Ell(ctx, b, t, max_terms=3)
+    assert ei.value.status == 1
+
+
+def test_bad_requests_are_errors(ctx):
+    case = chain_case(10, flip=True)
+    say("bad requests")
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    with pytest.raises(q.QMBError):
+        q.Basis(ctx, g, 4, 0)  # flip needs n_up = N/2
+    with pytest.raises(q.QMBError):
+        q.Basis(ctx, g, 5, 99)  # irrep out of range
+    case = square_case(4, 3)
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    t = q.Terms(12, case["table"])
+    b = q.Basis(ctx, g, 6, 1)
+    with pytest.raises(q.QMBError):
+        q.Ell(ctx, b, t, value_type=q.REAL)  # complex characters
+
+
+def test_edge_sectors(ctx):
+    say("edge sectors")
+    case = chain_case(8)
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    t = q.Terms(8, case["table"])
+    for n_up in (0, 8):
+        b = q.Basis(ctx, g, n_up, 0)
+        assert b.n_rows == 1
+        e = q.Ell(ctx, b, t)
+        ev, rep, _, _ = q.lanczos(e, max_iter=5)
+        assert abs(ev[0] - 8 * 0.25) < 1e-14  # all aligned: N * Jz/4
+    # a sector whose irrep keeps no orbit at all: k=pi for n_up=0 -> zero rows
+    b = q.Basis(ctx, g, 0, 4)
+    assert b.n_rows == 0
+
+
+@pytest.mark.parametrize("label,mk,n_up,a", [
+    ("chain16_k0", lambda: chain_case(16), 8, 0),
+    ("chain14_k3", lambda: chain_case(14, 1.0, 0.8, 0.25), 7, 3),
+    ("square4x4_flip_k0", lambda: square_case(4, 4, flip=True), 8, 0),
+    ("square4x3_k1", lambda: square_case(4, 3, 1.0, 0.8), 6, 1),
+])
+def test_lanczos_matches_oracle_and_dense(ctx, label, mk, n_up, a):
+    case = mk()
+    orb = oracle_orbits(case, n_up)
+    say(f"lanczos {label}")
+    ref = oracle_ell(case, orb, a)
+    st, e_or, it_or, al, be = ref.lanczos_converge(400, 2212456, tol=1e-13, check_every=10)
+    assert st == 0
+    dense = np.linalg.eigvalsh(ell_dense_nonnull(ref, orb, a))[0]
+    _, _, b, e = gpu_objects(ctx, case, n_up, a, q.LAYOUT_COMPACT)
+    ev, rep, kr, _ = q.lanczos(e, max_iter=400, tol=1e-13, seed=2212456)
+    assert rep.converged
+    sc = max(1.0, abs(dense))
+    assert abs(ev[0] - e_or) < 1e-10 * sc
+    assert abs(ev[0] - dense) < 1e-10 * sc
+
+
+def test_fixed_iterations_follow_oracle_recurrence(ctx):
+    case = chain_case(14, 1.0, 1.0, 0.2)
+    orb = oracle_orbits(case, 7)
+    say("fixed-iteration recurrence chain14")
+    ref = oracle_ell(case, orb, 0)
+    a_or, b_or = ref.lanczos(25, 31)
+    _, _, b, e = gpu_objects(ctx, case, 7, 0, q.LAYOUT_REFERENCE, max_terms=ref.W)
+    ev, rep, kr, _ = q.lanczos(e, max_iter=25, seed=31, fixed_iter=True)
+    w, z = oracle.tridiag_eig(a_or, b_or[1:25])
+    assert abs(ev[0] - w[0]) < 1e-11
+    zr = z[:, 0] * np.sign(z[0, 0])
+    np.testing.assert_allclose(kr, zr, atol=1e-8)
+
+
+def test_known_answers(ctx):
+    say("C1 N=16 chain k=0")
+    case = chain_case(16)
+    _, _, b, e = gpu_objects(ctx, case, 8, 0, q.LAYOUT_COMPACT)
+    assert b.n_rows == 810
+    ev, rep, _, _ = q.lanczos(e, max_iter=200, tol=1e-13)
+    assert abs(ev[0] - (-7.142296360617)) < 1e-11
+    say("4x4 square flip-even k=0")
+    case = square_case(4, 4, flip=True)
+    _, _, b, e = gpu_objects(ctx, case, 8, 0, q.LAYOUT_COMPACT)
+    ev, rep, _, _ = q.lanczos(e, max_iter=300, tol=1e-13)
+    assert abs(ev[0] - (-11.228483208429)) < 1e-10
+
+
+def test_c2_full_size_majumdar_ghosh(ctx):
+    """N=28 J1-J2 chain at J2/J1 = 1/2, Sz=0, k=0, flip-even: dim 716,728, E0 = -3N/8."""
+    import qmbcuda_amd.models as LM
+
+    say("C2 N=28 MG full size")
+    lat, info = LM.chain(28, 1.0, 1.0, 0.5)
+    ge, fl, ch = LM.translation_group(lat, spin_flip=True)
+    g = q.Group(ge, ch, fl)
+    t = q.Terms(28, LM.term_table(lat, info))
+    b = q.Basis(ctx, g, 14, 0, q.LAYOUT_COMPACT)
+    assert b.n_rows == 716728
+    e = q.Ell(ctx, b, t)
+    inf = e.info()
+    assert inf.max_row_len <= 57
+    # size-independent properties: symmetric matrix (<x,Hy> == <Hx,y>)
+    rng = np.random.default_rng(0)
+    x, y = rng.normal(size=b.n_rows), rng.normal(size=b.n_rows)
+    lhs, rhs = x @ e.spmv(y), e.spmv(x) @ y
+    assert abs(lhs - rhs) < 1e-9 * abs(lhs)
+    ev, rep, _, _ = q.lanczos(e, max_iter=500, tol=1e-13)
+    assert rep.converged
+    assert abs(ev[0] + 10.5) < 1e-10 * 10.5
+    # the representatives are the orbit minima: spot-check with the lookup
+    reps, stab, _ = b.export()
+    rows, el = b.lookup(reps[:: 997])
+    assert np.array_equal(rows, np.arange(0, b.n_rows, 997))
+    assert np.all(el == 0)
+
+
+def test_lowest_four_with_reorthogonalisation(ctx):
+    case = chain_case(14, 1.0, 0.9, 0.3)
+    orb = oracle_orbits(case, 7)
+    for a in (0, 3):
+        say(f"lowest-4 chain14 irrep={a}")
+        ref = oracle_ell(case, orb, a)
+        dense = np.linalg.eigvalsh(ell_dense_nonnull(ref, orb, a))[:4]
+        _, _, b, e = gpu_objects(ctx, case, 7, a, q.LAYOUT_COMPACT)
+        ev, rep, _, _ = q.lanczos(e, n_eigs=4, max_iter=300, tol=1e-13)
+        assert rep.converged
+        np.testing.assert_allclose(ev, dense, atol=1e-10 * max(1, abs(dense[0])))
+
+
+def test_eigenvector_second_pass(ctx):
+    case = chain_case(14, 1.0, 1.0, 0.2)
+    orb = oracle_orbits(case, 7)
+    say("eigenvector rebuild chain14")
+    ref = oracle_ell(case, orb, 0)
+    E = ell_dense_nonnull(ref, orb, 0)
+    _, _, b, e = gpu_objects(ctx, case, 7, 0, q.LAYOUT_COMPACT)
+    ev, rep, kr, x = q.lanczos(e, max_iter=300, tol=1e-13, want_eigvec=True)
+    assert abs(np.linalg.norm(x) - 1) < 1e-8
+    assert rep.eig_residual < 1e-5
+    np.testing.assert_allclose(E @ x, ev[0] * x, atol=1e-5)
+
+
+def test_reference_signature_drop_in(ctx):
+    """qmb_lanczos_eigvals_ref on the reference's own row-major arrays (fixed NIter)."""
+    case = chain_case(16)
+    orb = oracle_orbits(case, 8)
+    ref = oracle_ell(case, orb, 0)
+    say("eigvals_ref drop-in chain16")
+    a_or, b_or = ref.lanczos(50, 2212456)
+    w, z = oracle.tridiag_eig(a_or, b_or[1:50])
+    E, r, kr = q.eigvals_ref(None, 50, ref.n_rows, ref.W, 2212456, ref.ind, ref.val, ref.track, False, check=True)
+    assert abs(E - w[0]) < 1e-11
+    res_or, _ = ref.eigvec_residual(2212456, z[:, 0] * np.sign(z[0, 0]), w[0])
+    assert abs(r - res_or / ref.n_rows) < 1e-8
