@@ -1,0 +1,250 @@
+// ED driver and Lanczos free function of the C++ facade, on top of the C ABI.
+// Reference: Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.h:27-89
+// and .cu:20-251 (driver), Q/exact_diagonalization/lanczos.h:21-32 (free function).
+//
+// HeisenbergXXZAbelianSymms<T> keeps the reference constructor and getters.
+// Differences, all documented at the ABI: FP64 results; states may have up to
+// 64 sites; Lanczos runs to a tolerance (EDOptions::fixed_iter restores the
+// reference's fixed NIter + residual check + reseed loop); groups may carry
+// spin flips; the compact layout drops null orbits (the reference keeps them as
+// empty rows; EDOptions::reference_layout restores that).
+#pragma once
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../qmb.h"
+#include "lattice.hpp"
+#include "operators.hpp"
+#include "symmetry.hpp"
+
+namespace qmb {
+
+struct Status : std::runtime_error {
+    qmb_status code;
+    Status(qmb_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(qmb_status s, const char *what) {
+    if (s != QMB_OK) throw Status(s, std::string(what) + ": " + qmb_last_error());
+}
+
+struct EDOptions {
+    int irrep = -1;                // -1: every irrep (the reference's loop); else only this one
+    bool reference_layout = false; // keep null orbits as empty rows
+    bool fixed_iter = false;       // exactly NIter steps, residual check and reseeding (reference semantics)
+    double tol = 1e-13;            // relative eigenvalue tolerance when not fixed_iter
+    int max_iter = 2000;           // iteration cap when not fixed_iter
+    int device = 0;
+};
+
+template <class T>
+class HeisenbergXXZAbelianSymms {
+  public:
+    HeisenbergXXZAbelianSymms(Heisenberg<T> model, SymmetryClass<T> *group, int hop_f = -1, int GS_sector = -1,
+                              int NStates = 1, long long seed = 2212456, int NIter = 50, double tol_norm = 1e-5,
+                              EDOptions opts = EDOptions())
+        : lattice_model(std::move(model)), SymmGroup(group), hop_f_(hop_f), NStates_(NStates), NIter_(NIter),
+          tol_norm_(tol_norm), opts_(opts) {
+        if (NStates != 1) throw Status(QMB_ERR_INVALID, "NStates must be 1 (as in the reference)");
+        LS = lattice_model.GetLS();
+        GS_sector_ = GS_sector < 0 ? LS / 2 : GS_sector;
+        const auto chars = group->GetCharTable();
+        const auto elems = group->GetGroupElemVec();
+        auto flips = group->GetSpinFlips();
+        const int G = static_cast<int>(elems.size()), NIr = static_cast<int>(chars.size());
+        group_el_arr.assign((size_t)G * LS, 0);
+        group->GetGroupElemArr(lattice_model.GetR1(), lattice_model.GetR2(), group_el_arr.data());
+        std::vector<double> ch((size_t)NIr * G * 2);
+        for (int a = 0; a < NIr; ++a)
+            for (int g = 0; g < G; ++g) {
+                const std::complex<double> c(chars[a][g]);
+                ch[((size_t)a * G + g) * 2] = c.real();
+                ch[((size_t)a * G + g) * 2 + 1] = c.imag();
+            }
+        if (flips.empty()) flips.assign(G, 0);
+        const OperatorVectors<T> ov = BuildOperatorVecs(lattice_model.GetH());
+        std::vector<double> sc((size_t)ov.H_size * 2);
+        for (int h = 0; h < ov.H_size; ++h) {
+            sc[2 * h] = ov.H_scalar_table[h].real();
+            sc[2 * h + 1] = ov.H_scalar_table[h].imag();
+        }
+        // the reference's row width (…cu:71-72)
+        max_terms = hop_f > 0 ? GS_sector_ * hop_f : ov.H_size * GS_sector_ / LS + 1;
+
+        qmb_ctx *ctx = nullptr;
+        qmb_group *grp = nullptr;
+        qmb_terms *terms = nullptr;
+        check(qmb_ctx_create(opts_.device, &ctx), "qmb_ctx_create");
+        try {
+            check(qmb_group_create(LS, G, group_el_arr.data(), flips.data(), NIr, ch.data(), &grp), "qmb_group_create");
+            check(qmb_terms_create(LS, ov.H_size, ov.H_col_size, ov.H_decode_table.data(), ov.H_site_table.data(),
+                                   sc.data(), ov.H_NOTerms.data(), &terms),
+                  "qmb_terms_create");
+            const int a0 = opts_.irrep < 0 ? 0 : opts_.irrep;
+            const int a1 = opts_.irrep < 0 ? NIr : opts_.irrep + 1;
+            for (int a = 0; a < NIr; ++a) {
+                if (a < a0 || a >= a1) {
+                    E_vecs.push_back({std::numeric_limits<double>::infinity()});
+                    seed_vecs.push_back({seed});
+                    KrylovGS_vecs.push_back({});
+                    continue;
+                }
+                solve_irrep(ctx, grp, terms, a, ch, G, seed);
+            }
+        } catch (...) {
+            qmb_terms_destroy(terms);
+            qmb_group_destroy(grp);
+            qmb_ctx_destroy(ctx);
+            throw;
+        }
+        qmb_terms_destroy(terms);
+        qmb_group_destroy(grp);
+        qmb_ctx_destroy(ctx);
+        E_gs = E_vecs[0][0];
+        GS_irrep = 0;
+        GS_seed = seed_vecs[0][0];
+        for (int a = 1; a < NIr; ++a)
+            if (E_vecs[a][0] < E_gs) {
+                E_gs = E_vecs[a][0];
+                GS_irrep = a;
+                GS_seed = seed_vecs[a][0];
+            }
+        KrylovGS_vec = KrylovGS_vecs[GS_irrep];
+    }
+    virtual ~HeisenbergXXZAbelianSymms() = default;
+
+    std::vector<std::vector<double>> get_E_vecs() const { return E_vecs; }
+    std::vector<std::vector<long long>> get_seed_vecs() const { return seed_vecs; }
+    double GetGroundStateEnergy() const { return E_gs; }
+    int GetGroundStateIrrep() const { return GS_irrep; }
+    long long GetGroundStateSeed() const { return GS_seed; }
+    std::vector<double> GetKrylovGS() const { return KrylovGS_vec; }
+    std::vector<int64_t> get_sector_dims() const { return sector_dims; }
+    std::vector<int> get_iterations() const { return iterations; }
+
+  protected:
+    void solve_irrep(qmb_ctx *ctx, qmb_group *grp, qmb_terms *terms, int a, const std::vector<double> &ch, int G,
+                     long long seed) {
+        bool real = true;
+        for (int g = 0; g < G; ++g) real = real && ch[((size_t)a * G + g) * 2 + 1] == 0.0;
+        if constexpr (std::is_same_v<T, std::complex<double>>) {
+            for (const auto &t : lattice_model.GetH().GetElems()) {
+                std::complex<double> s(1.0);
+                for (const auto &o : t) s *= o.GetScalar();
+                real = real && s.imag() == 0.0;
+            }
+        }
+        qmb_basis *b = nullptr;
+        qmb_ell *e = nullptr;
+        check(qmb_basis_build(ctx, grp, GS_sector_, a,
+                              opts_.reference_layout ? QMB_LAYOUT_REFERENCE : QMB_LAYOUT_COMPACT, &b),
+              "qmb_basis_build");
+        qmb_basis_info bi;
+        qmb_basis_get_info(b, &bi);
+        sector_dims.push_back(bi.n_rows - bi.n_null);
+        if (bi.n_rows == 0) {  // no orbit survives: the irrep does not occur in this sector
+            qmb_basis_destroy(b);
+            E_vecs.push_back({std::numeric_limits<double>::infinity()});
+            seed_vecs.push_back({seed});
+            KrylovGS_vecs.push_back({});
+            iterations.push_back(0);
+            return;
+        }
+        qmb_status st = qmb_ell_build(ctx, b, terms, real ? QMB_REAL : QMB_COMPLEX, max_terms, &e);
+        if (st != QMB_OK) {
+            qmb_basis_destroy(b);
+            check(st, "qmb_ell_build");
+        }
+        qmb_lanczos_opts o;
+        qmb_lanczos_default_opts(&o);
+        o.seed = (uint64_t)seed;
+        std::vector<double> kr;
+        double E = 0.0;
+        long long used_seed = seed;
+        if (opts_.fixed_iter) {
+            // the reference's loop: fixed NIter, residual/nobv > 1e-4 -> reseed, at most 10 tries
+            // (…cu:189-213); reseeding uses a fixed LCG instead of rand()
+            o.max_iter = NIter_;
+            o.fixed_iter = 1;
+            o.want_eigvec = 1;
+            kr.assign(NIter_, 0.0);
+            uint64_t lcg = (uint64_t)seed;
+            for (int attempt = 0; attempt < 10; ++attempt) {
+                qmb_lanczos_report rep;
+                o.seed = (uint64_t)used_seed;
+                const qmb_status s = qmb_lanczos(ctx, e, &o, &E, kr.data(), nullptr, &rep);
+                if (s != QMB_OK && s != QMB_ERR_NOT_CONVERGED) {
+                    qmb_ell_destroy(e);
+                    qmb_basis_destroy(b);
+                    check(s, "qmb_lanczos");
+                }
+                if (rep.eig_residual / (double)bi.n_rows <= 1e-4) break;
+                if (attempt == 9) E = 1e7;  // the reference's "not found" sentinel
+                lcg = lcg * 6364136223846793005ULL + 1442695040888963407ULL;
+                used_seed = (long long)(lcg >> 33);
+            }
+            iterations.push_back(NIter_);
+        } else {
+            o.max_iter = opts_.max_iter;
+            o.tol = opts_.tol;
+            kr.assign(o.max_iter, 0.0);
+            qmb_lanczos_report rep;
+            const qmb_status s = qmb_lanczos(ctx, e, &o, &E, kr.data(), nullptr, &rep);
+            if (s != QMB_OK) {
+                qmb_ell_destroy(e);
+                qmb_basis_destroy(b);
+                check(s, "qmb_lanczos");
+            }
+            kr.resize(rep.iters);
+            iterations.push_back(rep.iters);
+        }
+        qmb_ell_destroy(e);
+        qmb_basis_destroy(b);
+        E_vecs.push_back({E});
+        seed_vecs.push_back({used_seed});
+        KrylovGS_vecs.push_back(kr);
+    }
+
+    Heisenberg<T> lattice_model;
+    int LS = 0;
+    int GS_sector_ = 0;
+    SymmetryClass<T> *SymmGroup;
+    int hop_f_, NStates_, NIter_;
+    double tol_norm_;
+    EDOptions opts_;
+    int max_terms = 0;
+    std::vector<uint32_t> group_el_arr;
+    std::vector<std::vector<double>> E_vecs;
+    std::vector<std::vector<long long>> seed_vecs;
+    std::vector<std::vector<double>> KrylovGS_vecs;
+    std::vector<double> KrylovGS_vec;
+    std::vector<int64_t> sector_dims;
+    std::vector<int> iterations;
+    int GS_irrep = 0;
+    double E_gs = 0.0;
+    long long GS_seed = 0;
+};
+
+// The reference's free function (lanczos.h:21-32) on host reference-layout
+// arrays: fixed NIter steps; E_tmp = lowest eigenvalue; returns ||Hx-Ex||/nobv
+// when checkEigState. T = double or std::complex<double>.
+template <class T>
+double LanczosEigVals(double *E_tmp, int NStates, int NIter, int nobv, int max_terms, long long seed,
+                      const uint32_t *index_mat, const T *vals_mat, const int16_t *track_vec,
+                      double *eig_vec_Krylov_out, bool exportKrylovVec, bool checkEigState) {
+    qmb_status st = QMB_OK;
+    const double r = qmb_lanczos_eigvals_ref(E_tmp, NStates, NIter, nobv, max_terms, seed, index_mat,
+                                             reinterpret_cast<const double *>(vals_mat), track_vec,
+                                             eig_vec_Krylov_out, exportKrylovVec ? 1 : 0, checkEigState ? 1 : 0,
+                                             std::is_same_v<T, std::complex<double>> ? 1 : 0, &st);
+    check(st, "LanczosEigVals");
+    return r;
+}
+
+}  // namespace qmb

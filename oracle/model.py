@@ -130,7 +130,9 @@ def extended_xxx_info(geom: Geometry, J_vec, intra=True, inter=True) -> Heisenbe
                     rows.append((ni, nj, ai, bi))
                     dx = f32(ni) * f32(A[0][0]) + f32(nj) * f32(A[0][1]) + f32(geom.rx_alpha[ai]) - f32(geom.rx_alpha[bi])
                     dy = f32(ni) * f32(A[1][0]) + f32(nj) * f32(A[1][1]) + f32(geom.ry_alpha[ai]) - f32(geom.ry_alpha[bi])
-                    dist.append(float(np.sqrt(f32(dx * dx + dy * dy))))
+                    # pow(float, 2) promotes to double; sqrtf of the float-rounded sum
+                    d2 = float(dx) * float(dx) + float(dy) * float(dy)
+                    dist.append(float(np.sqrt(f32(d2))))
     order = sorted(range(len(rows)), key=lambda k: dist[k])  # Python sort is stable
     rows = [rows[k] for k in order]
     dist = [dist[k] for k in order]

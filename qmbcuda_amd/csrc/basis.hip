@@ -44,41 +44,41 @@ __device__ inline bool compatible(uint64_t slo, uint64_t shi, uint64_t tlo, uint
     return ((slo & ~tlo) | (shi & ~thi)) == 0;
 }
 
+template <class S>
 __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const uint64_t *binom, int n_up,
                                                             int64_t dim, int64_t n_words, int compact,
                                                             uint64_t triv_lo, uint64_t triv_hi,
                                                             uint64_t *bitmap) {
-    __shared__ GroupLds lds;
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
-    stage_group(lds, perm_lds, gv);
+    stage_perm(perm_lds, gv);
     stage_binom(bin, binom);
     __syncthreads();
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n_words) return;
     const int64_t r0 = w * 64;
     const int cnt = (int)((dim - r0) < 64 ? (dim - r0) : 64);
-    uint64_t x = colex_unrank(r0, n_up, gv.n_sites, bin);
+    S x = colex_unrank<S>(r0, n_up, gv.n_sites, bin);
     uint64_t bits = 0;
     for (int i = 0; i < cnt; ++i) {
         uint64_t slo, shi;
-        if (is_representative(x, gv, lds, perm_lds, slo, shi)) {
+        if (is_representative<S>(x, gv, perm_lds, slo, shi)) {
             if (!compact || compatible(slo, shi, triv_lo, triv_hi)) bits |= 1ULL << i;
         }
-        if (i + 1 < cnt) x = next_comb(x);
+        if (i + 1 < cnt) x = next_comb<S>(x);
     }
     bitmap[w] = bits;
 }
 
+template <class S>
 __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, const uint64_t *binom, int n_up,
                                                                int64_t dim, int64_t n_words,
                                                                uint64_t triv_lo, uint64_t triv_hi,
                                                                const uint64_t *bitmap, const uint32_t *prefix,
                                                                uint64_t *reps, uint8_t *stab, uint8_t *nullf) {
-    __shared__ GroupLds lds;
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
-    stage_group(lds, perm_lds, gv);
+    stage_perm(perm_lds, gv);
     stage_binom(bin, binom);
     __syncthreads();
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -88,35 +88,35 @@ __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, con
     int64_t pos = prefix[w];
     const int64_t r0 = w * 64;
     const int last = 63 - __builtin_clzll(bits);
-    uint64_t x = colex_unrank(r0, n_up, gv.n_sites, bin);
+    S x = colex_unrank<S>(r0, n_up, gv.n_sites, bin);
     for (int i = 0; i <= last; ++i) {
         if ((bits >> i) & 1ULL) {
             uint64_t slo, shi;
-            is_representative(x, gv, lds, perm_lds, slo, shi);
-            reps[pos] = x;
+            is_representative<S>(x, gv, perm_lds, slo, shi);
+            reps[pos] = (uint64_t)x;
             stab[pos] = (uint8_t)(__popcll(slo) + __popcll(shi));
             nullf[pos] = compatible(slo, shi, triv_lo, triv_hi) ? 0 : 1;
             ++pos;
         }
-        if (i < last) x = next_comb(x);
+        if (i < last) x = next_comb<S>(x);
     }
 }
 
+template <class S>
 __global__ __launch_bounds__(kBasisThreads) void basis_lookup(GroupView gv, const uint64_t *binom, int64_t n,
                                                               const uint64_t *states, const uint64_t *bitmap,
                                                               const uint32_t *prefix, int64_t *rows,
                                                               int32_t *elems) {
-    __shared__ GroupLds lds;
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
-    stage_group(lds, perm_lds, gv);
+    stage_perm(perm_lds, gv);
     stage_binom(bin, binom);
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int gm;
-    const uint64_t rep = orbit_min(states[i], gv, lds, perm_lds, gm);
-    rows[i] = row_of_rep(rep, bin, bitmap, prefix);
+    const S rep = orbit_min<S>((S)states[i], gv, perm_lds, gm);
+    rows[i] = row_of_rep<S>(rep, bin, bitmap, prefix);
     elems[i] = gm;
 }
 
@@ -195,7 +195,8 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
         b->bitmap.alloc((size_t)b->n_words * sizeof(uint64_t));
         b->prefix.alloc((size_t)b->n_words * sizeof(uint32_t));
         const unsigned blocks = (unsigned)((b->n_words + kBasisThreads - 1) / kBasisThreads);
-        hipLaunchKernelGGL(basis_mark, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, gv,
+        const bool narrow = g->n_sites <= 32;
+        hipLaunchKernelGGL(narrow ? basis_mark<uint32_t> : basis_mark<uint64_t>, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, gv,
                            ctx->binom.as<uint64_t>(), n_up, b->sector_dim, b->n_words,
                            layout == QMB_LAYOUT_COMPACT ? 1 : 0, b->triv_lo, b->triv_hi, b->bitmap.as<uint64_t>());
         QMB_HIP(hipGetLastError());
@@ -206,7 +207,7 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
         b->stab.alloc((size_t)std::max<int64_t>(rows, 1));
         b->nullf.alloc((size_t)std::max<int64_t>(rows, 1));
         if (rows > 0) {
-            hipLaunchKernelGGL(basis_compact, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, gv,
+            hipLaunchKernelGGL(narrow ? basis_compact<uint32_t> : basis_compact<uint64_t>, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, gv,
                                ctx->binom.as<uint64_t>(), n_up, b->sector_dim, b->n_words, b->triv_lo, b->triv_hi,
                                b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>());
@@ -276,7 +277,7 @@ qmb_status qmb_basis_lookup(const qmb_basis *b, int64_t n, const uint64_t *state
         DevBuf ds(n * sizeof(uint64_t)), dr(n * sizeof(int64_t)), de(n * sizeof(int32_t));
         QMB_HIP(hipMemcpyAsync(ds.p, states, n * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
         const unsigned blocks = (unsigned)((n + kBasisThreads - 1) / kBasisThreads);
-        hipLaunchKernelGGL(basis_lookup, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, b->view(),
+        hipLaunchKernelGGL(b->n_sites <= 32 ? basis_lookup<uint32_t> : basis_lookup<uint64_t>, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, b->view(),
                            ctx->binom.as<uint64_t>(), n, ds.as<uint64_t>(), b->bitmap.as<uint64_t>(),
                            b->prefix.as<uint32_t>(), dr.as<int64_t>(), de.as<int32_t>());
         QMB_HIP(hipGetLastError());

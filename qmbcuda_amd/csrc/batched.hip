@@ -1,0 +1,635 @@
+// Batched Lanczos over many independent sectors (no reference counterpart; the
+// reference solves irreps one after another,
+// Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:127-229).
+//
+// All instances advance in lockstep with one iteration counter on the device.
+// Every phase is ONE launch covering every instance: a block task table maps
+// each block to (instance, chunk or element range); each instance owns a
+// contiguous block range, so its dot products are reduced deterministically by
+// a finishing kernel with one block per instance. Instances with real and complex matrices form two
+// families with their own kernel instantiations. A chunk of iterations is
+// captured once as a HIP graph and replayed; between replays the host solves
+// the tridiagonal problems and clears the active flag of converged instances
+// (their blocks return at once). With n_eigs > 1 the Krylov vectors are kept and
+// every new vector is reorthogonalised twice (classical Gram-Schmidt).
+// SpMV tile shape (chunks per block) is picked per family by timing the
+// candidates on the actual batch (autotuner with a per-process cache).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <tuple>
+
+#include "internal.hpp"
+#include "reduce.hpp"
+
+namespace qmb {
+
+struct BInst {
+    const int64_t *chunk_ptr;
+    const uint16_t *row_len;
+    const int32_t *col;
+    const double *val;
+    int64_t n_rows, n_chunks;
+    double *P, *Y, *VB;          // w, H phi, Krylov basis [M][n*vs]
+    double *alpha, *beta, *invb;  // [M], [M+1], [M+1]
+    double *coef;                 // [M*2] reorthogonalisation coefficients
+    unsigned *counter;            // [4]
+    int spmv_first, spmv_nblk;
+    int vec_first, vec_nblk;
+};
+
+struct BTask {
+    int inst;
+    int pad;
+    int64_t lo, hi;
+};
+
+constexpr int kVecPerBlock = kThreads * 8;
+
+template <bool CPLX>
+__global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
+                                                    const int *__restrict__ active, const int *__restrict__ it_p,
+                                                    double *__restrict__ part) {
+    __shared__ double lds[kWaves];
+    const BTask t = tasks[blockIdx.x];
+    if (!active[t.inst]) return;
+    const BInst I = insts[t.inst];
+    const int it = *it_p;
+    const double s = I.invb[it];
+    const int vs = CPLX ? 2 : 1;
+    double *vout = I.VB + (size_t)it * I.n_rows * vs;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double dot = 0.0;
+    for (int64_t c = t.lo + wv; c < t.hi; c += kWaves) {
+        const int64_t r = c * 64 + lane;
+        const int64_t base = I.chunk_ptr[c];
+        const int L = (int)((I.chunk_ptr[c + 1] - base) >> 6);
+        const int len = r < I.n_rows ? (int)I.row_len[r] : 0;
+        const int64_t p0 = base + lane;
+        if (!CPLX) {
+            double acc = 0.0;
+            for (int k = 0; k < L; ++k) {
+                if (k < len) {
+                    const int64_t p = p0 + (int64_t)k * 64;
+                    const int32_t j = __builtin_nontemporal_load(I.col + p);
+                    const double v = __builtin_nontemporal_load(I.val + p);
+                    acc += v * (I.P[j] * s);
+                }
+            }
+            if (r < I.n_rows) {
+                I.Y[r] = acc;
+                const double vr = I.P[r] * s;
+                vout[r] = vr;
+                dot += vr * acc;
+            }
+        } else {
+            const double2 *val2 = reinterpret_cast<const double2 *>(I.val);
+            const double2 *x2 = reinterpret_cast<const double2 *>(I.P);
+            double ar = 0.0, ai = 0.0;
+            for (int k = 0; k < L; ++k) {
+                if (k < len) {
+                    const int64_t p = p0 + (int64_t)k * 64;
+                    const int32_t j = __builtin_nontemporal_load(I.col + p);
+                    const double2 v = val2[p];
+                    double2 g = x2[j];
+                    g.x = g.x * s;
+                    g.y = g.y * s;
+                    ar += v.x * g.x - v.y * g.y;
+                    ai += v.x * g.y + v.y * g.x;
+                }
+            }
+            if (r < I.n_rows) {
+                reinterpret_cast<double2 *>(I.Y)[r] = make_double2(ar, ai);
+                double2 vr = x2[r];
+                vr.x = vr.x * s;
+                vr.y = vr.y * s;
+                reinterpret_cast<double2 *>(vout)[r] = vr;
+                dot += vr.x * ar + vr.y * ai;
+            }
+        }
+    }
+    const double tb = block_sum(dot, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = tb;
+}
+
+// One block per instance of the family: fixed-order sum of the instance's block
+// partials. mode 0: alpha[it] = s; 1: beta[it+1] = sqrt(s), invb[it+1];
+// 2 (start vector): invb[0] = 1/sqrt(s), beta[0] = 0.
+__global__ __launch_bounds__(kThreads) void bk_finish(const BInst *__restrict__ insts, const int *__restrict__ active,
+                                                      const int *__restrict__ it_p, const double *__restrict__ part,
+                                                      int first_inst, int mode) {
+    __shared__ double lds[kWaves];
+    const int inst = first_inst + blockIdx.x;
+    if (active && !active[inst]) return;
+    const BInst I = insts[inst];
+    const int it = it_p ? *it_p : 0;
+    const int first = mode == 0 ? I.spmv_first : I.vec_first;
+    const int nblk = mode == 0 ? I.spmv_nblk : I.vec_nblk;
+    const double s = sum_partials(part + first, nblk, lds);
+    if (threadIdx.x == 0) {
+        if (mode == 0) {
+            I.alpha[it] = s;
+        } else if (mode == 1) {
+            const double nb = sqrt(s);
+            I.beta[it + 1] = nb;
+            I.invb[it + 1] = nb > 0.0 ? 1.0 / nb : 0.0;
+        } else {
+            const double nr = sqrt(s);
+            I.invb[0] = nr > 0.0 ? 1.0 / nr : 0.0;
+            I.beta[0] = 0.0;
+        }
+    }
+}
+
+// w = (-a) phi_it + (y + (-b) phi_{it-1}); ||w|| -> beta[it+1]. Ranges are in doubles.
+__global__ __launch_bounds__(kThreads) void bk_update(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
+                                                      const int *__restrict__ active, const int *__restrict__ it_p,
+                                                      double *__restrict__ part, int vs) {
+    __shared__ double lds[kWaves];
+    const BTask t = tasks[blockIdx.x];
+    if (!active[t.inst]) return;
+    const BInst I = insts[t.inst];
+    const int it = *it_p;
+    const int64_t cnt = I.n_rows * vs;
+    const double a = I.alpha[it], b = I.beta[it];
+    const double *v = I.VB + (size_t)it * cnt;
+    const double *vp = it > 0 ? I.VB + (size_t)(it - 1) * cnt : nullptr;
+    double acc = 0.0;
+    for (int64_t i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
+        const double tt = vp ? I.Y[i] + (-b) * vp[i] : I.Y[i] + (-b) * 0.0;
+        const double wi = (-a) * v[i] + tt;
+        I.P[i] = wi;
+        acc += wi * wi;
+    }
+    const double tb = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = tb;
+}
+
+// part2[(j*nbv + block)*2 + {0,1}] = <V_j, w> over the block's range, j <= it.
+template <bool CPLX>
+__global__ __launch_bounds__(kThreads) void bk_rdots(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
+                                                     const int *__restrict__ active, const int *__restrict__ it_p,
+                                                     double *__restrict__ part2, int nbv) {
+    __shared__ double lds[kWaves];
+    const BTask t = tasks[blockIdx.x];
+    if (!active[t.inst]) return;
+    const BInst I = insts[t.inst];
+    const int it = *it_p;
+    const int vs = CPLX ? 2 : 1;
+    const int64_t cnt = I.n_rows * vs;
+    for (int j = 0; j <= it; ++j) {
+        const double *vj = I.VB + (size_t)j * cnt;
+        double sr = 0.0, si = 0.0;
+        if (CPLX) {
+            for (int64_t i = t.lo / 2 + threadIdx.x; i < t.hi / 2; i += blockDim.x) {
+                const double vr = vj[2 * i], vi = vj[2 * i + 1], wr = I.P[2 * i], wi = I.P[2 * i + 1];
+                sr += vr * wr + vi * wi;
+                si += vr * wi - vi * wr;
+            }
+        } else {
+            for (int64_t i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) sr += vj[i] * I.P[i];
+        }
+        const double tr = block_sum(sr, lds);
+        const double ti = CPLX ? block_sum(si, lds) : 0.0;
+        if (threadIdx.x == 0) {
+            part2[((size_t)j * nbv + blockIdx.x) * 2] = tr;
+            part2[((size_t)j * nbv + blockIdx.x) * 2 + 1] = ti;
+        }
+    }
+}
+
+// coef_j = sum of the instance's block partials, fixed order; one block per instance.
+__global__ __launch_bounds__(kThreads) void bk_rreduce(const BInst *__restrict__ insts, const int *__restrict__ active,
+                                                       const int *__restrict__ it_p, const double *__restrict__ part2,
+                                                       int nbv, int first_inst) {
+    __shared__ double lds[kWaves];
+    const int inst = first_inst + blockIdx.x;
+    if (!active[inst]) return;
+    const BInst I = insts[inst];
+    const int it = *it_p;
+    for (int j = 0; j <= it; ++j) {
+        double sr = 0.0, si = 0.0;
+        for (int b = threadIdx.x; b < I.vec_nblk; b += blockDim.x) {
+            sr += part2[((size_t)j * nbv + I.vec_first + b) * 2];
+            si += part2[((size_t)j * nbv + I.vec_first + b) * 2 + 1];
+        }
+        const double tr = block_sum(sr, lds);
+        const double ti = block_sum(si, lds);
+        if (threadIdx.x == 0) {
+            I.coef[2 * j] = tr;
+            I.coef[2 * j + 1] = ti;
+        }
+    }
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(kThreads) void bk_rapply(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
+                                                      const int *__restrict__ active, const int *__restrict__ it_p,
+                                                      double *__restrict__ part) {
+    __shared__ double lds[kWaves];
+    const BTask t = tasks[blockIdx.x];
+    if (!active[t.inst]) return;
+    const BInst I = insts[t.inst];
+    const int it = *it_p;
+    const int vs = CPLX ? 2 : 1;
+    const int64_t cnt = I.n_rows * vs;
+    double acc = 0.0;
+    if (CPLX) {
+        for (int64_t i = t.lo / 2 + threadIdx.x; i < t.hi / 2; i += blockDim.x) {
+            double wr = I.P[2 * i], wi = I.P[2 * i + 1];
+            for (int j = 0; j <= it; ++j) {
+                const double cr = I.coef[2 * j], ci = I.coef[2 * j + 1];
+                const double vr = I.VB[(size_t)j * cnt + 2 * i], vi = I.VB[(size_t)j * cnt + 2 * i + 1];
+                wr -= cr * vr - ci * vi;
+                wi -= cr * vi + ci * vr;
+            }
+            I.P[2 * i] = wr;
+            I.P[2 * i + 1] = wi;
+            acc += wr * wr + wi * wi;
+        }
+    } else {
+        for (int64_t i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
+            double wr = I.P[i];
+            for (int j = 0; j <= it; ++j) wr -= I.coef[2 * j] * I.VB[(size_t)j * cnt + i];
+            I.P[i] = wr;
+            acc += wr * wr;
+        }
+    }
+    const double tb = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = tb;
+}
+
+__global__ void bk_advance(int *it_p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *it_p = *it_p + 1;
+}
+
+// start vectors and their norms (outside the graph)
+__global__ __launch_bounds__(kThreads) void bk_start(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
+                                                     double *__restrict__ part, uint64_t seed, int vs) {
+    __shared__ double lds[kWaves];
+    const BTask t = tasks[blockIdx.x];
+    const BInst I = insts[t.inst];
+    double acc = 0.0;
+    for (int64_t i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
+        uint64_t z = seed + ((uint64_t)i + 1ULL) * 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        const double u = (double)((z >> 11) + 1ULL) * (1.0 / 9007199254740992.0);
+        I.P[i] = u;
+        acc += u * u;
+    }
+    const double tb = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = tb;
+    (void)vs;
+}
+
+struct Family {
+    bool cplx = false;
+    std::vector<int> members;  // instance indices (global numbering)
+    int first_inst = 0;        // position of the family's first instance in the device BInst array
+    std::vector<BTask> spmv_tasks, vec_tasks;
+    DevBuf d_spmv_tasks, d_vec_tasks;
+    int spmv_first_block = 0, vec_first_block = 0;
+    int cpb = 4;
+};
+
+static std::map<std::tuple<int, int, int>, int> g_tile_cache;
+
+}  // namespace qmb
+
+using namespace qmb;
+
+extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const qmb_lanczos_opts *opts,
+                                          double *evals, qmb_lanczos_report *reports) {
+    return guarded([&] {
+        QMB_REQUIRE(ctx && ells && opts && evals && n >= 1, "qmb_lanczos_batched: bad argument");
+        const qmb_lanczos_opts o = *opts;
+        QMB_REQUIRE(o.max_iter >= 1 && o.max_iter <= 5000, "qmb_lanczos_batched: max_iter must be in [1, 5000]");
+        QMB_REQUIRE(o.n_eigs >= 1 && o.n_eigs <= 8, "qmb_lanczos_batched: n_eigs must be in [1, 8]");
+        QMB_REQUIRE(o.n_eigs == 1 || o.reorth, "qmb_lanczos_batched: n_eigs > 1 needs reorth = 1");
+        QMB_REQUIRE(o.start == QMB_START_RANDOM, "qmb_lanczos_batched: only the seeded random start");
+        for (int i = 0; i < n; ++i) {
+            QMB_REQUIRE(ells[i] && ells[i]->ctx == ctx, "qmb_lanczos_batched: ELL missing or from another context");
+            QMB_REQUIRE(ells[i]->n_rows > 0, "qmb_lanczos_batched: empty matrix in the batch");
+        }
+        auto wall0 = std::chrono::steady_clock::now();
+        QMB_HIP(hipSetDevice(ctx->device));
+        const int chunk = o.check_every > 0 ? o.check_every : 10;
+        const int M = ((o.max_iter + chunk - 1) / chunk) * chunk;
+        const double tol = o.tol > 0 ? o.tol : 1e-13;
+        const bool reorth = o.reorth != 0;
+        const int k_eigs = o.n_eigs;
+
+        // families: real first, then complex
+        Family fam[2];
+        fam[0].cplx = false;
+        fam[1].cplx = true;
+        for (int i = 0; i < n; ++i) fam[ells[i]->value_type == QMB_COMPLEX ? 1 : 0].members.push_back(i);
+        std::vector<int> order;  // device instance position -> global index
+        for (auto &f : fam) {
+            f.first_inst = (int)order.size();
+            for (int i : f.members) order.push_back(i);
+        }
+        // per-instance device buffers
+        std::vector<BInst> hin(n);
+        std::vector<DevBuf> bufs;
+        bufs.reserve((size_t)n * 6);
+        DevBuf scal((size_t)n * (3 * M + 2) * sizeof(double));
+        DevBuf coefs((size_t)n * M * 2 * sizeof(double));
+        DevBuf counters((size_t)n * 4 * sizeof(unsigned));
+        QMB_HIP(hipMemsetAsync(scal.p, 0, scal.bytes, ctx->stream));
+        QMB_HIP(hipMemsetAsync(counters.p, 0, counters.bytes, ctx->stream));
+        for (int pos = 0; pos < n; ++pos) {
+            const qmb_ell *e = ells[order[pos]];
+            const int vs = e->value_type == QMB_COMPLEX ? 2 : 1;
+            const int64_t cnt = e->n_rows * vs;
+            BInst &I = hin[pos];
+            I.chunk_ptr = e->chunk_ptr.as<int64_t>();
+            I.row_len = e->row_len.as<uint16_t>();
+            I.col = e->col.as<int32_t>();
+            I.val = e->val.as<double>();
+            I.n_rows = e->n_rows;
+            I.n_chunks = e->n_chunks;
+            bufs.emplace_back(cnt * sizeof(double));
+            I.P = bufs.back().as<double>();
+            bufs.emplace_back(cnt * sizeof(double));
+            I.Y = bufs.back().as<double>();
+            bufs.emplace_back((size_t)M * cnt * sizeof(double));
+            I.VB = bufs.back().as<double>();
+            double *sc = scal.as<double>() + (size_t)pos * (3 * M + 2);
+            I.alpha = sc;
+            I.beta = sc + M;
+            I.invb = sc + 2 * M + 1;
+            I.coef = coefs.as<double>() + (size_t)pos * M * 2;
+            I.counter = counters.as<unsigned>() + 4 * pos;
+        }
+        // vector task tables (ranges in doubles, both families in one table per family)
+        int total_vec_blocks = 0, total_spmv_blocks = 0;
+        for (auto &f : fam) {
+            f.vec_first_block = total_vec_blocks;
+            for (size_t q = 0; q < f.members.size(); ++q) {
+                const int pos = f.first_inst + (int)q;
+                const qmb_ell *e = ells[order[pos]];
+                const int64_t cnt = e->n_rows * (f.cplx ? 2 : 1);
+                const int64_t per = f.cplx ? 2 * kVecPerBlock : kVecPerBlock;  // keep complex pairs together
+                hin[pos].vec_first = total_vec_blocks;
+                int nb = 0;
+                for (int64_t lo = 0; lo < cnt; lo += per, ++nb)
+                    f.vec_tasks.push_back(BTask{pos, 0, lo, std::min<int64_t>(cnt, lo + per)});
+                hin[pos].vec_nblk = nb;
+                total_vec_blocks += nb;
+            }
+        }
+        const int nbv = total_vec_blocks;
+        auto build_spmv_tasks = [&](Family &f, int cpb, int &first_block_io) {
+            f.spmv_tasks.clear();
+            f.spmv_first_block = first_block_io;
+            for (size_t q = 0; q < f.members.size(); ++q) {
+                const int pos = f.first_inst + (int)q;
+                hin[pos].spmv_first = first_block_io + (int)f.spmv_tasks.size();
+                int nb = 0;
+                for (int64_t lo = 0; lo < hin[pos].n_chunks; lo += cpb, ++nb)
+                    f.spmv_tasks.push_back(BTask{pos, 0, lo, std::min<int64_t>(hin[pos].n_chunks, lo + cpb)});
+                hin[pos].spmv_nblk = nb;
+            }
+            first_block_io += (int)f.spmv_tasks.size();
+        };
+        // device tables
+        DevBuf d_inst((size_t)n * sizeof(BInst)), d_active((size_t)n * sizeof(int)), d_it(sizeof(int));
+        std::vector<int> active(n, 1);
+        const int zero = 0;
+        QMB_HIP(hipMemcpyAsync(d_it.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        QMB_HIP(hipMemcpyAsync(d_active.p, active.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        for (auto &f : fam) {
+            if (f.vec_tasks.empty()) continue;
+            f.d_vec_tasks.alloc(f.vec_tasks.size() * sizeof(BTask));
+            QMB_HIP(hipMemcpy(f.d_vec_tasks.p, f.vec_tasks.data(), f.vec_tasks.size() * sizeof(BTask),
+                              hipMemcpyHostToDevice));
+        }
+        // partial slots: block ids are offsets into one array per phase type
+        int64_t max_spmv_blocks = 0;
+        for (auto &f : fam)
+            for (int i : f.members) max_spmv_blocks += ells[i]->n_chunks;
+        DevBuf part((size_t)std::max<int64_t>(max_spmv_blocks, nbv) * sizeof(double) + 64);
+        DevBuf part2(reorth ? (size_t)M * nbv * 2 * sizeof(double) : 64);
+
+        auto upload_inst = [&]() {
+            QMB_HIP(hipMemcpy(d_inst.p, hin.data(), n * sizeof(BInst), hipMemcpyHostToDevice));
+        };
+        // start vectors: per family, the vector task table; block ids must index `part`
+        upload_inst();
+        for (auto &f : fam) {
+            if (f.vec_tasks.empty()) continue;
+            // bk_start uses blockIdx.x as the partial slot: launch with the family's table, slots offset
+            // by shifting the instance's vec_first to the family-local numbering
+            std::vector<BInst> local = hin;
+            for (size_t q = 0; q < f.members.size(); ++q) local[f.first_inst + q].vec_first -= f.vec_first_block;
+            DevBuf d_local((size_t)n * sizeof(BInst));
+            QMB_HIP(hipMemcpy(d_local.p, local.data(), n * sizeof(BInst), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(bk_start, dim3((unsigned)f.vec_tasks.size()), dim3(kThreads), 0, ctx->stream,
+                               d_local.as<BInst>(), f.d_vec_tasks.as<BTask>(), part.as<double>(), o.seed,
+                               f.cplx ? 2 : 1);
+            QMB_HIP(hipGetLastError());
+            hipLaunchKernelGGL(bk_finish, dim3((unsigned)f.members.size()), dim3(kThreads), 0, ctx->stream,
+                               d_local.as<BInst>(), nullptr, nullptr, part.as<double>(), f.first_inst, 2);
+            QMB_HIP(hipGetLastError());
+            QMB_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        // The vector-phase kernels address `part` by family-local block id; make the
+        // instance descriptors family-local for the vector ranges.
+        for (auto &f : fam)
+            for (size_t q = 0; q < f.members.size(); ++q) hin[f.first_inst + q].vec_first -= f.vec_first_block;
+
+        // autotune the SpMV tile (chunks per block) per family
+        for (auto &f : fam) {
+            if (f.members.empty()) continue;
+            int64_t rows = 0;
+            for (int i : f.members) rows += ells[i]->n_rows;
+            const auto key = std::make_tuple((int)f.cplx, (int)std::log2((double)rows + 1.0), (int)f.members.size());
+            auto it = g_tile_cache.find(key);
+            if (it != g_tile_cache.end()) {
+                f.cpb = it->second;
+                continue;
+            }
+            float best = 1e30f;
+            int best_cpb = 4;
+            hipEvent_t ea, eb;
+            QMB_HIP(hipEventCreate(&ea));
+            QMB_HIP(hipEventCreate(&eb));
+            for (int cpb : {1, 2, 4, 8, 16, 32}) {
+                int fb = 0;
+                build_spmv_tasks(f, cpb, fb);
+                upload_inst();
+                DevBuf dt(f.spmv_tasks.size() * sizeof(BTask));
+                QMB_HIP(hipMemcpy(dt.p, f.spmv_tasks.data(), f.spmv_tasks.size() * sizeof(BTask),
+                                  hipMemcpyHostToDevice));
+                auto k = f.cplx ? bk_spmv<true> : bk_spmv<false>;
+                hipLaunchKernelGGL(k, dim3((unsigned)f.spmv_tasks.size()), dim3(kThreads), 0, ctx->stream,
+                                   d_inst.as<BInst>(), dt.as<BTask>(), d_active.as<int>(), d_it.as<int>(),
+                                   part.as<double>());
+                QMB_HIP(hipEventRecord(ea, ctx->stream));
+                for (int rep = 0; rep < 3; ++rep)
+                    hipLaunchKernelGGL(k, dim3((unsigned)f.spmv_tasks.size()), dim3(kThreads), 0, ctx->stream,
+                                       d_inst.as<BInst>(), dt.as<BTask>(), d_active.as<int>(), d_it.as<int>(),
+                                       part.as<double>());
+                QMB_HIP(hipEventRecord(eb, ctx->stream));
+                QMB_HIP(hipEventSynchronize(eb));
+                float t = 0.f;
+                QMB_HIP(hipEventElapsedTime(&t, ea, eb));
+                if (t < best) {
+                    best = t;
+                    best_cpb = cpb;
+                }
+            }
+            (void)hipEventDestroy(ea);
+            (void)hipEventDestroy(eb);
+            f.cpb = best_cpb;
+            g_tile_cache[key] = best_cpb;
+        }
+        // final SpMV task tables; spmv block ids are family-local too
+        for (auto &f : fam) {
+            if (f.members.empty()) continue;
+            int fb = 0;
+            build_spmv_tasks(f, f.cpb, fb);
+            f.d_spmv_tasks.alloc(f.spmv_tasks.size() * sizeof(BTask));
+            QMB_HIP(hipMemcpy(f.d_spmv_tasks.p, f.spmv_tasks.data(), f.spmv_tasks.size() * sizeof(BTask),
+                              hipMemcpyHostToDevice));
+        }
+        upload_inst();
+        // tuning ran real SpMVs at it = 0: restore the state the first iteration expects
+        QMB_HIP(hipMemcpyAsync(d_it.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+
+        // one iteration = every family's phases, then the counter advance
+        auto enqueue_iteration = [&](hipStream_t s) {
+            for (auto &f : fam) {
+                if (f.members.empty()) continue;
+                const unsigned nbs = (unsigned)f.spmv_tasks.size(), nvb = (unsigned)f.vec_tasks.size();
+                const int vs = f.cplx ? 2 : 1;
+                const unsigned nm = (unsigned)f.members.size();
+                hipLaunchKernelGGL(f.cplx ? bk_spmv<true> : bk_spmv<false>, dim3(nbs), dim3(kThreads), 0, s,
+                                   d_inst.as<BInst>(), f.d_spmv_tasks.as<BTask>(), d_active.as<int>(), d_it.as<int>(),
+                                   part.as<double>());
+                hipLaunchKernelGGL(bk_finish, dim3(nm), dim3(kThreads), 0, s, d_inst.as<BInst>(), d_active.as<int>(),
+                                   d_it.as<int>(), part.as<double>(), f.first_inst, 0);
+                hipLaunchKernelGGL(bk_update, dim3(nvb), dim3(kThreads), 0, s, d_inst.as<BInst>(),
+                                   f.d_vec_tasks.as<BTask>(), d_active.as<int>(), d_it.as<int>(), part.as<double>(),
+                                   vs);
+                hipLaunchKernelGGL(bk_finish, dim3(nm), dim3(kThreads), 0, s, d_inst.as<BInst>(), d_active.as<int>(),
+                                   d_it.as<int>(), part.as<double>(), f.first_inst, 1);
+                if (reorth) {
+                    for (int pass = 0; pass < 2; ++pass) {
+                        hipLaunchKernelGGL(f.cplx ? bk_rdots<true> : bk_rdots<false>, dim3(nvb), dim3(kThreads), 0,
+                                           s, d_inst.as<BInst>(), f.d_vec_tasks.as<BTask>(), d_active.as<int>(),
+                                           d_it.as<int>(), part2.as<double>(), (int)nvb);
+                        hipLaunchKernelGGL(bk_rreduce, dim3((unsigned)f.members.size()), dim3(kThreads), 0, s,
+                                           d_inst.as<BInst>(), d_active.as<int>(), d_it.as<int>(), part2.as<double>(),
+                                           (int)nvb, f.first_inst);
+                        hipLaunchKernelGGL(f.cplx ? bk_rapply<true> : bk_rapply<false>, dim3(nvb), dim3(kThreads), 0,
+                                           s, d_inst.as<BInst>(), f.d_vec_tasks.as<BTask>(), d_active.as<int>(),
+                                           d_it.as<int>(), part.as<double>());
+                        hipLaunchKernelGGL(bk_finish, dim3(nm), dim3(kThreads), 0, s, d_inst.as<BInst>(),
+                                           d_active.as<int>(), d_it.as<int>(), part.as<double>(), f.first_inst, 1);
+                    }
+                }
+            }
+            hipLaunchKernelGGL(bk_advance, dim3(1), dim3(64), 0, s, d_it.as<int>());
+        };
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        QMB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < chunk; ++q) enqueue_iteration(ctx->stream);
+        QMB_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        QMB_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+
+        std::vector<double> ha((size_t)n * (3 * M + 2));
+        std::vector<std::vector<double>> ev(n, std::vector<double>(k_eigs, std::nan(""))),
+            res(n, std::vector<double>(k_eigs, std::nan("")));
+        std::vector<int> iters(n, 0), conv(n, 0), brk(n, 0);
+        int it = 0, n_active = n;
+        double ms_host = 0.0;
+        std::vector<double> w(k_eigs + 1), zl(k_eigs + 1);
+        while (it < M && n_active > 0) {
+            QMB_HIP(hipGraphLaunch(exec, ctx->stream));
+            it += chunk;
+            QMB_HIP(hipMemcpyAsync(ha.data(), scal.p, ha.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            QMB_HIP(hipStreamSynchronize(ctx->stream));
+            auto th0 = std::chrono::steady_clock::now();
+            bool changed = false;
+            for (int pos = 0; pos < n; ++pos) {
+                if (!active[pos]) continue;
+                const double *a = ha.data() + (size_t)pos * (3 * M + 2);
+                const double *b = a + M;
+                double tsc = 0.0;
+                for (int j = 0; j < it; ++j) tsc = std::max(tsc, std::fabs(a[j]) + b[j] + b[j + 1]);
+                int m = it;
+                bool bd = false;
+                for (int j = 1; j <= it; ++j)
+                    if (!(b[j] > 1e-13 * std::max(tsc, 1e-300))) {
+                        m = j;
+                        bd = true;
+                        break;
+                    }
+                const int kk = std::min(k_eigs + 1, m);
+                tridiag_lowest(m, a, b, kk, w.data(), zl.data(), nullptr);
+                bool ok = bd && m >= k_eigs;
+                if (!bd && m >= k_eigs) {
+                    ok = true;
+                    const double sq = std::sqrt(tol);
+                    for (int j = 0; j < k_eigs && ok; ++j) {
+                        const double sc = std::max(1.0, std::fabs(w[j]));
+                        const double r = std::fabs(b[m] * zl[j]);
+                        double gap = 1.0;
+                        if (j + 1 < kk) gap = w[j + 1] - w[j];
+                        if (j > 0) gap = std::min(gap, w[j] - w[j - 1]);
+                        ok = r <= sq * sc && r * r / std::max(gap, sq) <= tol * sc;
+                    }
+                }
+                const int kq = std::min(k_eigs, m);
+                for (int j = 0; j < k_eigs; ++j) {
+                    ev[pos][j] = j < kq ? w[j] : std::nan("");
+                    res[pos][j] = j < kq ? std::fabs(b[m] * zl[j]) : std::nan("");
+                }
+                iters[pos] = m;
+                if (ok || bd || it >= M) {
+                    conv[pos] = ok ? 1 : 0;
+                    brk[pos] = bd ? 1 : 0;
+                    active[pos] = 0;
+                    --n_active;
+                    changed = true;
+                }
+            }
+            ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
+            if (changed && n_active > 0)
+                QMB_HIP(hipMemcpyAsync(d_active.p, active.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                                       ctx->stream));
+        }
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        const double wall =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+        bool all_conv = true;
+        for (int pos = 0; pos < n; ++pos) {
+            const int gi = order[pos];
+            for (int j = 0; j < k_eigs; ++j) evals[(size_t)gi * k_eigs + j] = ev[pos][j];
+            all_conv = all_conv && conv[pos];
+            if (reports) {
+                qmb_lanczos_report &r = reports[gi];
+                std::memset(&r, 0, sizeof(r));
+                r.iters = iters[pos];
+                r.converged = conv[pos];
+                r.breakdown = brk[pos];
+                r.n_spmv = iters[pos];
+                for (int j = 0; j < std::min(k_eigs, 8); ++j) r.residual[j] = res[pos][j];
+                r.ms_total = wall;
+                r.ms_host = ms_host;
+            }
+        }
+        if (!all_conv) throw Error{QMB_ERR_NOT_CONVERGED, "qmb_lanczos_batched: some instances did not converge"};
+    });
+}

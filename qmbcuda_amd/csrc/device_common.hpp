@@ -1,5 +1,11 @@
 // Device helpers shared by the basis and ELL kernels: group action, colex
 // rank/unrank, next combination, orbit minimum and row lookup.
+//
+// S is the state word: uint32_t when the lattice has at most 32 sites (half the
+// VALU work of 64-bit words), uint64_t otherwise. The group loops are uniform
+// across a wavefront, so the per-element tables are read straight from global
+// memory (scalar loads); only lane-divergent tables (binomials, generic
+// permutations) are staged in LDS.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,26 +16,15 @@
 
 namespace qmb {
 
-// Group tables staged in LDS by every block that applies group elements.
-struct GroupLds {
-    uint64_t keep[kMaxGroup];
-    uint8_t small[kMaxGroup];
-    uint8_t big[kMaxGroup];
-    uint8_t flip[kMaxGroup];
-    uint8_t inv[kMaxGroup];
-};
-
 // Binomial table C(p, j), p < 64, j <= 32, staged in LDS.
 constexpr int kBinJ = 33;
 
-__device__ inline void stage_group(GroupLds &lds, uint8_t *perm_lds, const GroupView &g) {
-    for (int i = threadIdx.x; i < g.G; i += blockDim.x) {
-        lds.keep[i] = g.keep[i];
-        lds.small[i] = g.small[i];
-        lds.big[i] = g.big[i];
-        lds.flip[i] = g.flip[i];
-        lds.inv[i] = g.inv[i];
-    }
+__device__ inline int ctz_s(uint32_t x) { return __builtin_ctz(x); }
+__device__ inline int ctz_s(uint64_t x) { return __builtin_ctzll(x); }
+__device__ inline int popc_s(uint32_t x) { return __popc(x); }
+__device__ inline int popc_s(uint64_t x) { return __popcll(x); }
+
+__device__ inline void stage_perm(uint8_t *perm_lds, const GroupView &g) {
     if (!g.torus)
         for (int i = threadIdx.x; i < g.G * g.n_sites; i += blockDim.x) perm_lds[i] = g.perm[i];
 }
@@ -38,41 +33,48 @@ __device__ inline void stage_binom(uint64_t *bin_lds, const uint64_t *bin) {
     for (int i = threadIdx.x; i < 64 * kBinJ; i += blockDim.x) bin_lds[i] = bin[i];
 }
 
+template <class S>
+__device__ inline S full_mask(const GroupView &gv) {
+    return (S)gv.full;
+}
+
 // Image of state x under element g: the reference's bit move s -> group_el[g][s]
 // (Q/exact_diagonalization/utils.cu:52-59), plus the spin flip of this build.
-__device__ inline uint64_t apply_elem(uint64_t x, int g, const GroupView &gv, const GroupLds &lds,
-                                      const uint8_t *perm_lds) {
-    uint64_t y;
+template <class S>
+__device__ inline S apply_elem(S x, int g, const GroupView &gv, const uint8_t *perm_lds) {
+    const S full = (S)gv.full;
+    S y;
     if (gv.torus) {
         y = x;
-        const unsigned s = lds.small[g];
+        const unsigned s = gv.small[g];
         if (s) {
-            const uint64_t k = lds.keep[g];
-            y = ((y & k) << s) | ((y & ~k & gv.full) >> (gv.block - s));
+            const S k = (S)gv.keep[g];
+            y = (S)(((y & k) << s) | ((y & (S)~k & full) >> (gv.block - s)));
         }
-        const unsigned b = lds.big[g];
-        if (b) y = ((y << b) | (y >> (gv.n_sites - b))) & gv.full;
+        const unsigned b = gv.big[g];
+        if (b) y = (S)(((y << b) | (y >> (gv.n_sites - b))) & full);
     } else {
         y = 0;
         const uint8_t *p = perm_lds + g * gv.n_sites;
-        uint64_t t = x;
+        S t = x;
         while (t) {
-            const int bit = __builtin_ctzll(t);
-            y |= 1ULL << p[bit];
+            const int bit = ctz_s(t);
+            y |= (S)1 << p[bit];
             t &= t - 1;
         }
     }
-    if (lds.flip[g]) y ^= gv.full;
+    if (gv.flip[g]) y ^= full;
     return y;
 }
 
 // Colex rank: sum over set bits p_0 < p_1 < ... of C(p_j, j+1); equals the
 // reference's GetIndexInHilbertSubspace (utils.cu:9-36).
-__device__ inline int64_t colex_rank(uint64_t x, const uint64_t *bin_lds) {
+template <class S>
+__device__ inline int64_t colex_rank(S x, const uint64_t *bin_lds) {
     int64_t r = 0;
     int j = 1;
     while (x) {
-        const int p = __builtin_ctzll(x);
+        const int p = ctz_s(x);
         r += (int64_t)bin_lds[p * kBinJ + j];
         ++j;
         x &= x - 1;
@@ -81,14 +83,15 @@ __device__ inline int64_t colex_rank(uint64_t x, const uint64_t *bin_lds) {
 }
 
 // Inverse of colex_rank for popcount k (k <= 32).
-__device__ inline uint64_t colex_unrank(int64_t rank, int k, int n_sites, const uint64_t *bin_lds) {
-    uint64_t x = 0;
+template <class S>
+__device__ inline S colex_unrank(int64_t rank, int k, int n_sites, const uint64_t *bin_lds) {
+    S x = 0;
     uint64_t r = (uint64_t)rank;
     int p = n_sites - 1;
     for (int j = k; j >= 1; --j) {
         while (p >= j && bin_lds[p * kBinJ + j] > r) --p;
         if (p < j - 1) p = j - 1;
-        x |= 1ULL << p;
+        x |= (S)1 << p;
         r -= bin_lds[p * kBinJ + j];
         --p;
     }
@@ -96,21 +99,22 @@ __device__ inline uint64_t colex_unrank(int64_t rank, int k, int n_sites, const 
 }
 
 // Next larger integer with the same popcount (ascending = colex order).
-__device__ inline uint64_t next_comb(uint64_t x) {
-    const uint64_t c = x & (~x + 1ULL);
-    const uint64_t r = x + c;
-    return (((r ^ x) >> 2) >> __builtin_ctzll(x)) | r;
+template <class S>
+__device__ inline S next_comb(S x) {
+    const S c = x & (S)(~x + (S)1);
+    const S r = x + c;
+    return (S)((((r ^ x) >> 2) >> ctz_s(x)) | r);
 }
 
 // Representative test. Returns true when x is the smallest integer of its
 // orbit; on success stab_lo/hi hold the stabiliser (elements with image x).
-__device__ inline bool is_representative(uint64_t x, const GroupView &gv, const GroupLds &lds,
-                                         const uint8_t *perm_lds, uint64_t &stab_lo,
+template <class S>
+__device__ inline bool is_representative(S x, const GroupView &gv, const uint8_t *perm_lds, uint64_t &stab_lo,
                                          uint64_t &stab_hi) {
     stab_lo = 0;
     stab_hi = 0;
     for (int g = 0; g < gv.G; ++g) {
-        const uint64_t y = apply_elem(x, g, gv, lds, perm_lds);
+        const S y = apply_elem<S>(x, g, gv, perm_lds);
         if (y < x) return false;
         if (y == x) {
             if (g < 64)
@@ -125,13 +129,13 @@ __device__ inline bool is_representative(uint64_t x, const GroupView &gv, const 
 // Orbit minimum of a state and the canonical element gm = the smallest g with
 // image_g(min) == x (the reference's orbit_indices entry, utils.cu:66-80, made
 // deterministic).
-__device__ inline uint64_t orbit_min(uint64_t x, const GroupView &gv, const GroupLds &lds,
-                                     const uint8_t *perm_lds, int &gm) {
-    uint64_t best = ~0ULL;
-    int bi = kMaxGroup;
-    for (int g = 0; g < gv.G; ++g) {
-        const uint64_t y = apply_elem(x, g, gv, lds, perm_lds);
-        const int ig = lds.inv[g];
+template <class S>
+__device__ inline S orbit_min(S x, const GroupView &gv, const uint8_t *perm_lds, int &gm) {
+    S best = x;
+    int bi = gv.inv[0];
+    for (int g = 1; g < gv.G; ++g) {
+        const S y = apply_elem<S>(x, g, gv, perm_lds);
+        const int ig = gv.inv[g];
         if (y < best) {
             best = y;
             bi = ig;
@@ -145,9 +149,9 @@ __device__ inline uint64_t orbit_min(uint64_t x, const GroupView &gv, const Grou
 
 // Row of a representative through the basis bitmap (rank -> word -> prefix +
 // popcount). Returns -1 when the representative is not a row.
-__device__ inline int64_t row_of_rep(uint64_t rep, const uint64_t *bin_lds, const uint64_t *bitmap,
-                                     const uint32_t *prefix) {
-    const int64_t rk = colex_rank(rep, bin_lds);
+template <class S>
+__device__ inline int64_t row_of_rep(S rep, const uint64_t *bin_lds, const uint64_t *bitmap, const uint32_t *prefix) {
+    const int64_t rk = colex_rank<S>(rep, bin_lds);
     const uint64_t w = bitmap[rk >> 6];
     const int b = (int)(rk & 63);
     if (!((w >> b) & 1ULL)) return -1;

@@ -28,19 +28,18 @@ struct TermsView {
 
 constexpr int kEllThreads = 256;
 
-template <bool FILL, bool CPLX>
+template <class S, bool FILL, bool CPLX>
 __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
                                                         int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
                                                         const uint8_t *nullf, const uint64_t *bitmap,
                                                         const uint32_t *prefix, TermsView tv, const double2 *chi,
                                                         const double *nrm, int W, uint16_t *row_len, int *err,
                                                         const int64_t *chunk_ptr, int32_t *col, double *val) {
-    __shared__ GroupLds lds;
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
     __shared__ double2 s_chi[kMaxGroup];
     __shared__ double s_nrm[kMaxGroup + 1];
-    stage_group(lds, perm_lds, gv);
+    stage_perm(perm_lds, gv);
     stage_binom(bin, binom);
     for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
     for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
@@ -59,10 +58,10 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
     const bool null_row = (layout == QMB_LAYOUT_REFERENCE) && nullf[r];
     double dr = 0.0, di = 0.0;
     if (!null_row) {
-        const uint64_t src = reps[r];
+        const S src = (S)reps[r];
         const double n_src = s_nrm[stab[r]];
         for (int h = 0; h < tv.n_terms; ++h) {
-            uint64_t tgt = src;
+            S tgt = src;
             const double2 c0 = tv.coef[h];
             double cr = c0.x, ci = c0.y;
             bool dead = false;
@@ -70,13 +69,13 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
             for (int k = 0; k < nk; ++k) {
                 const int s = tv.site[h * tv.max_ops + k];
                 const int op = tv.op[h * tv.max_ops + k];
-                const int spin = (int)((tgt >> s) & 1ULL);
+                const int spin = (int)((tgt >> s) & (S)1);
                 if (op == QMB_OP_SM) {
                     if (!spin) { dead = true; break; }
-                    tgt &= ~(1ULL << s);
+                    tgt &= (S)~((S)1 << s);
                 } else if (op == QMB_OP_SP) {
                     if (spin) { dead = true; break; }
-                    tgt |= 1ULL << s;
+                    tgt |= (S)1 << s;
                 } else {
                     const double f = -1.0 + 2.0 * (double)spin;
                     cr = cr * 0.5 * f;
@@ -90,8 +89,8 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                 continue;
             }
             int gm;
-            const uint64_t rep = orbit_min(tgt, gv, lds, perm_lds, gm);
-            const int64_t r2 = row_of_rep(rep, bin, bitmap, prefix);
+            const S rep = orbit_min<S>(tgt, gv, perm_lds, gm);
+            const int64_t r2 = row_of_rep<S>(rep, bin, bitmap, prefix);
             if (r2 < 0) continue;
             if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
             if (FILL) {
@@ -239,7 +238,9 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         const unsigned blocks = (unsigned)std::max<int64_t>(1, (n + kEllThreads - 1) / kEllThreads);
         const bool cplx = value_type == QMB_COMPLEX;
         if (n > 0) {
-            auto kcount = cplx ? ell_rows<false, true> : ell_rows<false, false>;
+            const bool narrow = b->n_sites <= 32;
+            auto kcount = narrow ? (cplx ? ell_rows<uint32_t, false, true> : ell_rows<uint32_t, false, false>)
+                                 : (cplx ? ell_rows<uint64_t, false, true> : ell_rows<uint64_t, false, false>);
             hipLaunchKernelGGL(kcount, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
                                ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
@@ -259,7 +260,9 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         e->col.alloc((size_t)std::max<int64_t>(e->stored, 1) * sizeof(int32_t));
         e->val.alloc((size_t)std::max<int64_t>(e->stored, 1) * (cplx ? 16 : 8));
         if (n > 0) {
-            auto kfill = cplx ? ell_rows<true, true> : ell_rows<true, false>;
+            const bool narrow = b->n_sites <= 32;
+            auto kfill = narrow ? (cplx ? ell_rows<uint32_t, true, true> : ell_rows<uint32_t, true, false>)
+                                : (cplx ? ell_rows<uint64_t, true, true> : ell_rows<uint64_t, true, false>);
             hipLaunchKernelGGL(kfill, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
                                ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),

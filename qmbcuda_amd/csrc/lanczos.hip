@@ -6,14 +6,14 @@
 //   b[0] = 0; phi_0 = start / ||start||;
 //   y = H phi_n; a[n] = Re<phi_n, y>; w = (-a[n]) phi_n + (y + (-b[n]) phi_{n-1});
 //   b[n+1] = ||w||; phi_{n+1} = w * (1/b[n+1]).
-// Per step there are two kernels and no host round trip:
+// Per step there are two streaming kernels and no host round trip:
 //   K1 spmv_sell<LZ>: y = H (w * 1/b) gathered on the fly, phi_n = w/b written
-//      out, a[n] reduced in the same pass (block partials + last-block sum);
-//   K2 lz_update: w = (-a) phi_n + (y - b phi_{n-1}), ||w||^2 reduced the same
-//      way -> b[n+1] and 1/b[n+1] stay on device.
-// The host reads a[], b[] every check_every steps and solves the tridiagonal
-// problem in FP64 (lanczos_host.cpp). Reductions are deterministic: fixed
-// block partial slots, summed in a fixed order by whichever block finishes last.
+//      out, the block partials of a[n] in the same pass;
+//   K2 lz_update: w = (-a) phi_n + (y - b phi_{n-1}) and the partials of ||w||^2;
+// each followed by a one-block finish_sum that leaves a[n], b[n+1], 1/b[n+1]
+// on the device. The host reads a[], b[] every check_every steps and solves the
+// tridiagonal problem in FP64 (lanczos_host.cpp). Reductions are deterministic:
+// fixed block partial slots summed in a fixed order.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -22,51 +22,11 @@
 #include <mutex>
 
 #include "internal.hpp"
+#include "reduce.hpp"
 
 namespace qmb {
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
 constexpr int kMaxBlocks = 4096;
-
-__device__ inline double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// Sum over the block, valid in thread 0.
-__device__ inline double block_sum(double v, double *lds) {
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) lds[wv] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < kWaves; ++i) t += lds[i];
-    __syncthreads();
-    return t;
-}
-
-// Writes the block's partial and returns true in the block that finished last.
-__device__ inline bool publish_partial(double v_thread0, double *partials, unsigned *counter) {
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = v_thread0;
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last != 0;
-}
-
-// Deterministic sum of partials[0..n) by one block (fixed order).
-__device__ inline double sum_partials(const double *partials, int n, double *lds) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += __builtin_nontemporal_load(partials + i);
-    return block_sum(s, lds);
-}
 
 // ---- K1: SELL-64 SpMV --------------------------------------------------------
 // One wavefront per 64-row chunk; slot-major storage makes every slot one
@@ -195,12 +155,27 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
     }
     if (LZ) {
         const double t = block_sum(dot, lds);
-        if (publish_partial(t, partials, counter)) {
-            const double a = sum_partials(partials, gridDim.x, lds);
-            if (threadIdx.x == 0) {
-                *alpha_out = a;
-                *counter = 0u;
-            }
+        if (threadIdx.x == 0) partials[blockIdx.x] = t;
+    }
+}
+
+// Fixed-order sum of the block partials of the previous kernel (one block):
+// mode 0 -> out = s; mode 1 -> out = sqrt(s), out_inv = 1/sqrt(s) (0 if s = 0).
+// A separate launch instead of a "last block" finish: on this chip a
+// device-scope fence in every block writes back and invalidates the L2 and
+// costs more than the SpMV's own traffic.
+__global__ __launch_bounds__(kThreads) void finish_sum(const double *__restrict__ partials, int n,
+                                                       double *__restrict__ out, double *__restrict__ out_inv,
+                                                       int mode) {
+    __shared__ double lds[kWaves];
+    const double s = sum_partials(partials, n, lds);
+    if (threadIdx.x == 0) {
+        if (mode == 0) {
+            *out = s;
+        } else {
+            const double nb = sqrt(s);
+            if (out) *out = nb;
+            if (out_inv) *out_inv = nb > 0.0 ? 1.0 / nb : 0.0;
         }
     }
 }
@@ -223,15 +198,7 @@ __global__ __launch_bounds__(kThreads) void lz_update(int64_t cnt, const double 
         acc += wi * wi;
     }
     const double tb = block_sum(acc, lds);
-    if (publish_partial(tb, partials, counter)) {
-        const double s = sum_partials(partials, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            const double nb = sqrt(s);
-            *beta_next = nb;
-            *invb_next = nb > 0.0 ? 1.0 / nb : 0.0;
-            *counter = 0u;
-        }
-    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = tb;
 }
 
 // ---- start vector, norms, eigenvector helpers -------------------------------
@@ -261,15 +228,7 @@ __global__ __launch_bounds__(kThreads) void vec_norm(int64_t cnt, const double *
         acc += d * d;
     }
     const double t = block_sum(acc, lds);
-    if (publish_partial(t, partials, counter)) {
-        const double s = sum_partials(partials, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            const double nr = sqrt(s);
-            if (out_norm) *out_norm = nr;
-            if (out_inv) *out_inv = nr > 0.0 ? 1.0 / nr : 0.0;
-            *counter = 0u;
-        }
-    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 // x = x + c * v (the reference's saxpy_functor2 accumulation of the eigenvector).
@@ -353,15 +312,7 @@ __global__ __launch_bounds__(kThreads) void reorth_apply(int64_t n, int64_t stri
         }
     }
     const double t = block_sum(acc, lds);
-    if (publish_partial(t, partials, counter)) {
-        const double s = sum_partials(partials, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            const double nb = sqrt(s);
-            *beta_next = nb;
-            *invb_next = nb > 0.0 ? 1.0 / nb : 0.0;
-            *counter = 0u;
-        }
-    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 // x = sum_j z_j V_j (eigenvector from a stored basis).
@@ -398,6 +349,11 @@ struct LzWork {
     }
 };
 
+static void launch_finish(qmb_ctx *ctx, const double *partials, unsigned n, double *out, double *out_inv, int mode) {
+    hipLaunchKernelGGL(finish_sum, dim3(1), dim3(kThreads), 0, ctx->stream, partials, (int)n, out, out_inv, mode);
+    QMB_HIP(hipGetLastError());
+}
+
 void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, bool lz, const double *invb,
                  double *vout, LzWork *wk, double *alpha_out) {
     const unsigned blocks = grid_for(e->n_chunks, kWaves);
@@ -407,15 +363,18 @@ void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, boo
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, ctx->stream, e->n_rows, e->n_chunks,
                        e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
                        e->val.as<double>(), x, y, invb, vout, wk ? wk->partials.as<double>() : nullptr,
-                       wk ? wk->counter.as<unsigned>() : nullptr, alpha_out);
+                       nullptr, alpha_out);
     QMB_HIP(hipGetLastError());
+    if (lz) launch_finish(ctx, wk->partials.as<double>(), blocks, alpha_out, nullptr, 0);
 }
 
 static void launch_norm(qmb_ctx *ctx, int64_t cnt, const double *a, const double *b, double e, LzWork &wk,
                         double *out_norm, double *out_inv) {
-    hipLaunchKernelGGL(vec_norm, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt, a, b, e,
-                       wk.partials.as<double>(), wk.counter.as<unsigned>() + 1, out_norm, out_inv);
+    const unsigned blocks = grid_for(cnt, kThreads * 4);
+    hipLaunchKernelGGL(vec_norm, dim3(blocks), dim3(kThreads), 0, ctx->stream, cnt, a, b, e,
+                       wk.partials.as<double>(), nullptr, out_norm, out_inv);
     QMB_HIP(hipGetLastError());
+    launch_finish(ctx, wk.partials.as<double>(), blocks, out_norm, out_inv, 1);
 }
 
 }  // namespace qmb
@@ -488,8 +447,9 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     auto k2 = [&](int it) {
         hipLaunchKernelGGL(lz_update, dim3(ublocks), dim3(kThreads), 0, ctx->stream, cnt, Y.as<double>(), vcur(it),
                            vprev(it), P.as<double>(), wk.alpha() + it, wk.beta() + it, wk.partials.as<double>(),
-                           wk.counter.as<unsigned>() + 2, wk.beta() + it + 1, wk.invb() + it + 1);
+                           nullptr, wk.beta() + it + 1, wk.invb() + it + 1);
         QMB_HIP(hipGetLastError());
+        launch_finish(ctx, wk.partials.as<double>(), ublocks, wk.beta() + it + 1, wk.invb() + it + 1, 1);
         if (reorth) {
             const int nv = it + 1;
             for (int pass = 0; pass < 2; ++pass) {
@@ -503,8 +463,9 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
                 auto ka = cplx ? reorth_apply<true> : reorth_apply<false>;
                 hipLaunchKernelGGL(ka, dim3(rblocks), dim3(kThreads), 0, ctx->stream, n, (int64_t)cnt, nv,
                                    VB.as<double>(), rcoef.as<double>(), P.as<double>(), wk.partials.as<double>(),
-                                   wk.counter.as<unsigned>() + 3, wk.beta() + it + 1, wk.invb() + it + 1);
+                                   nullptr, wk.beta() + it + 1, wk.invb() + it + 1);
                 QMB_HIP(hipGetLastError());
+                launch_finish(ctx, wk.partials.as<double>(), rblocks, wk.beta() + it + 1, wk.invb() + it + 1, 1);
             }
         }
     };
@@ -603,16 +564,11 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             QMB_HIP(hipMemsetAsync(Z.p, 0, cnt * sizeof(double), ctx->stream));
             init_start();
             for (int j = 0; j < m; ++j) {
-                launch_spmv(ctx, e, P.as<double>(), Y.as<double>(), true, wk.invb() + j, vcur(j), &wk,
-                            wk.alpha() + j);
+                k1(j);
                 hipLaunchKernelGGL(vec_axpy, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt,
                                    z0[j], vcur(j), X.as<double>());
                 QMB_HIP(hipGetLastError());
-                hipLaunchKernelGGL(lz_update, dim3(ublocks), dim3(kThreads), 0, ctx->stream, cnt, Y.as<double>(),
-                                   vcur(j), vprev(j), P.as<double>(), wk.alpha() + j, wk.beta() + j,
-                                   wk.partials.as<double>(), wk.counter.as<unsigned>() + 2, wk.beta() + j + 1,
-                                   wk.invb() + j + 1);
-                QMB_HIP(hipGetLastError());
+                k2(j);
             }
         }
         launch_spmv(ctx, e, X.as<double>(), Y.as<double>(), false, nullptr, nullptr, nullptr, nullptr);
@@ -791,14 +747,6 @@ qmb_status qmb_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts *o
         if (result == QMB_ERR_NOT_CONVERGED) set_error("qmb_lanczos: tolerance not met within max_iter");
     });
     return st != QMB_OK ? st : result;
-}
-
-qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const qmb_lanczos_opts *opts,
-                               double *evals, qmb_lanczos_report *reports) {
-    return guarded([&] {
-        QMB_REQUIRE(ctx && ells && opts && evals && n >= 1, "qmb_lanczos_batched: bad argument");
-        throw Error{QMB_ERR_UNSUPPORTED, "qmb_lanczos_batched: not built yet"};
-    });
 }
 
 double qmb_lanczos_eigvals_ref(double *E_tmp, int NStates, int NIter, int nobv, int max_terms, long long seed,

@@ -1,0 +1,39 @@
+// Deterministic block reductions shared by the Lanczos kernels: wave shuffles,
+// a fixed-order LDS combine and a fixed-order sum of per-block partial slots
+// (run by a separate one-block kernel; no floating-point atomics, no
+// device-scope fences inside the streaming kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace qmb {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Sum over the block, valid in thread 0.
+__device__ inline double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) lds[wv] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kWaves; ++i) t += lds[i];
+    __syncthreads();
+    return t;
+}
+
+// Deterministic sum of partials[0..n) by one block (fixed order).
+__device__ inline double sum_partials(const double *partials, int n, double *lds) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += __builtin_nontemporal_load(partials + i);
+    return block_sum(s, lds);
+}
+
+}  // namespace qmb

@@ -1,0 +1,151 @@
+// C++ facade checks, written the way the reference's own example drives its API
+// (Q/examples/examples.cu:885-913). Modes:
+//   tables <case>  print the facade's term table, group array and characters as
+//                  JSON (no GPU); tests compare them with the oracle's restatement
+//   solve <case>   run HeisenbergXXZAbelianSymms on the GPU and print per-irrep
+//                  energies as JSON
+//   ref <case>     run LanczosEigVals (the reference free function) on GPU-built
+//                  reference-layout arrays read from stdin (n W then ind/val/track)
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qmb/qmb.hpp"
+
+using namespace qmb;
+using cd = std::complex<double>;
+
+struct Case {
+    Heisenberg<cd> model;
+    SymmetryClass<cd> *group;
+    std::vector<SymmetryClass<cd> *> owned;
+};
+
+static HeisenbergInfo<cd> chain_info(double J1, double delta, double J2) {
+    HeisenbergInfo<cd> info;
+    for (int d = 1; d <= 2; ++d) {
+        const double J = d == 1 ? J1 : J2;
+        if (J == 0.0) continue;
+        info.Jxy_terms.push_back({d, 0, 0, 0});
+        info.Jxy_terms.push_back({-d, 0, 0, 0});
+        info.Jxy_couplings.push_back(cd(J / 2.0));
+        info.Jxy_couplings.push_back(cd(J / 2.0));
+        info.Jz_terms.push_back({d, 0, 0, 0});
+        info.Jz_couplings.push_back(J * delta);
+    }
+    return info;
+}
+
+static Case make_case(const std::string &name) {
+    Case c;
+    if (name == "chain16") {
+        c.model = Heisenberg<cd>(create_square_lattice_info(16, 1), chain_info(1.0, 1.0, 0.0));
+        auto *t = new Tnm_group(c.model.GetGeomInfo());
+        c.owned.push_back(t);
+        c.group = t;
+    } else if (name == "chain12_mg_flip") {
+        c.model = Heisenberg<cd>(create_square_lattice_info(12, 1), chain_info(1.0, 1.0, 0.5));
+        auto *t = new Tnm_group(c.model.GetGeomInfo());
+        auto *f = new SpinFlipGroup<cd>(*t);
+        c.owned = {t, f};
+        c.group = f;
+    } else if (name == "square2x4_example") {
+        // the reference's active example: CreateHeisenbergXXXSquare<complex>(2, 4, {1.0, 0.15})
+        c.model = CreateHeisenbergXXXSquare<cd>(2, 4, {1.0, 0.15});
+        auto *t = new Tnm_group(c.model.GetGeomInfo());
+        c.owned.push_back(t);
+        c.group = t;
+    } else if (name == "triangular3x3") {
+        c.model = CreateHeisenbergXXXTriangular<cd>(3, 3, {1.0});
+        auto *t = new Tnm_group(c.model.GetGeomInfo());
+        c.owned.push_back(t);
+        c.group = t;
+    } else if (name == "square4x4_flip") {
+        HeisenbergInfo<cd> info;
+        info.Jxy_terms = {{1, 0, 0, 0}, {-1, 0, 0, 0}, {0, 1, 0, 0}, {0, -1, 0, 0}};
+        info.Jxy_couplings = {cd(0.5), cd(0.5), cd(0.5), cd(0.5)};
+        info.Jz_terms = {{1, 0, 0, 0}, {0, 1, 0, 0}};
+        info.Jz_couplings = {1.0, 1.0};
+        c.model = Heisenberg<cd>(create_square_lattice_info(4, 4), info);
+        auto *t = new Tnm_group(c.model.GetGeomInfo());
+        auto *f = new SpinFlipGroup<cd>(*t);
+        c.owned = {t, f};
+        c.group = f;
+    } else {
+        std::fprintf(stderr, "unknown case %s\n", name.c_str());
+        std::exit(2);
+    }
+    return c;
+}
+
+static void print_tables(const Case &c) {
+    const auto ov = BuildOperatorVecs(c.model.GetH());
+    const int LS = c.model.GetLS();
+    const auto el = c.group->GetGroupElemVec();
+    std::vector<uint32_t> ge(el.size() * LS);
+    c.group->GetGroupElemArr(c.model.GetR1(), c.model.GetR2(), ge.data());
+    std::printf("{\"n_sites\": %d, \"H_size\": %d, \"H_col\": %d, \"op\": [", LS, ov.H_size, ov.H_col_size);
+    for (size_t i = 0; i < ov.H_decode_table.size(); ++i) std::printf("%s%d", i ? "," : "", ov.H_decode_table[i]);
+    std::printf("], \"site\": [");
+    for (size_t i = 0; i < ov.H_site_table.size(); ++i) std::printf("%s%d", i ? "," : "", ov.H_site_table[i]);
+    std::printf("], \"scalar\": [");
+    for (size_t i = 0; i < ov.H_scalar_table.size(); ++i)
+        std::printf("%s[%.17g,%.17g]", i ? "," : "", ov.H_scalar_table[i].real(), ov.H_scalar_table[i].imag());
+    std::printf("], \"group_el\": [");
+    for (size_t i = 0; i < ge.size(); ++i) std::printf("%s%u", i ? "," : "", ge[i]);
+    std::printf("], \"flip\": [");
+    const auto fl = c.group->GetSpinFlips();
+    for (size_t i = 0; i < fl.size(); ++i) std::printf("%s%d", i ? "," : "", fl[i]);
+    std::printf("], \"chars\": [");
+    const auto ch = c.group->GetCharTable();
+    bool first = true;
+    for (const auto &row : ch)
+        for (const auto &v : row) {
+            std::printf("%s[%.17g,%.17g]", first ? "" : ",", v.real(), v.imag());
+            first = false;
+        }
+    std::printf("]}\n");
+}
+
+static void solve(const Case &c, bool fixed) {
+    EDOptions o;
+    o.fixed_iter = fixed;
+    HeisenbergXXZAbelianSymms<cd> ham(c.model, c.group, -1, -1, 1, 2212456, fixed ? 60 : 50, 1e-5, o);
+    const auto E = ham.get_E_vecs();
+    std::printf("{\"E\": [");
+    for (size_t a = 0; a < E.size(); ++a) {
+        const double e = E[a][0];
+        if (std::isinf(e))
+            std::printf("%snull", a ? "," : "");
+        else
+            std::printf("%s%.15g", a ? "," : "", e);
+    }
+    std::printf("], \"E_gs\": %.15g, \"GS_irrep\": %d, \"krylov_len\": %zu}\n", ham.GetGroundStateEnergy(),
+                ham.GetGroundStateIrrep(), ham.GetKrylovGS().size());
+}
+
+int main(int argc, char **argv) {
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s tables|solve|solve_fixed <case>\n", argv[0]);
+        return 2;
+    }
+    const std::string mode = argv[1];
+    Case c = make_case(argv[2]);
+    try {
+        if (mode == "tables")
+            print_tables(c);
+        else if (mode == "solve")
+            solve(c, false);
+        else if (mode == "solve_fixed")
+            solve(c, true);
+        else
+            return 2;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    for (auto *g : c.owned) delete g;
+    return 0;
+}

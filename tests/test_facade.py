@@ -1,0 +1,108 @@
+"""The C++ facade (include/qmb/*.hpp), driven like the reference's own example.
+
+CPU part: the facade's term tables, group arrays and characters equal the
+oracle's restatement of the reference's model and symmetry code.
+GPU part: HeisenbergXXZAbelianSymms solves every irrep; energies match dense
+diagonalisation of the oracle's sector matrices.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import model as OM
+from tests.helpers import dense_spectrum, ell_dense_nonnull, oracle_ell, oracle_orbits
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "test_facade")
+
+
+def run(mode, case):
+    if not os.path.exists(EXE):
+        pytest.fail("tests/cpp/test_facade is not built (run __graft_entry__.build())")
+    out = subprocess.run([EXE, mode, case], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    return json.loads(out.stdout)
+
+
+def oracle_case(name):
+    if name == "chain16":
+        geom, info = OM.chain_xxz_info(16, 1.0, 1.0, 0.0)
+        flip = False
+    elif name == "chain12_mg_flip":
+        geom, info = OM.chain_xxz_info(12, 1.0, 1.0, 0.5)
+        flip = True
+    elif name == "square2x4_example":
+        geom = OM.square_geometry(2, 4)
+        info = OM.extended_xxx_info(geom, [1.0, 0.15])
+        flip = False
+    elif name == "triangular3x3":
+        geom = OM.triangular_geometry(3, 3)
+        info = OM.extended_xxx_info(geom, [1.0])
+        flip = False
+    elif name == "square4x4_flip":
+        geom, info = OM.square_xxz_info(4, 4, 1.0, 1.0)
+        flip = True
+    terms = OM.heisenberg_operator(geom, info)
+    table = OM.conjugated_term_table(terms)
+    ge, ch = OM.translation_group(geom)
+    fl = None
+    if flip:
+        ge, fl, ch = OM.with_spin_flip(ge, ch)
+    return dict(n_sites=geom.n_sites, terms=terms, table=table, group_el=ge, chars=ch, flip=fl)
+
+
+CASES = ["chain16", "chain12_mg_flip", "square2x4_example", "triangular3x3", "square4x4_flip"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_facade_tables_match_oracle(name):
+    got = run("tables", name)
+    ref = oracle_case(name)
+    op, site, sc, nops, hcol = ref["table"]
+    assert got["H_size"] == len(nops) and got["H_col"] == hcol
+    assert got["op"] == op.tolist()
+    assert got["site"] == site.tolist()
+    np.testing.assert_array_equal(np.array(got["scalar"]).reshape(-1, 2), np.c_[sc.real, sc.imag])
+    G = ref["group_el"].shape[0]
+    assert np.array_equal(np.array(got["group_el"]).reshape(G, -1), ref["group_el"])
+    if ref["flip"] is not None:
+        assert got["flip"] == ref["flip"].tolist()
+    else:
+        assert not any(got["flip"])
+    chars = np.array(got["chars"]).reshape(ref["chars"].shape[0], G, 2)
+    np.testing.assert_array_equal(chars[..., 0] + 1j * chars[..., 1], ref["chars"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,n_up", [("chain16", 8), ("chain12_mg_flip", 6), ("square2x4_example", 4),
+                                       ("triangular3x3", 4)])
+def test_facade_driver_solves_every_irrep(name, n_up):
+    print(f"[gpu-case] facade solve {name}", flush=True)
+    got = run("solve", name)
+    ref = oracle_case(name)
+    orb = oracle_orbits(ref, n_up)
+    E = got["E"]
+    assert len(E) == orb.n_irreps
+    lowest = []
+    for a in range(orb.n_irreps):
+        ell = oracle_ell(ref, orb, a)
+        M = ell_dense_nonnull(ell, orb, a)
+        if M.shape[0] == 0:
+            assert E[a] is None
+            continue
+        e0 = np.linalg.eigvalsh(M)[0]
+        lowest.append(e0)
+        assert abs(E[a] - e0) < 1e-10 * max(1, abs(e0))
+    dense = dense_spectrum(ref, n_up)
+    assert abs(got["E_gs"] - dense[0]) < 1e-10 * max(1, abs(dense[0]))
+
+
+@pytest.mark.gpu
+def test_facade_fixed_iteration_mode():
+    print("[gpu-case] facade solve_fixed chain16", flush=True)
+    got = run("solve_fixed", "chain16")
+    assert abs(got["E_gs"] - (-7.142296360617)) < 1e-8
+    assert got["krylov_len"] == 60
