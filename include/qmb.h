@@ -165,7 +165,8 @@ qmb_status qmb_spmv_benchmark(qmb_ctx *ctx, const qmb_ell *e, int reps, double *
  * LanczosEigenStateFromSeed (:31-104) with an FP64 device loop: the same
  * three-term recurrence, a convergence test on the Ritz residual, optional
  * full reorthogonalisation (needed for n_eigs > 1), host FP64 tridiagonal solve. */
-enum { QMB_START_RANDOM = 0, QMB_START_USER = 1 };
+/* start vector: seeded random (device generated), user host array, user device array */
+enum { QMB_START_RANDOM = 0, QMB_START_USER = 1, QMB_START_DEVICE = 2 };
 
 typedef struct {
     int32_t max_iter;    /* upper bound on iterations */
@@ -173,12 +174,13 @@ typedef struct {
     int32_t check_every; /* iterations between convergence checks (0 -> 10) */
     int32_t reorth;      /* 0 plain recurrence, 1 full reorthogonalisation */
     int32_t fixed_iter;  /* 1: run exactly max_iter steps, no test (reference semantics) */
-    int32_t start;       /* QMB_START_RANDOM or QMB_START_USER */
+    int32_t start;       /* QMB_START_RANDOM, QMB_START_USER or QMB_START_DEVICE */
     int32_t want_eigvec; /* 1: rebuild the ground-state vector (second pass) */
     int32_t profile;     /* 1: time every SpMV with HIP events */
     double tol;          /* eigenvalue tolerance, relative to max(1,|E|) (0 -> 1e-13) */
     uint64_t seed;       /* start-vector seed (reference default 2212456) */
-    const double *start_vec; /* host start vector when start == QMB_START_USER */
+    const double *start_vec; /* start vector: host (USER) or device (DEVICE) pointer, n_rows
+                              * values (x2 interleaved re/im for a complex ELL) */
 } qmb_lanczos_opts;
 
 typedef struct {
@@ -224,6 +226,43 @@ double qmb_lanczos_eigvals_ref(double *E_tmp, int NStates, int NIter, int nobv, 
 qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
                                const qmb_lanczos_opts *opts, double *evals,
                                qmb_lanczos_report *reports);
+
+/* ---- learned start vector (new; no reference counterpart) -------------------
+ * A translation-invariant network over 8-spin windows (+-1 spins, Delta, J2/J1
+ * -> tanh 128 -> tanh 128 -> log-amplitude, phase), log psi(s) = sum over the N
+ * windows of an N-site ring. Bf16 weights and activations on MFMA, FP32
+ * accumulation, FP32 Adam master weights. Parameter layout: W1[128][10], b1[128],
+ * W2[128][128], b2[128], W3[2][128], b3[2]. Samples are k=0 ring bases (compact
+ * layout, plain translation group) with their ED ground states; the loss is the
+ * mean infidelity 1 - |<u, c>|^2/(<c,c><u,u>), c_r = psi(r) sqrt(G/m_r). */
+typedef struct qmb_mlp qmb_mlp;
+
+int qmb_mlp_num_params(void);
+qmb_status qmb_mlp_create(qmb_ctx *ctx, uint64_t seed, qmb_mlp **out);
+qmb_status qmb_mlp_destroy(qmb_mlp *m);
+qmb_status qmb_mlp_get_params(const qmb_mlp *m, float *theta);
+qmb_status qmb_mlp_set_params(qmb_mlp *m, const float *theta);
+/* Network outputs for all 256 window patterns of n couplings: out[(i*256 + p)*2 + {a, phi}]. */
+qmb_status qmb_mlp_forward(qmb_mlp *m, int n, const double *delta, const double *j2, float *out);
+/* The basis must outlive the network's use of the sample. */
+qmb_status qmb_mlp_add_sample(qmb_mlp *m, const qmb_basis *b, const double *ground_state, double delta, double j2);
+qmb_status qmb_mlp_clear_samples(qmb_mlp *m);
+/* Mean loss over the samples and (grad != NULL) its gradient, host FP32 [num_params]. */
+qmb_status qmb_mlp_loss_grad(qmb_mlp *m, double *loss, float *grad);
+/* Full-batch Adam steps; loss_history[steps] optional (loss before each step). */
+qmb_status qmb_mlp_train(qmb_mlp *m, int steps, double lr, double *loss_history);
+/* Real start vector x_r = |c_r| cos(beta_r - beta_ref) on a k=0 ring basis
+ * (host, n_rows). *fallback = 1 when it is not finite or zero: use the random start. */
+qmb_status qmb_mlp_start_vector(qmb_mlp *m, const qmb_basis *b, double delta, double j2, double *x,
+                                int *fallback);
+/* Lanczos on e (built on basis b) from the learned start vector, which stays on
+ * the device: the network, the start vector and the Lanczos run share one
+ * stream and only ||x||^2 is read back. opts may be NULL (defaults); its start
+ * fields are overridden. *fallback (optional) = 1 when the learned vector was
+ * unusable and the seeded random start was used instead. eigvec as in qmb_lanczos. */
+qmb_status qmb_lanczos_learned(qmb_mlp *m, const qmb_basis *b, const qmb_ell *e, double delta, double j2,
+                               const qmb_lanczos_opts *opts, double *evals, double *eigvec,
+                               qmb_lanczos_report *report, int *fallback);
 
 #ifdef __cplusplus
 }

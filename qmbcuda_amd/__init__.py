@@ -251,6 +251,82 @@ def lanczos_batched(ells, n_eigs=1, max_iter=300, tol=1e-13, seed=2212456, reort
     return ev.reshape(len(ells), n_eigs), list(reps)
 
 
+class Mlp:
+    """Learned start vector network (see include/qmb.h)."""
+
+    def __init__(self, ctx: Context, seed: int = 20261202):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().qmb_mlp_create(ctx.h, seed, C.byref(h)))
+        self.h = h
+        self.n_params = int(lib().qmb_mlp_num_params())
+        self._samples = []
+
+    def params(self):
+        th = np.zeros(self.n_params, np.float32)
+        _check(lib().qmb_mlp_get_params(self.h, _ptr(th)))
+        return th
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        assert len(th) == self.n_params
+        _check(lib().qmb_mlp_set_params(self.h, _ptr(th)))
+
+    def forward(self, deltas, j2s):
+        d = np.ascontiguousarray(deltas, np.float64)
+        j = np.ascontiguousarray(j2s, np.float64)
+        out = np.zeros((len(d), 256, 2), np.float32)
+        _check(lib().qmb_mlp_forward(self.h, len(d), _ptr(d), _ptr(j), _ptr(out)))
+        return out
+
+    def add_sample(self, basis: Basis, ground_state, delta: float, j2: float):
+        u = np.ascontiguousarray(ground_state, np.float64)
+        _check(lib().qmb_mlp_add_sample(self.h, basis.h, _ptr(u), delta, j2))
+        self._samples.append(basis)  # keep the basis alive
+
+    def clear_samples(self):
+        _check(lib().qmb_mlp_clear_samples(self.h))
+        self._samples = []
+
+    def loss_grad(self, want_grad=True):
+        L = C.c_double(0.0)
+        g = np.zeros(self.n_params, np.float32) if want_grad else None
+        _check(lib().qmb_mlp_loss_grad(self.h, C.byref(L), _ptr(g)))
+        return L.value, g
+
+    def train(self, steps: int, lr: float):
+        hist = np.zeros(steps, np.float64)
+        _check(lib().qmb_mlp_train(self.h, steps, lr, _ptr(hist)))
+        return hist
+
+    def start_vector(self, basis: Basis, delta: float, j2: float):
+        x = np.zeros(basis.n_rows, np.float64)
+        fb = C.c_int(0)
+        _check(lib().qmb_mlp_start_vector(self.h, basis.h, delta, j2, _ptr(x), C.byref(fb)))
+        return x, bool(fb.value)
+
+    def lanczos(self, basis: Basis, ell: "Ell", delta: float, j2: float, n_eigs=1, max_iter=300, tol=1e-13,
+                seed=2212456, check_every=10, want_eigvec=False, profile=False):
+        """Lanczos from the learned start vector kept on the device
+        (qmb_lanczos_learned). Returns (evals, report, fallback, eigvec)."""
+        o = default_opts(max_iter=max_iter, n_eigs=n_eigs, tol=tol, seed=seed, reorth=int(n_eigs > 1),
+                         check_every=check_every, want_eigvec=int(want_eigvec), profile=int(profile))
+        ev = np.zeros(n_eigs, np.float64)
+        vec = np.zeros(basis.n_rows, np.float64) if want_eigvec else None
+        rep = LanczosReport()
+        fb = C.c_int(0)
+        _check(lib().qmb_lanczos_learned(self.h, basis.h, ell.h, delta, j2, C.byref(o), _ptr(ev), _ptr(vec),
+                                         C.byref(rep), C.byref(fb)))
+        return ev, rep, bool(fb.value), vec
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().qmb_mlp_destroy(self.h)
+        except Exception:
+            pass
+
+
 def eigvals_ref(E_out, NIter, nobv, max_terms, seed, ind, val, track, is_complex, check=True):
     """The reference-signature drop-in (qmb_lanczos_eigvals_ref) on host arrays."""
     E = C.c_double(0.0)

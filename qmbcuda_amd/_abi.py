@@ -113,6 +113,19 @@ SIGNATURES = [
      [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
       C.POINTER(C.c_int)]),
     ("qmb_lanczos_batched", C.c_int, [_vp, C.c_int, _vp, C.POINTER(LanczosOpts), _vp, _vp]),
+    ("qmb_mlp_num_params", C.c_int, []),
+    ("qmb_mlp_create", C.c_int, [_vp, C.c_uint64, _pp]),
+    ("qmb_mlp_destroy", C.c_int, [_vp]),
+    ("qmb_mlp_get_params", C.c_int, [_vp, _vp]),
+    ("qmb_mlp_set_params", C.c_int, [_vp, _vp]),
+    ("qmb_mlp_forward", C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
+    ("qmb_mlp_add_sample", C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double]),
+    ("qmb_mlp_clear_samples", C.c_int, [_vp]),
+    ("qmb_mlp_loss_grad", C.c_int, [_vp, C.POINTER(C.c_double), _vp]),
+    ("qmb_mlp_train", C.c_int, [_vp, C.c_int, C.c_double, _vp]),
+    ("qmb_mlp_start_vector", C.c_int, [_vp, _vp, C.c_double, C.c_double, _vp, C.POINTER(C.c_int)]),
+    ("qmb_lanczos_learned", C.c_int,
+     [_vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp, C.POINTER(C.c_int)]),
 ]
 
 _LIB = None

@@ -51,8 +51,10 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
                                                             uint64_t *bitmap) {
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
+    __shared__ GroupLds gt;
     stage_perm(perm_lds, gv);
     stage_binom(bin, binom);
+    stage_group(gt, gv);
     __syncthreads();
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n_words) return;
@@ -60,12 +62,38 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
     const int cnt = (int)((dim - r0) < 64 ? (dim - r0) : 64);
     S x = colex_unrank<S>(r0, n_up, gv.n_sites, bin);
     uint64_t bits = 0;
-    for (int i = 0; i < cnt; ++i) {
-        uint64_t slo, shi;
-        if (is_representative<S>(x, gv, perm_lds, slo, shi)) {
-            if (!compact || compatible(slo, shi, triv_lo, triv_hi)) bits |= 1ULL << i;
+    // One flat loop over (state, element) steps per lane: a lane moves to its
+    // next state as soon as an image is smaller (most states exit after a few
+    // images), so lanes never wait for a neighbour's full orbit scan.
+    int i = 0, g = 1;
+    uint64_t slo = 1, shi = 0;  // element 0 (identity) always fixes x
+    while (i < cnt) {
+        bool done = false, rep = false;
+        if (g >= gv.G) {
+            done = true;
+            rep = true;
+        } else {
+            const S y = apply_elem_div<S>(x, g, gv, gt, perm_lds);
+            if (y < x) {
+                done = true;
+            } else {
+                if (y == x) {
+                    if (g < 64)
+                        slo |= 1ULL << g;
+                    else
+                        shi |= 1ULL << (g - 64);
+                }
+                ++g;
+            }
         }
-        if (i + 1 < cnt) x = next_comb<S>(x);
+        if (done) {
+            if (rep && (!compact || compatible(slo, shi, triv_lo, triv_hi))) bits |= 1ULL << i;
+            ++i;
+            if (i < cnt) x = next_comb<S>(x);
+            g = 1;
+            slo = 1;
+            shi = 0;
+        }
     }
     bitmap[w] = bits;
 }
@@ -87,18 +115,17 @@ __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, con
     if (!bits) return;
     int64_t pos = prefix[w];
     const int64_t r0 = w * 64;
-    const int last = 63 - __builtin_clzll(bits);
-    S x = colex_unrank<S>(r0, n_up, gv.n_sites, bin);
-    for (int i = 0; i <= last; ++i) {
-        if ((bits >> i) & 1ULL) {
-            uint64_t slo, shi;
-            is_representative<S>(x, gv, perm_lds, slo, shi);
-            reps[pos] = (uint64_t)x;
-            stab[pos] = (uint8_t)(__popcll(slo) + __popcll(shi));
-            nullf[pos] = compatible(slo, shi, triv_lo, triv_hi) ? 0 : 1;
-            ++pos;
-        }
-        if (i < last) x = next_comb<S>(x);
+    // representatives are sparse among the ranks: unrank each one directly
+    while (bits) {
+        const int i = __builtin_ctzll(bits);
+        bits &= bits - 1;
+        const S x = colex_unrank<S>(r0 + i, n_up, gv.n_sites, bin);
+        uint64_t slo, shi;
+        is_representative<S>(x, gv, perm_lds, slo, shi);
+        reps[pos] = (uint64_t)x;
+        stab[pos] = (uint8_t)(__popcll(slo) + __popcll(shi));
+        nullf[pos] = compatible(slo, shi, triv_lo, triv_hi) ? 0 : 1;
+        ++pos;
     }
 }
 

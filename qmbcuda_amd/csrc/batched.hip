@@ -368,7 +368,7 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             I.counter = counters.as<unsigned>() + 4 * pos;
         }
         // vector task tables (ranges in doubles, both families in one table per family)
-        int total_vec_blocks = 0, total_spmv_blocks = 0;
+        int total_vec_blocks = 0;
         for (auto &f : fam) {
             f.vec_first_block = total_vec_blocks;
             for (size_t q = 0; q < f.members.size(); ++q) {

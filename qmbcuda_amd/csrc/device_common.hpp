@@ -67,6 +67,51 @@ __device__ inline S apply_elem(S x, int g, const GroupView &gv, const uint8_t *p
     return y;
 }
 
+// LDS copy of the per-element tables for lane-divergent element indices.
+struct GroupLds {
+    uint64_t keep[kMaxGroup];
+    uint8_t small[kMaxGroup];
+    uint8_t big[kMaxGroup];
+    uint8_t flip[kMaxGroup];
+};
+
+__device__ inline void stage_group(GroupLds &t, const GroupView &g) {
+    for (int i = threadIdx.x; i < g.G; i += blockDim.x) {
+        t.keep[i] = g.keep[i];
+        t.small[i] = g.small[i];
+        t.big[i] = g.big[i];
+        t.flip[i] = g.flip[i];
+    }
+}
+
+// apply_elem with the element index varying across lanes (tables from LDS).
+template <class S>
+__device__ inline S apply_elem_div(S x, int g, const GroupView &gv, const GroupLds &t, const uint8_t *perm_lds) {
+    const S full = (S)gv.full;
+    S y;
+    if (gv.torus) {
+        y = x;
+        const unsigned s = t.small[g];
+        if (s) {
+            const S k = (S)t.keep[g];
+            y = (S)(((y & k) << s) | ((y & (S)~k & full) >> (gv.block - s)));
+        }
+        const unsigned b = t.big[g];
+        if (b) y = (S)(((y << b) | (y >> (gv.n_sites - b))) & full);
+    } else {
+        y = 0;
+        const uint8_t *p = perm_lds + g * gv.n_sites;
+        S u = x;
+        while (u) {
+            const int bit = ctz_s(u);
+            y |= (S)1 << p[bit];
+            u &= u - 1;
+        }
+    }
+    if (t.flip[g]) y ^= full;
+    return y;
+}
+
 // Colex rank: sum over set bits p_0 < p_1 < ... of C(p_j, j+1); equals the
 // reference's GetIndexInHilbertSubspace (utils.cu:9-36).
 template <class S>

@@ -28,6 +28,34 @@ struct TermsView {
 
 constexpr int kEllThreads = 256;
 
+// Conjugated operators of term h applied left to right to src (the reference's
+// inner loop, utils.cu:133-161). Returns false when the term annihilates src.
+template <class S>
+__device__ inline bool apply_term(const TermsView &tv, int h, S src, S &tgt, double &cr, double &ci) {
+    tgt = src;
+    const double2 c0 = tv.coef[h];
+    cr = c0.x;
+    ci = c0.y;
+    const int nk = tv.n_ops[h];
+    for (int k = 0; k < nk; ++k) {
+        const int s = tv.site[h * tv.max_ops + k];
+        const int op = tv.op[h * tv.max_ops + k];
+        const int spin = (int)((tgt >> s) & (S)1);
+        if (op == QMB_OP_SM) {
+            if (!spin) return false;
+            tgt &= (S)~((S)1 << s);
+        } else if (op == QMB_OP_SP) {
+            if (spin) return false;
+            tgt |= (S)1 << s;
+        } else {
+            const double f = -1.0 + 2.0 * (double)spin;
+            cr = cr * 0.5 * f;
+            ci = ci * 0.5 * f;
+        }
+    }
+    return true;
+}
+
 template <class S, bool FILL, bool CPLX>
 __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
                                                         int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
@@ -60,56 +88,54 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
     if (!null_row) {
         const S src = (S)reps[r];
         const double n_src = s_nrm[stab[r]];
-        for (int h = 0; h < tv.n_terms; ++h) {
-            S tgt = src;
-            const double2 c0 = tv.coef[h];
-            double cr = c0.x, ci = c0.y;
-            bool dead = false;
-            const int nk = tv.n_ops[h];
-            for (int k = 0; k < nk; ++k) {
-                const int s = tv.site[h * tv.max_ops + k];
-                const int op = tv.op[h * tv.max_ops + k];
-                const int spin = (int)((tgt >> s) & (S)1);
-                if (op == QMB_OP_SM) {
-                    if (!spin) { dead = true; break; }
-                    tgt &= (S)~((S)1 << s);
-                } else if (op == QMB_OP_SP) {
-                    if (spin) { dead = true; break; }
-                    tgt |= (S)1 << s;
-                } else {
-                    const double f = -1.0 + 2.0 * (double)spin;
-                    cr = cr * 0.5 * f;
-                    ci = ci * 0.5 * f;
+        // Terms are taken 64 at a time. Pass A applies the operators (cheap bit
+        // tests, wave-uniform term index), adds diagonal terms in term order and
+        // marks the lane's off-diagonal candidates; pass B pops each lane's
+        // candidates in term order, so every iteration of the expensive orbit
+        // lookup keeps almost all lanes busy.
+        for (int h0 = 0; h0 < tv.n_terms; h0 += 64) {
+            const int h1 = min(h0 + 64, tv.n_terms);
+            uint64_t cand = 0;
+            for (int h = h0; h < h1; ++h) {
+                double cr, ci;
+                S tgt;
+                if (!apply_term<S>(tv, h, src, tgt, cr, ci)) continue;
+                if (tgt == src) {
+                    dr += cr;
+                    di += ci;
+                    continue;
                 }
+                cand |= 1ULL << (h - h0);
             }
-            if (dead) continue;
-            if (tgt == src) {
-                dr += cr;
-                di += ci;
-                continue;
-            }
-            int gm;
-            const S rep = orbit_min<S>(tgt, gv, perm_lds, gm);
-            const int64_t r2 = row_of_rep<S>(rep, bin, bitmap, prefix);
-            if (r2 < 0) continue;
-            if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
-            if (FILL) {
-                if (slot < L) {
-                    const double ratio = s_nrm[stab[r2]] / n_src;
-                    const double2 x = s_chi[gm];
-                    const double pr = cr * x.x - ci * x.y;
-                    const double pi = cr * x.y + ci * x.x;
-                    const int64_t pos = base + (int64_t)slot * kChunk + lane;
-                    col[pos] = (int32_t)r2;
-                    if (CPLX) {
-                        val[2 * pos] = pr * ratio;
-                        val[2 * pos + 1] = pi * ratio;
-                    } else {
-                        val[pos] = pr * ratio;
+            while (cand) {
+                const int h = h0 + __builtin_ctzll(cand);
+                cand &= cand - 1;
+                double cr, ci;
+                S tgt;
+                apply_term<S>(tv, h, src, tgt, cr, ci);
+                int gm;
+                const S rep = orbit_min<S>(tgt, gv, perm_lds, gm);
+                const int64_t r2 = row_of_rep<S>(rep, bin, bitmap, prefix);
+                if (r2 < 0) continue;
+                if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
+                if (FILL) {
+                    if (slot < L) {
+                        const double ratio = s_nrm[stab[r2]] / n_src;
+                        const double2 x = s_chi[gm];
+                        const double pr = cr * x.x - ci * x.y;
+                        const double pi = cr * x.y + ci * x.x;
+                        const int64_t pos = base + (int64_t)slot * kChunk + lane;
+                        col[pos] = (int32_t)r2;
+                        if (CPLX) {
+                            val[2 * pos] = pr * ratio;
+                            val[2 * pos + 1] = pi * ratio;
+                        } else {
+                            val[pos] = pr * ratio;
+                        }
                     }
                 }
+                ++slot;
             }
-            ++slot;
         }
     }
     const int len = null_row ? 0 : slot;

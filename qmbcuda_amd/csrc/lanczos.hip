@@ -433,6 +433,9 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         if (o.start == QMB_START_USER) {
             QMB_REQUIRE(o.start_vec != nullptr, "qmb_lanczos: start == USER needs start_vec");
             QMB_HIP(hipMemcpyAsync(P.p, o.start_vec, cnt * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        } else if (o.start == QMB_START_DEVICE) {
+            QMB_REQUIRE(o.start_vec != nullptr, "qmb_lanczos: start == DEVICE needs start_vec");
+            QMB_HIP(hipMemcpyAsync(P.p, o.start_vec, cnt * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         } else {
             hipLaunchKernelGGL(lz_random, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt,
                                o.seed, P.as<double>());
@@ -665,7 +668,8 @@ void check_opts(const qmb_lanczos_opts &o) {
     QMB_REQUIRE(o.max_iter >= 1 && o.max_iter <= 100000, "qmb_lanczos: max_iter must be in [1, 100000]");
     QMB_REQUIRE(o.n_eigs >= 1 && o.n_eigs <= 8, "qmb_lanczos: n_eigs must be in [1, 8]");
     QMB_REQUIRE(o.n_eigs == 1 || o.reorth, "qmb_lanczos: n_eigs > 1 needs reorth = 1");
-    QMB_REQUIRE(o.start == QMB_START_RANDOM || o.start == QMB_START_USER, "qmb_lanczos: bad start mode");
+    QMB_REQUIRE(o.start == QMB_START_RANDOM || o.start == QMB_START_USER || o.start == QMB_START_DEVICE,
+                "qmb_lanczos: bad start mode");
     QMB_REQUIRE(!(o.profile && o.reorth), "qmb_lanczos: profile with reorth is not supported");
 }
 

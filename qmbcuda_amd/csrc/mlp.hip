@@ -1,0 +1,864 @@
+// Learned Lanczos start vector (no reference counterpart).
+//
+// Network: input = 8 spins of a window (+-1) + Delta + J2/J1 (10 values);
+// tanh(128) -> tanh(128) -> (log-amplitude a, phase phi). For an N-site ring
+// state s: log psi(s) = sum_t a(w_t) + i phi(w_t), w_t = spins t..t+7 (mod N),
+// translation invariant by construction. The k=0 coefficient of representative
+// r is c_r = psi(r) sqrt(G/m_r).
+//
+// Because a window takes only 256 values, the network is evaluated once per
+// (pattern, coupling) row; states then cost N table lookups. Training
+// back-propagates per-pattern gradient histograms. The 128x128 layer - the
+// bulk of the arithmetic - runs on gfx950 MFMA (v_mfma_f32_32x32x16_bf16:
+// bf16 weights and activations, FP32 accumulation) in the forward pass, the
+// activation backward pass (dH1 = dA2 W2) and the weight gradient
+// (dW2 = dA2^T H1); the 10-wide input layer and the 2-wide head use FP32 VALU.
+// Weight-gradient partial sums, the pattern histograms (fixed-point int64) and
+// every reduction are deterministic.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "internal.hpp"
+#include "reduce.hpp"
+
+namespace qmb {
+
+constexpr int kIn = 10, kH = 128, kOut = 2, kWin = 8;
+constexpr int oW1 = 0, ob1 = oW1 + kH * kIn, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, oW3 = ob2 + kH,
+              ob3 = oW3 + kOut * kH, kParams = ob3 + kOut;
+constexpr double kFix = 68719476736.0;  // 2^36 fixed-point scale of the pattern histograms
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline uint16_t to_bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float from_bf16_bits(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline bf16x8 load8(const uint16_t *p) {
+    bf16x8 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+
+// C/D layout of the 32x32 MFMA tile held by lane l: element v sits at
+// row 8*(v/4) + 4*(l/32) + (v%4), column l%32. A fragment: row l%32, k = 8*(l/32)..+7;
+// B fragment: column l%32, k = 8*(l/32)..+7 (16 k per instruction).
+__device__ inline int mfma_row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+
+// ---- forward: one wavefront per 32 rows --------------------------------------
+// Row r = c*256 + p: pattern p, coupling c. Writes out[r][2]; with store, also
+// H1 (FP32 and bf16) and H2 (FP32) for the backward pass.
+constexpr int kTileLd = kH + 8;  // bf16 row stride of the LDS activation tile
+
+__global__ __launch_bounds__(64) void mlp_forward(const float *__restrict__ theta, const uint16_t *__restrict__ w2bf,
+                                                  const double *__restrict__ coup, int n_rows,
+                                                  float *__restrict__ out, float *__restrict__ H1,
+                                                  uint16_t *__restrict__ H1q, float *__restrict__ H2, int store) {
+    __shared__ float x[32][kIn];
+    __shared__ uint16_t a_tile[32 * kTileLd];
+    __shared__ float h2[32][kH + 1];
+    const int lane = threadIdx.x;
+    const int r0 = blockIdx.x * 32;
+    if (lane < 32) {
+        const int r = r0 + lane;
+        const int p = r & 255;
+        const int c = r >> 8;
+        for (int i = 0; i < kWin; ++i) x[lane][i] = ((p >> i) & 1) ? 1.f : -1.f;
+        x[lane][8] = r < n_rows ? (float)coup[2 * c] : 0.f;
+        x[lane][9] = r < n_rows ? (float)coup[2 * c + 1] : 0.f;
+    }
+    __syncthreads();
+    // layer 1 (VALU): lane owns hidden units 2*lane, 2*lane+1
+    for (int u = 0; u < 2; ++u) {
+        const int k = 2 * lane + u;
+        float w[kIn];
+        for (int i = 0; i < kIn; ++i) w[i] = theta[oW1 + k * kIn + i];
+        const float b = theta[ob1 + k];
+        for (int row = 0; row < 32; ++row) {
+            float s = b;
+            for (int i = 0; i < kIn; ++i) s += w[i] * x[row][i];
+            const float h = tanhf(s);
+            const uint16_t q = to_bf16_bits(h);
+            a_tile[row * kTileLd + k] = q;
+            const int r = r0 + row;
+            if (store && r < n_rows) {
+                H1[(size_t)r * kH + k] = h;
+                H1q[(size_t)r * kH + k] = q;
+            }
+        }
+    }
+    __syncthreads();
+    // layer 2 (MFMA): H2 = tanh(H1q W2^T + b2)
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x16 acc = {};
+        for (int kb = 0; kb < kH / 16; ++kb) {
+            const bf16x8 a = load8(a_tile + (lane & 31) * kTileLd + kb * 16 + 8 * (lane >> 5));
+            const bf16x8 b = load8(w2bf + (size_t)(nt * 32 + (lane & 31)) * kH + kb * 16 + 8 * (lane >> 5));
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        const int col = nt * 32 + (lane & 31);
+        const float b2 = theta[ob2 + col];
+        for (int v = 0; v < 16; ++v) h2[mfma_row(v, lane)][col] = tanhf(acc[v] + b2);
+    }
+    __syncthreads();
+    // head (VALU): lane -> (row = lane%32, output = lane/32)
+    {
+        const int row = lane & 31, o = lane >> 5;
+        float s = theta[ob3 + o];
+        for (int k = 0; k < kH; ++k) s += theta[oW3 + o * kH + k] * h2[row][k];
+        const int r = r0 + row;
+        if (r < n_rows) out[(size_t)r * 2 + o] = s;
+    }
+    if (store) {
+        for (int idx = lane; idx < 32 * kH; idx += 64) {
+            const int row = idx / kH, k = idx % kH;
+            const int r = r0 + row;
+            if (r < n_rows) H2[(size_t)r * kH + k] = h2[row][k];
+        }
+    }
+}
+
+// ---- backward, activations: dA2 = (dout W3) (1 - H2^2); dA1 = (dA2 W2) (1 - H1^2)
+// dA1 uses MFMA with B = W2 read column-wise through its transpose w2t[j][n].
+__global__ __launch_bounds__(64) void mlp_backward_act(const float *__restrict__ theta,
+                                                       const uint16_t *__restrict__ w2t, int n_rows,
+                                                       const float *__restrict__ dout, const float *__restrict__ H1,
+                                                       const float *__restrict__ H2, float *__restrict__ dA2,
+                                                       uint16_t *__restrict__ dA2q, float *__restrict__ dA1) {
+    __shared__ uint16_t a_tile[32 * kTileLd];
+    const int lane = threadIdx.x;
+    const int r0 = blockIdx.x * 32;
+    for (int idx = lane; idx < 32 * kH; idx += 64) {
+        const int row = idx / kH, k = idx % kH;
+        const int r = r0 + row;
+        float g = 0.f;
+        if (r < n_rows) {
+            const float h = H2[(size_t)r * kH + k];
+            g = (dout[(size_t)r * 2] * theta[oW3 + k] + dout[(size_t)r * 2 + 1] * theta[oW3 + kH + k]) * (1.f - h * h);
+            dA2[(size_t)r * kH + k] = g;
+        }
+        const uint16_t q = to_bf16_bits(g);
+        a_tile[row * kTileLd + k] = q;
+        if (r < n_rows) dA2q[(size_t)r * kH + k] = q;
+    }
+    __syncthreads();
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x16 acc = {};
+        for (int kb = 0; kb < kH / 16; ++kb) {
+            const bf16x8 a = load8(a_tile + (lane & 31) * kTileLd + kb * 16 + 8 * (lane >> 5));
+            const bf16x8 b = load8(w2t + (size_t)(nt * 32 + (lane & 31)) * kH + kb * 16 + 8 * (lane >> 5));
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        const int col = nt * 32 + (lane & 31);
+        for (int v = 0; v < 16; ++v) {
+            const int r = r0 + mfma_row(v, lane);
+            if (r < n_rows) {
+                const float h = H1[(size_t)r * kH + col];
+                dA1[(size_t)r * kH + col] = acc[v] * (1.f - h * h);
+            }
+        }
+    }
+}
+
+// ---- backward, dW2 = dA2^T H1q over a group of rows (MFMA, K = rows) ---------
+// grid (groups, 16): blockIdx.y = output tile (mt, nt); one partial tile per group.
+constexpr int kRowsPerGroup = 1024;
+
+__global__ __launch_bounds__(64) void mlp_wgrad2(int n_rows, const uint16_t *__restrict__ dA2q,
+                                                 const uint16_t *__restrict__ H1q, float *__restrict__ part) {
+    __shared__ uint16_t at[32][40];  // dA2^T tile: [m][row]
+    __shared__ uint16_t bt[32][40];  // H1q^T tile: [n][row]
+    const int lane = threadIdx.x;
+    const int mt = blockIdx.y >> 2, nt = blockIdx.y & 3;
+    const int g0 = blockIdx.x * kRowsPerGroup;
+    f32x16 acc = {};
+    for (int t0 = g0; t0 < g0 + kRowsPerGroup && t0 < n_rows; t0 += 32) {
+        for (int idx = lane; idx < 32 * 32; idx += 64) {
+            const int row = idx / 32, m = idx % 32;
+            const int r = t0 + row;
+            at[m][row] = r < n_rows ? dA2q[(size_t)r * kH + mt * 32 + m] : (uint16_t)0;
+            bt[m][row] = r < n_rows ? H1q[(size_t)r * kH + nt * 32 + m] : (uint16_t)0;
+        }
+        __syncthreads();
+        for (int kb = 0; kb < 2; ++kb) {
+            const bf16x8 a = load8(&at[lane & 31][kb * 16 + 8 * (lane >> 5)]);
+            const bf16x8 b = load8(&bt[lane & 31][kb * 16 + 8 * (lane >> 5)]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float *dst = part + ((size_t)blockIdx.x * kH * kH);
+    for (int v = 0; v < 16; ++v) {
+        const int m = mt * 32 + mfma_row(v, lane), n = nt * 32 + (lane & 31);
+        dst[(size_t)m * kH + n] = acc[v];
+    }
+}
+
+// ---- backward, small parameters (VALU) over a group of rows ----------------
+// per group partial of W1, b1, b2, W3, b3 (W2 comes from mlp_wgrad2).
+__global__ __launch_bounds__(kH) void mlp_wgrad_small(const double *__restrict__ coup, int n_rows,
+                                                      const float *__restrict__ dout, const float *__restrict__ H2,
+                                                      const float *__restrict__ dA2, const float *__restrict__ dA1,
+                                                      float *__restrict__ part) {
+    const int k = threadIdx.x;  // hidden unit
+    const int g0 = blockIdx.x * kRowsPerGroup;
+    float gw1[kIn] = {}, gb1 = 0.f, gb2 = 0.f, gw3a = 0.f, gw3b = 0.f, gb3a = 0.f, gb3b = 0.f;
+    for (int r = g0; r < g0 + kRowsPerGroup && r < n_rows; ++r) {
+        const int p = r & 255, c = r >> 8;
+        const float d1 = dA1[(size_t)r * kH + k];
+        for (int i = 0; i < kWin; ++i) gw1[i] += d1 * (((p >> i) & 1) ? 1.f : -1.f);
+        gw1[8] += d1 * (float)coup[2 * c];
+        gw1[9] += d1 * (float)coup[2 * c + 1];
+        gb1 += d1;
+        gb2 += dA2[(size_t)r * kH + k];
+        const float h2 = H2[(size_t)r * kH + k];
+        gw3a += dout[(size_t)r * 2] * h2;
+        gw3b += dout[(size_t)r * 2 + 1] * h2;
+        if (k == 0) {
+            gb3a += dout[(size_t)r * 2];
+            gb3b += dout[(size_t)r * 2 + 1];
+        }
+    }
+    float *dst = part + (size_t)blockIdx.x * kParams;
+    for (int i = 0; i < kIn; ++i) dst[oW1 + k * kIn + i] = gw1[i];
+    dst[ob1 + k] = gb1;
+    dst[ob2 + k] = gb2;
+    dst[oW3 + k] = gw3a;
+    dst[oW3 + kH + k] = gw3b;
+    if (k == 0) {
+        dst[ob3] = gb3a;
+        dst[ob3 + 1] = gb3b;
+    }
+}
+
+// grad = scale * sum over groups of the partials (W2 from part2, the rest from part).
+__global__ __launch_bounds__(256) void mlp_reduce_grad(int n_groups, const float *__restrict__ part,
+                                                       const float *__restrict__ part2, float scale,
+                                                       float *__restrict__ grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kParams) return;
+    float s = 0.f;
+    if (i >= oW2 && i < ob2) {
+        for (int g = 0; g < n_groups; ++g) s += part2[(size_t)g * kH * kH + (i - oW2)];
+    } else {
+        for (int g = 0; g < n_groups; ++g) s += part[(size_t)g * kParams + i];
+    }
+    grad[i] = s * scale;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights; refresh the bf16 W2 and its transpose.
+__global__ __launch_bounds__(256) void mlp_adam(float *__restrict__ theta, const float *__restrict__ grad,
+                                                float *__restrict__ m1, float *__restrict__ m2, float lr, float b1,
+                                                float b2, float bc1, float bc2, uint16_t *__restrict__ w2bf,
+                                                uint16_t *__restrict__ w2t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kParams) return;
+    const float g = grad[i];
+    const float a = b1 * m1[i] + (1.f - b1) * g;
+    const float v = b2 * m2[i] + (1.f - b2) * g * g;
+    m1[i] = a;
+    m2[i] = v;
+    const float t = theta[i] - lr * (a / bc1) / (sqrtf(v / bc2) + 1e-8f);
+    theta[i] = t;
+    if (i >= oW2 && i < ob2) {
+        const int n = (i - oW2) / kH, k = (i - oW2) % kH;
+        const uint16_t q = to_bf16_bits(t);
+        w2bf[n * kH + k] = q;
+        w2t[k * kH + n] = q;
+    }
+}
+
+__global__ __launch_bounds__(256) void mlp_refresh_w2(const float *__restrict__ theta, uint16_t *__restrict__ w2bf,
+                                                      uint16_t *__restrict__ w2t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kH * kH) return;
+    const int n = i / kH, k = i % kH;
+    const uint16_t q = to_bf16_bits(theta[oW2 + i]);
+    w2bf[n * kH + k] = q;
+    w2t[k * kH + n] = q;
+}
+
+// ---- per-state kernels (one sample = one ground state) ---------------------
+__device__ inline int window_pattern(uint64_t s, int t, int n) {
+    const uint64_t ss = (n == 64) ? s : (s | (s << n));  // ring wrap (n <= 56 used)
+    return (int)((ss >> t) & 0xFFull);
+}
+
+// alpha_r, beta_r from the pattern table of this coupling; block max of alpha.
+__global__ __launch_bounds__(256) void mlp_state_logpsi(int64_t n, int n_sites, const uint64_t *__restrict__ reps,
+                                                        const float *__restrict__ tab, double *__restrict__ alpha,
+                                                        double *__restrict__ beta, double *__restrict__ part) {
+    __shared__ float t[512];
+    __shared__ double lds[kWaves];
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) t[i] = tab[i];
+    __syncthreads();
+    double mx = -1e300;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = reps[r];
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < n_sites; ++q) {
+            const int p = window_pattern(s, q, n_sites);
+            a += (double)t[2 * p];
+            b += (double)t[2 * p + 1];
+        }
+        alpha[r] = a;
+        beta[r] = b;
+        mx = fmax(mx, a);
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lds[0];
+        for (int i = 1; i < kWaves; ++i) m = fmax(m, lds[i]);
+        part[blockIdx.x] = m;
+    }
+}
+
+__global__ __launch_bounds__(256) void mlp_finish_max(const double *__restrict__ part, int n, double *__restrict__ out) {
+    __shared__ double lds[kWaves];
+    double mx = -1e300;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = fmax(mx, part[i]);
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lds[0];
+        for (int i = 1; i < kWaves; ++i) m = fmax(m, lds[i]);
+        *out = m;
+    }
+}
+
+// c_r = exp(alpha - amax + i beta) sqrt(G/m_r); partials of Re/Im <u, c>, <c, c>, <u, u>.
+__global__ __launch_bounds__(256) void mlp_state_overlap(int64_t n, int G, const uint8_t *__restrict__ stab,
+                                                         const double *__restrict__ alpha,
+                                                         const double *__restrict__ beta,
+                                                         const double *__restrict__ u, const double *__restrict__ amax,
+                                                         double *__restrict__ part, int nb) {
+    __shared__ double lds[kWaves];
+    const double am = *amax;
+    double o_re = 0, o_im = 0, nc = 0, nu = 0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const double mag = exp(alpha[r] - am) * sqrt((double)G / (double)stab[r]);
+        const double cr = mag * cos(beta[r]), ci = mag * sin(beta[r]);
+        const double ur = u ? u[r] : 0.0;
+        o_re += ur * cr;
+        o_im += ur * ci;
+        nc += mag * mag;
+        nu += ur * ur;
+    }
+    const double v[4] = {o_re, o_im, nc, nu};
+    for (int q = 0; q < 4; ++q) {
+        const double t = block_sum(v[q], lds);
+        if (threadIdx.x == 0) part[q * nb + blockIdx.x] = t;
+    }
+}
+
+// One block: sums of the 4 partial rows -> stats[0..3] = Re O, Im O, Nc, Nu; stats[4] = loss.
+__global__ __launch_bounds__(256) void mlp_finish_overlap(const double *__restrict__ part, int nb,
+                                                          double *__restrict__ stats) {
+    __shared__ double lds[kWaves];
+    double s[4];
+    for (int q = 0; q < 4; ++q) s[q] = sum_partials(part + q * nb, nb, lds);
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 4; ++q) stats[q] = s[q];
+        const double o2 = s[0] * s[0] + s[1] * s[1];
+        stats[4] = 1.0 - o2 / (s[2] * s[3]);
+    }
+}
+
+// dL/dalpha_r, dL/dbeta_r (L = 1 - F) accumulated per window pattern in
+// fixed point (order-independent integer sums): hist[block][256][2].
+__global__ __launch_bounds__(256) void mlp_state_grad(int64_t n, int n_sites, int G, const uint64_t *__restrict__ reps,
+                                                      const uint8_t *__restrict__ stab,
+                                                      const double *__restrict__ alpha,
+                                                      const double *__restrict__ beta, const double *__restrict__ u,
+                                                      const double *__restrict__ amax,
+                                                      const double *__restrict__ stats,
+                                                      unsigned long long *__restrict__ hist) {
+    __shared__ unsigned long long h[512];
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) h[i] = 0ull;
+    __syncthreads();
+    const double am = *amax;
+    const double ore = stats[0], oim = stats[1], nc = stats[2], nu = stats[3];
+    const double o2 = ore * ore + oim * oim;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const double mag = exp(alpha[r] - am) * sqrt((double)G / (double)stab[r]);
+        const double cr = mag * cos(beta[r]), ci = mag * sin(beta[r]);
+        // w = conj(O) u c
+        const double wr = u[r] * (ore * cr + oim * ci);
+        const double wi = u[r] * (ore * ci - oim * cr);
+        const double ga = -(2.0 * wr * nc - o2 * 2.0 * mag * mag) / (nc * nc * nu);
+        const double gb = -(-2.0 * wi) / (nc * nu);
+        const long long qa = (long long)llrint(ga * kFix), qb = (long long)llrint(gb * kFix);
+        const uint64_t s = reps[r];
+        for (int q = 0; q < n_sites; ++q) {
+            const int p = window_pattern(s, q, n_sites);
+            atomicAdd(&h[2 * p], (unsigned long long)qa);
+            atomicAdd(&h[2 * p + 1], (unsigned long long)qb);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) hist[(size_t)blockIdx.x * 512 + i] = h[i];
+}
+
+// dout[row][o] for the sample's 256 rows = sum over blocks of the histograms / 2^36.
+__global__ __launch_bounds__(256) void mlp_finish_hist(const unsigned long long *__restrict__ hist, int nb,
+                                                       float *__restrict__ dout_rows, float scale) {
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+        long long s = 0;
+        for (int b = 0; b < nb; ++b) s += (long long)hist[(size_t)b * 512 + i];
+        dout_rows[i] = (float)((double)s / kFix * scale);
+    }
+}
+
+// Real start vector x_r = |c_r| cos(beta_r - beta_ref) with beta_ref the phase
+// of the first representative of largest amplitude (alpha == amax); the
+// partials of ||x||^2 go to part[block].
+__global__ __launch_bounds__(256) void mlp_state_start(int64_t n, int G, const uint8_t *__restrict__ stab,
+                                                       const double *__restrict__ alpha,
+                                                       const double *__restrict__ beta, const double *__restrict__ amax,
+                                                       const double *__restrict__ beta_ref, double *__restrict__ x,
+                                                       double *__restrict__ part) {
+    __shared__ double lds[kWaves];
+    const double am = *amax, br = *beta_ref;
+    double acc = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const double mag = exp(alpha[r] - am) * sqrt((double)G / (double)stab[r]);
+        const double v = mag * cos(beta[r] - br);
+        x[r] = v;
+        acc += v * v;
+    }
+    const double t = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Smallest index whose alpha equals amax: per-block minima (part[block] =
+// index as a double, exact below 2^53), then one block picks the minimum and
+// stores beta at that index.
+__device__ inline long long block_min_ll(long long v, long long *lds) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const long long o = __shfl_down(v, off, 64);
+        v = o < v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    long long b = lds[0];
+    for (int i = 1; i < kWaves; ++i) b = lds[i] < b ? lds[i] : b;
+    __syncthreads();
+    return b;
+}
+
+__global__ __launch_bounds__(256) void mlp_argmax_part(int64_t n, const double *__restrict__ alpha,
+                                                       const double *__restrict__ amax, double *__restrict__ part) {
+    __shared__ long long lds[kWaves];
+    const double am = *amax;
+    long long best = LLONG_MAX;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        if (alpha[r] == am && r < best) best = r;
+    const long long b = block_min_ll(best, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = b == LLONG_MAX ? -1.0 : (double)b;
+}
+
+__global__ __launch_bounds__(256) void mlp_beta_ref(int nb, const double *__restrict__ part,
+                                                    const double *__restrict__ beta, double *__restrict__ beta_ref) {
+    __shared__ long long lds[kWaves];
+    long long best = LLONG_MAX;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        const double v = part[i];
+        if (v >= 0.0 && (long long)v < best) best = (long long)v;
+    }
+    const long long b = block_min_ll(best, lds);
+    if (threadIdx.x == 0) *beta_ref = b == LLONG_MAX ? 0.0 : beta[b];
+}
+
+// out[0] = fixed-order sum of part[0..n).
+__global__ __launch_bounds__(256) void mlp_finish_sum(const double *__restrict__ part, int n, double *__restrict__ out) {
+    __shared__ double lds[kWaves];
+    const double t = sum_partials(part, n, lds);
+    if (threadIdx.x == 0) out[0] = t;
+}
+
+}  // namespace qmb
+
+using namespace qmb;
+
+struct MlpSample {
+    const qmb_basis *basis;
+    DevBuf u;  // target ground state
+    double delta, j2;
+};
+
+struct qmb_mlp {
+    qmb_ctx *ctx = nullptr;
+    DevBuf theta, m1, m2, grad, w2bf, w2t;
+    int64_t step = 0;
+    std::vector<MlpSample> samples;
+    // row buffers (sized for the current row count)
+    int rows_cap = 0;
+    DevBuf coup, out, H1, H1q, H2, dout, dA2, dA2q, dA1, part, part2;
+    void ensure_rows(int rows);
+};
+
+void qmb_mlp::ensure_rows(int rows) {
+    if (rows <= rows_cap) return;
+    const int cap = ((rows + 255) / 256) * 256;
+    coup.alloc((size_t)(cap / 256) * 2 * sizeof(double));
+    out.alloc((size_t)cap * 2 * sizeof(float));
+    H1.alloc((size_t)cap * kH * sizeof(float));
+    H1q.alloc((size_t)cap * kH * sizeof(uint16_t));
+    H2.alloc((size_t)cap * kH * sizeof(float));
+    dout.alloc((size_t)cap * 2 * sizeof(float));
+    dA2.alloc((size_t)cap * kH * sizeof(float));
+    dA2q.alloc((size_t)cap * kH * sizeof(uint16_t));
+    dA1.alloc((size_t)cap * kH * sizeof(float));
+    const int groups = (cap + kRowsPerGroup - 1) / kRowsPerGroup;
+    part.alloc((size_t)groups * kParams * sizeof(float));
+    part2.alloc((size_t)groups * kH * kH * sizeof(float));
+    rows_cap = cap;
+}
+
+namespace {
+
+void check_ring_basis(const qmb_basis *b) {
+    const qmb_group *g = b->group;
+    QMB_REQUIRE(g->torus && g->block == g->n_sites && g->G == g->n_sites,
+                "mlp: the basis must use the plain ring translation group (G = N)");
+    QMB_REQUIRE(b->layout == QMB_LAYOUT_COMPACT, "mlp: the basis must use the compact layout");
+    QMB_REQUIRE(b->n_sites >= kWin && b->n_sites <= 56, "mlp: ring size must be in [8, 56]");
+    const double *ch = &g->chars[(size_t)b->irrep * g->G * 2];
+    for (int e = 0; e < g->G; ++e)
+        QMB_REQUIRE(ch[2 * e] == 1.0 && ch[2 * e + 1] == 0.0, "mlp: the basis must be the k=0 sector");
+}
+
+// Forward pass over the given couplings (rows = 256 per coupling).
+void forward_rows(qmb_mlp *m, const std::vector<double> &coup, bool store) {
+    qmb_ctx *ctx = m->ctx;
+    const int rows = (int)(coup.size() / 2) * 256;
+    m->ensure_rows(rows);
+    QMB_HIP(hipMemcpyAsync(m->coup.p, coup.data(), coup.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(mlp_forward, dim3((rows + 31) / 32), dim3(64), 0, ctx->stream, m->theta.as<float>(),
+                       m->w2bf.as<uint16_t>(), m->coup.as<double>(), rows, m->out.as<float>(), m->H1.as<float>(),
+                       m->H1q.as<uint16_t>(), m->H2.as<float>(), store ? 1 : 0);
+    QMB_HIP(hipGetLastError());
+}
+
+// alpha/beta/amax of one state set for the coupling whose table starts at tab.
+struct StateWork {
+    DevBuf alpha, beta, part, amax, stats, hist;
+    int64_t cap = 0;
+    int nb = 0;
+    void ensure(int64_t n) {
+        if (n <= cap) return;
+        alpha.alloc(n * sizeof(double));
+        beta.alloc(n * sizeof(double));
+        nb = (int)std::min<int64_t>(1024, (n + 255) / 256);
+        part.alloc((size_t)nb * 4 * sizeof(double));
+        amax.alloc(sizeof(double));
+        stats.alloc(8 * sizeof(double));
+        hist.alloc((size_t)nb * 512 * sizeof(unsigned long long));
+        cap = n;
+    }
+};
+
+void state_logpsi(qmb_ctx *ctx, const qmb_basis *b, const float *tab, StateWork &w) {
+    w.ensure(b->n_rows);
+    hipLaunchKernelGGL(mlp_state_logpsi, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, b->n_sites,
+                       b->reps.as<uint64_t>(), tab, w.alpha.as<double>(), w.beta.as<double>(), w.part.as<double>());
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_finish_max, dim3(1), dim3(256), 0, ctx->stream, w.part.as<double>(), w.nb,
+                       w.amax.as<double>());
+    QMB_HIP(hipGetLastError());
+}
+
+// Loss (and dout rows of this sample when dout_rows != NULL) of one sample.
+void sample_loss(qmb_ctx *ctx, const MlpSample &s, const float *tab, StateWork &w, float *dout_rows, float scale) {
+    const qmb_basis *b = s.basis;
+    state_logpsi(ctx, b, tab, w);
+    hipLaunchKernelGGL(mlp_state_overlap, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, b->group->G,
+                       b->stab.as<uint8_t>(), w.alpha.as<double>(), w.beta.as<double>(), s.u.as<double>(),
+                       w.amax.as<double>(), w.part.as<double>(), w.nb);
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_finish_overlap, dim3(1), dim3(256), 0, ctx->stream, w.part.as<double>(), w.nb,
+                       w.stats.as<double>());
+    QMB_HIP(hipGetLastError());
+    if (dout_rows) {
+        hipLaunchKernelGGL(mlp_state_grad, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, b->n_sites,
+                           b->group->G, b->reps.as<uint64_t>(), b->stab.as<uint8_t>(), w.alpha.as<double>(),
+                           w.beta.as<double>(), s.u.as<double>(), w.amax.as<double>(), w.stats.as<double>(),
+                           w.hist.as<unsigned long long>());
+        QMB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(mlp_finish_hist, dim3(1), dim3(256), 0, ctx->stream, w.hist.as<unsigned long long>(), w.nb,
+                           dout_rows, scale);
+        QMB_HIP(hipGetLastError());
+    }
+}
+
+// Learned start vector of a k=0 ring basis into dx (device, n_rows values),
+// entirely on the device: network rows for (delta, j2), log-amplitude and
+// phase per representative, reference phase of the first largest amplitude,
+// x_r = |c_r| cos(beta_r - beta_ref). Returns false when ||x||^2 is zero or
+// not finite (the caller then starts from the seeded random vector); only
+// that one number travels to the host.
+bool build_start(qmb_mlp *m, const qmb_basis *b, double delta, double j2, double *dx) {
+    check_ring_basis(b);
+    qmb_ctx *ctx = m->ctx;
+    std::vector<double> coup = {delta, j2};
+    forward_rows(m, coup, false);
+    StateWork w;
+    state_logpsi(ctx, b, m->out.as<float>(), w);
+    DevBuf bref(sizeof(double));
+    hipLaunchKernelGGL(mlp_argmax_part, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, w.alpha.as<double>(),
+                       w.amax.as<double>(), w.part.as<double>());
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_beta_ref, dim3(1), dim3(256), 0, ctx->stream, w.nb, w.part.as<double>(),
+                       w.beta.as<double>(), bref.as<double>());
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_state_start, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, b->group->G,
+                       b->stab.as<uint8_t>(), w.alpha.as<double>(), w.beta.as<double>(), w.amax.as<double>(),
+                       bref.as<double>(), dx, w.part.as<double>());
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_finish_sum, dim3(1), dim3(256), 0, ctx->stream, w.part.as<double>(), w.nb,
+                       w.stats.as<double>());
+    QMB_HIP(hipGetLastError());
+    double nrm2 = 0.0;
+    QMB_HIP(hipMemcpyAsync(&nrm2, w.stats.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    QMB_HIP(hipStreamSynchronize(ctx->stream));
+    return std::isfinite(nrm2) && nrm2 > 0.0;
+}
+
+// Mean loss over the samples and, optionally, the gradient (device grad buffer).
+double loss_and_grad(qmb_mlp *m, bool want_grad) {
+    qmb_ctx *ctx = m->ctx;
+    const int ns = (int)m->samples.size();
+    std::vector<double> coup(2 * ns);
+    for (int i = 0; i < ns; ++i) {
+        coup[2 * i] = m->samples[i].delta;
+        coup[2 * i + 1] = m->samples[i].j2;
+    }
+    forward_rows(m, coup, want_grad);
+    const int rows = ns * 256;
+    StateWork w;
+    DevBuf losses((size_t)ns * sizeof(double));
+    for (int i = 0; i < ns; ++i) {
+        sample_loss(ctx, m->samples[i], m->out.as<float>() + (size_t)i * 512, w,
+                    want_grad ? m->dout.as<float>() + (size_t)i * 512 : nullptr, 1.0f / ns);
+        QMB_HIP(hipMemcpyAsync(losses.as<double>() + i, w.stats.as<double>() + 4, sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (want_grad) {
+        hipLaunchKernelGGL(mlp_backward_act, dim3((rows + 31) / 32), dim3(64), 0, ctx->stream, m->theta.as<float>(),
+                           m->w2t.as<uint16_t>(), rows, m->dout.as<float>(), m->H1.as<float>(), m->H2.as<float>(),
+                           m->dA2.as<float>(), m->dA2q.as<uint16_t>(), m->dA1.as<float>());
+        QMB_HIP(hipGetLastError());
+        const int groups = (rows + kRowsPerGroup - 1) / kRowsPerGroup;
+        hipLaunchKernelGGL(mlp_wgrad2, dim3(groups, 16), dim3(64), 0, ctx->stream, rows, m->dA2q.as<uint16_t>(),
+                           m->H1q.as<uint16_t>(), m->part2.as<float>());
+        QMB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(mlp_wgrad_small, dim3(groups), dim3(kH), 0, ctx->stream, m->coup.as<double>(), rows,
+                           m->dout.as<float>(), m->H2.as<float>(), m->dA2.as<float>(), m->dA1.as<float>(),
+                           m->part.as<float>());
+        QMB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(mlp_reduce_grad, dim3((kParams + 255) / 256), dim3(256), 0, ctx->stream, groups,
+                           m->part.as<float>(), m->part2.as<float>(), 1.0f, m->grad.as<float>());
+        QMB_HIP(hipGetLastError());
+    }
+    std::vector<double> hl(ns);
+    QMB_HIP(hipMemcpyAsync(hl.data(), losses.p, ns * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    QMB_HIP(hipStreamSynchronize(ctx->stream));
+    double mean = 0.0;
+    for (double v : hl) mean += v;
+    return mean / ns;
+}
+
+std::vector<float> init_params(uint64_t seed) {
+    std::vector<float> th(kParams);
+    const int fan_for[] = {kIn, kIn, kH, kH, kH, kH};
+    const int ends[] = {ob1, oW2, ob2, oW3, ob3, kParams};
+    int seg = 0;
+    for (int i = 0; i < kParams; ++i) {
+        while (i >= ends[seg]) ++seg;
+        uint64_t z = seed + ((uint64_t)i + 1ULL) * 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        const double u = (double)((z >> 11) + 1ULL) * (1.0 / 9007199254740992.0);
+        th[i] = (float)((2.0 * u - 1.0) / std::sqrt((double)fan_for[seg]));
+    }
+    return th;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qmb_mlp_num_params(void) { return kParams; }
+
+qmb_status qmb_mlp_create(qmb_ctx *ctx, uint64_t seed, qmb_mlp **out) {
+    return guarded([&] {
+        QMB_REQUIRE(ctx && out, "qmb_mlp_create: NULL argument");
+        *out = nullptr;
+        QMB_HIP(hipSetDevice(ctx->device));
+        auto m = std::make_unique<qmb_mlp>();
+        m->ctx = ctx;
+        m->theta.alloc(kParams * sizeof(float));
+        m->m1.alloc(kParams * sizeof(float));
+        m->m2.alloc(kParams * sizeof(float));
+        m->grad.alloc(kParams * sizeof(float));
+        m->w2bf.alloc(kH * kH * sizeof(uint16_t));
+        m->w2t.alloc(kH * kH * sizeof(uint16_t));
+        const std::vector<float> th = init_params(seed);
+        QMB_HIP(hipMemcpy(m->theta.p, th.data(), kParams * sizeof(float), hipMemcpyHostToDevice));
+        QMB_HIP(hipMemset(m->m1.p, 0, kParams * sizeof(float)));
+        QMB_HIP(hipMemset(m->m2.p, 0, kParams * sizeof(float)));
+        hipLaunchKernelGGL(mlp_refresh_w2, dim3(kH * kH / 256), dim3(256), 0, ctx->stream, m->theta.as<float>(),
+                           m->w2bf.as<uint16_t>(), m->w2t.as<uint16_t>());
+        QMB_HIP(hipGetLastError());
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        *out = m.release();
+    });
+}
+
+qmb_status qmb_mlp_destroy(qmb_mlp *m) {
+    if (m && m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+    return QMB_OK;
+}
+
+qmb_status qmb_mlp_get_params(const qmb_mlp *m, float *theta) {
+    return guarded([&] {
+        QMB_REQUIRE(m && theta, "qmb_mlp_get_params: NULL argument");
+        QMB_HIP(hipStreamSynchronize(m->ctx->stream));
+        QMB_HIP(hipMemcpy(theta, m->theta.p, kParams * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+qmb_status qmb_mlp_set_params(qmb_mlp *m, const float *theta) {
+    return guarded([&] {
+        QMB_REQUIRE(m && theta, "qmb_mlp_set_params: NULL argument");
+        QMB_HIP(hipMemcpy(m->theta.p, theta, kParams * sizeof(float), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(mlp_refresh_w2, dim3(kH * kH / 256), dim3(256), 0, m->ctx->stream, m->theta.as<float>(),
+                           m->w2bf.as<uint16_t>(), m->w2t.as<uint16_t>());
+        QMB_HIP(hipGetLastError());
+        QMB_HIP(hipStreamSynchronize(m->ctx->stream));
+    });
+}
+
+qmb_status qmb_mlp_forward(qmb_mlp *m, int n, const double *delta, const double *j2, float *out) {
+    return guarded([&] {
+        QMB_REQUIRE(m && delta && j2 && out && n >= 1, "qmb_mlp_forward: bad argument");
+        std::vector<double> coup(2 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            coup[2 * i] = delta[i];
+            coup[2 * i + 1] = j2[i];
+        }
+        forward_rows(m, coup, false);
+        QMB_HIP(hipMemcpyAsync(out, m->out.p, (size_t)n * 512 * sizeof(float), hipMemcpyDeviceToHost, m->ctx->stream));
+        QMB_HIP(hipStreamSynchronize(m->ctx->stream));
+    });
+}
+
+qmb_status qmb_mlp_add_sample(qmb_mlp *m, const qmb_basis *b, const double *ground_state, double delta, double j2) {
+    return guarded([&] {
+        QMB_REQUIRE(m && b && ground_state, "qmb_mlp_add_sample: NULL argument");
+        QMB_REQUIRE(b->ctx == m->ctx, "qmb_mlp_add_sample: basis from another context");
+        check_ring_basis(b);
+        MlpSample s;
+        s.basis = b;
+        s.delta = delta;
+        s.j2 = j2;
+        s.u.alloc((size_t)b->n_rows * sizeof(double));
+        QMB_HIP(hipMemcpy(s.u.p, ground_state, (size_t)b->n_rows * sizeof(double), hipMemcpyHostToDevice));
+        m->samples.push_back(std::move(s));
+    });
+}
+
+qmb_status qmb_mlp_clear_samples(qmb_mlp *m) {
+    return guarded([&] {
+        QMB_REQUIRE(m, "qmb_mlp_clear_samples: NULL argument");
+        QMB_HIP(hipStreamSynchronize(m->ctx->stream));
+        m->samples.clear();
+    });
+}
+
+qmb_status qmb_mlp_loss_grad(qmb_mlp *m, double *loss, float *grad) {
+    return guarded([&] {
+        QMB_REQUIRE(m && loss, "qmb_mlp_loss_grad: NULL argument");
+        QMB_REQUIRE(!m->samples.empty(), "qmb_mlp_loss_grad: no samples");
+        *loss = loss_and_grad(m, grad != nullptr);
+        if (grad) QMB_HIP(hipMemcpy(grad, m->grad.p, kParams * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+qmb_status qmb_mlp_train(qmb_mlp *m, int steps, double lr, double *loss_history) {
+    return guarded([&] {
+        QMB_REQUIRE(m && steps >= 1 && lr > 0, "qmb_mlp_train: bad argument");
+        QMB_REQUIRE(!m->samples.empty(), "qmb_mlp_train: no samples");
+        const float b1 = 0.9f, b2 = 0.999f;
+        for (int s = 0; s < steps; ++s) {
+            const double L = loss_and_grad(m, true);
+            if (loss_history) loss_history[s] = L;
+            ++m->step;
+            const float bc1 = 1.f - std::pow(b1, (float)m->step), bc2 = 1.f - std::pow(b2, (float)m->step);
+            hipLaunchKernelGGL(mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, m->ctx->stream,
+                               m->theta.as<float>(), m->grad.as<float>(), m->m1.as<float>(), m->m2.as<float>(),
+                               (float)lr, b1, b2, bc1, bc2, m->w2bf.as<uint16_t>(), m->w2t.as<uint16_t>());
+            QMB_HIP(hipGetLastError());
+        }
+        QMB_HIP(hipStreamSynchronize(m->ctx->stream));
+    });
+}
+
+qmb_status qmb_mlp_start_vector(qmb_mlp *m, const qmb_basis *b, double delta, double j2, double *x, int *fallback) {
+    return guarded([&] {
+        QMB_REQUIRE(m && b && x, "qmb_mlp_start_vector: NULL argument");
+        DevBuf dx((size_t)b->n_rows * sizeof(double));
+        const bool ok = build_start(m, b, delta, j2, dx.as<double>());
+        QMB_HIP(hipMemcpyAsync(x, dx.p, b->n_rows * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+        QMB_HIP(hipStreamSynchronize(m->ctx->stream));
+        if (fallback) *fallback = ok ? 0 : 1;
+    });
+}
+
+qmb_status qmb_lanczos_learned(qmb_mlp *m, const qmb_basis *b, const qmb_ell *e, double delta, double j2,
+                               const qmb_lanczos_opts *opts, double *evals, double *eigvec, qmb_lanczos_report *report,
+                               int *fallback) {
+    qmb_status st = guarded([&] {
+        QMB_REQUIRE(m && b && e && evals, "qmb_lanczos_learned: NULL argument");
+        QMB_REQUIRE(e->basis == b, "qmb_lanczos_learned: the ELL was not built on this basis");
+        QMB_REQUIRE(e->ctx == m->ctx, "qmb_lanczos_learned: network and ELL live on different contexts");
+        QMB_REQUIRE(e->value_type == QMB_REAL, "qmb_lanczos_learned: the learned start vector is real; "
+                                               "build the ELL with QMB_REAL");
+    });
+    if (st != QMB_OK) return st;
+    qmb_lanczos_opts o;
+    if (opts)
+        o = *opts;
+    else
+        qmb_lanczos_default_opts(&o);
+    DevBuf dx;
+    bool ok = false;
+    st = guarded([&] {
+        dx.alloc((size_t)b->n_rows * sizeof(double));
+        ok = build_start(m, b, delta, j2, dx.as<double>());
+    });
+    if (st != QMB_OK) return st;
+    if (fallback) *fallback = ok ? 0 : 1;
+    if (ok) {
+        o.start = QMB_START_DEVICE;
+        o.start_vec = dx.as<double>();
+    } else {
+        o.start = QMB_START_RANDOM;
+        o.start_vec = nullptr;
+    }
+    return qmb_lanczos(m->ctx, e, &o, evals, nullptr, eigvec, report);
+}
+
+}  // extern "C"

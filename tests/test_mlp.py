@@ -1,0 +1,142 @@
+"""Learned start vector: oracle pins (CPU) and GPU MFMA forward/backward checks.
+
+CPU: the FP64 oracle's back-propagation equals central finite differences of
+its own loss; its bfloat16 rounding equals torch's.
+GPU: parameter initialisation bit-equal to the oracle's; pattern-table forward
+pass and loss within bf16 tolerance of the oracle; the MFMA gradient against
+the oracle gradient (same bf16 rounding points); training lowers the loss; a
+trained network starts Lanczos closer to the ground state than the random
+start (fewer iterations, same E0).
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp as OMLP
+
+
+def small_problem(n=10, delta=0.8, j2=0.3, seed=3):
+    """A random target vector over the k=0 ring representatives (CPU only)."""
+    from tests.helpers import chain_case, oracle_orbits
+
+    case = chain_case(n)
+    orb = oracle_orbits(case, n // 2)
+    reps = orb.reps
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=len(reps))
+    u /= np.linalg.norm(u)
+    return reps, orb.stab, n, u, delta, j2
+
+
+def test_oracle_backprop_matches_finite_differences():
+    reps, stab, n, u, d, j2 = small_problem()
+    pats = OMLP.window_patterns(reps, n)
+    theta = OMLP.init_params(11) * 3.0
+    L, g = OMLP.fidelity_loss(theta, pats, stab, n, u, d, j2, with_grad=True, round_like_gpu=False)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(OMLP.N_PARAMS, 40, replace=False)
+    h = 1e-6
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (OMLP.fidelity_loss(tp, pats, stab, n, u, d, j2, round_like_gpu=False)
+              - OMLP.fidelity_loss(tm, pats, stab, n, u, d, j2, round_like_gpu=False)) / (2 * h)
+        assert abs(fd - g[i]) < 1e-7 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_bf16_rounding_matches_torch():
+    import torch
+
+    x = np.random.default_rng(1).normal(size=10000) * 10.0 ** np.random.default_rng(2).integers(-6, 6, 10000)
+    want = torch.tensor(x, dtype=torch.float32).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(OMLP.bf16(x), want)
+
+
+@pytest.mark.gpu
+def test_mlp_init_forward_loss_and_gradient(ctx):
+    import qmbcuda_amd as q
+    from qmbcuda_amd import models as LM
+
+    print("[gpu-case] mlp init/forward/grad", flush=True)
+    net = q.Mlp(ctx, seed=20261202)
+    th = net.params().astype(np.float64)
+    assert np.array_equal(th, OMLP.init_params(20261202))
+    theta = th * 3.0  # larger weights so tanh saturation and phases matter
+    net.set_params(theta.astype(np.float32))
+    theta = theta.astype(np.float32).astype(np.float64)
+    # forward pass over all 256 patterns of two couplings
+    out = net.forward([0.7, 1.3], [0.1, 0.55])
+    for i, (d, j2) in enumerate([(0.7, 0.1), (1.3, 0.55)]):
+        ref, _ = OMLP.forward(theta, OMLP.pattern_inputs(d, j2))
+        err = np.abs(out[i] - ref)
+        assert err.max() < 3e-2 and np.median(err) < 2e-3, err.max()
+    # one sample: the exact ground state of an N=12 chain sector (k=0)
+    lat, info = LM.chain(12, 1.0, 0.8, 0.3)
+    ge, fl, ch = LM.translation_group(lat)
+    g = q.Group(ge, ch)
+    t = q.Terms(12, LM.term_table(lat, info))
+    b = q.Basis(ctx, g, 6, 0, q.LAYOUT_COMPACT)
+    e = q.Ell(ctx, b, t)
+    ev, rep, _, u = q.lanczos(e, max_iter=300, want_eigvec=True)
+    net.add_sample(b, u, 0.8, 0.3)
+    L, grad = net.loss_grad()
+    reps, stab, _ = b.export()
+    pats = OMLP.window_patterns(reps, 12)
+    L_or, g_or = OMLP.fidelity_loss(theta, pats, stab, 12, u, 0.8, 0.3, with_grad=True, round_like_gpu=True)
+    assert abs(L - L_or) < 2e-3 * max(1.0, abs(L_or))
+    cos = grad @ g_or / (np.linalg.norm(grad) * np.linalg.norm(g_or))
+    assert cos > 0.999, cos
+    assert abs(np.linalg.norm(grad) / np.linalg.norm(g_or) - 1) < 3e-2
+    # the big blocks separately: W2 (MFMA weight gradient) and W1 (through the MFMA dH1)
+    for name, sl in (("W2", slice(1408, 1408 + 16384)), ("W1", slice(0, 1280))):
+        a, r = grad[sl], g_or[sl]
+        assert a @ r / (np.linalg.norm(a) * np.linalg.norm(r)) > 0.995, name
+
+
+@pytest.mark.gpu
+def test_mlp_training_and_learned_start_gain(ctx):
+    """Train on N=12,14,16 k=0 ground states over a 3x3 coupling grid; evaluate
+    on N=18 at held-out couplings: the learned start needs fewer Lanczos steps
+    than the seeded random start and reaches the same E0."""
+    import qmbcuda_amd as q
+    from qmbcuda_amd import models as LM
+
+    print("[gpu-case] mlp training + gain", flush=True)
+    keep = []
+    net = q.Mlp(ctx, seed=7)
+
+    def sector(n, d, j2):
+        lat, info = LM.chain(n, 1.0, d, j2)
+        ge, fl, ch = LM.translation_group(lat)
+        g = q.Group(ge, ch)
+        t = q.Terms(n, LM.term_table(lat, info))
+        b = q.Basis(ctx, g, n // 2, 0, q.LAYOUT_COMPACT)
+        e = q.Ell(ctx, b, t)
+        keep.extend([g, t, b, e])
+        return b, e
+
+    for n in (12, 14, 16):
+        for d in np.linspace(0.6, 1.4, 3):
+            for j2 in np.linspace(0.0, 0.5, 3):
+                b, e = sector(n, d, j2)
+                ev, rep, _, u = q.lanczos(e, max_iter=400, want_eigvec=True)
+                net.add_sample(b, u, float(d), float(j2))
+    L0, _ = net.loss_grad(want_grad=False)
+    hist = net.train(300, 3e-3)
+    L1, _ = net.loss_grad(want_grad=False)
+    print(f"loss {L0:.4f} -> {L1:.4f}", flush=True)
+    assert L1 < 0.5 * L0
+    gains = []
+    for d, j2 in [(0.9, 0.2), (1.1, 0.35)]:
+        b, e = sector(18, d, j2)
+        x, fb = net.start_vector(b, d, j2)
+        assert not fb
+        ev_r, rep_r, _, _ = q.lanczos(e, max_iter=600, tol=1e-13)
+        ev_l, rep_l, _, _ = q.lanczos(e, max_iter=600, tol=1e-13, start_vec=x)
+        assert rep_r.converged and rep_l.converged
+        assert abs(ev_l[0] - ev_r[0]) < 1e-10 * abs(ev_r[0])
+        gains.append(rep_r.iters - rep_l.iters)
+        # the device-resident path (qmb_lanczos_learned) starts from the same vector
+        ev_d, rep_d, fb_d, _ = net.lanczos(b, e, d, j2, max_iter=600, tol=1e-13)
+        assert not fb_d and rep_d.iters == rep_l.iters and ev_d[0] == ev_l[0]
+    assert min(gains) > 0, gains
