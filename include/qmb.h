@@ -12,7 +12,9 @@
  *  - every function returns a qmb_status; qmb_last_error() gives the text of the
  *    last failure on the calling thread;
  *  - opaque handles own their device memory and are released by *_destroy;
- *    host arrays always belong to the caller;
+ *    a handle holds the handles it was made from (a basis its context and
+ *    group, an ELL its basis, a network its context and sample bases), so
+ *    *_destroy calls may come in any order; host arrays belong to the caller;
  *  - a handle is used by one host thread at a time; every handle made from a
  *    context runs on that context's HIP stream;
  *  - states are uint64 (at most 64 sites); Lanczos runs in FP64.
@@ -230,8 +232,10 @@ qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
 /* ---- learned start vector (new; no reference counterpart) -------------------
  * A translation-invariant network over 8-spin windows (+-1 spins, Delta, J2/J1
  * -> tanh 128 -> tanh 128 -> log-amplitude, phase), log psi(s) = sum over the N
- * windows of an N-site ring. Bf16 weights and activations on MFMA, FP32
- * accumulation, FP32 Adam master weights. Parameter layout: W1[128][10], b1[128],
+ * windows of an N-site ring, plus i*pi*(up spins on even sites) (the Marshall
+ * sign rule as a fixed prior: the network learns the phase relative to it).
+ * Bf16 weights and activations on MFMA, FP32 accumulation, FP32 Adam master
+ * weights. Parameter layout: W1[128][10], b1[128],
  * W2[128][128], b2[128], W3[2][128], b3[2]. Samples are k=0 ring bases (compact
  * layout, plain translation group) with their ED ground states; the loss is the
  * mean infidelity 1 - |<u, c>|^2/(<c,c><u,u>), c_r = psi(r) sqrt(G/m_r). */
@@ -244,7 +248,7 @@ qmb_status qmb_mlp_get_params(const qmb_mlp *m, float *theta);
 qmb_status qmb_mlp_set_params(qmb_mlp *m, const float *theta);
 /* Network outputs for all 256 window patterns of n couplings: out[(i*256 + p)*2 + {a, phi}]. */
 qmb_status qmb_mlp_forward(qmb_mlp *m, int n, const double *delta, const double *j2, float *out);
-/* The basis must outlive the network's use of the sample. */
+/* The network keeps the basis alive while the sample is held. */
 qmb_status qmb_mlp_add_sample(qmb_mlp *m, const qmb_basis *b, const double *ground_state, double delta, double j2);
 qmb_status qmb_mlp_clear_samples(qmb_mlp *m);
 /* Mean loss over the samples and (grad != NULL) its gradient, host FP32 [num_params]. */
@@ -257,9 +261,12 @@ qmb_status qmb_mlp_start_vector(qmb_mlp *m, const qmb_basis *b, double delta, do
                                 int *fallback);
 /* Lanczos on e (built on basis b) from the learned start vector, which stays on
  * the device: the network, the start vector and the Lanczos run share one
- * stream and only ||x||^2 is read back. opts may be NULL (defaults); its start
- * fields are overridden. *fallback (optional) = 1 when the learned vector was
- * unusable and the seeded random start was used instead. eigvec as in qmb_lanczos. */
+ * stream and only two norms are read back. The start is x/||x|| + 0.1 xi/||xi||
+ * with xi_r = 2u_r - 1 from the SplitMix64 stream of opts->seed (the noise keeps
+ * every eigenvector in reach when x misses the ground state). opts may be NULL
+ * (defaults); its start fields are overridden. *fallback (optional) = 1 when
+ * the learned vector was unusable (zero or not finite) and the seeded random
+ * start was used instead. eigvec as in qmb_lanczos. */
 qmb_status qmb_lanczos_learned(qmb_mlp *m, const qmb_basis *b, const qmb_ell *e, double delta, double j2,
                                const qmb_lanczos_opts *opts, double *evals, double *eigvec,
                                qmb_lanczos_report *report, int *fallback);

@@ -3,7 +3,10 @@
 The network (no reference counterpart): input = 8 spins of a window (+-1,
 spin up = +1) plus Delta and J2/J1; two tanh hidden layers of 128; two
 outputs (log-amplitude a, phase phi). For a basis state s of an N-site ring,
-log psi(s) = sum_{t<N} a(w_t(s)) + i phi(w_t(s)), w_t(s) = spins t..t+7 (mod N).
+log psi(s) = sum_{t<N} a(w_t(s)) + i phi(w_t(s)) + i pi M(s), w_t(s) = spins
+t..t+7 (mod N), M(s) = number of up spins on even sites (Marshall's sign rule
+for a bipartite ring, a fixed prior; the network learns the phase relative
+to it).
 The k=0 symmetric-basis coefficient of representative r is
 c_r = psi(r) * sqrt(G/m_r). Training loss per ground state u (real, unit):
 L = 1 - |<u, c>|^2 / (<c, c> <u, u>).
@@ -92,13 +95,45 @@ def window_patterns(reps, n_sites):
     return out
 
 
-def fidelity_loss(theta, pats, stab, G, u, delta, j2, with_grad=False, round_like_gpu=True):
-    """Loss 1 - F for one ground state; optional gradient wrt theta (FP64)."""
-    X = pattern_inputs(delta, j2)
-    out, cache = forward(theta, X, round_like_gpu)
-    a_tab, p_tab = out[:, 0], out[:, 1]
-    alpha = a_tab[pats].sum(1)
-    beta = p_tab[pats].sum(1)
+def marshall_phase(reps, n_sites):
+    """pi * (up spins on the even sites) of each representative."""
+    even = sum(1 << j for j in range(0, n_sites, 2))
+    return np.array([np.pi * bin(int(r) & even).count("1") for r in np.asarray(reps, np.uint64)], np.float64)
+
+
+def log_psi(theta, pats, prior, delta, j2, round_like_gpu=True):
+    """(alpha_r, beta_r): log-amplitude and phase of each representative."""
+    out, cache = forward(theta, pattern_inputs(delta, j2), round_like_gpu)
+    alpha = out[:, 0][pats].sum(1)
+    beta = out[:, 1][pats].sum(1) + (0.0 if prior is None else prior)
+    return alpha, beta, cache
+
+
+def start_vector(theta, pats, prior, stab, G, delta, j2):
+    """Real start vector |c_r| cos(beta_r - beta_ref), beta_ref = phase of the
+    first representative of largest amplitude (before the noise admixture)."""
+    alpha, beta, _ = log_psi(theta, pats, prior, delta, j2)
+    i = int(np.argmax(alpha))
+    return np.exp(alpha - alpha[i]) * np.sqrt(G / np.asarray(stab, np.float64)) * np.cos(beta - beta[i])
+
+
+def start_noise(n, seed):
+    """xi_r = 2 u_r - 1 with u_r element r of the SplitMix64 stream of seed."""
+    m = (1 << 64) - 1
+    out = np.empty(n, np.float64)
+    for c in range(n):
+        z = (seed + (c + 1) * 0x9E3779B97F4A7C15) & m
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+        z ^= z >> 31
+        out[c] = 2.0 * (((z >> 11) + 1) * 2.0 ** -53) - 1.0
+    return out
+
+
+def fidelity_loss(theta, pats, stab, G, u, delta, j2, with_grad=False, round_like_gpu=True, prior=None):
+    """Loss 1 - F for one ground state; optional gradient wrt theta (FP64).
+    prior: the fixed phase per representative (marshall_phase), or None."""
+    alpha, beta, cache = log_psi(theta, pats, prior, delta, j2, round_like_gpu)
     amax = alpha.max()
     c = np.exp(alpha - amax + 1j * beta) * np.sqrt(G / np.asarray(stab, np.float64))
     O = np.sum(u * c)

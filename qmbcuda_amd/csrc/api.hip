@@ -279,13 +279,13 @@ qmb_status qmb_ctx_create(int device, qmb_ctx **out) {
         c->device = device;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
+            c->stream = nullptr;
             delete c;
             throw Error{QMB_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
         }
         try {
             ensure_binom(c);
         } catch (...) {
-            (void)hipStreamDestroy(c->stream);
             delete c;
             throw;
         }
@@ -297,9 +297,7 @@ qmb_status qmb_ctx_destroy(qmb_ctx *ctx) {
     if (!ctx) return QMB_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    ctx->binom.release();
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    qmb::release(ctx);  // torn down once no basis, ELL or network uses it
     return QMB_OK;
 }
 
@@ -357,7 +355,7 @@ qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el, 
 }
 
 qmb_status qmb_group_destroy(qmb_group *g) {
-    delete g;
+    qmb::release(g);
     return QMB_OK;
 }
 

@@ -254,7 +254,7 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
 
 qmb_status qmb_basis_destroy(qmb_basis *b) {
     if (b && b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
-    delete b;
+    qmb::release(b);
     return QMB_OK;
 }
 

@@ -274,11 +274,7 @@ __global__ __launch_bounds__(kThreads) void bk_start(const BInst *__restrict__ i
     const BInst I = insts[t.inst];
     double acc = 0.0;
     for (int64_t i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
-        uint64_t z = seed + ((uint64_t)i + 1ULL) * 0x9E3779B97F4A7C15ULL;
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-        z ^= z >> 31;
-        const double u = (double)((z >> 11) + 1ULL) * (1.0 / 9007199254740992.0);
+        const double u = splitmix_uniform(seed, (uint64_t)i);
         I.P[i] = u;
         acc += u * u;
     }

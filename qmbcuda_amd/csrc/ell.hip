@@ -310,7 +310,7 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
 
 qmb_status qmb_ell_destroy(qmb_ell *e) {
     if (e && e->ctx) (void)hipStreamSynchronize(e->ctx->stream);
-    delete e;
+    qmb::release(e);
     return QMB_OK;
 }
 

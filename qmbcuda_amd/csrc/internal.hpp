@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -102,15 +103,58 @@ struct GroupView {
     const uint8_t *inv;    // [G]
 };
 
+// Handles are reference counted: a child (basis, ELL, network) holds its
+// parents (context, group, basis), so the C ABI's destroy calls may come in any
+// order; the last holder frees the object.
+struct Counted {
+    mutable std::atomic<int> refs{1};
+};
+
+template <class T>
+inline void retain(const T *p) {
+    if (p) p->refs.fetch_add(1, std::memory_order_relaxed);
+}
+
+template <class T>
+inline void release(const T *p) {
+    if (p && p->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete p;
+}
+
+template <class T>
+struct Ref {
+    T *p = nullptr;
+    Ref() = default;
+    Ref(const Ref &) = delete;
+    Ref &operator=(const Ref &) = delete;
+    Ref(Ref &&o) noexcept : p(o.p) { o.p = nullptr; }
+    Ref &operator=(T *q) {
+        retain(q);
+        release(p);
+        p = q;
+        return *this;
+    }
+    ~Ref() { release(p); }
+    T *operator->() const { return p; }
+    operator T *() const { return p; }
+};
+
 }  // namespace qmb
 
-struct qmb_ctx {
+struct qmb_ctx : qmb::Counted {
     int device = 0;
     hipStream_t stream = nullptr;
     qmb::DevBuf binom;  // uint64 [64*33] C(p, j)
+    ~qmb_ctx() {
+        if (stream) {
+            (void)hipSetDevice(device);
+            (void)hipStreamSynchronize(stream);
+        }
+        binom.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
-struct qmb_group {
+struct qmb_group : qmb::Counted {
     int n_sites = 0, G = 0, n_irreps = 0;
     std::vector<uint32_t> perm;   // [G*n_sites]
     std::vector<uint8_t> flip;    // [G]
@@ -127,9 +171,9 @@ struct qmb_terms {
     std::vector<double> scalar;  // re, im pairs
 };
 
-struct qmb_basis {
-    qmb_ctx *ctx = nullptr;
-    const qmb_group *group = nullptr;
+struct qmb_basis : qmb::Counted {
+    qmb::Ref<qmb_ctx> ctx;
+    qmb::Ref<const qmb_group> group;
     int n_sites = 0, n_up = 0, irrep = 0, layout = 0;
     int64_t sector_dim = 0, n_words = 0, n_rows = 0, n_null = 0;
     qmb::DevBuf bitmap;  // uint64 [n_words]: bit = representative kept as a row
@@ -143,9 +187,9 @@ struct qmb_basis {
     uint64_t triv_lo = 0, triv_hi = 0;  // elements with character 1 for the irrep
 };
 
-struct qmb_ell {
-    qmb_ctx *ctx = nullptr;
-    const qmb_basis *basis = nullptr;
+struct qmb_ell : qmb::Counted {
+    qmb::Ref<qmb_ctx> ctx;
+    qmb::Ref<const qmb_basis> basis;
     int value_type = QMB_REAL;
     int max_terms = 0;
     int64_t n_rows = 0, n_chunks = 0, nnz = 0, stored = 0;

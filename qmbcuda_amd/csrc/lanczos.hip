@@ -202,15 +202,6 @@ __global__ __launch_bounds__(kThreads) void lz_update(int64_t cnt, const double 
 }
 
 // ---- start vector, norms, eigenvector helpers -------------------------------
-// Counter-based SplitMix64 (Steele, Lea, Flood 2014) -> uniform (0, 1].
-__device__ inline double splitmix_uniform(uint64_t seed, uint64_t c) {
-    uint64_t z = seed + (c + 1ULL) * 0x9E3779B97F4A7C15ULL;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    z ^= z >> 31;
-    return (double)((z >> 11) + 1ULL) * (1.0 / 9007199254740992.0);
-}
-
 __global__ __launch_bounds__(kThreads) void lz_random(int64_t cnt, uint64_t seed, double *w) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
         w[i] = splitmix_uniform(seed, (uint64_t)i);

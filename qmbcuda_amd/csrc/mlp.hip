@@ -30,6 +30,8 @@ constexpr int kIn = 10, kH = 128, kOut = 2, kWin = 8;
 constexpr int oW1 = 0, ob1 = oW1 + kH * kIn, oW2 = ob1 + kH, ob2 = oW2 + kH * kH, oW3 = ob2 + kH,
               ob3 = oW3 + kOut * kH, kParams = ob3 + kOut;
 constexpr double kFix = 68719476736.0;  // 2^36 fixed-point scale of the pattern histograms
+constexpr double kMarshallPi = 3.14159265358979323846;
+constexpr double kStartNoise = 0.1;  // weight of the unit seeded noise in the Lanczos start
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -292,6 +294,10 @@ __device__ inline int window_pattern(uint64_t s, int t, int n) {
 }
 
 // alpha_r, beta_r from the pattern table of this coupling; block max of alpha.
+// beta_r also carries the Marshall sign prior pi * (up spins on even sites):
+// the network learns the phase relative to the bipartite sign rule, whose
+// state (and not the uniform one, which is the fully polarised multiplet and
+// orthogonal to a singlet) is the natural starting point of training.
 __global__ __launch_bounds__(256) void mlp_state_logpsi(int64_t n, int n_sites, const uint64_t *__restrict__ reps,
                                                         const float *__restrict__ tab, double *__restrict__ alpha,
                                                         double *__restrict__ beta, double *__restrict__ part) {
@@ -299,6 +305,7 @@ __global__ __launch_bounds__(256) void mlp_state_logpsi(int64_t n, int n_sites, 
     __shared__ double lds[kWaves];
     for (int i = threadIdx.x; i < 512; i += blockDim.x) t[i] = tab[i];
     __syncthreads();
+    const uint64_t even = 0x5555555555555555ULL & ((1ULL << n_sites) - 1ULL);  // n_sites <= 56
     double mx = -1e300;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t s = reps[r];
@@ -309,7 +316,7 @@ __global__ __launch_bounds__(256) void mlp_state_logpsi(int64_t n, int n_sites, 
             b += (double)t[2 * p + 1];
         }
         alpha[r] = a;
-        beta[r] = b;
+        beta[r] = b + kMarshallPi * (double)__popcll(s & even);
         mx = fmax(mx, a);
     }
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
@@ -420,24 +427,42 @@ __global__ __launch_bounds__(256) void mlp_finish_hist(const unsigned long long 
 }
 
 // Real start vector x_r = |c_r| cos(beta_r - beta_ref) with beta_ref the phase
-// of the first representative of largest amplitude (alpha == amax); the
-// partials of ||x||^2 go to part[block].
+// of the first representative of largest amplitude (alpha == amax). Block
+// partials: part[block] = sum of x_r^2, part[gridDim.x + block] = sum of xi_r^2
+// for the seeded noise xi_r = 2 u_r - 1 (u_r = element r of the SplitMix64
+// stream of `seed`) that the Lanczos start mixes in.
 __global__ __launch_bounds__(256) void mlp_state_start(int64_t n, int G, const uint8_t *__restrict__ stab,
                                                        const double *__restrict__ alpha,
                                                        const double *__restrict__ beta, const double *__restrict__ amax,
-                                                       const double *__restrict__ beta_ref, double *__restrict__ x,
-                                                       double *__restrict__ part) {
+                                                       const double *__restrict__ beta_ref, uint64_t seed,
+                                                       double *__restrict__ x, double *__restrict__ part) {
     __shared__ double lds[kWaves];
     const double am = *amax, br = *beta_ref;
-    double acc = 0.0;
+    double acc = 0.0, accn = 0.0;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         const double mag = exp(alpha[r] - am) * sqrt((double)G / (double)stab[r]);
         const double v = mag * cos(beta[r] - br);
         x[r] = v;
         acc += v * v;
+        const double xi = 2.0 * splitmix_uniform(seed, (uint64_t)r) - 1.0;
+        accn += xi * xi;
     }
     const double t = block_sum(acc, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    const double tn = block_sum(accn, lds);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = t;
+        part[gridDim.x + blockIdx.x] = tn;
+    }
+}
+
+// x_r <- x_r / ||x|| + eta * xi_r / ||xi|| (norms squared in sums[0], sums[1]).
+__global__ __launch_bounds__(256) void mlp_mix_noise(int64_t n, uint64_t seed, double eta,
+                                                     const double *__restrict__ sums, double *__restrict__ x) {
+    const double ix = 1.0 / sqrt(sums[0]), in = eta / sqrt(sums[1]);
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const double xi = 2.0 * splitmix_uniform(seed, (uint64_t)r) - 1.0;
+        x[r] = x[r] * ix + xi * in;
+    }
 }
 
 // Smallest index whose alpha equals amax: per-block minima (part[block] =
@@ -479,11 +504,16 @@ __global__ __launch_bounds__(256) void mlp_beta_ref(int nb, const double *__rest
     if (threadIdx.x == 0) *beta_ref = b == LLONG_MAX ? 0.0 : beta[b];
 }
 
-// out[0] = fixed-order sum of part[0..n).
-__global__ __launch_bounds__(256) void mlp_finish_sum(const double *__restrict__ part, int n, double *__restrict__ out) {
+// out[0], out[1] = fixed-order sums of part[0..n) and part[n..2n).
+__global__ __launch_bounds__(256) void mlp_finish_sum2(const double *__restrict__ part, int n,
+                                                       double *__restrict__ out) {
     __shared__ double lds[kWaves];
-    const double t = sum_partials(part, n, lds);
-    if (threadIdx.x == 0) out[0] = t;
+    const double t0 = sum_partials(part, n, lds);
+    const double t1 = sum_partials(part + n, n, lds);
+    if (threadIdx.x == 0) {
+        out[0] = t0;
+        out[1] = t1;
+    }
 }
 
 }  // namespace qmb
@@ -491,13 +521,13 @@ __global__ __launch_bounds__(256) void mlp_finish_sum(const double *__restrict__
 using namespace qmb;
 
 struct MlpSample {
-    const qmb_basis *basis;
+    Ref<const qmb_basis> basis;
     DevBuf u;  // target ground state
     double delta, j2;
 };
 
-struct qmb_mlp {
-    qmb_ctx *ctx = nullptr;
+struct qmb_mlp : Counted {
+    Ref<qmb_ctx> ctx;
     DevBuf theta, m1, m2, grad, w2bf, w2t;
     int64_t step = 0;
     std::vector<MlpSample> samples;
@@ -604,15 +634,16 @@ void sample_loss(qmb_ctx *ctx, const MlpSample &s, const float *tab, StateWork &
 // Learned start vector of a k=0 ring basis into dx (device, n_rows values),
 // entirely on the device: network rows for (delta, j2), log-amplitude and
 // phase per representative, reference phase of the first largest amplitude,
-// x_r = |c_r| cos(beta_r - beta_ref). Returns false when ||x||^2 is zero or
-// not finite (the caller then starts from the seeded random vector); only
-// that one number travels to the host.
-bool build_start(qmb_mlp *m, const qmb_basis *b, double delta, double j2, double *dx) {
+// x_r = |c_r| cos(beta_r - beta_ref); w.stats[0] = ||x||^2 and w.stats[1] =
+// ||xi||^2 of the seeded noise. Returns false when ||x||^2 is zero or not
+// finite (the caller then starts from the seeded random vector); only those
+// two numbers travel to the host.
+bool build_start(qmb_mlp *m, const qmb_basis *b, double delta, double j2, uint64_t seed, double *dx,
+                 StateWork &w) {
     check_ring_basis(b);
     qmb_ctx *ctx = m->ctx;
     std::vector<double> coup = {delta, j2};
     forward_rows(m, coup, false);
-    StateWork w;
     state_logpsi(ctx, b, m->out.as<float>(), w);
     DevBuf bref(sizeof(double));
     hipLaunchKernelGGL(mlp_argmax_part, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, w.alpha.as<double>(),
@@ -623,15 +654,15 @@ bool build_start(qmb_mlp *m, const qmb_basis *b, double delta, double j2, double
     QMB_HIP(hipGetLastError());
     hipLaunchKernelGGL(mlp_state_start, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, b->group->G,
                        b->stab.as<uint8_t>(), w.alpha.as<double>(), w.beta.as<double>(), w.amax.as<double>(),
-                       bref.as<double>(), dx, w.part.as<double>());
+                       bref.as<double>(), seed, dx, w.part.as<double>());
     QMB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(mlp_finish_sum, dim3(1), dim3(256), 0, ctx->stream, w.part.as<double>(), w.nb,
+    hipLaunchKernelGGL(mlp_finish_sum2, dim3(1), dim3(256), 0, ctx->stream, w.part.as<double>(), w.nb,
                        w.stats.as<double>());
     QMB_HIP(hipGetLastError());
-    double nrm2 = 0.0;
-    QMB_HIP(hipMemcpyAsync(&nrm2, w.stats.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    double sums[2] = {0.0, 0.0};
+    QMB_HIP(hipMemcpyAsync(sums, w.stats.p, sizeof(sums), hipMemcpyDeviceToHost, ctx->stream));
     QMB_HIP(hipStreamSynchronize(ctx->stream));
-    return std::isfinite(nrm2) && nrm2 > 0.0;
+    return std::isfinite(sums[0]) && sums[0] > 0.0 && sums[1] > 0.0;
 }
 
 // Mean loss over the samples and, optionally, the gradient (device grad buffer).
@@ -728,7 +759,7 @@ qmb_status qmb_mlp_create(qmb_ctx *ctx, uint64_t seed, qmb_mlp **out) {
 
 qmb_status qmb_mlp_destroy(qmb_mlp *m) {
     if (m && m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
-    delete m;
+    release(m);
     return QMB_OK;
 }
 
@@ -820,7 +851,8 @@ qmb_status qmb_mlp_start_vector(qmb_mlp *m, const qmb_basis *b, double delta, do
     return guarded([&] {
         QMB_REQUIRE(m && b && x, "qmb_mlp_start_vector: NULL argument");
         DevBuf dx((size_t)b->n_rows * sizeof(double));
-        const bool ok = build_start(m, b, delta, j2, dx.as<double>());
+        StateWork w;
+        const bool ok = build_start(m, b, delta, j2, 0, dx.as<double>(), w);
         QMB_HIP(hipMemcpyAsync(x, dx.p, b->n_rows * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
         QMB_HIP(hipStreamSynchronize(m->ctx->stream));
         if (fallback) *fallback = ok ? 0 : 1;
@@ -844,10 +876,19 @@ qmb_status qmb_lanczos_learned(qmb_mlp *m, const qmb_basis *b, const qmb_ell *e,
     else
         qmb_lanczos_default_opts(&o);
     DevBuf dx;
+    StateWork w;
     bool ok = false;
     st = guarded([&] {
         dx.alloc((size_t)b->n_rows * sizeof(double));
-        ok = build_start(m, b, delta, j2, dx.as<double>());
+        ok = build_start(m, b, delta, j2, o.seed, dx.as<double>(), w);
+        if (ok) {
+            // unit learned vector + kStartNoise x unit seeded noise: the noise keeps
+            // every eigenvector in the Krylov space even where the network's
+            // vector misses the ground state
+            hipLaunchKernelGGL(mlp_mix_noise, dim3(w.nb), dim3(256), 0, m->ctx->stream, b->n_rows, o.seed,
+                               kStartNoise, w.stats.as<double>(), dx.as<double>());
+            QMB_HIP(hipGetLastError());
+        }
     });
     if (st != QMB_OK) return st;
     if (fallback) *fallback = ok ? 0 : 1;

@@ -1,7 +1,8 @@
 // Deterministic block reductions shared by the Lanczos kernels: wave shuffles,
 // a fixed-order LDS combine and a fixed-order sum of per-block partial slots
 // (run by a separate one-block kernel; no floating-point atomics, no
-// device-scope fences inside the streaming kernels).
+// device-scope fences inside the streaming kernels). Also the counter-based
+// generator of the seeded start vectors.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,6 +35,16 @@ __device__ inline double sum_partials(const double *partials, int n, double *lds
     double s = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += __builtin_nontemporal_load(partials + i);
     return block_sum(s, lds);
+}
+
+// Counter-based SplitMix64 (Steele, Lea, Flood 2014) -> uniform (0, 1]:
+// element c of the stream of `seed`, independent of the launch shape.
+__device__ inline double splitmix_uniform(uint64_t seed, uint64_t c) {
+    uint64_t z = seed + (c + 1ULL) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return (double)((z >> 11) + 1ULL) * (1.0 / 9007199254740992.0);
 }
 
 }  // namespace qmb
