@@ -174,7 +174,9 @@ typedef struct {
     int32_t max_iter;    /* upper bound on iterations */
     int32_t n_eigs;      /* lowest eigenvalues wanted (1..8) */
     int32_t check_every; /* iterations between convergence checks (0 -> 10) */
-    int32_t reorth;      /* 0 plain recurrence, 1 full reorthogonalisation */
+    int32_t reorth;      /* 0 plain recurrence, 1 full reorthogonalisation every step,
+                          * 2 partial: only when the estimated loss of orthogonality
+                          * passes sqrt(eps) (Simon 1984); 1 and 2 keep max_iter vectors */
     int32_t fixed_iter;  /* 1: run exactly max_iter steps, no test (reference semantics) */
     int32_t start;       /* QMB_START_RANDOM, QMB_START_USER or QMB_START_DEVICE */
     int32_t want_eigvec; /* 1: rebuild the ground-state vector (second pass) */
@@ -190,6 +192,8 @@ typedef struct {
     int32_t converged;   /* 1 if the tolerance was met */
     int32_t breakdown;   /* 1 if an invariant subspace was hit (exact result) */
     int32_t n_spmv;
+    int32_t n_reorth;    /* steps that reorthogonalised */
+    int32_t reserved;
     double residual[8];  /* Ritz residuals of the returned eigenvalues */
     double eig_residual; /* ||H x - E x|| of the rebuilt vector (want_eigvec) */
     double ms_total;     /* wall time of the call */
@@ -224,7 +228,8 @@ double qmb_lanczos_eigvals_ref(double *E_tmp, int NStates, int NIter, int nobv, 
  * This solves n independent ELLs together: one launch per phase covers every
  * instance (a work table of row chunks), the iteration chunk is captured once as
  * a HIP graph and replayed, converged instances drop out. evals[n*n_eigs],
- * reports[n] (may be NULL). */
+ * reports[n] (may be NULL). reorth = 2 runs as 1 (every step): the captured
+ * iteration has no host decision point. */
 qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
                                const qmb_lanczos_opts *opts, double *evals,
                                qmb_lanczos_report *reports);

@@ -98,9 +98,13 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
     bitmap[w] = bits;
 }
 
+// One thread per row: the row's bitmap word is the last word whose row offset
+// is <= row (binary search over the offsets), its bit the (row - offset)-th
+// set bit of that word. Every thread does the same work (one unrank, one
+// stabiliser count) and the writes are coalesced.
 template <class S>
 __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, const uint64_t *binom, int n_up,
-                                                               int64_t dim, int64_t n_words,
+                                                               int64_t n_rows, int64_t n_words,
                                                                uint64_t triv_lo, uint64_t triv_hi,
                                                                const uint64_t *bitmap, const uint32_t *prefix,
                                                                uint64_t *reps, uint8_t *stab, uint8_t *nullf) {
@@ -109,24 +113,25 @@ __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, con
     stage_perm(perm_lds, gv);
     stage_binom(bin, binom);
     __syncthreads();
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= n_words) return;
-    uint64_t bits = bitmap[w];
-    if (!bits) return;
-    int64_t pos = prefix[w];
-    const int64_t r0 = w * 64;
-    // representatives are sparse among the ranks: unrank each one directly
-    while (bits) {
-        const int i = __builtin_ctzll(bits);
-        bits &= bits - 1;
-        const S x = colex_unrank<S>(r0 + i, n_up, gv.n_sites, bin);
-        uint64_t slo, shi;
-        is_representative<S>(x, gv, perm_lds, slo, shi);
-        reps[pos] = (uint64_t)x;
-        stab[pos] = (uint8_t)(__popcll(slo) + __popcll(shi));
-        nullf[pos] = compatible(slo, shi, triv_lo, triv_hi) ? 0 : 1;
-        ++pos;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    int64_t lo = 0, hi = n_words - 1;  // prefix[0] = 0 <= r
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if ((int64_t)prefix[mid] <= r)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
+    uint64_t bits = bitmap[lo];
+    for (int64_t k = r - (int64_t)prefix[lo]; k > 0; --k) bits &= bits - 1;
+    const int i = __builtin_ctzll(bits);
+    const S x = colex_unrank<S>(lo * 64 + i, n_up, gv.n_sites, bin);
+    uint64_t slo, shi;
+    is_representative<S>(x, gv, perm_lds, slo, shi);
+    reps[r] = (uint64_t)x;
+    stab[r] = (uint8_t)(__popcll(slo) + __popcll(shi));
+    nullf[r] = compatible(slo, shi, triv_lo, triv_hi) ? 0 : 1;
 }
 
 template <class S>
@@ -208,8 +213,10 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
         b->sector_dim = (int64_t)dim;
         b->n_words = (int64_t)((dim + 63) / 64);
         const double *ch = &g->chars[(size_t)irrep * g->G * 2];
+        b->all_rows = true;
         for (int e = 0; e < g->G; ++e) {
             const bool one = std::fabs(ch[2 * e] - 1.0) < 1e-9 && std::fabs(ch[2 * e + 1]) < 1e-9;
+            b->all_rows = b->all_rows && one;
             if (one) {
                 if (e < 64)
                     b->triv_lo |= 1ULL << e;
@@ -234,8 +241,9 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
         b->stab.alloc((size_t)std::max<int64_t>(rows, 1));
         b->nullf.alloc((size_t)std::max<int64_t>(rows, 1));
         if (rows > 0) {
-            hipLaunchKernelGGL(narrow ? basis_compact<uint32_t> : basis_compact<uint64_t>, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, gv,
-                               ctx->binom.as<uint64_t>(), n_up, b->sector_dim, b->n_words, b->triv_lo, b->triv_hi,
+            const unsigned rblocks = (unsigned)((rows + kBasisThreads - 1) / kBasisThreads);
+            hipLaunchKernelGGL(narrow ? basis_compact<uint32_t> : basis_compact<uint64_t>, dim3(rblocks), dim3(kBasisThreads), 0, ctx->stream, gv,
+                               ctx->binom.as<uint64_t>(), n_up, rows, b->n_words, b->triv_lo, b->triv_hi,
                                b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>());
             QMB_HIP(hipGetLastError());

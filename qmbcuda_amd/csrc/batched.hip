@@ -24,6 +24,7 @@
 
 #include "internal.hpp"
 #include "reduce.hpp"
+#include "sell.hpp"
 
 namespace qmb {
 
@@ -70,15 +71,7 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
         const int len = r < I.n_rows ? (int)I.row_len[r] : 0;
         const int64_t p0 = base + lane;
         if (!CPLX) {
-            double acc = 0.0;
-            for (int k = 0; k < L; ++k) {
-                if (k < len) {
-                    const int64_t p = p0 + (int64_t)k * 64;
-                    const int32_t j = __builtin_nontemporal_load(I.col + p);
-                    const double v = __builtin_nontemporal_load(I.val + p);
-                    acc += v * (I.P[j] * s);
-                }
-            }
+            const double acc = sell_row_real(I.col, I.val, I.P, p0, L, len) * s;
             if (r < I.n_rows) {
                 I.Y[r] = acc;
                 const double vr = I.P[r] * s;
@@ -86,28 +79,16 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
                 dot += vr * acc;
             }
         } else {
-            const double2 *val2 = reinterpret_cast<const double2 *>(I.val);
-            const double2 *x2 = reinterpret_cast<const double2 *>(I.P);
-            double ar = 0.0, ai = 0.0;
-            for (int k = 0; k < L; ++k) {
-                if (k < len) {
-                    const int64_t p = p0 + (int64_t)k * 64;
-                    const int32_t j = __builtin_nontemporal_load(I.col + p);
-                    const double2 v = val2[p];
-                    double2 g = x2[j];
-                    g.x = g.x * s;
-                    g.y = g.y * s;
-                    ar += v.x * g.x - v.y * g.y;
-                    ai += v.x * g.y + v.y * g.x;
-                }
-            }
+            double2 a = sell_row_cplx(I.col, I.val, I.P, p0, L, len);
+            a.x *= s;
+            a.y *= s;
             if (r < I.n_rows) {
-                reinterpret_cast<double2 *>(I.Y)[r] = make_double2(ar, ai);
-                double2 vr = x2[r];
+                reinterpret_cast<double2 *>(I.Y)[r] = a;
+                double2 vr = reinterpret_cast<const double2 *>(I.P)[r];
                 vr.x = vr.x * s;
                 vr.y = vr.y * s;
                 reinterpret_cast<double2 *>(vout)[r] = vr;
-                dot += vr.x * ar + vr.y * ai;
+                dot += vr.x * a.x + vr.y * a.y;
             }
         }
     }
@@ -306,7 +287,8 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         const qmb_lanczos_opts o = *opts;
         QMB_REQUIRE(o.max_iter >= 1 && o.max_iter <= 5000, "qmb_lanczos_batched: max_iter must be in [1, 5000]");
         QMB_REQUIRE(o.n_eigs >= 1 && o.n_eigs <= 8, "qmb_lanczos_batched: n_eigs must be in [1, 8]");
-        QMB_REQUIRE(o.n_eigs == 1 || o.reorth, "qmb_lanczos_batched: n_eigs > 1 needs reorth = 1");
+        QMB_REQUIRE(o.reorth >= 0 && o.reorth <= 2, "qmb_lanczos_batched: reorth must be 0, 1 or 2");
+        QMB_REQUIRE(o.n_eigs == 1 || o.reorth, "qmb_lanczos_batched: n_eigs > 1 needs reorth = 1 or 2");
         QMB_REQUIRE(o.start == QMB_START_RANDOM, "qmb_lanczos_batched: only the seeded random start");
         for (int i = 0; i < n; ++i) {
             QMB_REQUIRE(ells[i] && ells[i]->ctx == ctx, "qmb_lanczos_batched: ELL missing or from another context");
@@ -621,6 +603,7 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                 r.converged = conv[pos];
                 r.breakdown = brk[pos];
                 r.n_spmv = iters[pos];
+                r.n_reorth = reorth ? iters[pos] : 0;
                 for (int j = 0; j < std::min(k_eigs, 8); ++j) r.residual[j] = res[pos][j];
                 r.ms_total = wall;
                 r.ms_host = ms_host;

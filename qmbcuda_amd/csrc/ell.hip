@@ -62,7 +62,8 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                                                         const uint8_t *nullf, const uint64_t *bitmap,
                                                         const uint32_t *prefix, TermsView tv, const double2 *chi,
                                                         const double *nrm, int W, uint16_t *row_len, int *err,
-                                                        const int64_t *chunk_ptr, int32_t *col, double *val) {
+                                                        int all_rows, const int64_t *chunk_ptr, int32_t *col,
+                                                        double *val) {
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
     __shared__ double2 s_chi[kMaxGroup];
@@ -106,6 +107,12 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                     continue;
                 }
                 cand |= 1ULL << (h - h0);
+            }
+            if (!FILL && all_rows) {
+                // every orbit is a row (no null orbits for this irrep): the row
+                // length needs no orbit lookup
+                slot += __popcll(cand);
+                continue;
             }
             while (cand) {
                 const int h = h0 + __builtin_ctzll(cand);
@@ -271,7 +278,8 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
                                b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
-                               e->row_len.as<uint16_t>(), d_err.as<int>(), nullptr, nullptr, nullptr);
+                               e->row_len.as<uint16_t>(), d_err.as<int>(), b->all_rows ? 1 : 0, nullptr, nullptr,
+                               nullptr);
             QMB_HIP(hipGetLastError());
         }
         int herr = 0;
@@ -293,7 +301,7 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
                                b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
-                               e->row_len.as<uint16_t>(), d_err.as<int>(), e->chunk_ptr.as<int64_t>(),
+                               e->row_len.as<uint16_t>(), d_err.as<int>(), 0, e->chunk_ptr.as<int64_t>(),
                                e->col.as<int32_t>(), e->val.as<double>());
             QMB_HIP(hipGetLastError());
             if (n % kChunk) {

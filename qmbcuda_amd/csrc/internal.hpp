@@ -185,6 +185,7 @@ struct qmb_basis : qmb::Counted {
     qmb::DevBuf g_keep, g_small, g_big, g_flip, g_perm, g_inv;
     qmb::GroupView view() const;
     uint64_t triv_lo = 0, triv_hi = 0;  // elements with character 1 for the irrep
+    bool all_rows = false;              // every element has character 1: no null orbits
 };
 
 struct qmb_ell : qmb::Counted {

@@ -7,8 +7,9 @@
 //   y = H phi_n; a[n] = Re<phi_n, y>; w = (-a[n]) phi_n + (y + (-b[n]) phi_{n-1});
 //   b[n+1] = ||w||; phi_{n+1} = w * (1/b[n+1]).
 // Per step there are two streaming kernels and no host round trip:
-//   K1 spmv_sell<LZ>: y = H (w * 1/b) gathered on the fly, phi_n = w/b written
-//      out, the block partials of a[n] in the same pass;
+//   K1 spmv_sell<LZ>: y = (1/b) (H w) (the scale applied once per row after
+//      the gathers), phi_n = w/b written out, the block partials of a[n] in
+//      the same pass;
 //   K2 lz_update: w = (-a) phi_n + (y - b phi_{n-1}) and the partials of ||w||^2;
 // each followed by a one-block finish_sum that leaves a[n], b[n+1], 1/b[n+1]
 // on the device. The host reads a[], b[] every check_every steps and solves the
@@ -23,15 +24,16 @@
 
 #include "internal.hpp"
 #include "reduce.hpp"
+#include "sell.hpp"
 
 namespace qmb {
 
 constexpr int kMaxBlocks = 4096;
 
 // ---- K1: SELL-64 SpMV --------------------------------------------------------
-// One wavefront per 64-row chunk; slot-major storage makes every slot one
-// coalesced 256 B (col) + 512 B/1 KB (val) read; the x gathers are the only
-// random accesses. LZ fuses the Lanczos normalisation and the a[n] dot.
+// One wavefront per 64-row chunk (row products in sell.hpp); the x gathers are
+// the only random accesses. LZ fuses the Lanczos normalisation and the a[n]
+// dot: y = s (H x), phi = s x, partial sums of phi . y.
 template <bool CPLX, bool LZ>
 __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_chunks,
                                                       const int64_t *__restrict__ chunk_ptr,
@@ -40,8 +42,7 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ x, double *__restrict__ y,
                                                       const double *__restrict__ invb, double *__restrict__ vout,
-                                                      double *__restrict__ partials, unsigned *__restrict__ counter,
-                                                      double *__restrict__ alpha_out) {
+                                                      double *__restrict__ partials) {
     __shared__ double lds[kWaves];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -54,36 +55,8 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
         const int len = r < n_rows ? (int)row_len[r] : 0;
         const int64_t p0 = base + lane;
         if (!CPLX) {
-            double acc = 0.0;
-            int k = 0;
-            for (; k + 4 <= L; k += 4) {
-                int32_t j[4];
-                double v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (k + u < len) {
-                        const int64_t p = p0 + (int64_t)(k + u) * 64;
-                        j[u] = __builtin_nontemporal_load(col + p);
-                        v[u] = __builtin_nontemporal_load(val + p);
-                    }
-                }
-                double g[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (k + u < len) g[u] = LZ ? x[j[u]] * s : x[j[u]];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (k + u < len) acc += v[u] * g[u];
-            }
-            for (; k < L; ++k) {
-                if (k < len) {
-                    const int64_t p = p0 + (int64_t)k * 64;
-                    const int32_t jj = __builtin_nontemporal_load(col + p);
-                    const double vv = __builtin_nontemporal_load(val + p);
-                    const double gg = LZ ? x[jj] * s : x[jj];
-                    acc += vv * gg;
-                }
-            }
+            double acc = sell_row_real(col, val, x, p0, L, len);
+            if (LZ) acc *= s;
             if (r < n_rows) {
                 y[r] = acc;
                 if (LZ) {
@@ -93,62 +66,19 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                 }
             }
         } else {
-            const double2 *val2 = reinterpret_cast<const double2 *>(val);
-            const double2 *x2 = reinterpret_cast<const double2 *>(x);
-            double ar = 0.0, ai = 0.0;
-            int k = 0;
-            for (; k + 4 <= L; k += 4) {
-                int32_t j[4];
-                double2 v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (k + u < len) {
-                        const int64_t p = p0 + (int64_t)(k + u) * 64;
-                        j[u] = __builtin_nontemporal_load(col + p);
-                        v[u] = val2[p];
-                    }
-                }
-                double2 g[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (k + u < len) {
-                        g[u] = x2[j[u]];
-                        if (LZ) {
-                            g[u].x = g[u].x * s;
-                            g[u].y = g[u].y * s;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (k + u < len) {
-                        ar += v[u].x * g[u].x - v[u].y * g[u].y;
-                        ai += v[u].x * g[u].y + v[u].y * g[u].x;
-                    }
-                }
-            }
-            for (; k < L; ++k) {
-                if (k < len) {
-                    const int64_t p = p0 + (int64_t)k * 64;
-                    const int32_t jj = __builtin_nontemporal_load(col + p);
-                    const double2 vv = val2[p];
-                    double2 gg = x2[jj];
-                    if (LZ) {
-                        gg.x = gg.x * s;
-                        gg.y = gg.y * s;
-                    }
-                    ar += vv.x * gg.x - vv.y * gg.y;
-                    ai += vv.x * gg.y + vv.y * gg.x;
-                }
+            double2 a = sell_row_cplx(col, val, x, p0, L, len);
+            if (LZ) {
+                a.x *= s;
+                a.y *= s;
             }
             if (r < n_rows) {
-                reinterpret_cast<double2 *>(y)[r] = make_double2(ar, ai);
+                reinterpret_cast<double2 *>(y)[r] = a;
                 if (LZ) {
-                    double2 vr = x2[r];
+                    double2 vr = reinterpret_cast<const double2 *>(x)[r];
                     vr.x = vr.x * s;
                     vr.y = vr.y * s;
                     reinterpret_cast<double2 *>(vout)[r] = vr;
-                    dot += vr.x * ar + vr.y * ai;  // Re(conj(v) y)
+                    dot += vr.x * a.x + vr.y * a.y;  // Re(conj(v) y)
                 }
             }
         }
@@ -353,8 +283,7 @@ void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, boo
                   : (lz ? spmv_sell<false, true> : spmv_sell<false, false>);
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, ctx->stream, e->n_rows, e->n_chunks,
                        e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
-                       e->val.as<double>(), x, y, invb, vout, wk ? wk->partials.as<double>() : nullptr,
-                       nullptr, alpha_out);
+                       e->val.as<double>(), x, y, invb, vout, wk ? wk->partials.as<double>() : nullptr);
     QMB_HIP(hipGetLastError());
     if (lz) launch_finish(ctx, wk->partials.as<double>(), blocks, alpha_out, nullptr, 0);
 }
@@ -390,7 +319,10 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     const int64_t cnt = cplx ? 2 * n : n;
     const int M = o.max_iter;
     const int k_eigs = o.n_eigs;
-    const bool reorth = o.reorth != 0;
+    // reorth 0: plain recurrence; 1: full (every step); 2: partial, driven by
+    // the estimated loss of orthogonality. 1 and 2 keep every phi_n (VB).
+    const int rmode = o.reorth;
+    const bool reorth = rmode != 0;
     const int check = o.fixed_iter ? M : (o.check_every > 0 ? o.check_every : 10);
     const double tol = o.tol > 0 ? o.tol : 1e-13;
     QMB_HIP(hipSetDevice(ctx->device));
@@ -438,13 +370,61 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     auto k1 = [&](int it) {
         launch_spmv(ctx, e, P.as<double>(), Y.as<double>(), true, wk.invb() + it, vcur(it), &wk, wk.alpha() + it);
     };
+    // Partial reorthogonalisation (H. D. Simon, Math. Comp. 42 (1984) 115-142):
+    // om_cur[k] estimates <phi_j, phi_k> through the recurrence the Lanczos
+    // vectors themselves obey,
+    //   b[j+1] om'[k] = b[k+1] om[k+1] + (a[k] - a[j]) om[k] + b[k] om[k-1] - b[j] om_prev[k],
+    // plus a rounding term of size eps1 = eps sqrt(n) per step. When an estimate
+    // passes sqrt(eps), the new vector and the one after it are orthogonalised
+    // against every kept phi and the estimates restart at eps1.
+    const double eps1 = 2.220446049250313e-16 * std::sqrt((double)n);
+    std::vector<double> om_prev, om_cur, om_next, pa, pb;
+    if (rmode == 2) {
+        om_prev.assign(M + 2, 0.0);
+        om_cur.assign(M + 2, 0.0);
+        om_next.assign(M + 2, 0.0);
+        om_cur[0] = 1.0;
+        pa.assign(M + 1, 0.0);
+        pb.assign(M + 2, 0.0);
+    }
+    bool pro_second = false;
+    int n_reorth = 0;
+    auto pro_decide = [&](int it) -> bool {
+        // a[it], b[it+1] of this step (one small read per step)
+        QMB_HIP(hipMemcpyAsync(&pa[it], wk.alpha() + it, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipMemcpyAsync(&pb[it + 1], wk.beta() + it + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        const int j = it;
+        const double bj1 = pb[j + 1];
+        if (!(bj1 > 0.0)) return false;  // breakdown: nothing to orthogonalise
+        double worst = 0.0;
+        for (int k = 0; k < j; ++k) {
+            double t = pb[k + 1] * om_cur[k + 1] + (pa[k] - pa[j]) * om_cur[k] - pb[j] * om_prev[k];
+            if (k > 0) t += pb[k] * om_cur[k - 1];
+            t /= bj1;
+            t += (t >= 0.0 ? eps1 : -eps1);
+            om_next[k] = t;
+            worst = std::max(worst, std::fabs(t));
+        }
+        om_next[j] = eps1;
+        om_next[j + 1] = 1.0;
+        const bool run = pro_second || worst > 1.4901161193847656e-08;  // sqrt(eps)
+        pro_second = run && !pro_second;
+        if (run)
+            for (int k = 0; k <= j; ++k) om_next[k] = eps1;
+        std::swap(om_prev, om_cur);
+        std::swap(om_cur, om_next);
+        return run;
+    };
     auto k2 = [&](int it) {
         hipLaunchKernelGGL(lz_update, dim3(ublocks), dim3(kThreads), 0, ctx->stream, cnt, Y.as<double>(), vcur(it),
                            vprev(it), P.as<double>(), wk.alpha() + it, wk.beta() + it, wk.partials.as<double>(),
                            nullptr, wk.beta() + it + 1, wk.invb() + it + 1);
         QMB_HIP(hipGetLastError());
         launch_finish(ctx, wk.partials.as<double>(), ublocks, wk.beta() + it + 1, wk.invb() + it + 1, 1);
-        if (reorth) {
+        const bool run = rmode == 1 || (rmode == 2 && pro_decide(it));
+        if (run) {
+            ++n_reorth;
             const int nv = it + 1;
             for (int pass = 0; pass < 2; ++pass) {
                 auto kd = cplx ? reorth_dots<true> : reorth_dots<false>;
@@ -460,6 +440,12 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
                                    nullptr, wk.beta() + it + 1, wk.invb() + it + 1);
                 QMB_HIP(hipGetLastError());
                 launch_finish(ctx, wk.partials.as<double>(), rblocks, wk.beta() + it + 1, wk.invb() + it + 1, 1);
+            }
+            if (rmode == 2) {
+                // the orthogonalised vector has a new norm b[it+1]
+                QMB_HIP(hipMemcpyAsync(&pb[it + 1], wk.beta() + it + 1, sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+                QMB_HIP(hipStreamSynchronize(ctx->stream));
             }
         }
     };
@@ -580,6 +566,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         rep->converged = converged ? 1 : 0;
         rep->breakdown = breakdown ? 1 : 0;
         rep->n_spmv = n_spmv;
+        rep->n_reorth = n_reorth;
         for (int j = 0; j < std::min(k_eigs, 8); ++j)
             rep->residual[j] = j < kk ? std::fabs(hb[m] * zl[j]) : std::nan("");
         rep->eig_residual = eig_res;
@@ -658,10 +645,10 @@ qmb_ctx *default_ctx() {
 void check_opts(const qmb_lanczos_opts &o) {
     QMB_REQUIRE(o.max_iter >= 1 && o.max_iter <= 100000, "qmb_lanczos: max_iter must be in [1, 100000]");
     QMB_REQUIRE(o.n_eigs >= 1 && o.n_eigs <= 8, "qmb_lanczos: n_eigs must be in [1, 8]");
-    QMB_REQUIRE(o.n_eigs == 1 || o.reorth, "qmb_lanczos: n_eigs > 1 needs reorth = 1");
+    QMB_REQUIRE(o.reorth >= 0 && o.reorth <= 2, "qmb_lanczos: reorth must be 0, 1 or 2");
+    QMB_REQUIRE(o.n_eigs == 1 || o.reorth, "qmb_lanczos: n_eigs > 1 needs reorth = 1 or 2");
     QMB_REQUIRE(o.start == QMB_START_RANDOM || o.start == QMB_START_USER || o.start == QMB_START_DEVICE,
                 "qmb_lanczos: bad start mode");
-    QMB_REQUIRE(!(o.profile && o.reorth), "qmb_lanczos: profile with reorth is not supported");
 }
 
 }  // namespace

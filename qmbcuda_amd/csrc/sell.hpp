@@ -1,0 +1,95 @@
+// SELL-64 row products shared by the single and the batched SpMV kernels.
+//
+// One lane owns one row of a 64-row chunk; the chunk is stored slot-major, so
+// slot k of the 64 rows is one coalesced 256 B (col) + 512 B / 1 KB (val)
+// read. Slots are taken four at a time: the four column and value loads are
+// issued together, then the four x gathers, then the four products are added
+// in slot order, so the sum has the same rounding sequence as a plain
+// slot-by-slot loop (and as the FP64 oracle's row sum).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qmb {
+
+// sum over slots k < len of val[p0 + 64k] * x[col[p0 + 64k]] (L = chunk slots)
+__device__ inline double sell_row_real(const int32_t *__restrict__ col, const double *__restrict__ val,
+                                       const double *__restrict__ x, int64_t p0, int L, int len) {
+    double acc = 0.0;
+    int k = 0;
+    for (; k + 4 <= L; k += 4) {
+        int32_t j[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k + u < len) {
+                const int64_t p = p0 + (int64_t)(k + u) * 64;
+                j[u] = __builtin_nontemporal_load(col + p);
+                v[u] = __builtin_nontemporal_load(val + p);
+            }
+        }
+        double g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < len) g[u] = x[j[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < len) acc += v[u] * g[u];
+    }
+    for (; k < L; ++k) {
+        if (k < len) {
+            const int64_t p = p0 + (int64_t)k * 64;
+            const int32_t jj = __builtin_nontemporal_load(col + p);
+            const double vv = __builtin_nontemporal_load(val + p);
+            acc += vv * x[jj];
+        }
+    }
+    return acc;
+}
+
+// complex rows: val and x are interleaved (re, im) pairs
+__device__ inline double2 sell_row_cplx(const int32_t *__restrict__ col, const double *__restrict__ val,
+                                        const double *__restrict__ x, int64_t p0, int L, int len) {
+    const double2 *val2 = reinterpret_cast<const double2 *>(val);
+    const double2 *x2 = reinterpret_cast<const double2 *>(x);
+    double ar = 0.0, ai = 0.0;
+    int k = 0;
+    for (; k + 4 <= L; k += 4) {
+        int32_t j[4];
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k + u < len) {
+                const int64_t p = p0 + (int64_t)(k + u) * 64;
+                j[u] = __builtin_nontemporal_load(col + p);
+                v[u] = val2[p];
+            }
+        }
+        double2 g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < len) g[u] = x2[j[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k + u < len) {
+                ar += v[u].x * g[u].x - v[u].y * g[u].y;
+                ai += v[u].x * g[u].y + v[u].y * g[u].x;
+            }
+        }
+    }
+    for (; k < L; ++k) {
+        if (k < len) {
+            const int64_t p = p0 + (int64_t)k * 64;
+            const int32_t jj = __builtin_nontemporal_load(col + p);
+            const double2 vv = val2[p];
+            const double2 gg = x2[jj];
+            ar += vv.x * gg.x - vv.y * gg.y;
+            ai += vv.x * gg.y + vv.y * gg.x;
+        }
+    }
+    return make_double2(ar, ai);
+}
+
+}  // namespace qmb

@@ -245,6 +245,26 @@ def test_lowest_four_with_reorthogonalisation(ctx):
         np.testing.assert_allclose(ev, dense, atol=1e-10 * max(1, abs(dense[0])))
 
 
+def test_partial_reorthogonalisation(ctx):
+    """reorth = 2 (estimated-loss driven) gives the same lowest eigenvalues as
+    dense diagonalisation while orthogonalising on only part of the steps."""
+    for label, case, n_up, a in (("chain14", chain_case(14, 1.0, 0.9, 0.3), 7, 3),
+                                 ("chain16", chain_case(16, 1.0, 1.0, 0.0), 8, 0)):
+        say(f"partial reorthogonalisation {label} irrep={a}")
+        orb = oracle_orbits(case, n_up)
+        ref = oracle_ell(case, orb, a)
+        dense = np.linalg.eigvalsh(ell_dense_nonnull(ref, orb, a))[:3]
+        _, _, b, e = gpu_objects(ctx, case, n_up, a, q.LAYOUT_COMPACT)
+        ev, rep, _, _ = q.lanczos(e, n_eigs=3, max_iter=400, tol=1e-13, reorth=2)
+        print(f"{label}: iterations {rep.iters}, reorthogonalised steps {rep.n_reorth}", flush=True)
+        assert rep.converged
+        assert rep.n_reorth < rep.iters
+        np.testing.assert_allclose(ev, dense, atol=1e-10 * max(1, abs(dense[0])))
+        ev1, rep1, _, _ = q.lanczos(e, n_eigs=3, max_iter=400, tol=1e-13, reorth=1)
+        assert rep1.n_reorth == rep1.iters
+        np.testing.assert_allclose(ev, ev1, atol=1e-10 * max(1, abs(dense[0])))
+
+
 def test_eigenvector_second_pass(ctx):
     case = chain_case(14, 1.0, 1.0, 0.2)
     orb = oracle_orbits(case, 7)
