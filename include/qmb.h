@@ -166,7 +166,9 @@ qmb_status qmb_spmv_benchmark(qmb_ctx *ctx, const qmb_ell *e, int reps, double *
  * Replaces LanczosEigVals_CUDA (Q/exact_diagonalization/lanczos.cu:128-317) and
  * LanczosEigenStateFromSeed (:31-104) with an FP64 device loop: the same
  * three-term recurrence, a convergence test on the Ritz residual, optional
- * full reorthogonalisation (needed for n_eigs > 1), host FP64 tridiagonal solve. */
+ * reorthogonalisation (needed for n_eigs > 1: classical Gram-Schmidt against
+ * the kept vectors, repeated once when the first pass removed more than 1-1/sqrt(2)
+ * of the norm), host FP64 tridiagonal solve. */
 /* start vector: seeded random (device generated), user host array, user device array */
 enum { QMB_START_RANDOM = 0, QMB_START_USER = 1, QMB_START_DEVICE = 2 };
 

@@ -10,8 +10,10 @@
 // families with their own kernel instantiations. A chunk of iterations is
 // captured once as a HIP graph and replayed; between replays the host solves
 // the tridiagonal problems and clears the active flag of converged instances
-// (their blocks return at once). With n_eigs > 1 the Krylov vectors are kept and
-// every new vector is reorthogonalised twice (classical Gram-Schmidt).
+// (their blocks return at once). With reorthogonalisation the Krylov vectors are
+// kept and every new vector is orthogonalised against them by classical
+// Gram-Schmidt, a second time only for the instances whose first pass removed
+// most of the norm (decided on the device, so the graph stays fixed).
 // SpMV tile shape (chunks per block) is picked per family by timing the
 // candidates on the actual batch (autotuner with a per-process cache).
 #include <algorithm>
@@ -37,7 +39,7 @@ struct BInst {
     double *P, *Y, *VB;          // w, H phi, Krylov basis [M][n*vs]
     double *alpha, *beta, *invb;  // [M], [M+1], [M+1]
     double *coef;                 // [M*2] reorthogonalisation coefficients
-    unsigned *counter;            // [4]
+    double *gate;                 // [1] second Gram-Schmidt pass wanted (1) or not (0)
     int spmv_first, spmv_nblk;
     int vec_first, vec_nblk;
 };
@@ -96,9 +98,15 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
     if (threadIdx.x == 0) part[blockIdx.x] = tb;
 }
 
+// Second Gram-Schmidt pass only when the first removed more than this share
+// of the norm (Daniel, Gragg, Kaufman, Stewart 1976).
+constexpr double kDgks = 0.7071067811865476;
+
 // One block per instance of the family: fixed-order sum of the instance's block
 // partials. mode 0: alpha[it] = s; 1: beta[it+1] = sqrt(s), invb[it+1];
-// 2 (start vector): invb[0] = 1/sqrt(s), beta[0] = 0.
+// 2 (start vector): invb[0] = 1/sqrt(s), beta[0] = 0; 3 (after the first
+// Gram-Schmidt pass): as 1, and gate = 1 if sqrt(s) < kDgks * the old
+// beta[it+1]; 4 (after the second pass): as 1 where the gate is 1.
 __global__ __launch_bounds__(kThreads) void bk_finish(const BInst *__restrict__ insts, const int *__restrict__ active,
                                                       const int *__restrict__ it_p, const double *__restrict__ part,
                                                       int first_inst, int mode) {
@@ -106,6 +114,7 @@ __global__ __launch_bounds__(kThreads) void bk_finish(const BInst *__restrict__ 
     const int inst = first_inst + blockIdx.x;
     if (active && !active[inst]) return;
     const BInst I = insts[inst];
+    if (mode == 4 && *I.gate == 0.0) return;
     const int it = it_p ? *it_p : 0;
     const int first = mode == 0 ? I.spmv_first : I.vec_first;
     const int nblk = mode == 0 ? I.spmv_nblk : I.vec_nblk;
@@ -113,8 +122,9 @@ __global__ __launch_bounds__(kThreads) void bk_finish(const BInst *__restrict__ 
     if (threadIdx.x == 0) {
         if (mode == 0) {
             I.alpha[it] = s;
-        } else if (mode == 1) {
+        } else if (mode == 1 || mode == 3 || mode == 4) {
             const double nb = sqrt(s);
+            if (mode == 3) *I.gate = nb < kDgks * I.beta[it + 1] ? 1.0 : 0.0;
             I.beta[it + 1] = nb;
             I.invb[it + 1] = nb > 0.0 ? 1.0 / nb : 0.0;
         } else {
@@ -153,11 +163,12 @@ __global__ __launch_bounds__(kThreads) void bk_update(const BInst *__restrict__ 
 template <bool CPLX>
 __global__ __launch_bounds__(kThreads) void bk_rdots(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
                                                      const int *__restrict__ active, const int *__restrict__ it_p,
-                                                     double *__restrict__ part2, int nbv) {
+                                                     double *__restrict__ part2, int nbv, int gated) {
     __shared__ double lds[kWaves];
     const BTask t = tasks[blockIdx.x];
     if (!active[t.inst]) return;
     const BInst I = insts[t.inst];
+    if (gated && *I.gate == 0.0) return;
     const int it = *it_p;
     const int vs = CPLX ? 2 : 1;
     const int64_t cnt = I.n_rows * vs;
@@ -185,11 +196,12 @@ __global__ __launch_bounds__(kThreads) void bk_rdots(const BInst *__restrict__ i
 // coef_j = sum of the instance's block partials, fixed order; one block per instance.
 __global__ __launch_bounds__(kThreads) void bk_rreduce(const BInst *__restrict__ insts, const int *__restrict__ active,
                                                        const int *__restrict__ it_p, const double *__restrict__ part2,
-                                                       int nbv, int first_inst) {
+                                                       int nbv, int first_inst, int gated) {
     __shared__ double lds[kWaves];
     const int inst = first_inst + blockIdx.x;
     if (!active[inst]) return;
     const BInst I = insts[inst];
+    if (gated && *I.gate == 0.0) return;
     const int it = *it_p;
     for (int j = 0; j <= it; ++j) {
         double sr = 0.0, si = 0.0;
@@ -209,11 +221,12 @@ __global__ __launch_bounds__(kThreads) void bk_rreduce(const BInst *__restrict__
 template <bool CPLX>
 __global__ __launch_bounds__(kThreads) void bk_rapply(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
                                                       const int *__restrict__ active, const int *__restrict__ it_p,
-                                                      double *__restrict__ part) {
+                                                      double *__restrict__ part, int gated) {
     __shared__ double lds[kWaves];
     const BTask t = tasks[blockIdx.x];
     if (!active[t.inst]) return;
     const BInst I = insts[t.inst];
+    if (gated && *I.gate == 0.0) return;
     const int it = *it_p;
     const int vs = CPLX ? 2 : 1;
     const int64_t cnt = I.n_rows * vs;
@@ -318,9 +331,9 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         bufs.reserve((size_t)n * 6);
         DevBuf scal((size_t)n * (3 * M + 2) * sizeof(double));
         DevBuf coefs((size_t)n * M * 2 * sizeof(double));
-        DevBuf counters((size_t)n * 4 * sizeof(unsigned));
+        DevBuf gates((size_t)n * sizeof(double));
         QMB_HIP(hipMemsetAsync(scal.p, 0, scal.bytes, ctx->stream));
-        QMB_HIP(hipMemsetAsync(counters.p, 0, counters.bytes, ctx->stream));
+        QMB_HIP(hipMemsetAsync(gates.p, 0, gates.bytes, ctx->stream));
         for (int pos = 0; pos < n; ++pos) {
             const qmb_ell *e = ells[order[pos]];
             const int vs = e->value_type == QMB_COMPLEX ? 2 : 1;
@@ -343,7 +356,7 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             I.beta = sc + M;
             I.invb = sc + 2 * M + 1;
             I.coef = coefs.as<double>() + (size_t)pos * M * 2;
-            I.counter = counters.as<unsigned>() + 4 * pos;
+            I.gate = gates.as<double>() + pos;
         }
         // vector task tables (ranges in doubles, both families in one table per family)
         int total_vec_blocks = 0;
@@ -502,15 +515,17 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                     for (int pass = 0; pass < 2; ++pass) {
                         hipLaunchKernelGGL(f.cplx ? bk_rdots<true> : bk_rdots<false>, dim3(nvb), dim3(kThreads), 0,
                                            s, d_inst.as<BInst>(), f.d_vec_tasks.as<BTask>(), d_active.as<int>(),
-                                           d_it.as<int>(), part2.as<double>(), (int)nvb);
+                                           d_it.as<int>(), part2.as<double>(), (int)nvb, pass);
                         hipLaunchKernelGGL(bk_rreduce, dim3((unsigned)f.members.size()), dim3(kThreads), 0, s,
                                            d_inst.as<BInst>(), d_active.as<int>(), d_it.as<int>(), part2.as<double>(),
-                                           (int)nvb, f.first_inst);
+                                           (int)nvb, f.first_inst, pass);
                         hipLaunchKernelGGL(f.cplx ? bk_rapply<true> : bk_rapply<false>, dim3(nvb), dim3(kThreads), 0,
                                            s, d_inst.as<BInst>(), f.d_vec_tasks.as<BTask>(), d_active.as<int>(),
-                                           d_it.as<int>(), part.as<double>());
+                                           d_it.as<int>(), part.as<double>(), pass);
+                        // pass 0 decides whether pass 1 runs, per instance, on the device
                         hipLaunchKernelGGL(bk_finish, dim3(nm), dim3(kThreads), 0, s, d_inst.as<BInst>(),
-                                           d_active.as<int>(), d_it.as<int>(), part.as<double>(), f.first_inst, 1);
+                                           d_active.as<int>(), d_it.as<int>(), part.as<double>(), f.first_inst,
+                                           pass ? 4 : 3);
                     }
                 }
             }

@@ -89,21 +89,34 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
     }
 }
 
+// Second Gram-Schmidt pass only when the first one removed more than this
+// share of the norm (Daniel, Gragg, Kaufman, Stewart, Math. Comp. 30 (1976)):
+// one pass is then already orthogonal to working precision.
+constexpr double kDgks = 0.7071067811865476;
+
+// gate: NULL, or a device flag; kernels of a gated second pass return at once
+// when it is 0.
+__device__ inline bool gated_off(const double *gate) { return gate && *gate == 0.0; }
+
 // Fixed-order sum of the block partials of the previous kernel (one block):
-// mode 0 -> out = s; mode 1 -> out = sqrt(s), out_inv = 1/sqrt(s) (0 if s = 0).
+// mode 0 -> out = s; mode 1 -> out = sqrt(s), out_inv = 1/sqrt(s) (0 if s = 0),
+// skipped when the gate is 0; mode 2 (after a first Gram-Schmidt pass) -> as
+// mode 1, and *gate = 1 if sqrt(s) < kDgks * (the previous *out), else 0.
 // A separate launch instead of a "last block" finish: on this chip a
 // device-scope fence in every block writes back and invalidates the L2 and
 // costs more than the SpMV's own traffic.
 __global__ __launch_bounds__(kThreads) void finish_sum(const double *__restrict__ partials, int n,
                                                        double *__restrict__ out, double *__restrict__ out_inv,
-                                                       int mode) {
+                                                       int mode, double *__restrict__ gate) {
     __shared__ double lds[kWaves];
+    if (mode == 1 && gated_off(gate)) return;
     const double s = sum_partials(partials, n, lds);
     if (threadIdx.x == 0) {
         if (mode == 0) {
             *out = s;
         } else {
             const double nb = sqrt(s);
+            if (mode == 2) *gate = nb < kDgks * *out ? 1.0 : 0.0;
             if (out) *out = nb;
             if (out_inv) *out_inv = nb > 0.0 ? 1.0 / nb : 0.0;
         }
@@ -116,8 +129,7 @@ __global__ __launch_bounds__(kThreads) void lz_update(int64_t cnt, const double 
                                                       const double *__restrict__ vprev, double *__restrict__ w,
                                                       const double *__restrict__ alpha_p,
                                                       const double *__restrict__ beta_p,
-                                                      double *__restrict__ partials, unsigned *__restrict__ counter,
-                                                      double *__restrict__ beta_next, double *__restrict__ invb_next) {
+                                                      double *__restrict__ partials) {
     __shared__ double lds[kWaves];
     const double a = *alpha_p, b = *beta_p;
     double acc = 0.0;
@@ -140,8 +152,7 @@ __global__ __launch_bounds__(kThreads) void lz_random(int64_t cnt, uint64_t seed
 // out_norm = ||a - e*b|| (b may be NULL -> ||a||); out_inv = 1/out_norm.
 __global__ __launch_bounds__(kThreads) void vec_norm(int64_t cnt, const double *__restrict__ a,
                                                      const double *__restrict__ b, double e,
-                                                     double *__restrict__ partials, unsigned *__restrict__ counter,
-                                                     double *__restrict__ out_norm, double *__restrict__ out_inv) {
+                                                     double *__restrict__ partials) {
     __shared__ double lds[kWaves];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
@@ -159,13 +170,14 @@ __global__ __launch_bounds__(kThreads) void vec_axpy(int64_t cnt, double c, cons
         x[i] = x[i] + c * v[i];
 }
 
-// ---- full reorthogonalisation (classical Gram-Schmidt, applied twice) -------
+// ---- reorthogonalisation (classical Gram-Schmidt, second pass on demand) ------
 // dots: coef[j] = <V_j, w> for j < nv; grid (blocks, nv).
 template <bool CPLX>
 __global__ __launch_bounds__(kThreads) void reorth_dots(int64_t n, int64_t stride, const double *__restrict__ V,
                                                         const double *__restrict__ w, double *__restrict__ part,
-                                                        int nb) {
+                                                        int nb, const double *__restrict__ gate) {
     __shared__ double lds[kWaves];
+    if (gated_off(gate)) return;
     const int j = blockIdx.y;
     const double *vj = V + (size_t)j * stride;
     double sr = 0.0, si = 0.0;
@@ -187,8 +199,9 @@ __global__ __launch_bounds__(kThreads) void reorth_dots(int64_t n, int64_t strid
 }
 
 __global__ __launch_bounds__(kThreads) void reorth_reduce(const double *__restrict__ part, int nb,
-                                                          double *__restrict__ coef) {
+                                                          double *__restrict__ coef, const double *__restrict__ gate) {
     __shared__ double lds[kWaves];
+    if (gated_off(gate)) return;
     const int j = blockIdx.x;
     double sr = 0.0, si = 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
@@ -203,15 +216,15 @@ __global__ __launch_bounds__(kThreads) void reorth_reduce(const double *__restri
     }
 }
 
-// w -= sum_j coef_j V_j; ||w|| -> beta_next, invb_next.
+// w -= sum_j coef_j V_j; block partials of ||w||^2.
 template <bool CPLX>
 __global__ __launch_bounds__(kThreads) void reorth_apply(int64_t n, int64_t stride, int nv,
                                                          const double *__restrict__ V,
                                                          const double *__restrict__ coef, double *__restrict__ w,
-                                                         double *__restrict__ partials, unsigned *__restrict__ counter,
-                                                         double *__restrict__ beta_next,
-                                                         double *__restrict__ invb_next) {
+                                                         double *__restrict__ partials,
+                                                         const double *__restrict__ gate) {
     __shared__ double lds[kWaves];
+    if (gated_off(gate)) return;
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (CPLX) {
@@ -254,7 +267,7 @@ static unsigned grid_for(int64_t work_items, int per_block) {
 }
 
 struct LzWork {
-    DevBuf partials, counter, scal;  // scal: alpha[M], beta[M+1], invb[M+1], misc[8]
+    DevBuf partials, scal;  // scal: alpha[M], beta[M+1], invb[M+1], misc[8]
     int M = 0;
     double *alpha() const { return scal.as<double>(); }
     double *beta() const { return scal.as<double>() + M; }
@@ -263,15 +276,15 @@ struct LzWork {
     void init(qmb_ctx *ctx, int max_iter) {
         M = max_iter;
         partials.alloc(kMaxBlocks * sizeof(double) * 2);
-        counter.alloc(sizeof(unsigned) * 4);
         scal.alloc((size_t)(3 * M + 2 + 8) * sizeof(double));
-        QMB_HIP(hipMemsetAsync(counter.p, 0, sizeof(unsigned) * 4, ctx->stream));
         QMB_HIP(hipMemsetAsync(scal.p, 0, (size_t)(3 * M + 2 + 8) * sizeof(double), ctx->stream));
     }
 };
 
-static void launch_finish(qmb_ctx *ctx, const double *partials, unsigned n, double *out, double *out_inv, int mode) {
-    hipLaunchKernelGGL(finish_sum, dim3(1), dim3(kThreads), 0, ctx->stream, partials, (int)n, out, out_inv, mode);
+static void launch_finish(qmb_ctx *ctx, const double *partials, unsigned n, double *out, double *out_inv, int mode,
+                          double *gate = nullptr) {
+    hipLaunchKernelGGL(finish_sum, dim3(1), dim3(kThreads), 0, ctx->stream, partials, (int)n, out, out_inv, mode,
+                       gate);
     QMB_HIP(hipGetLastError());
 }
 
@@ -292,7 +305,7 @@ static void launch_norm(qmb_ctx *ctx, int64_t cnt, const double *a, const double
                         double *out_norm, double *out_inv) {
     const unsigned blocks = grid_for(cnt, kThreads * 4);
     hipLaunchKernelGGL(vec_norm, dim3(blocks), dim3(kThreads), 0, ctx->stream, cnt, a, b, e,
-                       wk.partials.as<double>(), nullptr, out_norm, out_inv);
+                       wk.partials.as<double>());
     QMB_HIP(hipGetLastError());
     launch_finish(ctx, wk.partials.as<double>(), blocks, out_norm, out_inv, 1);
 }
@@ -418,28 +431,31 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     };
     auto k2 = [&](int it) {
         hipLaunchKernelGGL(lz_update, dim3(ublocks), dim3(kThreads), 0, ctx->stream, cnt, Y.as<double>(), vcur(it),
-                           vprev(it), P.as<double>(), wk.alpha() + it, wk.beta() + it, wk.partials.as<double>(),
-                           nullptr, wk.beta() + it + 1, wk.invb() + it + 1);
+                           vprev(it), P.as<double>(), wk.alpha() + it, wk.beta() + it, wk.partials.as<double>());
         QMB_HIP(hipGetLastError());
         launch_finish(ctx, wk.partials.as<double>(), ublocks, wk.beta() + it + 1, wk.invb() + it + 1, 1);
         const bool run = rmode == 1 || (rmode == 2 && pro_decide(it));
         if (run) {
             ++n_reorth;
             const int nv = it + 1;
+            // pass 0 always; pass 1 only where pass 0 cancelled most of the norm
+            // (the flag in misc[3] is set by pass 0's finish on the device)
+            double *gate = wk.misc() + 3;
             for (int pass = 0; pass < 2; ++pass) {
+                const double *g = pass ? gate : nullptr;
                 auto kd = cplx ? reorth_dots<true> : reorth_dots<false>;
                 hipLaunchKernelGGL(kd, dim3(rblocks, nv), dim3(kThreads), 0, ctx->stream, n, (int64_t)cnt,
-                                   VB.as<double>(), P.as<double>(), rpart.as<double>(), (int)rblocks);
+                                   VB.as<double>(), P.as<double>(), rpart.as<double>(), (int)rblocks, g);
                 QMB_HIP(hipGetLastError());
                 hipLaunchKernelGGL(reorth_reduce, dim3(nv), dim3(kThreads), 0, ctx->stream, rpart.as<double>(),
-                                   (int)rblocks, rcoef.as<double>());
+                                   (int)rblocks, rcoef.as<double>(), g);
                 QMB_HIP(hipGetLastError());
                 auto ka = cplx ? reorth_apply<true> : reorth_apply<false>;
                 hipLaunchKernelGGL(ka, dim3(rblocks), dim3(kThreads), 0, ctx->stream, n, (int64_t)cnt, nv,
-                                   VB.as<double>(), rcoef.as<double>(), P.as<double>(), wk.partials.as<double>(),
-                                   nullptr, wk.beta() + it + 1, wk.invb() + it + 1);
+                                   VB.as<double>(), rcoef.as<double>(), P.as<double>(), wk.partials.as<double>(), g);
                 QMB_HIP(hipGetLastError());
-                launch_finish(ctx, wk.partials.as<double>(), rblocks, wk.beta() + it + 1, wk.invb() + it + 1, 1);
+                launch_finish(ctx, wk.partials.as<double>(), rblocks, wk.beta() + it + 1, wk.invb() + it + 1,
+                              pass ? 1 : 2, gate);
             }
             if (rmode == 2) {
                 // the orthogonalised vector has a new norm b[it+1]
