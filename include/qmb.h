@@ -152,6 +152,10 @@ qmb_status qmb_ell_get_info(const qmb_ell *e, qmb_ell_info *info);
 /* Reference row-major layout (ind[n_rows*W], val[n_rows*W] or [n_rows*W*2] for
  * complex, track[n_rows]), exactly as the reference kernel leaves its buffers. */
 qmb_status qmb_ell_export_reference(const qmb_ell *e, uint32_t *ind, double *val, int16_t *track);
+/* Rows [row0, row0 + n_rows) in the same layout (ind[n_rows*W], ...): a shard of
+ * a matrix too large to export whole; column indices stay global. */
+qmb_status qmb_ell_export_rows(const qmb_ell *e, int64_t row0, int64_t n_rows, uint32_t *ind, double *val,
+                               int16_t *track);
 
 /* ---- SpMV -------------------------------------------------------------------
  * Replaces SparseMatDenseVec_prod_EEL (utils.cu:601-624). Vectors are n_rows

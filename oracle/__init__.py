@@ -267,6 +267,16 @@ class Ell:
         return r, x
 
 
+def spmv_rows(W, ind, val, track, x, is_complex=False):
+    """y = H x over a shard of rows given in the reference layout (global
+    column indices into the full-length x)."""
+    y = np.zeros(len(track) * (2 if is_complex else 1), np.float64)
+    lib().orc_spmv(len(track), W, np.ascontiguousarray(ind, np.uint32),
+                   np.ascontiguousarray(np.asarray(val).view(np.float64)), np.ascontiguousarray(track, np.int16),
+                   int(is_complex), np.ascontiguousarray(np.asarray(x).view(np.float64)), y)
+    return y.view(np.complex128) if is_complex else y
+
+
 def rank_colex_many(states):
     st = np.ascontiguousarray(states, np.uint64)
     out = np.zeros(len(st), np.int64)

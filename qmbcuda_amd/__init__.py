@@ -184,6 +184,15 @@ class Ell:
         _check(lib().qmb_ell_export_reference(self.h, _ptr(ind), _ptr(val), _ptr(track)))
         return ind, (val.view(np.complex128) if self.is_complex else val), track
 
+    def export_rows(self, row0, n_rows):
+        """Rows [row0, row0 + n_rows) in the reference layout (global column indices)."""
+        W = int(self.info().max_terms)
+        ind = np.zeros(n_rows * W, np.uint32)
+        val = np.zeros(n_rows * W * (2 if self.is_complex else 1), np.float64)
+        track = np.zeros(n_rows, np.int16)
+        _check(lib().qmb_ell_export_rows(self.h, int(row0), int(n_rows), _ptr(ind), _ptr(val), _ptr(track)))
+        return ind, (val.view(np.complex128) if self.is_complex else val), track
+
     def spmv(self, x):
         x = np.ascontiguousarray(x, np.complex128 if self.is_complex else np.float64)
         y = np.zeros_like(x)

@@ -337,38 +337,54 @@ qmb_status qmb_ell_get_info(const qmb_ell *e, qmb_ell_info *info) {
     });
 }
 
-qmb_status qmb_ell_export_reference(const qmb_ell *e, uint32_t *ind, double *val, int16_t *track) {
+qmb_status qmb_ell_export_rows(const qmb_ell *e, int64_t row0, int64_t n_rows, uint32_t *ind, double *val,
+                               int16_t *track) {
     return guarded([&] {
-        QMB_REQUIRE(e && ind && val && track, "qmb_ell_export_reference: NULL argument");
+        QMB_REQUIRE(e && (n_rows == 0 || (ind && val && track)), "qmb_ell_export_rows: NULL argument");
+        QMB_REQUIRE(row0 >= 0 && n_rows >= 0 && row0 + n_rows <= e->n_rows, "qmb_ell_export_rows: bad row range");
         QMB_HIP(hipSetDevice(e->ctx->device));
         QMB_HIP(hipStreamSynchronize(e->ctx->stream));
-        const int64_t n = e->n_rows;
         const int W = e->max_terms;
         const int vs = e->value_type == QMB_COMPLEX ? 2 : 1;
-        std::fill(ind, ind + (size_t)n * W, 0u);
-        std::fill(val, val + (size_t)n * W * vs, 0.0);
-        std::fill(track, track + n, (int16_t)0);
-        if (n == 0) return;
-        std::vector<int64_t> cp(e->n_chunks + 1);
-        std::vector<uint16_t> rl(n);
-        std::vector<int32_t> cl(e->stored);
-        std::vector<double> vl((size_t)e->stored * vs);
-        QMB_HIP(hipMemcpy(cp.data(), e->chunk_ptr.p, cp.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-        QMB_HIP(hipMemcpy(rl.data(), e->row_len.p, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
-        QMB_HIP(hipMemcpy(cl.data(), e->col.p, cl.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-        QMB_HIP(hipMemcpy(vl.data(), e->val.p, vl.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (int64_t r = 0; r < n; ++r) {
-            const int64_t c = r / kChunk, lane = r % kChunk;
-            ind[(size_t)r * W] = (uint32_t)r;
-            const int len = rl[r];
-            track[r] = (int16_t)len;
+        std::fill(ind, ind + (size_t)n_rows * W, 0u);
+        std::fill(val, val + (size_t)n_rows * W * vs, 0.0);
+        std::fill(track, track + n_rows, (int16_t)0);
+        if (n_rows == 0) return;
+        // the rows' chunks are one contiguous slice of the SELL arrays
+        const int64_t c0 = row0 / kChunk, c1 = (row0 + n_rows + kChunk - 1) / kChunk;
+        std::vector<int64_t> cp(c1 - c0 + 1);
+        std::vector<uint16_t> rl(n_rows);
+        QMB_HIP(hipMemcpy(cp.data(), e->chunk_ptr.as<int64_t>() + c0, cp.size() * sizeof(int64_t),
+                          hipMemcpyDeviceToHost));
+        QMB_HIP(hipMemcpy(rl.data(), e->row_len.as<uint16_t>() + row0, n_rows * sizeof(uint16_t),
+                          hipMemcpyDeviceToHost));
+        const int64_t base = cp[0], len_all = cp.back() - cp[0];
+        std::vector<int32_t> cl(len_all);
+        std::vector<double> vl((size_t)len_all * vs);
+        QMB_HIP(hipMemcpy(cl.data(), e->col.as<int32_t>() + base, cl.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        QMB_HIP(hipMemcpy(vl.data(), e->val.as<double>() + (size_t)base * vs, vl.size() * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n_rows; ++i) {
+            const int64_t r = row0 + i;
+            const int64_t c = r / kChunk - c0, lane = r % kChunk;
+            ind[(size_t)i * W] = (uint32_t)r;
+            const int len = rl[i];
+            track[i] = (int16_t)len;
             for (int s = 0; s < len; ++s) {
-                const int64_t pos = cp[c] + (int64_t)s * kChunk + lane;
-                ind[(size_t)r * W + s] = (uint32_t)cl[pos];
-                for (int q = 0; q < vs; ++q) val[((size_t)r * W + s) * vs + q] = vl[(size_t)pos * vs + q];
+                const int64_t pos = cp[c] - base + (int64_t)s * kChunk + lane;
+                ind[(size_t)i * W + s] = (uint32_t)cl[pos];
+                for (int q = 0; q < vs; ++q) val[((size_t)i * W + s) * vs + q] = vl[(size_t)pos * vs + q];
             }
         }
     });
+}
+
+qmb_status qmb_ell_export_reference(const qmb_ell *e, uint32_t *ind, double *val, int16_t *track) {
+    if (!e) {
+        qmb::set_error("qmb_ell_export_reference: NULL argument");
+        return QMB_ERR_INVALID;
+    }
+    return qmb_ell_export_rows(e, 0, e->n_rows, ind, val, track);
 }
 
 }  // extern "C"

@@ -110,6 +110,21 @@ def test_compact_layout_is_reference_without_null_rows(ctx):
         assert np.array_equal(E, E_ref)
 
 
+def test_export_rows_is_a_slice_of_the_full_export(ctx):
+    case = chain_case(16, 1.0, 0.8, 0.3)
+    say("export_rows chain16")
+    _, _, b, e = gpu_objects(ctx, case, 8, 3, q.LAYOUT_COMPACT)
+    ind, val, track = e.export_reference()
+    W = e.info().max_terms
+    for row0, k in ((0, 64), (37, 100), (b.n_rows - 5, 5), (b.n_rows, 0)):
+        i2, v2, t2 = e.export_rows(row0, k)
+        assert np.array_equal(t2, track[row0:row0 + k])
+        assert np.array_equal(i2, ind[row0 * W:(row0 + k) * W])
+        assert np.array_equal(v2, val[row0 * W:(row0 + k) * W])
+    with pytest.raises(q.QMBError):
+        e.export_rows(b.n_rows - 3, 4)
+
+
 def test_max_terms_overflow_is_an_error(ctx):
     case = chain_case(10)
     say("overflow chain10")
