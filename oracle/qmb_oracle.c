@@ -152,20 +152,7 @@ static int cmp_i64(const void *a, const void *b) {
     return (x > y) - (x < y);
 }
 
-/* Reference-form orbit construction, restating BuildSRStates
- * (Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:341-400):
- *   SetUpSRS (utils.cu:39-64)        SRS[i*G+g] = rank(image of state i under g)
- *   SolveSRS_min (utils.cu:750-762)  representative rank = min over g
- *   thrust sort/unique (…cu:367-370) sorted unique representatives
- *   SetUpSRS_from_indices (utils.cu:716-745)  orbit table of each representative
- *   ComputeSRSNorms (utils.cu:767-805)        norm[o*NIr+a] = sqrt|sum_{i,j: s_i=s_j}
- *                                             conj(chi_a(i)) chi_a(j)| / G
- *   ComputeOrbitIndices (utils.cu:66-80)      rank -> (orbit, g)
- * The reference's ComputeOrbitIndices lets all G threads of an orbit race; the
- * restatement keeps the smallest g (deterministic; any g of the coset gives the
- * same matrix element on compatible orbits).
- * Also returns the stabiliser size m of every orbit (the number of g whose image
- * is the representative itself). Returns n_orb, or -1 on bad input. */
+/* Colex ranks of many states (GetIndexInHilbertSubspace, utils.cu:9-36, per state). */
 void orc_rank_colex_many(const uint64_t *states, int64_t n, int64_t *ranks) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) ranks[i] = orc_rank_colex(states[i]);
@@ -245,6 +232,20 @@ int orc_orbit_tables(int n_sites, int G, const uint32_t *perm, const uint8_t *fl
     return 0;
 }
 
+/* Reference-form orbit construction, restating BuildSRStates
+ * (Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:341-400):
+ *   SetUpSRS (utils.cu:39-64)        SRS[i*G+g] = rank(image of state i under g)
+ *   SolveSRS_min (utils.cu:750-762)  representative rank = min over g
+ *   thrust sort/unique (…cu:367-370) sorted unique representatives
+ *   SetUpSRS_from_indices (utils.cu:716-745)  orbit table of each representative
+ *   ComputeSRSNorms (utils.cu:767-805)        norm[o*NIr+a] = sqrt|sum_{i,j: s_i=s_j}
+ *                                             conj(chi_a(i)) chi_a(j)| / G
+ *   ComputeOrbitIndices (utils.cu:66-80)      rank -> (orbit, g)
+ * The reference's ComputeOrbitIndices lets all G threads of an orbit race; the
+ * restatement keeps the smallest g (deterministic; any g of the coset gives the
+ * same matrix element on compatible orbits).
+ * Also returns the stabiliser size m of every orbit (the number of g whose image
+ * is the representative itself). Returns n_orb, or -1 on bad input. */
 int64_t orc_build_orbits(int n_sites, int n_up, int G, const uint32_t *perm,
                          const uint8_t *flip, int n_irreps, const double *chars,
                          const uint64_t *bas, int64_t n_bas, int64_t *rep_rank,
