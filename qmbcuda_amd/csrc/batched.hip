@@ -22,6 +22,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <tuple>
 
 #include "internal.hpp"
@@ -287,7 +288,23 @@ struct Family {
     int cpb = 4;
 };
 
+// (complex, log2 rows, instances) -> chunks per block; shared by every context
+// of the process, hence the lock
 static std::map<std::tuple<int, int, int>, int> g_tile_cache;
+static std::mutex g_tile_mutex;
+
+static bool tile_cache_get(const std::tuple<int, int, int> &key, int &cpb) {
+    std::lock_guard<std::mutex> lock(g_tile_mutex);
+    auto it = g_tile_cache.find(key);
+    if (it == g_tile_cache.end()) return false;
+    cpb = it->second;
+    return true;
+}
+
+static void tile_cache_put(const std::tuple<int, int, int> &key, int cpb) {
+    std::lock_guard<std::mutex> lock(g_tile_mutex);
+    g_tile_cache[key] = cpb;
+}
 
 }  // namespace qmb
 
@@ -441,11 +458,7 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             int64_t rows = 0;
             for (int i : f.members) rows += ells[i]->n_rows;
             const auto key = std::make_tuple((int)f.cplx, (int)std::log2((double)rows + 1.0), (int)f.members.size());
-            auto it = g_tile_cache.find(key);
-            if (it != g_tile_cache.end()) {
-                f.cpb = it->second;
-                continue;
-            }
+            if (tile_cache_get(key, f.cpb)) continue;
             float best = 1e30f;
             int best_cpb = 4;
             hipEvent_t ea, eb;
@@ -479,7 +492,7 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             (void)hipEventDestroy(ea);
             (void)hipEventDestroy(eb);
             f.cpb = best_cpb;
-            g_tile_cache[key] = best_cpb;
+            tile_cache_put(key, best_cpb);
         }
         // final SpMV task tables; spmv block ids are family-local too
         for (auto &f : fam) {
