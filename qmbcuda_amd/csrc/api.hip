@@ -250,6 +250,55 @@ static bool detect_torus(qmb_group &g) {
     return false;
 }
 
+// Orbit-minimum tables for a group that is the whole product of block shifts
+// a < nb, in-block shifts s < B (and, with flip, both halves), B <= 8 and
+// nb <= 8: the top block of an image is one source block rotated (and maybe
+// complemented), so the minimum's top block and the elements that can reach it
+// follow from per-block-value tables instead of all G images.
+//   tab[v]        = min over s of rot_B(v, s)          (v < 2^B)
+//   tab[256 + v]  = bit s set when rot_B(v, s) equals it
+//   tab[512 + (f*8 + a)*8 + s] = element index of (flip f, block shift a, s)
+static void build_lut(qmb_group &g) {
+    g.lut = 0;
+    g.lut_flip = 0;
+    g.lut_tab.clear();
+    if (!g.torus) return;
+    const int N = g.n_sites, B = g.block, nb = N / B;
+    if (B > 8 || nb > 8) return;
+    int hf = 0;
+    for (int e = 0; e < g.G; ++e) hf |= g.flip[e] ? 1 : 0;
+    std::vector<int> gid(2 * 8 * 8, -1);
+    for (int e = 0; e < g.G; ++e) {
+        const int a = g.big[e] / B, s = g.small[e], f = g.flip[e] ? 1 : 0;
+        const int i = (f * 8 + a) * 8 + s;
+        if (gid[i] >= 0) return;
+        gid[i] = e;
+    }
+    for (int f = 0; f <= hf; ++f)
+        for (int a = 0; a < nb; ++a)
+            for (int s = 0; s < B; ++s)
+                if (gid[(f * 8 + a) * 8 + s] < 0) return;
+    g.lut_tab.assign(kLutBytes, 0);
+    const unsigned mB = (1u << B) - 1u;
+    for (unsigned v = 0; v <= mB; ++v) {
+        unsigned best = 0xFF, mask = 0;
+        for (int s = 0; s < B; ++s) {
+            const unsigned r = s ? (((v << s) | (v >> (B - s))) & mB) : v;
+            if (r < best) {
+                best = r;
+                mask = 1u << s;
+            } else if (r == best) {
+                mask |= 1u << s;
+            }
+        }
+        g.lut_tab[v] = (uint8_t)best;
+        g.lut_tab[256 + v] = (uint8_t)mask;
+    }
+    for (int i = 0; i < 2 * 8 * 8; ++i) g.lut_tab[512 + i] = (uint8_t)(gid[i] < 0 ? 0 : gid[i]);
+    g.lut = 1;
+    g.lut_flip = hf;
+}
+
 }  // namespace qmb
 
 using namespace qmb;
@@ -350,6 +399,7 @@ qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el, 
             QMB_REQUIRE(g->inv[e] >= 0, "qmb_group_create: element list is not closed under inversion");
         }
         detect_torus(*g);
+        build_lut(*g);
         *out = g.release();
     });
 }

@@ -33,6 +33,9 @@ qmb::GroupView qmb_basis::view() const {
     v.flip = g_flip.as<uint8_t>();
     v.perm = g_perm.as<uint8_t>();
     v.inv = g_inv.as<uint8_t>();
+    v.lut = group->lut;
+    v.lut_flip = group->lut_flip;
+    v.lut_tab = group->lut ? g_lut.as<uint8_t>() : nullptr;
     return v;
 }
 
@@ -169,6 +172,10 @@ static void upload_group(qmb_basis *b, const qmb_group *g) {
     QMB_HIP(hipMemcpy(b->g_flip.p, g->flip.data(), G, hipMemcpyHostToDevice));
     QMB_HIP(hipMemcpy(b->g_inv.p, inv8.data(), G, hipMemcpyHostToDevice));
     QMB_HIP(hipMemcpy(b->g_perm.p, perm8.data(), perm8.size(), hipMemcpyHostToDevice));
+    if (g->lut) {
+        b->g_lut.alloc(kLutBytes);
+        QMB_HIP(hipMemcpy(b->g_lut.p, g->lut_tab.data(), kLutBytes, hipMemcpyHostToDevice));
+    }
 }
 
 static uint64_t binom_host(int n, int k) {

@@ -67,12 +67,15 @@ __device__ inline S apply_elem(S x, int g, const GroupView &gv, const uint8_t *p
     return y;
 }
 
-// LDS copy of the per-element tables for lane-divergent element indices.
+// LDS copy of the per-element tables for lane-divergent element indices, and
+// (when the group has them) the torus orbit-minimum tables.
 struct GroupLds {
     uint64_t keep[kMaxGroup];
     uint8_t small[kMaxGroup];
     uint8_t big[kMaxGroup];
     uint8_t flip[kMaxGroup];
+    uint8_t inv[kMaxGroup];
+    uint8_t lut[kLutBytes];
 };
 
 __device__ inline void stage_group(GroupLds &t, const GroupView &g) {
@@ -81,7 +84,10 @@ __device__ inline void stage_group(GroupLds &t, const GroupView &g) {
         t.small[i] = g.small[i];
         t.big[i] = g.big[i];
         t.flip[i] = g.flip[i];
+        t.inv[i] = g.inv[i];
     }
+    if (g.lut)
+        for (int i = threadIdx.x; i < kLutBytes; i += blockDim.x) t.lut[i] = g.lut_tab[i];
 }
 
 // apply_elem with the element index varying across lanes (tables from LDS).
@@ -186,6 +192,49 @@ __device__ inline S orbit_min(S x, const GroupView &gv, const uint8_t *perm_lds,
             bi = ig;
         } else if (y == best && ig < bi) {
             bi = ig;
+        }
+    }
+    gm = bi;
+    return best;
+}
+
+// orbit_min for a group that is the whole torus product (gv.lut): an image's
+// top block is source block j rotated by the in-block shift s (complemented
+// under flip), placed on top by the block shift a = nb-1-j. Integer order is
+// decided by the top block first, so the minimum image has the smallest top
+// block tmin over all (j, s, f), and only the elements reaching tmin need full
+// images. Same result and same gm as orbit_min.
+template <class S>
+__device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, int &gm) {
+    const int B = gv.block, nb = gv.n_sites / B;
+    const unsigned mB = (1u << B) - 1u;
+    const int nf = gv.lut_flip ? 2 : 1;
+    unsigned tmin = 0xFFu;
+    for (int j = 0; j < nb; ++j) {
+        const unsigned v = (unsigned)(x >> (j * B)) & mB;
+        tmin = min(tmin, (unsigned)t.lut[v]);
+        if (nf == 2) tmin = min(tmin, (unsigned)t.lut[v ^ mB]);
+    }
+    S best = 0;
+    int bi = -1;
+    for (int j = 0; j < nb; ++j) {
+        const unsigned v0 = (unsigned)(x >> (j * B)) & mB;
+        const int a = nb - 1 - j;
+        for (int f = 0; f < nf; ++f) {
+            const unsigned v = f ? (v0 ^ mB) : v0;
+            if (t.lut[v] != tmin) continue;
+            unsigned sm = t.lut[256 + v];
+            while (sm) {
+                const int sh = __builtin_ctz(sm);
+                sm &= sm - 1;
+                const int g = t.lut[512 + (f * 8 + a) * 8 + sh];
+                const S y = apply_elem_div<S>(x, g, gv, t, nullptr);
+                const int ig = t.inv[g];
+                if (bi < 0 || y < best || (y == best && ig < bi)) {
+                    best = y;
+                    bi = ig;
+                }
+            }
         }
     }
     gm = bi;

@@ -68,7 +68,9 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
     __shared__ uint64_t bin[64 * kBinJ];
     __shared__ double2 s_chi[kMaxGroup];
     __shared__ double s_nrm[kMaxGroup + 1];
+    __shared__ GroupLds gt;
     stage_perm(perm_lds, gv);
+    stage_group(gt, gv);
     stage_binom(bin, binom);
     for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
     for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                 S tgt;
                 apply_term<S>(tv, h, src, tgt, cr, ci);
                 int gm;
-                const S rep = orbit_min<S>(tgt, gv, perm_lds, gm);
+                const S rep = gv.lut ? orbit_min_lut<S>(tgt, gv, gt, gm) : orbit_min<S>(tgt, gv, perm_lds, gm);
                 const int64_t r2 = row_of_rep<S>(rep, bin, bitmap, prefix);
                 if (r2 < 0) continue;
                 if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;

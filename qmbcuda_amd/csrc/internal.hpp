@@ -101,7 +101,12 @@ struct GroupView {
     const uint8_t *flip;   // [G]
     const uint8_t *perm;   // [G*n_sites] generic destination table
     const uint8_t *inv;    // [G]
+    int lut;               // 1: the group is every (block shift, in-block shift[, flip]) of a torus
+    int lut_flip;          //    with a spin-flip half
+    const uint8_t *lut_tab;  // [kLutBytes] rotation minima and element index table (see device_common.hpp)
 };
+
+constexpr int kLutBytes = 256 + 256 + 2 * 8 * 8;
 
 // Handles are reference counted: a child (basis, ELL, network) holds its
 // parents (context, group, basis), so the C ABI's destroy calls may come in any
@@ -163,6 +168,8 @@ struct qmb_group : qmb::Counted {
     int torus = 0, block = 0;
     std::vector<uint64_t> keep;
     std::vector<uint8_t> small, big;
+    int lut = 0, lut_flip = 0;     // full torus product with block width <= 8
+    std::vector<uint8_t> lut_tab;  // [kLutBytes]
 };
 
 struct qmb_terms {
@@ -182,7 +189,7 @@ struct qmb_basis : qmb::Counted {
     qmb::DevBuf stab;    // uint8 [n_rows]: stabiliser size
     qmb::DevBuf nullf;   // uint8 [n_rows]: 1 if the orbit is null for the irrep
     // device copies of the group tables
-    qmb::DevBuf g_keep, g_small, g_big, g_flip, g_perm, g_inv;
+    qmb::DevBuf g_keep, g_small, g_big, g_flip, g_perm, g_inv, g_lut;
     qmb::GroupView view() const;
     uint64_t triv_lo = 0, triv_hi = 0;  // elements with character 1 for the irrep
     bool all_rows = false;              // every element has character 1: no null orbits
