@@ -288,8 +288,10 @@ static void launch_finish(qmb_ctx *ctx, const double *partials, unsigned n, doub
     QMB_HIP(hipGetLastError());
 }
 
+// after (optional): an event recorded between the SpMV and the a[n] finish, so
+// that profiling brackets the SpMV kernel alone.
 void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, bool lz, const double *invb,
-                 double *vout, LzWork *wk, double *alpha_out) {
+                 double *vout, LzWork *wk, double *alpha_out, hipEvent_t after = nullptr) {
     const unsigned blocks = grid_for(e->n_chunks, kWaves);
     const bool cplx = e->value_type == QMB_COMPLEX;
     auto k = cplx ? (lz ? spmv_sell<true, true> : spmv_sell<true, false>)
@@ -298,6 +300,7 @@ void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, boo
                        e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
                        e->val.as<double>(), x, y, invb, vout, wk ? wk->partials.as<double>() : nullptr);
     QMB_HIP(hipGetLastError());
+    if (after) QMB_HIP(hipEventRecord(after, ctx->stream));
     if (lz) launch_finish(ctx, wk->partials.as<double>(), blocks, alpha_out, nullptr, 0);
 }
 
@@ -380,8 +383,9 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         launch_norm(ctx, cnt, P.as<double>(), nullptr, 0.0, wk, wk.misc(), wk.invb());
     };
     const unsigned ublocks = grid_for(cnt, kThreads * 4);
-    auto k1 = [&](int it) {
-        launch_spmv(ctx, e, P.as<double>(), Y.as<double>(), true, wk.invb() + it, vcur(it), &wk, wk.alpha() + it);
+    auto k1 = [&](int it, hipEvent_t after) {
+        launch_spmv(ctx, e, P.as<double>(), Y.as<double>(), true, wk.invb() + it, vcur(it), &wk, wk.alpha() + it,
+                    after);
     };
     // Partial reorthogonalisation (H. D. Simon, Math. Comp. 42 (1984) 115-142):
     // om_cur[k] estimates <phi_j, phi_k> through the recurrence the Lanczos
@@ -486,8 +490,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         const int win1 = std::min(M, it + check);
         for (; it < win1; ++it) {
             if (o.profile) QMB_HIP(hipEventRecord(ev[2 * (it - win0)], ctx->stream));
-            k1(it);
-            if (o.profile) QMB_HIP(hipEventRecord(ev[2 * (it - win0) + 1], ctx->stream));
+            k1(it, o.profile ? ev[2 * (it - win0) + 1] : nullptr);
             k2(it);
             ++n_spmv;
         }
@@ -560,7 +563,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             QMB_HIP(hipMemsetAsync(Z.p, 0, cnt * sizeof(double), ctx->stream));
             init_start();
             for (int j = 0; j < m; ++j) {
-                k1(j);
+                k1(j, nullptr);
                 hipLaunchKernelGGL(vec_axpy, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt,
                                    z0[j], vcur(j), X.as<double>());
                 QMB_HIP(hipGetLastError());

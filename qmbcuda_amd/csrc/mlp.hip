@@ -417,12 +417,16 @@ __global__ __launch_bounds__(256) void mlp_state_grad(int64_t n, int n_sites, in
 }
 
 // dout[row][o] for the sample's 256 rows = sum over blocks of the histograms / 2^36.
+// Grid (512 / 64): a wavefront per bin, lanes striding over the block
+// histograms; integer sums, so the result does not depend on the order.
 __global__ __launch_bounds__(256) void mlp_finish_hist(const unsigned long long *__restrict__ hist, int nb,
                                                        float *__restrict__ dout_rows, float scale) {
-    for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    for (int i = blockIdx.x * kWaves + (threadIdx.x >> 6); i < 512; i += gridDim.x * kWaves) {
         long long s = 0;
-        for (int b = 0; b < nb; ++b) s += (long long)hist[(size_t)b * 512 + i];
-        dout_rows[i] = (float)((double)s / kFix * scale);
+        for (int b = lane; b < nb; b += 64) s += (long long)hist[(size_t)b * 512 + i];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) dout_rows[i] = (float)((double)s / kFix * scale);
     }
 }
 
@@ -625,8 +629,8 @@ void sample_loss(qmb_ctx *ctx, const MlpSample &s, const float *tab, StateWork &
                            w.beta.as<double>(), s.u.as<double>(), w.amax.as<double>(), w.stats.as<double>(),
                            w.hist.as<unsigned long long>());
         QMB_HIP(hipGetLastError());
-        hipLaunchKernelGGL(mlp_finish_hist, dim3(1), dim3(256), 0, ctx->stream, w.hist.as<unsigned long long>(), w.nb,
-                           dout_rows, scale);
+        hipLaunchKernelGGL(mlp_finish_hist, dim3(512 / kWaves), dim3(256), 0, ctx->stream,
+                           w.hist.as<unsigned long long>(), w.nb, dout_rows, scale);
         QMB_HIP(hipGetLastError());
     }
 }
