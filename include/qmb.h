@@ -242,13 +242,14 @@ qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
 
 /* ---- learned start vector (new; no reference counterpart) -------------------
  * A translation-invariant network over 8-spin windows (+-1 spins, Delta, J2/J1
- * -> tanh 128 -> tanh 128 -> log-amplitude, phase), log psi(s) = sum over the N
- * windows of an N-site ring, plus i*pi*(up spins on even sites) (the Marshall
+ * -> tanh 128 -> tanh 128 -> log-amplitude, phase), log psi(s) = sum over the
+ * windows of the lattice (the N windows of an N-site ring, or the windows set
+ * by qmb_mlp_set_windows), plus i*pi*(up spins on sublattice A) (the Marshall
  * sign rule as a fixed prior: the network learns the phase relative to it).
  * Bf16 weights and activations on MFMA, FP32 accumulation, FP32 Adam master
  * weights. Parameter layout: W1[128][10], b1[128],
- * W2[128][128], b2[128], W3[2][128], b3[2]. Samples are k=0 ring bases (compact
- * layout, plain translation group) with their ED ground states; the loss is the
+ * W2[128][128], b2[128], W3[2][128], b3[2]. Samples are fully symmetric bases
+ * (k = 0, flip-even; compact layout) with their ED ground states; the loss is the
  * mean infidelity 1 - |<u, c>|^2/(<c,c><u,u>), c_r = psi(r) sqrt(G/m_r). */
 typedef struct qmb_mlp qmb_mlp;
 
@@ -259,6 +260,14 @@ qmb_status qmb_mlp_get_params(const qmb_mlp *m, float *theta);
 qmb_status qmb_mlp_set_params(qmb_mlp *m, const float *theta);
 /* Network outputs for all 256 window patterns of n couplings: out[(i*256 + p)*2 + {a, phi}]. */
 qmb_status qmb_mlp_forward(qmb_mlp *m, int n, const double *delta, const double *j2, float *out);
+/* Windows of a basis that is not a plain ring (e.g. a 2-D torus): n_win windows
+ * of 8 site indices (sites[w*8 + k] = site of bit k of window w), and the
+ * sublattice-A mask of the Marshall sign prior. A plain ring (translation
+ * group G = N) defaults to the N windows q..q+7 (mod N), sublattice A = even
+ * sites. The basis must be the fully symmetric sector (k = 0, flip-even),
+ * compact layout; call before adding its samples. */
+qmb_status qmb_mlp_set_windows(qmb_mlp *m, const qmb_basis *b, int n_win, const uint8_t *sites,
+                               uint64_t sublattice_a);
 /* The network keeps the basis alive while the sample is held. */
 qmb_status qmb_mlp_add_sample(qmb_mlp *m, const qmb_basis *b, const double *ground_state, double delta, double j2);
 qmb_status qmb_mlp_clear_samples(qmb_mlp *m);
@@ -266,7 +275,7 @@ qmb_status qmb_mlp_clear_samples(qmb_mlp *m);
 qmb_status qmb_mlp_loss_grad(qmb_mlp *m, double *loss, float *grad);
 /* Full-batch Adam steps; loss_history[steps] optional (loss before each step). */
 qmb_status qmb_mlp_train(qmb_mlp *m, int steps, double lr, double *loss_history);
-/* Real start vector x_r = |c_r| cos(beta_r - beta_ref) on a k=0 ring basis
+/* Real start vector x_r = |c_r| cos(beta_r - beta_ref) on a fully symmetric basis
  * (host, n_rows). *fallback = 1 when it is not finite or zero: use the random start. */
 qmb_status qmb_mlp_start_vector(qmb_mlp *m, const qmb_basis *b, double delta, double j2, double *x,
                                 int *fallback);

@@ -3,9 +3,10 @@
 The network (no reference counterpart): input = 8 spins of a window (+-1,
 spin up = +1) plus Delta and J2/J1; two tanh hidden layers of 128; two
 outputs (log-amplitude a, phase phi). For a basis state s of an N-site ring,
-log psi(s) = sum_{t<N} a(w_t(s)) + i phi(w_t(s)) + i pi M(s), w_t(s) = spins
-t..t+7 (mod N), M(s) = number of up spins on even sites (Marshall's sign rule
-for a bipartite ring, a fixed prior; the network learns the phase relative
+log psi(s) = sum_t a(w_t(s)) + i phi(w_t(s)) + i pi M(s), w_t(s) = the spins
+of window t (a ring: sites t..t+7 mod N; a 2-D lattice: the windows it
+registers), M(s) = number of up spins on sublattice A (Marshall's sign rule
+for a bipartite lattice, a fixed prior; the network learns the phase relative
 to it).
 The k=0 symmetric-basis coefficient of representative r is
 c_r = psi(r) * sqrt(G/m_r). Training loss per ground state u (real, unit):
@@ -81,24 +82,26 @@ def backward(theta, cache, dout, round_like_gpu=True):
     return pack(g)
 
 
-def window_patterns(reps, n_sites):
-    """patterns[r, t] = bits t..t+7 (mod N) of representative r."""
-    reps = np.asarray(reps, np.uint64).astype(object)
-    out = np.zeros((len(reps), n_sites), np.int64)
-    mask = (1 << n_sites) - 1
-    for i, s in enumerate(reps):
-        s = int(s)
-        ss = s | (s << n_sites)
-        for t in range(n_sites):
-            out[i, t] = (ss >> t) & 0xFF
-        assert s & ~mask == 0
+def window_patterns(reps, n_sites, windows=None):
+    """patterns[r, t] = pattern of window t of representative r: bit k = spin at
+    site windows[t][k]; without windows, the ring windows t..t+7 (mod N)."""
+    if windows is None:
+        windows = [[(t + k) % n_sites for k in range(WIN)] for t in range(n_sites)]
+    windows = np.asarray(windows, np.int64)
+    reps = np.asarray(reps, np.uint64)
+    out = np.zeros((len(reps), len(windows)), np.int64)
+    for k in range(WIN):
+        out |= ((reps[:, None] >> windows[None, :, k].astype(np.uint64)) & np.uint64(1)).astype(np.int64) << k
     return out
 
 
-def marshall_phase(reps, n_sites):
-    """pi * (up spins on the even sites) of each representative."""
-    even = sum(1 << j for j in range(0, n_sites, 2))
-    return np.array([np.pi * bin(int(r) & even).count("1") for r in np.asarray(reps, np.uint64)], np.float64)
+def marshall_phase(reps, n_sites, sublattice_a=None):
+    """pi * (up spins on sublattice A) of each representative; A defaults to the
+    even sites of a ring."""
+    if sublattice_a is None:
+        sublattice_a = sum(1 << j for j in range(0, n_sites, 2))
+    return np.array([np.pi * bin(int(r) & sublattice_a).count("1") for r in np.asarray(reps, np.uint64)],
+                    np.float64)
 
 
 def log_psi(theta, pats, prior, delta, j2, round_like_gpu=True):

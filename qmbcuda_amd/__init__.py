@@ -308,6 +308,14 @@ class Mlp:
         _check(lib().qmb_mlp_train(self.h, steps, lr, _ptr(hist)))
         return hist
 
+    def set_windows(self, basis: Basis, windows, sublattice_a: int):
+        """Register the 8-site windows (array [n_win, 8]) and the Marshall
+        sublattice mask of a basis that is not a plain ring."""
+        w = np.ascontiguousarray(windows, np.uint8)
+        assert w.ndim == 2 and w.shape[1] == 8
+        self._keep_windows = getattr(self, "_keep_windows", []) + [basis]
+        _check(lib().qmb_mlp_set_windows(self.h, basis.h, int(w.shape[0]), _ptr(w), int(sublattice_a)))
+
     def start_vector(self, basis: Basis, delta: float, j2: float):
         x = np.zeros(basis.n_rows, np.float64)
         fb = C.c_int(0)

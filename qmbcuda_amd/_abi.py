@@ -126,6 +126,7 @@ SIGNATURES = [
     ("qmb_mlp_clear_samples", C.c_int, [_vp]),
     ("qmb_mlp_loss_grad", C.c_int, [_vp, C.POINTER(C.c_double), _vp]),
     ("qmb_mlp_train", C.c_int, [_vp, C.c_int, C.c_double, _vp]),
+    ("qmb_mlp_set_windows", C.c_int, [_vp, _vp, C.c_int, _vp, C.c_uint64]),
     ("qmb_mlp_start_vector", C.c_int, [_vp, _vp, C.c_double, C.c_double, _vp, C.POINTER(C.c_int)]),
     ("qmb_lanczos_learned", C.c_int,
      [_vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, _vp, C.POINTER(C.c_int)]),

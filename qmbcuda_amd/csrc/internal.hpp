@@ -132,6 +132,14 @@ struct Ref {
     Ref(const Ref &) = delete;
     Ref &operator=(const Ref &) = delete;
     Ref(Ref &&o) noexcept : p(o.p) { o.p = nullptr; }
+    Ref &operator=(Ref &&o) noexcept {
+        if (this != &o) {
+            release(p);
+            p = o.p;
+            o.p = nullptr;
+        }
+        return *this;
+    }
     Ref &operator=(T *q) {
         retain(q);
         release(p);

@@ -288,35 +288,47 @@ __global__ __launch_bounds__(256) void mlp_refresh_w2(const float *__restrict__ 
 }
 
 // ---- per-state kernels (one sample = one ground state) ---------------------
-__device__ inline int window_pattern(uint64_t s, int t, int n) {
-    const uint64_t ss = (n == 64) ? s : (s | (s << n));  // ring wrap (n <= 56 used)
-    return (int)((ss >> t) & 0xFFull);
+// A window is 8 site indices; its pattern has bit k = spin at site k of the
+// window. Rings use windows q..q+7 (mod N); 2-D lattices register their own.
+constexpr int kMaxWin = 128;
+
+__device__ inline int window_pattern(uint64_t s, const uint8_t *w) {
+    int p = 0;
+#pragma unroll
+    for (int k = 0; k < kWin; ++k) p |= (int)((s >> w[k]) & 1ull) << k;
+    return p;
+}
+
+__device__ inline void stage_windows(uint8_t *lds, const uint8_t *win, int n_win) {
+    for (int i = threadIdx.x; i < n_win * kWin; i += blockDim.x) lds[i] = win[i];
 }
 
 // alpha_r, beta_r from the pattern table of this coupling; block max of alpha.
-// beta_r also carries the Marshall sign prior pi * (up spins on even sites):
+// beta_r also carries the Marshall sign prior pi * (up spins on sublattice A):
 // the network learns the phase relative to the bipartite sign rule, whose
 // state (and not the uniform one, which is the fully polarised multiplet and
 // orthogonal to a singlet) is the natural starting point of training.
-__global__ __launch_bounds__(256) void mlp_state_logpsi(int64_t n, int n_sites, const uint64_t *__restrict__ reps,
+__global__ __launch_bounds__(256) void mlp_state_logpsi(int64_t n, int n_win, const uint8_t *__restrict__ win,
+                                                        uint64_t sub_a, const uint64_t *__restrict__ reps,
                                                         const float *__restrict__ tab, double *__restrict__ alpha,
                                                         double *__restrict__ beta, double *__restrict__ part) {
     __shared__ float t[512];
     __shared__ double lds[kWaves];
+    __shared__ uint8_t wl[kMaxWin * kWin];
     for (int i = threadIdx.x; i < 512; i += blockDim.x) t[i] = tab[i];
+    stage_windows(wl, win, n_win);
     __syncthreads();
-    const uint64_t even = 0x5555555555555555ULL & ((1ULL << n_sites) - 1ULL);  // n_sites <= 56
     double mx = -1e300;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t s = reps[r];
         double a = 0.0, b = 0.0;
-        for (int q = 0; q < n_sites; ++q) {
-            const int p = window_pattern(s, q, n_sites);
+        for (int q = 0; q < n_win; ++q) {
+            const int p = window_pattern(s, wl + q * kWin);
             a += (double)t[2 * p];
             b += (double)t[2 * p + 1];
         }
         alpha[r] = a;
-        beta[r] = b + kMarshallPi * (double)__popcll(s & even);
+        beta[r] = b + kMarshallPi * (double)__popcll(s & sub_a);
         mx = fmax(mx, a);
     }
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
@@ -383,7 +395,8 @@ __global__ __launch_bounds__(256) void mlp_finish_overlap(const double *__restri
 
 // dL/dalpha_r, dL/dbeta_r (L = 1 - F) accumulated per window pattern in
 // fixed point (order-independent integer sums): hist[block][256][2].
-__global__ __launch_bounds__(256) void mlp_state_grad(int64_t n, int n_sites, int G, const uint64_t *__restrict__ reps,
+__global__ __launch_bounds__(256) void mlp_state_grad(int64_t n, int n_win, const uint8_t *__restrict__ win, int G,
+                                                      const uint64_t *__restrict__ reps,
                                                       const uint8_t *__restrict__ stab,
                                                       const double *__restrict__ alpha,
                                                       const double *__restrict__ beta, const double *__restrict__ u,
@@ -391,7 +404,9 @@ __global__ __launch_bounds__(256) void mlp_state_grad(int64_t n, int n_sites, in
                                                       const double *__restrict__ stats,
                                                       unsigned long long *__restrict__ hist) {
     __shared__ unsigned long long h[512];
+    __shared__ uint8_t wl[kMaxWin * kWin];
     for (int i = threadIdx.x; i < 512; i += blockDim.x) h[i] = 0ull;
+    stage_windows(wl, win, n_win);
     __syncthreads();
     const double am = *amax;
     const double ore = stats[0], oim = stats[1], nc = stats[2], nu = stats[3];
@@ -406,8 +421,8 @@ __global__ __launch_bounds__(256) void mlp_state_grad(int64_t n, int n_sites, in
         const double gb = -(-2.0 * wi) / (nc * nu);
         const long long qa = (long long)llrint(ga * kFix), qb = (long long)llrint(gb * kFix);
         const uint64_t s = reps[r];
-        for (int q = 0; q < n_sites; ++q) {
-            const int p = window_pattern(s, q, n_sites);
+        for (int q = 0; q < n_win; ++q) {
+            const int p = window_pattern(s, wl + q * kWin);
             atomicAdd(&h[2 * p], (unsigned long long)qa);
             atomicAdd(&h[2 * p + 1], (unsigned long long)qb);
         }
@@ -524,9 +539,19 @@ __global__ __launch_bounds__(256) void mlp_finish_sum2(const double *__restrict_
 
 using namespace qmb;
 
+// Window geometry of one basis: n_win windows of 8 sites and the sublattice A
+// mask of the Marshall prior.
+struct MlpGeom {
+    Ref<const qmb_basis> basis;
+    int n_win = 0;
+    DevBuf win;  // uint8 [n_win * 8]
+    uint64_t sub_a = 0;
+};
+
 struct MlpSample {
     Ref<const qmb_basis> basis;
-    DevBuf u;  // target ground state
+    int geom = 0;  // index into qmb_mlp::geoms
+    DevBuf u;      // target ground state
     double delta, j2;
 };
 
@@ -534,6 +559,7 @@ struct qmb_mlp : Counted {
     Ref<qmb_ctx> ctx;
     DevBuf theta, m1, m2, grad, w2bf, w2t;
     int64_t step = 0;
+    std::vector<MlpGeom> geoms;
     std::vector<MlpSample> samples;
     // row buffers (sized for the current row count)
     int rows_cap = 0;
@@ -561,15 +587,36 @@ void qmb_mlp::ensure_rows(int rows) {
 
 namespace {
 
-void check_ring_basis(const qmb_basis *b) {
-    const qmb_group *g = b->group;
-    QMB_REQUIRE(g->torus && g->block == g->n_sites && g->G == g->n_sites,
-                "mlp: the basis must use the plain ring translation group (G = N)");
+// The network's bases: compact layout, every character 1 (k = 0, and
+// flip-even when the group has the spin flip).
+void check_mlp_basis(const qmb_basis *b) {
     QMB_REQUIRE(b->layout == QMB_LAYOUT_COMPACT, "mlp: the basis must use the compact layout");
-    QMB_REQUIRE(b->n_sites >= kWin && b->n_sites <= 56, "mlp: ring size must be in [8, 56]");
-    const double *ch = &g->chars[(size_t)b->irrep * g->G * 2];
-    for (int e = 0; e < g->G; ++e)
-        QMB_REQUIRE(ch[2 * e] == 1.0 && ch[2 * e + 1] == 0.0, "mlp: the basis must be the k=0 sector");
+    QMB_REQUIRE(b->all_rows, "mlp: the basis must be the fully symmetric sector (k = 0, flip-even)");
+    QMB_REQUIRE(b->n_sites >= kWin && b->n_sites <= 64, "mlp: lattice size must be in [8, 64]");
+}
+
+// Index of the geometry of basis b: registered windows, or for a plain ring
+// (translation group G = N) the N windows q..q+7 (mod N) with sublattice A =
+// the even sites.
+int geom_for(qmb_mlp *m, const qmb_basis *b) {
+    for (size_t i = 0; i < m->geoms.size(); ++i)
+        if (m->geoms[i].basis == b) return (int)i;
+    check_mlp_basis(b);
+    const qmb_group *g = b->group;
+    QMB_REQUIRE(g->torus && g->block == g->n_sites && g->G == g->n_sites && b->n_sites <= kMaxWin,
+                "mlp: no windows registered for this basis and it is not a plain ring (G = N)");
+    const int N = b->n_sites;
+    std::vector<uint8_t> w((size_t)N * kWin);
+    for (int q = 0; q < N; ++q)
+        for (int k = 0; k < kWin; ++k) w[(size_t)q * kWin + k] = (uint8_t)((q + k) % N);
+    MlpGeom gm;
+    gm.basis = b;
+    gm.n_win = N;
+    gm.win.alloc(w.size());
+    QMB_HIP(hipMemcpy(gm.win.p, w.data(), w.size(), hipMemcpyHostToDevice));
+    for (int q = 0; q < N; q += 2) gm.sub_a |= 1ULL << q;
+    m->geoms.push_back(std::move(gm));
+    return (int)m->geoms.size() - 1;
 }
 
 // Forward pass over the given couplings (rows = 256 per coupling).
@@ -602,10 +649,11 @@ struct StateWork {
     }
 };
 
-void state_logpsi(qmb_ctx *ctx, const qmb_basis *b, const float *tab, StateWork &w) {
+void state_logpsi(qmb_ctx *ctx, const qmb_basis *b, const MlpGeom &gm, const float *tab, StateWork &w) {
     w.ensure(b->n_rows);
-    hipLaunchKernelGGL(mlp_state_logpsi, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, b->n_sites,
-                       b->reps.as<uint64_t>(), tab, w.alpha.as<double>(), w.beta.as<double>(), w.part.as<double>());
+    hipLaunchKernelGGL(mlp_state_logpsi, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, gm.n_win,
+                       gm.win.as<uint8_t>(), gm.sub_a, b->reps.as<uint64_t>(), tab, w.alpha.as<double>(),
+                       w.beta.as<double>(), w.part.as<double>());
     QMB_HIP(hipGetLastError());
     hipLaunchKernelGGL(mlp_finish_max, dim3(1), dim3(256), 0, ctx->stream, w.part.as<double>(), w.nb,
                        w.amax.as<double>());
@@ -613,9 +661,10 @@ void state_logpsi(qmb_ctx *ctx, const qmb_basis *b, const float *tab, StateWork 
 }
 
 // Loss (and dout rows of this sample when dout_rows != NULL) of one sample.
-void sample_loss(qmb_ctx *ctx, const MlpSample &s, const float *tab, StateWork &w, float *dout_rows, float scale) {
+void sample_loss(qmb_ctx *ctx, const MlpSample &s, const MlpGeom &gm, const float *tab, StateWork &w,
+                 float *dout_rows, float scale) {
     const qmb_basis *b = s.basis;
-    state_logpsi(ctx, b, tab, w);
+    state_logpsi(ctx, b, gm, tab, w);
     hipLaunchKernelGGL(mlp_state_overlap, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, b->group->G,
                        b->stab.as<uint8_t>(), w.alpha.as<double>(), w.beta.as<double>(), s.u.as<double>(),
                        w.amax.as<double>(), w.part.as<double>(), w.nb);
@@ -624,8 +673,8 @@ void sample_loss(qmb_ctx *ctx, const MlpSample &s, const float *tab, StateWork &
                        w.stats.as<double>());
     QMB_HIP(hipGetLastError());
     if (dout_rows) {
-        hipLaunchKernelGGL(mlp_state_grad, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, b->n_sites,
-                           b->group->G, b->reps.as<uint64_t>(), b->stab.as<uint8_t>(), w.alpha.as<double>(),
+        hipLaunchKernelGGL(mlp_state_grad, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, gm.n_win,
+                           gm.win.as<uint8_t>(), b->group->G, b->reps.as<uint64_t>(), b->stab.as<uint8_t>(), w.alpha.as<double>(),
                            w.beta.as<double>(), s.u.as<double>(), w.amax.as<double>(), w.stats.as<double>(),
                            w.hist.as<unsigned long long>());
         QMB_HIP(hipGetLastError());
@@ -644,11 +693,11 @@ void sample_loss(qmb_ctx *ctx, const MlpSample &s, const float *tab, StateWork &
 // two numbers travel to the host.
 bool build_start(qmb_mlp *m, const qmb_basis *b, double delta, double j2, uint64_t seed, double *dx,
                  StateWork &w) {
-    check_ring_basis(b);
+    const int gi = geom_for(m, b);
     qmb_ctx *ctx = m->ctx;
     std::vector<double> coup = {delta, j2};
     forward_rows(m, coup, false);
-    state_logpsi(ctx, b, m->out.as<float>(), w);
+    state_logpsi(ctx, b, m->geoms[gi], m->out.as<float>(), w);
     DevBuf bref(sizeof(double));
     hipLaunchKernelGGL(mlp_argmax_part, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, w.alpha.as<double>(),
                        w.amax.as<double>(), w.part.as<double>());
@@ -683,7 +732,7 @@ double loss_and_grad(qmb_mlp *m, bool want_grad) {
     StateWork w;
     DevBuf losses((size_t)ns * sizeof(double));
     for (int i = 0; i < ns; ++i) {
-        sample_loss(ctx, m->samples[i], m->out.as<float>() + (size_t)i * 512, w,
+        sample_loss(ctx, m->samples[i], m->geoms[m->samples[i].geom], m->out.as<float>() + (size_t)i * 512, w,
                     want_grad ? m->dout.as<float>() + (size_t)i * 512 : nullptr, 1.0f / ns);
         QMB_HIP(hipMemcpyAsync(losses.as<double>() + i, w.stats.as<double>() + 4, sizeof(double),
                                hipMemcpyDeviceToDevice, ctx->stream));
@@ -804,14 +853,42 @@ qmb_status qmb_mlp_add_sample(qmb_mlp *m, const qmb_basis *b, const double *grou
     return guarded([&] {
         QMB_REQUIRE(m && b && ground_state, "qmb_mlp_add_sample: NULL argument");
         QMB_REQUIRE(b->ctx == m->ctx, "qmb_mlp_add_sample: basis from another context");
-        check_ring_basis(b);
         MlpSample s;
+        s.geom = geom_for(m, b);
         s.basis = b;
         s.delta = delta;
         s.j2 = j2;
         s.u.alloc((size_t)b->n_rows * sizeof(double));
         QMB_HIP(hipMemcpy(s.u.p, ground_state, (size_t)b->n_rows * sizeof(double), hipMemcpyHostToDevice));
         m->samples.push_back(std::move(s));
+    });
+}
+
+qmb_status qmb_mlp_set_windows(qmb_mlp *m, const qmb_basis *b, int n_win, const uint8_t *sites,
+                               uint64_t sublattice_a) {
+    return guarded([&] {
+        QMB_REQUIRE(m && b && sites, "qmb_mlp_set_windows: NULL argument");
+        QMB_REQUIRE(b->ctx == m->ctx, "qmb_mlp_set_windows: basis from another context");
+        check_mlp_basis(b);
+        QMB_REQUIRE(n_win >= 1 && n_win <= kMaxWin, "qmb_mlp_set_windows: n_win must be in [1, 128]");
+        for (int i = 0; i < n_win * kWin; ++i)
+            QMB_REQUIRE(sites[i] < b->n_sites, "qmb_mlp_set_windows: site index out of range");
+        const uint64_t full = b->n_sites == 64 ? ~0ULL : ((1ULL << b->n_sites) - 1ULL);
+        QMB_REQUIRE((sublattice_a & ~full) == 0, "qmb_mlp_set_windows: sublattice mask beyond the lattice");
+        for (const auto &s : m->samples)
+            QMB_REQUIRE(s.basis != b, "qmb_mlp_set_windows: the basis already has samples");
+        MlpGeom gm;
+        gm.basis = b;
+        gm.n_win = n_win;
+        gm.win.alloc((size_t)n_win * kWin);
+        QMB_HIP(hipMemcpy(gm.win.p, sites, (size_t)n_win * kWin, hipMemcpyHostToDevice));
+        gm.sub_a = sublattice_a;
+        for (auto &g : m->geoms)
+            if (g.basis == b) {
+                g = std::move(gm);
+                return;
+            }
+        m->geoms.push_back(std::move(gm));
     });
 }
 

@@ -159,3 +159,30 @@ def momentum_irrep(lat: Lattice, kx: int, ky: int = 0, flip_parity: int | None =
 
 def is_real_irrep(chars, irrep: int) -> bool:
     return bool(np.all(np.asarray(chars)[irrep].imag == 0.0))
+
+
+def square_windows(lat: Lattice):
+    """8-site windows of a periodic square lattice (one site per cell) for the
+    learned start: a 2x4 and a 4x2 block anchored at every cell, bit k of a
+    window = site k in row-major order inside the block. Returns [2*N, 8] uint8."""
+    import numpy as np
+
+    assert lat.M == 1
+    wins = []
+    for (h, w) in ((2, 4), (4, 2)):
+        for i in range(lat.N1):
+            for j in range(lat.N2):
+                wins.append([lat.site((i + di) % lat.N1, (j + dj) % lat.N2) for di in range(h) for dj in range(w)])
+    return np.array(wins, np.uint8)
+
+
+def sublattice_mask(lat: Lattice) -> int:
+    """Sites with (i + j) even: the A sublattice of the bipartite square lattice
+    (the Marshall sign prior counts up spins there)."""
+    assert lat.M == 1
+    m = 0
+    for i in range(lat.N1):
+        for j in range(lat.N2):
+            if (i + j) % 2 == 0:
+                m |= 1 << lat.site(i, j)
+    return m

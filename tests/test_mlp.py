@@ -70,6 +70,41 @@ def test_marshall_prior_is_the_unfrustrated_ground_state_sign():
     assert abs(uniform @ u) < 1e-10 * np.linalg.norm(uniform)
 
 
+def test_marshall_prior_on_the_square_lattice():
+    """4x4 Heisenberg, k=(0,0), flip-even: the sublattice-A Marshall sign is the
+    exact sign of every ground-state coefficient (bipartite lattice)."""
+    from qmbcuda_amd import models as LM
+    from tests.helpers import oracle_ell, oracle_orbits, square_case
+
+    case = square_case(4, 4, 1.0, 1.0, flip=True)
+    orb = oracle_orbits(case, 8)
+    H = oracle_ell(case, orb, 0).dense()
+    w, V = np.linalg.eigh(H)
+    u = V[:, 0]
+    lat, _ = LM.square(4, 4, 1.0, 1.0)
+    sgn = np.cos(OMLP.marshall_phase(orb.bas[orb.rep_rank], 16, LM.sublattice_mask(lat)))
+    prod = np.sign(u) * sgn
+    assert np.all(np.abs(u) > 1e-12) and np.all(prod == prod[0])
+    # the ring default (even sites) is not the square lattice's sublattice
+    assert not np.all(np.sign(u) * np.cos(OMLP.marshall_phase(orb.bas[orb.rep_rank], 16)) == prod[0])
+
+
+def test_square_windows():
+    from qmbcuda_amd import models as LM
+
+    lat, _ = LM.square(4, 6, 1.0, 1.0)
+    W = LM.square_windows(lat)
+    assert W.shape == (48, 8)
+    for w in W:
+        assert len(set(w.tolist())) == 8
+    # window 0: rows 0-1, columns 0-3; window 24: rows 0-3, columns 0-1
+    assert W[0].tolist() == [0, 1, 2, 3, 6, 7, 8, 9]
+    assert W[24].tolist() == [0, 1, 6, 7, 12, 13, 18, 19]
+    # every site is in 8 windows of each shape
+    assert np.all(np.bincount(W[:24].ravel(), minlength=24) == 8)
+    assert np.all(np.bincount(W[24:].ravel(), minlength=24) == 8)
+
+
 def test_start_noise_is_the_random_start_stream():
     import oracle
 
@@ -177,3 +212,61 @@ def test_mlp_training_and_learned_start_gain(ctx):
         assert abs(ev_l[0] - ev_r[0]) < 1e-10 * abs(ev_r[0])
         gains.append(rep_r.iters - rep_l.iters)
     assert min(gains) > 0, gains
+
+
+@pytest.mark.gpu
+def test_mlp_square_lattice_windows(ctx):
+    """2-D windows: the GPU loss and gradient on a 4x4 k=0 flip-even sample
+    against the oracle; after training on 4x4 and 4x6 ground states, the learned
+    start needs fewer Lanczos steps on a 4x6 coupling it was not trained on."""
+    import qmbcuda_amd as q
+    from qmbcuda_amd import models as LM
+
+    print("[gpu-case] mlp square-lattice windows", flush=True)
+    keep = []
+
+    def sector(n1, n2, d):
+        lat, info = LM.square(n1, n2, 1.0, d)
+        ge, fl, ch = LM.translation_group(lat, spin_flip=True)
+        g = q.Group(ge, ch, fl)
+        t = q.Terms(n1 * n2, LM.term_table(lat, info))
+        b = q.Basis(ctx, g, n1 * n2 // 2, 0, q.LAYOUT_COMPACT)
+        e = q.Ell(ctx, b, t)
+        keep.extend([g, t, b, e])
+        return lat, b, e
+
+    net = q.Mlp(ctx, seed=11)
+    lat, b, e = sector(4, 4, 1.0)
+    W, A = LM.square_windows(lat), LM.sublattice_mask(lat)
+    net.set_windows(b, W, A)
+    ev, rep, _, u = q.lanczos(e, max_iter=300, want_eigvec=True)
+    net.add_sample(b, u, 1.0, 0.0)
+    theta = net.params().astype(np.float64)
+    L, grad = net.loss_grad()
+    reps, stab, _ = b.export()
+    pats = OMLP.window_patterns(reps, 16, W)
+    prior = OMLP.marshall_phase(reps, 16, A)
+    L_or, g_or = OMLP.fidelity_loss(theta, pats, stab, 32, u, 1.0, 0.0, with_grad=True, round_like_gpu=True,
+                                    prior=prior)
+    assert abs(L - L_or) < 2e-3 * max(1.0, abs(L_or))
+    assert grad @ g_or / (np.linalg.norm(grad) * np.linalg.norm(g_or)) > 0.999
+    net.clear_samples()
+    for (n1, n2, deltas) in ((4, 4, (0.8, 1.0, 1.2)), (4, 6, (0.8, 1.2))):
+        for d in deltas:
+            lat, b, e = sector(n1, n2, d)
+            net.set_windows(b, LM.square_windows(lat), LM.sublattice_mask(lat))
+            ev, rep, _, u = q.lanczos(e, max_iter=300, want_eigvec=True)
+            net.add_sample(b, u, d, 0.0)
+    L0, _ = net.loss_grad(want_grad=False)
+    net.train(200, 3e-3)
+    L1, _ = net.loss_grad(want_grad=False)
+    print(f"square loss {L0:.4f} -> {L1:.4f}", flush=True)
+    assert L1 < L0
+    lat, b, e = sector(4, 6, 1.0)
+    net.set_windows(b, LM.square_windows(lat), LM.sublattice_mask(lat))
+    ev_r, rep_r, _, _ = q.lanczos(e, max_iter=300, tol=1e-13, check_every=1)
+    ev_l, rep_l, fb, _ = net.lanczos(b, e, 1.0, 0.0, max_iter=300, tol=1e-13, check_every=1)
+    print(f"4x6 iterations random {rep_r.iters}, learned {rep_l.iters}", flush=True)
+    assert not fb and rep_l.converged
+    assert abs(ev_l[0] - ev_r[0]) < 1e-10 * abs(ev_r[0])
+    assert rep_l.iters < rep_r.iters
