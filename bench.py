@@ -143,6 +143,47 @@ def cpu_baseline(cfg, last, table, threads, spmv_bytes):
     }
 
 
+def c5_learned(ctx, lat6, b6, e6, e0_plain, rep_plain, lanczos_s_plain, args):
+    """The same 6x6 solve from the learned start: the window network (2x4 and
+    4x2 windows, Marshall sublattice prior) trained on the ground states of 4x4,
+    4x6 and 6x4 lattices at Delta in {0.8, ..., 1.2}, then Lanczos on the 6x6
+    from its start vector (inference on the device, inside the timed span)."""
+    import numpy as np
+
+    import qmbcuda_amd as q
+    from qmbcuda_amd import models as LM
+
+    net = q.Mlp(ctx, seed=20261202)
+    keep = []
+    t0 = time.perf_counter()
+    for (n1, n2) in ((4, 4), (4, 6), (6, 4)):
+        for d in (0.8, 0.9, 1.0, 1.1, 1.2):
+            lat, info = LM.square(n1, n2, 1.0, d)
+            ge, fl, ch = LM.translation_group(lat, spin_flip=True)
+            g = q.Group(ge, ch, fl)
+            t = q.Terms(n1 * n2, LM.term_table(lat, info))
+            b = q.Basis(ctx, g, n1 * n2 // 2, 0, q.LAYOUT_COMPACT)
+            e = q.Ell(ctx, b, t)
+            net.set_windows(b, LM.square_windows(lat), LM.sublattice_mask(lat))
+            _, _, _, u = q.lanczos(e, max_iter=300, want_eigvec=True)
+            net.add_sample(b, u, d, 0.0)
+            keep += [g, t, b, e]
+    data_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = net.train(args.c5_steps, 3e-3)
+    train_s = time.perf_counter() - t0
+    net.set_windows(b6, LM.square_windows(lat6), LM.sublattice_mask(lat6))
+    t0 = time.perf_counter()
+    ev, rep, fb, _ = net.lanczos(b6, e6, 1.0, 0.0, max_iter=100, tol=1e-13, reorth=2, check_every=1)
+    wall = time.perf_counter() - t0
+    return {"lanczos_ms": round(wall * 1e3, 1), "iters": rep.iters, "plain_lanczos_ms": round(lanczos_s_plain * 1e3, 1),
+            "plain_iters": rep_plain.iters, "speedup": round(lanczos_s_plain / wall, 3),
+            "abs_dE_vs_plain": abs(ev[0] - e0_plain), "fallback": fb, "converged": bool(rep.converged),
+            "train_loss_first_last": [float(hist[0]), float(hist[-1])], "train_steps": args.c5_steps,
+            "train_s": round(train_s, 2), "data_gen_s": round(data_s, 2),
+            "training_lattices": "4x4, 4x6, 6x4 at Delta 0.8..1.2 (J2 = 0), k=(0,0) flip-even"}
+
+
 def cpu_baseline_shard(e, iters, threads, spmv_bytes, shard_rows=1 << 21):
     """CPU baseline where the reference's construction cannot run (6x6: 9e9-state
     tables): the oracle's FP64 SpMV timed on a row shard of the device-built
@@ -348,6 +389,8 @@ def run_c5(args):
                         "avg_launch_us": round(t_k1 * 1e6, 2), "bytes_per_launch": B_k1,
                         "traffic": profile_traffic("spmv_sell_square_6x6")},
            "dtype": "f64", "data": "synthetic", "n_gpus": 1, "cpu_baseline": None}
+    if not args.no_learned:
+        out["learned_start"] = c5_learned(ctx, lat, b, e, ev[0], rep, t3 - t2, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_shard(e, rep.iters, args.cpu_threads, inf.spmv_bytes)
     print(json.dumps(out), flush=True)
@@ -373,6 +416,8 @@ def main():
     ap.add_argument("--c4-steps", type=int, default=400)
     ap.add_argument("--c4-eval", type=int, default=16)
     ap.add_argument("--c4-n", type=int, default=30)
+    ap.add_argument("--c5-steps", type=int, default=300)
+    ap.add_argument("--no-learned", action="store_true", help="c5: skip the learned-start solve")
     ap.add_argument("--tol", type=float, default=1e-13)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)

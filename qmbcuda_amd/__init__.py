@@ -323,10 +323,12 @@ class Mlp:
         return x, bool(fb.value)
 
     def lanczos(self, basis: Basis, ell: "Ell", delta: float, j2: float, n_eigs=1, max_iter=300, tol=1e-13,
-                seed=2212456, check_every=10, want_eigvec=False, profile=False):
+                seed=2212456, check_every=10, want_eigvec=False, profile=False, reorth=None):
         """Lanczos from the learned start vector kept on the device
         (qmb_lanczos_learned). Returns (evals, report, fallback, eigvec)."""
-        o = default_opts(max_iter=max_iter, n_eigs=n_eigs, tol=tol, seed=seed, reorth=int(n_eigs > 1),
+        if reorth is None:
+            reorth = n_eigs > 1
+        o = default_opts(max_iter=max_iter, n_eigs=n_eigs, tol=tol, seed=seed, reorth=int(reorth),
                          check_every=check_every, want_eigvec=int(want_eigvec), profile=int(profile))
         ev = np.zeros(n_eigs, np.float64)
         vec = np.zeros(basis.n_rows, np.float64) if want_eigvec else None
