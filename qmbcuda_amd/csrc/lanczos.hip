@@ -485,9 +485,14 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     int it = 0;
     int m_eff = 0;
     bool converged = false, breakdown = false;
+    // Convergence is tested after `next` steps: check_every at first, then the
+    // number of steps the residual's observed geometric decay predicts it needs
+    // to reach the tolerance (at least 1, at most check_every).
+    int next = check, it_prev = -1;
+    double q_prev = 0.0;
     while (it < M) {
         const int win0 = it;
-        const int win1 = std::min(M, it + check);
+        const int win1 = std::min(M, it + next);
         for (; it < win1; ++it) {
             if (o.profile) QMB_HIP(hipEventRecord(ev[2 * (it - win0)], ctx->stream));
             k1(it, o.profile ? ev[2 * (it - win0) + 1] : nullptr);
@@ -526,19 +531,30 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         if (o.fixed_iter) break;
         if (m_eff >= k_eigs) {
             bool ok = true;
+            double q = 0.0;  // worst residual / its target over the wanted eigenvalues
             const double sq = std::sqrt(tol);
-            for (int j = 0; j < k_eigs && ok; ++j) {
+            for (int j = 0; j < k_eigs; ++j) {
                 const double sc = std::max(1.0, std::fabs(w[j]));
                 const double r = std::fabs(hb[m_eff] * zl[j]);
                 double gap = 1.0;
                 if (j + 1 < kk) gap = w[j + 1] - w[j];
                 if (j > 0) gap = std::min(gap, w[j] - w[j - 1]);
-                ok = r <= sq * sc && r * r / std::max(gap, sq) <= tol * sc;
+                ok = ok && r <= sq * sc && r * r / std::max(gap, sq) <= tol * sc;
+                const double target = std::min(sq * sc, std::sqrt(tol * sc * std::max(gap, sq)));
+                q = std::max(q, r / target);
             }
             if (ok) {
                 converged = true;
                 break;
             }
+            next = check;
+            if (it_prev >= 0 && q > 1.0 && q < q_prev) {
+                const double rate = std::log(q_prev / q) / (double)(it - it_prev);  // decay per step
+                const double steps = std::ceil(std::log(q) / rate);
+                if (steps < (double)check) next = std::max(1, (int)steps);
+            }
+            q_prev = q;
+            it_prev = it;
         }
     }
     if (o.fixed_iter) converged = true;
