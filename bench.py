@@ -262,18 +262,30 @@ def run_c3(args):
     build_s = time.perf_counter() - t0
     rows = [e.info().n_rows for e in ells]
     q.lanczos_batched(ells[:4], n_eigs=4, max_iter=400)  # warm-up (and tile autotune cache)
-    t0 = time.perf_counter()
-    evb, reps = q.lanczos_batched(ells, n_eigs=4, max_iter=400, tol=1e-13)
-    tb = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    evs = np.array([q.lanczos(e, n_eigs=4, max_iter=400, tol=1e-13)[0] for e in ells])
-    ts = time.perf_counter() - t0
+    walls, evals = {}, {}
+    for mode, rmode in (("partial", 2), ("full", 1)):
+        t0 = time.perf_counter()
+        evals["batched_" + mode], reps = q.lanczos_batched(ells, n_eigs=4, max_iter=400, tol=1e-13, reorth=rmode)
+        walls["batched_" + mode] = time.perf_counter() - t0
+        if mode == "partial":
+            reps_partial = reps
+        t0 = time.perf_counter()
+        evals["sequential_" + mode] = np.array([q.lanczos(e, n_eigs=4, max_iter=400, tol=1e-13, reorth=rmode)[0]
+                                               for e in ells])
+        walls["sequential_" + mode] = time.perf_counter() - t0
+    tb = walls["batched_partial"]
+    ts = min(walls["sequential_partial"], walls["sequential_full"])
+    dE = max(float(np.max(np.abs(evals[a] - evals["sequential_full"]))) for a in evals)
     out = {"metric": METRIC, "config": {"workload": "C3: XXZ chain N=24, 24 k x n_up {12,14,16}, 72 sectors, "
                                                     "lowest 4 each, FP64"},
-           "value": round(ts / tb, 3), "unit": "x (sequential wall / batched wall)", "higher_is_better": True,
-           "batched_ms": round(tb * 1e3, 2), "sequential_ms": round(ts * 1e3, 2), "ell_build_s": round(build_s, 3),
-           "max_abs_dE_batched_vs_sequential": float(np.max(np.abs(evb - evs))),
-           "iters_batched_max": int(max(r.iters for r in reps)), "sector_dims": [min(rows), max(rows)],
+           "value": round(ts / tb, 3), "unit": "x (best sequential wall / batched wall)", "higher_is_better": True,
+           "batched_ms": round(tb * 1e3, 2), "sequential_ms": round(ts * 1e3, 2),
+           "wall_ms": {k: round(v * 1e3, 2) for k, v in walls.items()},
+           "reorth": "partial (Simon 1984; batched: decided per instance on the device)",
+           "ell_build_s": round(build_s, 3), "max_abs_dE_vs_sequential_full": dE,
+           "iters_batched_max": int(max(r.iters for r in reps_partial)),
+           "reorth_steps_batched_partial": int(sum(r.n_reorth for r in reps_partial)),
+           "sector_dims": [min(rows), max(rows)],
            "n_sectors": len(ells), "dtype": "f64", "data": "synthetic", "n_gpus": 1}
     print(json.dumps(out), flush=True)
 

@@ -234,8 +234,8 @@ double qmb_lanczos_eigvals_ref(double *E_tmp, int NStates, int NIter, int nobv, 
  * This solves n independent ELLs together: one launch per phase covers every
  * instance (a work table of row chunks), the iteration chunk is captured once as
  * a HIP graph and replayed, converged instances drop out. evals[n*n_eigs],
- * reports[n] (may be NULL). reorth = 2 runs as 1 (every step): the captured
- * iteration has no host decision point. */
+ * reports[n] (may be NULL). reorth = 2 (partial) decides on the device, per
+ * instance and step, inside the captured iteration. */
 qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
                                const qmb_lanczos_opts *opts, double *evals,
                                qmb_lanczos_report *reports);

@@ -41,6 +41,8 @@ struct BInst {
     double *alpha, *beta, *invb;  // [M], [M+1], [M+1]
     double *coef;                 // [M*2] reorthogonalisation coefficients
     double *gate;                 // [1] second Gram-Schmidt pass wanted (1) or not (0)
+    double *pro;                  // [3] this step orthogonalises (1/0), second-step flag, steps that did
+    double *omega;                // [3][M+2] orthogonality estimates (partial reorthogonalisation)
     int spmv_first, spmv_nblk;
     int vec_first, vec_nblk;
 };
@@ -115,6 +117,7 @@ __global__ __launch_bounds__(kThreads) void bk_finish(const BInst *__restrict__ 
     const int inst = first_inst + blockIdx.x;
     if (active && !active[inst]) return;
     const BInst I = insts[inst];
+    if ((mode == 3 || mode == 4) && I.pro[0] == 0.0) return;
     if (mode == 4 && *I.gate == 0.0) return;
     const int it = it_p ? *it_p : 0;
     const int first = mode == 0 ? I.spmv_first : I.vec_first;
@@ -169,7 +172,7 @@ __global__ __launch_bounds__(kThreads) void bk_rdots(const BInst *__restrict__ i
     const BTask t = tasks[blockIdx.x];
     if (!active[t.inst]) return;
     const BInst I = insts[t.inst];
-    if (gated && *I.gate == 0.0) return;
+    if (I.pro[0] == 0.0 || (gated && *I.gate == 0.0)) return;
     const int it = *it_p;
     const int vs = CPLX ? 2 : 1;
     const int64_t cnt = I.n_rows * vs;
@@ -202,7 +205,7 @@ __global__ __launch_bounds__(kThreads) void bk_rreduce(const BInst *__restrict__
     const int inst = first_inst + blockIdx.x;
     if (!active[inst]) return;
     const BInst I = insts[inst];
-    if (gated && *I.gate == 0.0) return;
+    if (I.pro[0] == 0.0 || (gated && *I.gate == 0.0)) return;
     const int it = *it_p;
     for (int j = 0; j <= it; ++j) {
         double sr = 0.0, si = 0.0;
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void bk_rapply(const BInst *__restrict__ 
     const BTask t = tasks[blockIdx.x];
     if (!active[t.inst]) return;
     const BInst I = insts[t.inst];
-    if (gated && *I.gate == 0.0) return;
+    if (I.pro[0] == 0.0 || (gated && *I.gate == 0.0)) return;
     const int it = *it_p;
     const int vs = CPLX ? 2 : 1;
     const int64_t cnt = I.n_rows * vs;
@@ -255,6 +258,49 @@ __global__ __launch_bounds__(kThreads) void bk_rapply(const BInst *__restrict__ 
     }
     const double tb = block_sum(acc, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = tb;
+}
+
+// Partial reorthogonalisation on the device (H. D. Simon, Math. Comp. 42 (1984)
+// 115-142), one wavefront per instance of the family: the estimates
+// om_{j+1,k} of <phi_{j+1}, phi_k> follow the Lanczos recurrence,
+//   b[j+1] om_{j+1,k} = b[k+1] om_{j,k+1} + (a[k] - a[j]) om_{j,k} + b[k] om_{j,k-1} - b[j] om_{j-1,k},
+// plus a rounding term eps1 = eps sqrt(n); when one passes sqrt(eps), this
+// step and the next orthogonalise (pro[0] = 1) and the estimates restart at
+// eps1. Rows of omega rotate by step parity (j+1, j, j-1 mod 3).
+__global__ __launch_bounds__(64) void bk_pro(const BInst *__restrict__ insts, const int *__restrict__ active,
+                                             const int *__restrict__ it_p, int first_inst, int M) {
+    const int inst = first_inst + blockIdx.x;
+    if (!active[inst]) return;
+    const BInst I = insts[inst];
+    const int j = *it_p, lane = threadIdx.x;
+    const int64_t stride = M + 2;
+    double *om_next = I.omega + (size_t)((j + 1) % 3) * stride;
+    const double *om_cur = I.omega + (size_t)(j % 3) * stride;
+    const double *om_prev = I.omega + (size_t)((j + 2) % 3) * stride;
+    const double eps1 = 2.220446049250313e-16 * sqrt((double)I.n_rows);
+    const double bj1 = I.beta[j + 1];
+    double worst = 0.0;
+    for (int k = lane; k < j; k += 64) {
+        double t = I.beta[k + 1] * om_cur[k + 1] + (I.alpha[k] - I.alpha[j]) * om_cur[k] - I.beta[j] * om_prev[k];
+        if (k > 0) t += I.beta[k] * om_cur[k - 1];
+        t = bj1 > 0.0 ? t / bj1 : 0.0;
+        t += t >= 0.0 ? eps1 : -eps1;
+        om_next[k] = t;
+        worst = fmax(worst, fabs(t));
+    }
+    for (int off = 32; off > 0; off >>= 1) worst = fmax(worst, __shfl_down(worst, off, 64));
+    worst = __shfl(worst, 0, 64);
+    const bool second = I.pro[1] != 0.0;
+    const bool run = bj1 > 0.0 && (second || worst > 1.4901161193847656e-08);  // sqrt(eps)
+    if (run)
+        for (int k = lane; k <= j; k += 64) om_next[k] = eps1;
+    if (lane == 0) {
+        if (!run) om_next[j] = eps1;
+        om_next[j + 1] = 1.0;
+        I.pro[0] = run ? 1.0 : 0.0;
+        I.pro[1] = (run && !second) ? 1.0 : 0.0;
+        if (run) I.pro[2] += 1.0;
+    }
 }
 
 __global__ void bk_advance(int *it_p) {
@@ -330,6 +376,7 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         const int M = ((o.max_iter + chunk - 1) / chunk) * chunk;
         const double tol = o.tol > 0 ? o.tol : 1e-13;
         const bool reorth = o.reorth != 0;
+        const bool partial = o.reorth == 2;  // device-side decisions (bk_pro)
         const int k_eigs = o.n_eigs;
 
         // families: real first, then complex
@@ -351,6 +398,20 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         DevBuf gates((size_t)n * sizeof(double));
         QMB_HIP(hipMemsetAsync(scal.p, 0, scal.bytes, ctx->stream));
         QMB_HIP(hipMemsetAsync(gates.p, 0, gates.bytes, ctx->stream));
+        // orthogonality estimates: row 0 = om_0 = e_0, rows 1, 2 zero; flags:
+        // full reorthogonalisation runs every step (pro[0] = 1 throughout)
+        const size_t om_stride = (size_t)M + 2;
+        DevBuf omegas(partial ? (size_t)n * 3 * om_stride * sizeof(double) : 8), pros((size_t)n * 3 * sizeof(double));
+        {
+            std::vector<double> hp((size_t)n * 3, 0.0);
+            for (int i = 0; i < n; ++i) hp[(size_t)i * 3] = partial ? 0.0 : 1.0;
+            QMB_HIP(hipMemcpy(pros.p, hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice));
+            if (partial) {
+                std::vector<double> ho((size_t)n * 3 * om_stride, 0.0);
+                for (int i = 0; i < n; ++i) ho[(size_t)i * 3 * om_stride] = 1.0;
+                QMB_HIP(hipMemcpy(omegas.p, ho.data(), ho.size() * sizeof(double), hipMemcpyHostToDevice));
+            }
+        }
         for (int pos = 0; pos < n; ++pos) {
             const qmb_ell *e = ells[order[pos]];
             const int vs = e->value_type == QMB_COMPLEX ? 2 : 1;
@@ -374,6 +435,8 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             I.invb = sc + 2 * M + 1;
             I.coef = coefs.as<double>() + (size_t)pos * M * 2;
             I.gate = gates.as<double>() + pos;
+            I.pro = pros.as<double>() + 3 * (size_t)pos;
+            I.omega = partial ? omegas.as<double>() + (size_t)pos * 3 * om_stride : nullptr;
         }
         // vector task tables (ranges in doubles, both families in one table per family)
         int total_vec_blocks = 0;
@@ -524,6 +587,9 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                                    vs);
                 hipLaunchKernelGGL(bk_finish, dim3(nm), dim3(kThreads), 0, s, d_inst.as<BInst>(), d_active.as<int>(),
                                    d_it.as<int>(), part.as<double>(), f.first_inst, 1);
+                if (partial)
+                    hipLaunchKernelGGL(bk_pro, dim3(nm), dim3(64), 0, s, d_inst.as<BInst>(), d_active.as<int>(),
+                                       d_it.as<int>(), f.first_inst, M);
                 if (reorth) {
                     for (int pass = 0; pass < 2; ++pass) {
                         hipLaunchKernelGGL(f.cplx ? bk_rdots<true> : bk_rdots<false>, dim3(nvb), dim3(kThreads), 0,
@@ -620,6 +686,10 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         const double wall =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
         bool all_conv = true;
+        std::vector<double> hpro((size_t)n * 3, 0.0);
+        QMB_HIP(hipMemcpyAsync(hpro.data(), pros.p, hpro.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
         for (int pos = 0; pos < n; ++pos) {
             const int gi = order[pos];
             for (int j = 0; j < k_eigs; ++j) evals[(size_t)gi * k_eigs + j] = ev[pos][j];
@@ -631,7 +701,7 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                 r.converged = conv[pos];
                 r.breakdown = brk[pos];
                 r.n_spmv = iters[pos];
-                r.n_reorth = reorth ? iters[pos] : 0;
+                r.n_reorth = !reorth ? 0 : partial ? (int)hpro[(size_t)pos * 3 + 2] : iters[pos];
                 for (int j = 0; j < std::min(k_eigs, 8); ++j) r.residual[j] = res[pos][j];
                 r.ms_total = wall;
                 r.ms_host = ms_host;

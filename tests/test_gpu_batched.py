@@ -49,6 +49,31 @@ def test_batched_matches_dense_small(ctx):
             np.testing.assert_allclose(got, want, atol=1e-10 * max(1.0, abs(want[0])))
 
 
+def test_batched_partial_reorthogonalisation_matches_dense(ctx):
+    """reorth = 2 in the batch: the per-instance device decisions keep the lowest
+    four eigenvalues exact while orthogonalising on only part of the steps."""
+    print("[gpu-case] batched chain12 lowest-4 partial reorthogonalisation", flush=True)
+    keep, ells, labels = sector_ells(ctx, 12, [6, 4], J2=0.3, delta=0.9)
+    ev, reps = q.lanczos_batched(ells, n_eigs=4, max_iter=300, tol=1e-13, reorth=2)
+    full, _ = q.lanczos_batched(ells, n_eigs=4, max_iter=300, tol=1e-13, reorth=1)
+    case = chain_case(12, 1.0, 0.9, 0.3)
+    partial_steps = 0
+    for n_up in (6, 4):
+        orb = oracle_orbits(case, n_up)
+        for a in range(12):
+            if (n_up, a) not in labels:
+                continue
+            i = labels.index((n_up, a))
+            E = np.linalg.eigvalsh(ell_dense_nonnull(oracle_ell(case, orb, a), orb, a))
+            want = E[:4]
+            assert reps[i].converged
+            np.testing.assert_allclose(ev[i][: len(want)], want, atol=1e-10 * max(1.0, abs(want[0])))
+            np.testing.assert_allclose(ev[i][: len(want)], full[i][: len(want)], atol=1e-10 * max(1.0, abs(want[0])))
+            assert reps[i].n_reorth <= reps[i].iters + 10
+            partial_steps += reps[i].n_reorth < reps[i].iters
+    assert partial_steps > 0
+
+
 def test_batched_equals_sequential_ground_states(ctx):
     print("[gpu-case] batched chain16 ground states", flush=True)
     keep, ells, labels = sector_ells(ctx, 16, [8, 6])
