@@ -258,11 +258,38 @@ static bool detect_torus(qmb_group &g) {
 //   tab[v]        = min over s of rot_B(v, s)          (v < 2^B)
 //   tab[256 + v]  = bit s set when rot_B(v, s) equals it
 //   tab[512 + (f*8 + a)*8 + s] = element index of (flip f, block shift a, s)
+// Rings (one block of N sites, every rotation s and, with flip, both halves,
+// 8 <= N <= 56): the top 8 bits of an image are a circular 8-site window of
+// the source, so only the rotations whose window is the smallest need full
+// images. tab[512 + f*64 + s] = element index of (flip f, rotation s).
+static bool build_ring_lut(qmb_group &g) {
+    const int N = g.n_sites;
+    if (g.block != N || N < 8 || N > 56) return false;
+    int hf = 0;
+    for (int e = 0; e < g.G; ++e) hf |= g.flip[e] ? 1 : 0;
+    std::vector<int> gid(2 * 64, -1);
+    for (int e = 0; e < g.G; ++e) {
+        if (g.big[e] != 0) return false;
+        const int i = (g.flip[e] ? 1 : 0) * 64 + g.small[e];
+        if (gid[i] >= 0) return false;
+        gid[i] = e;
+    }
+    for (int f = 0; f <= hf; ++f)
+        for (int s = 0; s < N; ++s)
+            if (gid[f * 64 + s] < 0) return false;
+    g.lut_tab.assign(kLutBytes, 0);
+    for (int i = 0; i < 2 * 64; ++i) g.lut_tab[512 + i] = (uint8_t)(gid[i] < 0 ? 0 : gid[i]);
+    g.lut = 2;
+    g.lut_flip = hf;
+    return true;
+}
+
 static void build_lut(qmb_group &g) {
     g.lut = 0;
     g.lut_flip = 0;
     g.lut_tab.clear();
     if (!g.torus) return;
+    if (build_ring_lut(g)) return;
     const int N = g.n_sites, B = g.block, nb = N / B;
     if (B > 8 || nb > 8) return;
     int hf = 0;

@@ -241,6 +241,58 @@ __device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, i
     return best;
 }
 
+// orbit_min for a ring group (gv.lut == 2): the top 8 bits of rotation s of x
+// are the circular window of x starting at site p = (N - 8 - s) mod N
+// (complemented under flip). Pass 1 finds the smallest window and the lane's
+// candidate positions (bit masks); pass 2 takes full images of the candidates
+// only. Same result and same gm as orbit_min.
+template <class S>
+__device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, int &gm) {
+    const int N = gv.n_sites;  // 8 <= N <= 56: the doubled word holds bits p..p+7 for every p < N
+    const uint64_t xx = (uint64_t)x | ((uint64_t)x << N);
+    const bool hf = gv.lut_flip != 0;
+    unsigned tmin = 0x1FFu;
+    uint64_t c0 = 0, c1 = 0;
+    for (int p = 0; p < N; ++p) {
+        const unsigned w = (unsigned)(xx >> p) & 0xFFu;
+        if (w < tmin) {
+            tmin = w;
+            c0 = 0;
+            c1 = 0;
+        }
+        if (w == tmin) c0 |= 1ull << p;
+        if (hf) {
+            const unsigned wf = w ^ 0xFFu;
+            if (wf < tmin) {
+                tmin = wf;
+                c0 = 0;
+                c1 = 0;
+            }
+            if (wf == tmin) c1 |= 1ull << p;
+        }
+    }
+    S best = 0;
+    int bi = -1;
+    for (int f = 0; f < (hf ? 2 : 1); ++f) {
+        uint64_t c = f ? c1 : c0;
+        while (c) {
+            const int p = __builtin_ctzll(c);
+            c &= c - 1;
+            int s = N - 8 - p;
+            if (s < 0) s += N;
+            const int g = t.lut[512 + f * 64 + s];
+            const S y = apply_elem_div<S>(x, g, gv, t, nullptr);
+            const int ig = t.inv[g];
+            if (bi < 0 || y < best || (y == best && ig < bi)) {
+                best = y;
+                bi = ig;
+            }
+        }
+    }
+    gm = bi;
+    return best;
+}
+
 // Row of a representative through the basis bitmap (rank -> word -> prefix +
 // popcount). Returns -1 when the representative is not a row.
 template <class S>

@@ -123,7 +123,9 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                 S tgt;
                 apply_term<S>(tv, h, src, tgt, cr, ci);
                 int gm;
-                const S rep = gv.lut ? orbit_min_lut<S>(tgt, gv, gt, gm) : orbit_min<S>(tgt, gv, perm_lds, gm);
+                const S rep = gv.lut == 1   ? orbit_min_lut<S>(tgt, gv, gt, gm)
+                              : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm)
+                                            : orbit_min<S>(tgt, gv, perm_lds, gm);
                 const int64_t r2 = row_of_rep<S>(rep, bin, bitmap, prefix);
                 if (r2 < 0) continue;
                 if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;

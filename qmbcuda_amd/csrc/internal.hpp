@@ -101,7 +101,8 @@ struct GroupView {
     const uint8_t *flip;   // [G]
     const uint8_t *perm;   // [G*n_sites] generic destination table
     const uint8_t *inv;    // [G]
-    int lut;               // 1: the group is every (block shift, in-block shift[, flip]) of a torus
+    int lut;               // 1: the group is every (block shift, in-block shift[, flip]) of a torus;
+                           // 2: every rotation[, flip] of a ring
     int lut_flip;          //    with a spin-flip half
     const uint8_t *lut_tab;  // [kLutBytes] rotation minima and element index table (see device_common.hpp)
 };
@@ -176,7 +177,7 @@ struct qmb_group : qmb::Counted {
     int torus = 0, block = 0;
     std::vector<uint64_t> keep;
     std::vector<uint8_t> small, big;
-    int lut = 0, lut_flip = 0;     // full torus product with block width <= 8
+    int lut = 0, lut_flip = 0;     // 1: full torus product with block width <= 8; 2: full ring
     std::vector<uint8_t> lut_tab;  // [kLutBytes]
 };
 
