@@ -269,6 +269,7 @@ def run_c3(args):
         walls["batched_" + mode] = time.perf_counter() - t0
         if mode == "partial":
             reps_partial = reps
+    for mode, rmode in (("full", 1), ("partial", 2)):
         t0 = time.perf_counter()
         evals["sequential_" + mode] = np.array([q.lanczos(e, n_eigs=4, max_iter=400, tol=1e-13, reorth=rmode)[0]
                                                for e in ells])
