@@ -108,3 +108,37 @@ def test_ctx_create_without_gpu_reports_no_device():
         pytest.skip("a GPU is visible")
     h = C.c_void_p()
     assert q.lib().qmb_ctx_create(0, C.byref(h)) == 5
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of the ABI structs have the header's sizes and offsets
+    (a C program compiled against include/qmb.h prints them)."""
+    import subprocess
+
+    structs = {
+        "qmb_lanczos_opts": _abi.LanczosOpts,
+        "qmb_lanczos_report": _abi.LanczosReport,
+        "qmb_basis_info": _abi.BasisInfo,
+        "qmb_ell_info": _abi.EllInfo,
+    }
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "qmb.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            a, b, c = ln.split()
+            got[(a, b)] = int(c)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
