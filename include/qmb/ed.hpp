@@ -41,6 +41,8 @@ struct EDOptions {
     double tol = 1e-13;            // relative eigenvalue tolerance when not fixed_iter
     int max_iter = 2000;           // iteration cap when not fixed_iter
     int device = 0;
+    bool batched = false;          // not fixed_iter: solve all irreps in one qmb_lanczos_batched call
+                                   // (the ground-state irrep is then re-solved alone for its Krylov vector)
 };
 
 template <class T>
@@ -88,14 +90,18 @@ class HeisenbergXXZAbelianSymms {
                   "qmb_terms_create");
             const int a0 = opts_.irrep < 0 ? 0 : opts_.irrep;
             const int a1 = opts_.irrep < 0 ? NIr : opts_.irrep + 1;
-            for (int a = 0; a < NIr; ++a) {
-                if (a < a0 || a >= a1) {
-                    E_vecs.push_back({std::numeric_limits<double>::infinity()});
-                    seed_vecs.push_back({seed});
-                    KrylovGS_vecs.push_back({});
-                    continue;
+            if (opts_.batched && !opts_.fixed_iter) {
+                solve_batched(ctx, grp, terms, a0, a1, NIr, ch, G, seed);
+            } else {
+                for (int a = 0; a < NIr; ++a) {
+                    if (a < a0 || a >= a1) {
+                        E_vecs.push_back({std::numeric_limits<double>::infinity()});
+                        seed_vecs.push_back({seed});
+                        KrylovGS_vecs.push_back({});
+                        continue;
+                    }
+                    solve_irrep(ctx, grp, terms, a, ch, G, seed);
                 }
-                solve_irrep(ctx, grp, terms, a, ch, G, seed);
             }
         } catch (...) {
             qmb_terms_destroy(terms);
@@ -129,8 +135,7 @@ class HeisenbergXXZAbelianSymms {
     std::vector<int> get_iterations() const { return iterations; }
 
   protected:
-    void solve_irrep(qmb_ctx *ctx, qmb_group *grp, qmb_terms *terms, int a, const std::vector<double> &ch, int G,
-                     long long seed) {
+    bool irrep_is_real(int a, const std::vector<double> &ch, int G) const {
         bool real = true;
         for (int g = 0; g < G; ++g) real = real && ch[((size_t)a * G + g) * 2 + 1] == 0.0;
         if constexpr (std::is_same_v<T, std::complex<double>>) {
@@ -140,6 +145,75 @@ class HeisenbergXXZAbelianSymms {
                 real = real && s.imag() == 0.0;
             }
         }
+        return real;
+    }
+
+    // Every irrep in [a0, a1) in one batched solve; the ground-state irrep is then
+    // solved alone once more for its Krylov coefficients (batched solves keep none).
+    void solve_batched(qmb_ctx *ctx, qmb_group *grp, qmb_terms *terms, int a0, int a1, int NIr,
+                       const std::vector<double> &ch, int G, long long seed) {
+        std::vector<qmb_basis *> bases(NIr, nullptr);
+        std::vector<qmb_ell *> ells(NIr, nullptr);
+        auto cleanup = [&]() {
+            for (auto *e : ells) qmb_ell_destroy(e);
+            for (auto *b : bases) qmb_basis_destroy(b);
+        };
+        try {
+            std::vector<const qmb_ell *> batch;
+            std::vector<int> which;
+            E_vecs.assign(NIr, {std::numeric_limits<double>::infinity()});
+            seed_vecs.assign(NIr, {seed});
+            KrylovGS_vecs.assign(NIr, {});
+            iterations.assign(NIr, 0);
+            sector_dims.clear();
+            for (int a = a0; a < a1; ++a) {
+                check(qmb_basis_build(ctx, grp, GS_sector_, a,
+                                      opts_.reference_layout ? QMB_LAYOUT_REFERENCE : QMB_LAYOUT_COMPACT, &bases[a]),
+                      "qmb_basis_build");
+                qmb_basis_info bi;
+                qmb_basis_get_info(bases[a], &bi);
+                sector_dims.push_back(bi.n_rows - bi.n_null);
+                if (bi.n_rows == 0) continue;
+                check(qmb_ell_build(ctx, bases[a], terms, irrep_is_real(a, ch, G) ? QMB_REAL : QMB_COMPLEX,
+                                    max_terms, &ells[a]),
+                      "qmb_ell_build");
+                batch.push_back(ells[a]);
+                which.push_back(a);
+            }
+            if (!batch.empty()) {
+                qmb_lanczos_opts o;
+                qmb_lanczos_default_opts(&o);
+                o.seed = (uint64_t)seed;
+                o.max_iter = opts_.max_iter;
+                o.tol = opts_.tol;
+                std::vector<double> E(batch.size());
+                std::vector<qmb_lanczos_report> reps(batch.size());
+                check(qmb_lanczos_batched(ctx, (int)batch.size(), batch.data(), &o, E.data(), reps.data()),
+                      "qmb_lanczos_batched");
+                int best = 0;
+                for (size_t i = 0; i < batch.size(); ++i) {
+                    E_vecs[which[i]] = {E[i]};
+                    iterations[which[i]] = reps[i].iters;
+                    if (E[i] < E[best]) best = (int)i;
+                }
+                // Krylov coefficients of the ground-state irrep
+                std::vector<double> kr(o.max_iter, 0.0);
+                qmb_lanczos_report rep;
+                double Eg = 0.0;
+                check(qmb_lanczos(ctx, batch[best], &o, &Eg, kr.data(), nullptr, &rep), "qmb_lanczos");
+                kr.resize(rep.iters);
+                KrylovGS_vecs[which[best]] = kr;
+            }
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    }
+
+    void solve_irrep(qmb_ctx *ctx, qmb_group *grp, qmb_terms *terms, int a, const std::vector<double> &ch, int G,
+                     long long seed) {
+        const bool real = irrep_is_real(a, ch, G);
         qmb_basis *b = nullptr;
         qmb_ell *e = nullptr;
         check(qmb_basis_build(ctx, grp, GS_sector_, a,

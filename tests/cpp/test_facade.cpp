@@ -109,9 +109,10 @@ static void print_tables(const Case &c) {
     std::printf("]}\n");
 }
 
-static void solve(const Case &c, bool fixed) {
+static void solve(const Case &c, bool fixed, bool batched = false) {
     EDOptions o;
     o.fixed_iter = fixed;
+    o.batched = batched;
     HeisenbergXXZAbelianSymms<cd> ham(c.model, c.group, -1, -1, 1, 2212456, fixed ? 60 : 50, 1e-5, o);
     const auto E = ham.get_E_vecs();
     std::printf("{\"E\": [");
@@ -128,7 +129,7 @@ static void solve(const Case &c, bool fixed) {
 
 int main(int argc, char **argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s tables|solve|solve_fixed <case>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s tables|solve|solve_fixed|solve_batched <case>\n", argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -140,6 +141,8 @@ int main(int argc, char **argv) {
             solve(c, false);
         else if (mode == "solve_fixed")
             solve(c, true);
+        else if (mode == "solve_batched")
+            solve(c, false, true);
         else
             return 2;
     } catch (const std::exception &e) {

@@ -101,6 +101,23 @@ def test_facade_driver_solves_every_irrep(name, n_up):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", ["chain16", "square2x4_example"])
+def test_facade_driver_batched_equals_sequential(name):
+    """EDOptions::batched solves every irrep in one batched call: same energies,
+    same ground-state irrep, a Krylov vector for it."""
+    print(f"[gpu-case] facade solve_batched {name}", flush=True)
+    seq = run("solve", name)
+    bat = run("solve_batched", name)
+    assert bat["GS_irrep"] == seq["GS_irrep"]
+    assert abs(bat["E_gs"] - seq["E_gs"]) < 1e-10 * max(1, abs(seq["E_gs"]))
+    for a, b in zip(seq["E"], bat["E"]):
+        assert (a is None) == (b is None)
+        if a is not None:
+            assert abs(a - b) < 1e-10 * max(1, abs(a))
+    assert bat["krylov_len"] > 0
+
+
+@pytest.mark.gpu
 def test_facade_fixed_iteration_mode():
     print("[gpu-case] facade solve_fixed chain16", flush=True)
     got = run("solve_fixed", "chain16")
