@@ -229,6 +229,27 @@ double qmb_lanczos_eigvals_ref(double *E_tmp, int NStates, int NIter, int nobv, 
                                int exportKrylovVec, int checkEigState, int is_complex,
                                qmb_status *status);
 
+/* ---- expectation values ------------------------------------------------------
+ * Replaces the device part of ComputeStaticExpValZeroT
+ * (Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:421-601): the kernel
+ * ExpecValArbOperatorAbelianIrrepProjHeisenbergZeroT (Q/exact_diagonalization/utils.cu:451-548)
+ * and its reduction (:583-590). out[0..1] = <Psi|A|Psi> (re, im), where Psi is the
+ * many-body state of the vector x of the basis's sector (n_rows values, interleaved
+ * re/im when is_complex; x on the host, or on the device when x_on_device): for an
+ * eigenvector of the sector matrix, its eigenstate (no normalisation is applied).
+ * A is any Sz-conserving operator as a conjugated term table, in the same form as
+ * the Hamiltonian's. It need not commute with the group: the sector sees its group
+ * average (1/G) sum_g g A g^-1, taken over every image of each row's representative;
+ * when A commutes with the group (checked on the host) the average is skipped.
+ * flags = QMB_EXPECT_GROUP_AVERAGE forces the average; *averaged (optional) = 1 when
+ * it was taken. Differences from the reference kernel: every image of the source
+ * contributes (it stops at the first image a term annihilates), and the pairing of
+ * x with its conjugate is that of <Psi|A|Psi> (the reference's equals it only for
+ * vectors that are real up to a phase); FP64, no tol_norm. */
+enum { QMB_EXPECT_GROUP_AVERAGE = 1 };
+qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a, int is_complex,
+                           const double *x, int x_on_device, int flags, double *out, int *averaged);
+
 /* ---- batched Lanczos over many sectors (new; no reference counterpart) -------
  * The reference solves irreps one after another (HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:127-229).
  * This solves n independent ELLs together: one launch per phase covers every

@@ -70,6 +70,8 @@ class HeisenbergXXZAbelianSymms {
                 ch[((size_t)a * G + g) * 2 + 1] = c.imag();
             }
         if (flips.empty()) flips.assign(G, 0);
+        ch_ = ch;
+        flips_ = flips;
         const OperatorVectors<T> ov = BuildOperatorVecs(lattice_model.GetH());
         std::vector<double> sc((size_t)ov.H_size * 2);
         for (int h = 0; h < ov.H_size; ++h) {
@@ -133,6 +135,80 @@ class HeisenbergXXZAbelianSymms {
     std::vector<double> GetKrylovGS() const { return KrylovGS_vec; }
     std::vector<int64_t> get_sector_dims() const { return sector_dims; }
     std::vector<int> get_iterations() const { return iterations; }
+
+    // <Psi|A|Psi> / <Psi|Psi> in the ground state (ComputeStaticExpValZeroT,
+    // Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:421-601). Like the
+    // reference, it rebuilds the ground-state irrep's basis and Hamiltonian and regenerates
+    // the ground state from its seed, then evaluates qmb_expectation on the device. A is any
+    // Sz-conserving operator; it need not commute with the group. max_terms <= 0: the
+    // Hamiltonian's row width. The reference returns its sum without normalising by
+    // <Psi|Psi> (its regenerated vector has norm close to 1); this returns the normalised value.
+    std::complex<double> ComputeStaticExpValZeroT(const ManyBodyOperator<std::complex<double>> &A,
+                                                  int max_terms_in = 0) const {
+        struct Handles {
+            qmb_ctx *ctx = nullptr;
+            qmb_group *grp = nullptr;
+            qmb_terms *th = nullptr, *ta = nullptr;
+            qmb_basis *b = nullptr;
+            qmb_ell *e = nullptr;
+            ~Handles() {
+                qmb_ell_destroy(e);
+                qmb_basis_destroy(b);
+                qmb_terms_destroy(ta);
+                qmb_terms_destroy(th);
+                qmb_group_destroy(grp);
+                qmb_ctx_destroy(ctx);
+            }
+        } h;
+        const int G = static_cast<int>(flips_.size());
+        const int NIr = static_cast<int>(ch_.size() / (2 * (size_t)G));
+        check(qmb_ctx_create(opts_.device, &h.ctx), "qmb_ctx_create");
+        check(qmb_group_create(LS, G, group_el_arr.data(), flips_.data(), NIr, ch_.data(), &h.grp),
+              "qmb_group_create");
+        auto make_terms = [&](const auto &ov, qmb_terms **out) {
+            std::vector<double> sc((size_t)ov.H_size * 2);
+            for (int i = 0; i < ov.H_size; ++i) {
+                sc[2 * i] = std::complex<double>(ov.H_scalar_table[i]).real();
+                sc[2 * i + 1] = std::complex<double>(ov.H_scalar_table[i]).imag();
+            }
+            check(qmb_terms_create(LS, ov.H_size, ov.H_col_size, ov.H_decode_table.data(), ov.H_site_table.data(),
+                                   sc.data(), ov.H_NOTerms.data(), out),
+                  "qmb_terms_create");
+        };
+        make_terms(BuildOperatorVecs(lattice_model.GetH()), &h.th);
+        make_terms(BuildOperatorVecs(A), &h.ta);
+        const bool real = irrep_is_real(GS_irrep, ch_, G);
+        check(qmb_basis_build(h.ctx, h.grp, GS_sector_, GS_irrep,
+                              opts_.reference_layout ? QMB_LAYOUT_REFERENCE : QMB_LAYOUT_COMPACT, &h.b),
+              "qmb_basis_build");
+        qmb_basis_info bi;
+        qmb_basis_get_info(h.b, &bi);
+        if (bi.n_rows == 0) throw Status(QMB_ERR_INVALID, "ComputeStaticExpValZeroT: no ground state");
+        check(qmb_ell_build(h.ctx, h.b, h.th, real ? QMB_REAL : QMB_COMPLEX,
+                            max_terms_in > 0 ? max_terms_in : max_terms, &h.e),
+              "qmb_ell_build");
+        qmb_lanczos_opts o;
+        qmb_lanczos_default_opts(&o);
+        o.seed = (uint64_t)GS_seed;
+        o.want_eigvec = 1;
+        if (opts_.fixed_iter) {
+            o.max_iter = NIter_;
+            o.fixed_iter = 1;
+        } else {
+            o.max_iter = opts_.max_iter;
+            o.tol = opts_.tol;
+        }
+        std::vector<double> x((size_t)bi.n_rows * (real ? 1 : 2));
+        double E = 0.0;
+        qmb_lanczos_report rep;
+        const qmb_status st = qmb_lanczos(h.ctx, h.e, &o, &E, nullptr, x.data(), &rep);
+        if (st != QMB_OK && !(opts_.fixed_iter && st == QMB_ERR_NOT_CONVERGED)) check(st, "qmb_lanczos");
+        double out[2] = {0.0, 0.0};
+        check(qmb_expectation(h.ctx, h.b, h.ta, real ? 0 : 1, x.data(), 0, 0, out, nullptr), "qmb_expectation");
+        double nn = 0.0;
+        for (double v : x) nn += v * v;
+        return std::complex<double>(out[0], out[1]) / nn;
+    }
 
   protected:
     bool irrep_is_real(int a, const std::vector<double> &ch, int G) const {
@@ -292,6 +368,8 @@ class HeisenbergXXZAbelianSymms {
     int hop_f_, NStates_, NIter_;
     double tol_norm_;
     EDOptions opts_;
+    std::vector<double> ch_;      // characters [NIr][G] (re, im)
+    std::vector<uint8_t> flips_;  // spin flip of each element
     int max_terms = 0;
     std::vector<uint32_t> group_el_arr;
     std::vector<std::vector<double>> E_vecs;

@@ -6,5 +6,6 @@
 #include "../qmb.h"
 #include "ed.hpp"
 #include "lattice.hpp"
+#include "observables.hpp"
 #include "operators.hpp"
 #include "symmetry.hpp"

@@ -69,6 +69,10 @@ def lib():
     L.orc_ell_build.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, _f64p, c.c_int, c.c_int, c.c_int,
                                 _i32p, _i32p, _f64p, _i32p, _u64p, c.c_int64, _i64p, _f64p, _i64p,
                                 c.c_double, c.c_int, c.c_int, _u32p, _f64p, _i16p]
+    L.orc_expectation.restype = None
+    L.orc_expectation.argtypes = [c.c_int, c.c_int, _u32p, c.c_void_p, c.c_int, _f64p, c.c_int, c.c_int, c.c_int,
+                                  _i32p, _i32p, _f64p, _i32p, _u64p, c.c_int64, _i64p, _f64p, _i64p, c.c_double,
+                                  c.c_int, _f64p, _f64p]
     L.orc_spmv.restype = None
     L.orc_spmv.argtypes = [c.c_int64, c.c_int, _u32p, _f64p, _i16p, c.c_int, _f64p, _f64p]
     L.orc_start_vector.restype = None
@@ -265,6 +269,24 @@ class Ell:
         r = lib().orc_lanczos_eigvec(self.n_rows, self.W, self.ind, self.val, self.track, int(self.is_complex),
                                      len(kr), seed, st, kr, e0, x.view(np.float64))
         return r, x
+
+
+def expectation(orbits: Orbits, term_table, alpha, x):
+    """<Psi|A|Psi> of the reference-layout sector vector x (orc_expectation): A
+    given as a conjugated term table (op, site, scalar, n_ops, h_col), x real or
+    complex over the orbit rows. Returns a complex number."""
+    op, site, scalar, n_ops, h_col = term_table
+    o = orbits
+    cplx = np.iscomplexobj(x)
+    xx = np.ascontiguousarray(x, dtype=np.complex128 if cplx else np.float64)
+    sc = np.ascontiguousarray(np.asarray(scalar, np.complex128).view(np.float64))
+    out = np.zeros(2, np.float64)
+    fl = None if o.flip is None else o.flip.ctypes.data
+    lib().orc_expectation(o.n_sites, o.G, o.group_el, fl, o.n_irreps, o.chars, alpha, len(n_ops), h_col,
+                          np.ascontiguousarray(op, np.int32), np.ascontiguousarray(site, np.int32), sc,
+                          np.ascontiguousarray(n_ops, np.int32), o.bas, o.n_orb, o.rep_rank, o.norms,
+                          o.orbit_idx, o.tol_norm, int(cplx), xx.view(np.float64), out)
+    return complex(out[0], out[1])
 
 
 def spmv_rows(W, ind, val, track, x, is_complex=False):

@@ -19,6 +19,12 @@
  *                        equals the dense full-sector spectrum (numpy, N <= 14);
  *                        Hermiticity after merging duplicate columns
  *   orc_spmv             dense matvec of the exported matrix (numpy)
+ *   orc_expectation      <Psi|A|Psi> computed densely in the full Sz sector from
+ *                        the operator algebra (numpy), with Psi embedded through
+ *                        orbit states that are checked to carry H's eigenvectors
+ *                        (H Psi = E Psi); covers complex irreps, spin flips,
+ *                        operators that do not commute with the group and
+ *                        non-Hermitian operators
  *   orc_lanczos*         dense eigenvalues (numpy) and closed forms
  *                        (Majumdar-Ghosh, 2-site, 4-ring, known E0 values)
  *   orc_tridiag_ql       numpy.linalg.eigh on random symmetric tridiagonals
@@ -377,6 +383,78 @@ int orc_ell_build(int n_sites, int n_up, int G, int n_irreps, const double *char
         track[id] = (int16_t)slot;
     }
     return overflow;
+}
+
+/* <Psi|A|Psi> for the sector vector x (reference layout rows), the restatement of
+ * ExpecValArbOperatorAbelianIrrepProjHeisenbergZeroT (Q/exact_diagonalization/
+ * utils.cu:451-548) and its reduction and division by G
+ * (Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:583-590):
+ * for every source row, every image g.rep (gi = 0..G-1, the reference's
+ * SRS_states row) and every term of the conjugated table A: apply the term,
+ * find the target's orbit o and element gm (orbit_idx), skip null orbits and
+ * accumulate c * conj(chi(gi)) * chi(gm) * (n_o / n_src) * x_o into acc; then
+ * out = sum_src x_src * conj(acc) / G. Two corrections to the reference: the
+ * image loop continues past an image the term annihilates (the reference breaks
+ * out of it), and x is paired as in <Psi|A|Psi> with Psi = sum conj(x_r) v_r
+ * (the reference's conj(phi_src) ... phi_o differs for complex vectors). */
+void orc_expectation(int n_sites, int G, const uint32_t *group_el, const uint8_t *flip,
+                     int n_irreps, const double *chars, int alpha, int n_terms, int max_ops,
+                     const int32_t *op_type, const int32_t *site, const double *scalar,
+                     const int32_t *n_ops, const uint64_t *bas, int64_t n_orb,
+                     const int64_t *rep_rank, const double *norms, const int64_t *orbit_idx,
+                     double tol_norm, int is_complex, const double *x, double *out) {
+    const double *ch = chars + (size_t)alpha * G * 2;
+    double tr = 0.0, ti = 0.0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : tr, ti)
+    for (int64_t id = 0; id < n_orb; id++) {
+        double cur = norms[id * n_irreps + alpha];
+        if (cur < tol_norm) continue;
+        uint64_t rep = bas[rep_rank[id]];
+        double ar = 0.0, ai = 0.0;
+        for (int gi = 0; gi < G; gi++) {
+            uint64_t img = orc_apply(rep, group_el + (size_t)gi * n_sites, flip ? flip[gi] : 0, n_sites);
+            double jr = ch[2 * gi], ji = -ch[2 * gi + 1]; /* conj(chi(gi)) */
+            for (int hi = 0; hi < n_terms; hi++) {
+                uint64_t tgt = img;
+                double cr = scalar[2 * hi], ci = scalar[2 * hi + 1];
+                int dead = 0;
+                for (int k = 0; k < n_ops[hi]; k++) {
+                    int s = site[hi * max_ops + k];
+                    int op = op_type[hi * max_ops + k];
+                    int spin = (int)((tgt >> s) & 1ULL);
+                    if (op == 2) {
+                        if (!spin) { dead = 1; break; }
+                        tgt &= ~(1ULL << s);
+                    } else if (op == 1) {
+                        if (spin) { dead = 1; break; }
+                        tgt |= 1ULL << s;
+                    } else {
+                        double f = -1.0 + 2.0 * (double)spin;
+                        cr = cr * 0.5 * f;
+                        ci = ci * 0.5 * f;
+                    }
+                }
+                if (dead) continue;
+                int64_t t = orc_rank_colex(tgt);
+                int64_t o = orbit_idx[2 * t];
+                int64_t gm = orbit_idx[2 * t + 1];
+                double on = norms[o * n_irreps + alpha];
+                if (on < tol_norm) continue;
+                double ratio = on / cur;
+                double qr = jr * ch[2 * gm] - ji * ch[2 * gm + 1];
+                double qi = jr * ch[2 * gm + 1] + ji * ch[2 * gm];
+                double fr = (cr * qr - ci * qi) * ratio, fi = (cr * qi + ci * qr) * ratio;
+                double xr = is_complex ? x[2 * o] : x[o], xi = is_complex ? x[2 * o + 1] : 0.0;
+                ar += fr * xr - fi * xi;
+                ai += fr * xi + fi * xr;
+            }
+        }
+        double sr = is_complex ? x[2 * id] : x[id], si = is_complex ? x[2 * id + 1] : 0.0;
+        tr += (sr * ar + si * ai) / (double)G;
+        ti += (si * ar - sr * ai) / (double)G;
+    }
+    out[0] = tr;
+    out[1] = ti;
 }
 
 /* SparseMatDenseVec_prod_EEL (Q/exact_diagonalization/utils.cu:601-624):

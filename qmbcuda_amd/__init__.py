@@ -12,8 +12,8 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import (COMPLEX, LAYOUT_COMPACT, LAYOUT_REFERENCE, REAL, START_RANDOM, START_USER, BasisInfo,
-                   EllInfo, LanczosOpts, LanczosReport)
+from ._abi import (COMPLEX, EXPECT_GROUP_AVERAGE, LAYOUT_COMPACT, LAYOUT_REFERENCE, REAL, START_RANDOM,
+                   START_USER, BasisInfo, EllInfo, LanczosOpts, LanczosReport)
 from . import models
 
 __all__ = ["QMBError", "Context", "Group", "Terms", "Basis", "Ell", "lanczos", "lanczos_batched", "models",
@@ -146,6 +146,20 @@ class Basis:
         el = np.zeros(len(st), np.int32)
         _check(lib().qmb_basis_lookup(self.h, len(st), _ptr(st), _ptr(rows), _ptr(el)))
         return rows, el
+
+    def expectation(self, terms: "Terms", x, group_average: bool = False):
+        """<Psi|A|Psi> of the sector vector x (host array, n_rows values, complex
+        for a complex irrep); A = terms (a conjugated table). Returns (value, averaged):
+        averaged tells whether the group average of A was taken."""
+        cplx = np.iscomplexobj(x)
+        xx = np.ascontiguousarray(x, np.complex128 if cplx else np.float64)
+        if len(xx) != self.n_rows:
+            raise ValueError("x must have n_rows entries")
+        out = np.zeros(2, np.float64)
+        avg = C.c_int(0)
+        _check(lib().qmb_expectation(self.ctx.h, self.h, terms.h, int(cplx), _ptr(xx), 0,
+                                     EXPECT_GROUP_AVERAGE if group_average else 0, _ptr(out), C.byref(avg)))
+        return complex(out[0], out[1]), bool(avg.value)
 
     def __del__(self):
         try:

@@ -21,6 +21,7 @@ STATUS_NAMES = {
 LAYOUT_REFERENCE, LAYOUT_COMPACT = 0, 1
 REAL, COMPLEX = 0, 1
 START_RANDOM, START_USER = 0, 1
+EXPECT_GROUP_AVERAGE = 1
 
 
 class BasisInfo(C.Structure):
@@ -107,6 +108,7 @@ SIGNATURES = [
     ("qmb_ell_get_info", C.c_int, [_vp, C.POINTER(EllInfo)]),
     ("qmb_ell_export_reference", C.c_int, [_vp, _vp, _vp, _vp]),
     ("qmb_ell_export_rows", C.c_int, [_vp, C.c_int64, C.c_int64, _vp, _vp, _vp]),
+    ("qmb_expectation", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp, C.POINTER(C.c_int)]),
     ("qmb_spmv_host", C.c_int, [_vp, _vp, _vp, _vp]),
     ("qmb_spmv_device", C.c_int, [_vp, _vp, _vp, _vp]),
     ("qmb_spmv_benchmark", C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_double)]),

@@ -15,46 +15,11 @@
 #include <memory>
 
 #include "device_common.hpp"
+#include "terms.hpp"
 
 namespace qmb {
 
-struct TermsView {
-    int n_terms, max_ops;
-    const int8_t *op;      // [n_terms*max_ops]
-    const int8_t *site;    // [n_terms*max_ops]
-    const double2 *coef;   // [n_terms]
-    const int8_t *n_ops;   // [n_terms]
-};
-
 constexpr int kEllThreads = 256;
-
-// Conjugated operators of term h applied left to right to src (the reference's
-// inner loop, utils.cu:133-161). Returns false when the term annihilates src.
-template <class S>
-__device__ inline bool apply_term(const TermsView &tv, int h, S src, S &tgt, double &cr, double &ci) {
-    tgt = src;
-    const double2 c0 = tv.coef[h];
-    cr = c0.x;
-    ci = c0.y;
-    const int nk = tv.n_ops[h];
-    for (int k = 0; k < nk; ++k) {
-        const int s = tv.site[h * tv.max_ops + k];
-        const int op = tv.op[h * tv.max_ops + k];
-        const int spin = (int)((tgt >> s) & (S)1);
-        if (op == QMB_OP_SM) {
-            if (!spin) return false;
-            tgt &= (S)~((S)1 << s);
-        } else if (op == QMB_OP_SP) {
-            if (spin) return false;
-            tgt |= (S)1 << s;
-        } else {
-            const double f = -1.0 + 2.0 * (double)spin;
-            cr = cr * 0.5 * f;
-            ci = ci * 0.5 * f;
-        }
-    }
-    return true;
-}
 
 template <class S, bool FILL, bool CPLX>
 __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
@@ -243,31 +208,13 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         e->n_rows = b->n_rows;
         e->n_chunks = (b->n_rows + kChunk - 1) / kChunk;
         // term table and irrep data on device
-        const int nt = t->n_terms, mo = t->max_ops;
-        std::vector<int8_t> op8((size_t)std::max(1, nt * mo)), site8((size_t)std::max(1, nt * mo)),
-            nops8((size_t)std::max(1, nt));
-        std::vector<double> coef((size_t)std::max(1, nt) * 2, 0.0);
-        for (int i = 0; i < nt * mo; ++i) {
-            op8[i] = (int8_t)t->op_type[i];
-            site8[i] = (int8_t)(t->site[i] < 0 ? 0 : t->site[i]);
-        }
-        for (int h = 0; h < nt; ++h) {
-            nops8[h] = (int8_t)t->n_ops[h];
-            coef[2 * h] = t->scalar[2 * h];
-            coef[2 * h + 1] = t->scalar[2 * h + 1];
-        }
-        std::vector<double> nrm(g->G + 1, 0.0);
-        for (int m = 1; m <= g->G; ++m) nrm[m] = std::sqrt((double)g->G * (double)m) / (double)g->G;
-        DevBuf d_op(op8.size()), d_site(site8.size()), d_nops(nops8.size()), d_coef(coef.size() * sizeof(double)),
-            d_chi((size_t)g->G * 2 * sizeof(double)), d_nrm(nrm.size() * sizeof(double)), d_err(sizeof(int));
-        QMB_HIP(hipMemcpy(d_op.p, op8.data(), op8.size(), hipMemcpyHostToDevice));
-        QMB_HIP(hipMemcpy(d_site.p, site8.data(), site8.size(), hipMemcpyHostToDevice));
-        QMB_HIP(hipMemcpy(d_nops.p, nops8.data(), nops8.size(), hipMemcpyHostToDevice));
-        QMB_HIP(hipMemcpy(d_coef.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
+        const DeviceTerms dt(*t);
+        const TermsView tv = dt.view;
+        const std::vector<double> nrm = orbit_norm_table(g->G);
+        DevBuf d_chi((size_t)g->G * 2 * sizeof(double)), d_nrm(nrm.size() * sizeof(double)), d_err(sizeof(int));
         QMB_HIP(hipMemcpy(d_chi.p, ch, (size_t)g->G * 2 * sizeof(double), hipMemcpyHostToDevice));
         QMB_HIP(hipMemcpy(d_nrm.p, nrm.data(), nrm.size() * sizeof(double), hipMemcpyHostToDevice));
         QMB_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), ctx->stream));
-        TermsView tv{nt, mo, d_op.as<int8_t>(), d_site.as<int8_t>(), d_coef.as<double2>(), d_nops.as<int8_t>()};
         const GroupView gv = b->view();
         const int64_t n = b->n_rows;
         e->row_len.alloc((size_t)std::max<int64_t>(n, 1) * sizeof(uint16_t));

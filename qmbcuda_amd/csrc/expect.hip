@@ -1,0 +1,263 @@
+// Expectation values of Sz-conserving operators in a symmetry-reduced sector.
+//
+// Replaces the device part of ComputeStaticExpValZeroT
+// (Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:421-601):
+// the kernel ExpecValArbOperatorAbelianIrrepProjHeisenbergZeroT
+// (Q/exact_diagonalization/utils.cu:451-548) and its thrust::reduce.
+//
+// Rows of the sector matrix represent the states v_r = sum_g conj(chi(g)) g|rep_r>
+// (normalised); an eigenvector x of the sector matrix is the many-body state
+// Psi = sum_r conj(x_r) v_r. For any operator A with a conjugated term table,
+// <Psi|A|Psi> = conj(sum_r conj(x_r) (M x)_r), where M is the sector matrix of the
+// group average (1/G) sum_g g A g^-1 (it has the same expectation value as A in
+// every state of the sector). Row r of M x, for the element images u = j.rep_r:
+//   sum_j sum_terms c(u -> u') conj(chi(j)) chi(gm(u')) (n_t / n_r) x_t / G,
+// t = row of the orbit of u', gm(u') = the element taking its representative onto
+// u' (the ELL kernel's formula, applied to every image of the source). When A
+// commutes with the group the average is A itself and only j = identity is taken.
+//
+// Two deliberate differences from the reference kernel: it stops the loop over
+// the images of a term at the first image the term annihilates (utils.cu:523,
+// `if (interr_bool) break;`), dropping the remaining images of S+/S- terms, and it
+// accumulates conj(phi_r) conj(V) phi_t, which equals <Psi|A|Psi> only when the
+// vector is real up to a phase. Both agree with this kernel for Sz-diagonal
+// operators and real vectors.
+#include <cmath>
+#include <complex>
+#include <map>
+#include <memory>
+
+#include "device_common.hpp"
+#include "reduce.hpp"
+#include "terms.hpp"
+
+namespace qmb {
+
+constexpr int kExpThreads = 256;
+constexpr int kExpMaxBlocks = 4096;
+
+template <class S, bool CPLX, bool AVG>
+__global__ __launch_bounds__(kExpThreads) void expect_rows(GroupView gv, const uint64_t *binom, int layout,
+                                                           int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
+                                                           const uint8_t *nullf, const uint64_t *bitmap,
+                                                           const uint32_t *prefix, TermsView tv, const double2 *chi,
+                                                           const double *nrm, const double *x, double *partials) {
+    __shared__ uint8_t perm_lds[kMaxGroup * 64];
+    __shared__ uint64_t bin[64 * kBinJ];
+    __shared__ double2 s_chi[kMaxGroup];
+    __shared__ double s_nrm[kMaxGroup + 1];
+    __shared__ GroupLds gt;
+    __shared__ double red[kWaves];
+    stage_perm(perm_lds, gv);
+    stage_group(gt, gv);
+    stage_binom(bin, binom);
+    for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
+    for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
+    __syncthreads();
+    const double inv_g = AVG ? 1.0 / (double)gv.G : 1.0;
+    double out_r = 0.0, out_i = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        if (layout == QMB_LAYOUT_REFERENCE && nullf[r]) continue;
+        const S rep = (S)reps[r];
+        const double n_src = s_nrm[stab[r]];
+        double ar = 0.0, ai = 0.0;  // (M x)_r
+        const int n_img = AVG ? gv.G : 1;
+        for (int j = 0; j < n_img; ++j) {
+            const S u = AVG ? apply_elem<S>(rep, j, gv, perm_lds) : rep;
+            const double2 cj = AVG ? make_double2(s_chi[j].x, -s_chi[j].y) : make_double2(1.0, 0.0);
+            for (int h0 = 0; h0 < tv.n_terms; h0 += 64) {
+                const int h1 = min(h0 + 64, tv.n_terms);
+                uint64_t cand = 0;
+                double dr = 0.0, di = 0.0;
+                for (int h = h0; h < h1; ++h) {
+                    double cr, ci;
+                    S tgt;
+                    if (!apply_term<S>(tv, h, u, tgt, cr, ci)) continue;
+                    if (tgt == u) {  // same orbit element: conj(chi(j)) chi(gm) = 1, n_t = n_r
+                        dr += cr;
+                        di += ci;
+                    } else {
+                        cand |= 1ULL << (h - h0);
+                    }
+                }
+                if (CPLX) {
+                    const double xr = x[2 * r], xi = x[2 * r + 1];
+                    ar += dr * xr - di * xi;
+                    ai += dr * xi + di * xr;
+                } else {
+                    const double xr = x[r];
+                    ar += dr * xr;
+                    ai += di * xr;
+                }
+                while (cand) {
+                    const int h = h0 + __builtin_ctzll(cand);
+                    cand &= cand - 1;
+                    double cr, ci;
+                    S tgt;
+                    apply_term<S>(tv, h, u, tgt, cr, ci);
+                    int gm;
+                    const S rep2 = gv.lut == 1   ? orbit_min_lut<S>(tgt, gv, gt, gm)
+                                   : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm)
+                                                 : orbit_min<S>(tgt, gv, perm_lds, gm);
+                    const int64_t r2 = row_of_rep<S>(rep2, bin, bitmap, prefix);
+                    if (r2 < 0) continue;
+                    if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
+                    const double ratio = s_nrm[stab[r2]] / n_src;
+                    const double2 cg = s_chi[gm];
+                    // f = c * conj(chi(j)) * chi(gm) * ratio
+                    const double qr = cj.x * cg.x - cj.y * cg.y, qi = cj.x * cg.y + cj.y * cg.x;
+                    const double fr = (cr * qr - ci * qi) * ratio, fi = (cr * qi + ci * qr) * ratio;
+                    if (CPLX) {
+                        const double xr = x[2 * r2], xi = x[2 * r2 + 1];
+                        ar += fr * xr - fi * xi;
+                        ai += fr * xi + fi * xr;
+                    } else {
+                        const double xr = x[r2];
+                        ar += fr * xr;
+                        ai += fi * xr;
+                    }
+                }
+            }
+        }
+        ar *= inv_g;
+        ai *= inv_g;
+        // x_r conj((M x)_r)
+        const double xr = CPLX ? x[2 * r] : x[r], xi = CPLX ? x[2 * r + 1] : 0.0;
+        out_r += xr * ar + xi * ai;
+        out_i += xi * ar - xr * ai;
+    }
+    const double sr = block_sum(out_r, red);
+    const double si = block_sum(out_i, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sr;
+        partials[2 * blockIdx.x + 1] = si;
+    }
+}
+
+// True when the operator commutes with every element: the images g A g^-1 (sites
+// moved by the element, S+ <-> S- and Sz -> -Sz under a spin flip) have the same
+// terms with the same summed scalars. Operators on one site keep their order.
+static bool commutes_with_group(const qmb_terms &t, const qmb_group &g) {
+    using Key = std::vector<int16_t>;
+    auto canon = [&](int elem) {
+        std::map<Key, std::complex<double>> m;
+        const int n = t.n_sites;
+        for (int h = 0; h < t.n_terms; ++h) {
+            const int k = t.n_ops[h];
+            std::vector<std::pair<int, int>> ops(k);
+            std::complex<double> c(t.scalar[2 * h], t.scalar[2 * h + 1]);
+            for (int q = 0; q < k; ++q) {
+                int ty = t.op_type[(size_t)h * t.max_ops + q];
+                const int s = t.site[(size_t)h * t.max_ops + q];
+                const int d = elem < 0 ? s : (int)g.perm[(size_t)elem * n + s];
+                if (elem >= 0 && g.flip[elem]) {
+                    if (ty == QMB_OP_SP)
+                        ty = QMB_OP_SM;
+                    else if (ty == QMB_OP_SM)
+                        ty = QMB_OP_SP;
+                    else
+                        c = -c;
+                }
+                ops[q] = {d, ty};
+            }
+            std::stable_sort(ops.begin(), ops.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+            Key key;
+            for (const auto &o : ops) {
+                key.push_back((int16_t)o.first);
+                key.push_back((int16_t)o.second);
+            }
+            m[key] += c;
+        }
+        return m;
+    };
+    const auto base = canon(-1);
+    double scale = 0.0;
+    for (const auto &kv : base) scale = std::max(scale, std::abs(kv.second));
+    const double tol = 1e-12 * std::max(scale, 1e-300);
+    for (int e = 0; e < g.G; ++e) {
+        const auto img = canon(e);
+        for (const auto &kv : img) {
+            auto it = base.find(kv.first);
+            const std::complex<double> b = it == base.end() ? 0.0 : it->second;
+            if (std::abs(kv.second - b) > tol) return false;
+        }
+        for (const auto &kv : base)
+            if (img.find(kv.first) == img.end() && std::abs(kv.second) > tol) return false;
+    }
+    return true;
+}
+
+}  // namespace qmb
+
+using namespace qmb;
+
+extern "C" {
+
+qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a, int is_complex, const double *x,
+                           int x_on_device, int flags, double *out, int *averaged) {
+    return guarded([&] {
+        QMB_REQUIRE(ctx && b && a && x && out, "qmb_expectation: NULL argument");
+        QMB_REQUIRE(b->ctx == ctx, "qmb_expectation: basis belongs to another context");
+        QMB_REQUIRE(a->n_sites == b->n_sites, "qmb_expectation: term table and basis disagree on n_sites");
+        QMB_REQUIRE(is_complex == 0 || is_complex == 1, "qmb_expectation: is_complex must be 0 or 1");
+        QMB_REQUIRE(flags == 0 || flags == QMB_EXPECT_GROUP_AVERAGE, "qmb_expectation: unknown flags");
+        const qmb_group *g = b->group;
+        const double *ch = &g->chars[(size_t)b->irrep * g->G * 2];
+        if (!is_complex)
+            for (int e = 0; e < g->G; ++e)
+                QMB_REQUIRE(ch[2 * e + 1] == 0.0, "qmb_expectation: a real vector needs a real character row");
+        const bool avg = (flags & QMB_EXPECT_GROUP_AVERAGE) || !commutes_with_group(*a, *g);
+        if (averaged) *averaged = avg ? 1 : 0;
+        out[0] = 0.0;
+        out[1] = 0.0;
+        const int64_t n = b->n_rows;
+        if (n == 0 || a->n_terms == 0) return;
+        QMB_HIP(hipSetDevice(ctx->device));
+        const size_t xbytes = (size_t)n * (is_complex ? 16 : 8);
+        DevBuf d_x;
+        const double *xd = x;
+        if (!x_on_device) {
+            d_x.alloc(xbytes);
+            QMB_HIP(hipMemcpyAsync(d_x.p, x, xbytes, hipMemcpyHostToDevice, ctx->stream));
+            xd = d_x.as<double>();
+        }
+        const DeviceTerms dt(*a);
+        const std::vector<double> nrm = orbit_norm_table(g->G);
+        DevBuf d_chi((size_t)g->G * 2 * sizeof(double)), d_nrm(nrm.size() * sizeof(double));
+        QMB_HIP(hipMemcpy(d_chi.p, ch, (size_t)g->G * 2 * sizeof(double), hipMemcpyHostToDevice));
+        QMB_HIP(hipMemcpy(d_nrm.p, nrm.data(), nrm.size() * sizeof(double), hipMemcpyHostToDevice));
+        const int blocks = (int)std::min<int64_t>(kExpMaxBlocks, (n + kExpThreads - 1) / kExpThreads);
+        DevBuf d_part((size_t)blocks * 2 * sizeof(double));
+        const bool narrow = b->n_sites <= 32;
+        using K = void (*)(GroupView, const uint64_t *, int, int64_t, const uint64_t *, const uint8_t *,
+                           const uint8_t *, const uint64_t *, const uint32_t *, TermsView, const double2 *,
+                           const double *, const double *, double *);
+        K k;
+        if (narrow)
+            k = is_complex ? (avg ? expect_rows<uint32_t, true, true> : expect_rows<uint32_t, true, false>)
+                           : (avg ? expect_rows<uint32_t, false, true> : expect_rows<uint32_t, false, false>);
+        else
+            k = is_complex ? (avg ? expect_rows<uint64_t, true, true> : expect_rows<uint64_t, true, false>)
+                           : (avg ? expect_rows<uint64_t, false, true> : expect_rows<uint64_t, false, false>);
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(kExpThreads), 0, ctx->stream, b->view(), ctx->binom.as<uint64_t>(),
+                           b->layout, n, b->reps.as<uint64_t>(), b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(),
+                           b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), dt.view, d_chi.as<double2>(),
+                           d_nrm.as<double>(), xd, d_part.as<double>());
+        QMB_HIP(hipGetLastError());
+        std::vector<double> part((size_t)blocks * 2);
+        QMB_HIP(hipMemcpyAsync(part.data(), d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        // <Psi|A|Psi> = conj(sum_r conj(x_r) (M x)_r) = sum_r x_r conj((M x)_r), summed in block order
+        double sr = 0.0, si = 0.0;
+        for (int i = 0; i < blocks; ++i) {
+            sr += part[2 * i];
+            si += part[2 * i + 1];
+        }
+        out[0] = sr;
+        out[1] = si;
+    });
+}
+
+}  // extern "C"

@@ -186,3 +186,67 @@ def sublattice_mask(lat: Lattice) -> int:
             if (i + j) % 2 == 0:
                 m |= 1 << lat.site(i, j)
     return m
+
+
+# ---- operators for expectation values ----------------------------------------
+# An operator is a list of products; a product is a list of (site, op, scalar)
+# with op in {OP_SZ, OP_SP, OP_SM}, acting on a ket right to left (the last
+# factor first), like ManyBodyOperator (Q/quantum_operators/ArbitraryOperator.h:7-80).
+
+
+def operator_table(terms):
+    """Conjugated flat table of an operator (BuildOperatorVecs,
+    Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:306-336): op
+    types conjugated (Sp <-> Sm), scalar = product of the conjugated scalars.
+    Returns (op_type, site, scalar, n_ops, max_ops) for qmbcuda_amd.Terms."""
+    conj = {OP_SZ: OP_SZ, OP_SP: OP_SM, OP_SM: OP_SP}
+    h = len(terms)
+    w = max((len(t) for t in terms), default=0)
+    op = np.zeros(h * w, np.int32)
+    site = np.full(h * w, -1, np.int32)
+    scalar = np.ones(h, np.complex128)
+    n_ops = np.zeros(h, np.int32)
+    for i, t in enumerate(terms):
+        n_ops[i] = len(t)
+        for k, (s, ty, sc) in enumerate(t):
+            op[i * w + k] = conj[ty]
+            site[i * w + k] = s
+            scalar[i] *= np.conj(complex(sc))
+    return op, site, scalar, n_ops, w
+
+
+def szsz(i: int, j: int, scalar=1.0):
+    """Sz_i Sz_j (SzSz_correlator, Q/quantum_operators/ArbitraryOperator.cu:234-242)."""
+    return [[(i, OP_SZ, complex(scalar)), (j, OP_SZ, 1.0 + 0j)]]
+
+
+def spin_spin(i: int, j: int, scalar=1.0):
+    """S_i . S_j = (Sp_i Sm_j + Sm_i Sp_j)/2 + Sz_i Sz_j."""
+    c = complex(scalar)
+    return [[(i, OP_SP, c / 2), (j, OP_SM, 1.0 + 0j)], [(i, OP_SM, c / 2), (j, OP_SP, 1.0 + 0j)],
+            [(i, OP_SZ, c), (j, OP_SZ, 1.0 + 0j)]]
+
+
+def cell_positions(lat: Lattice):
+    """Integer cell coordinates (i, j) of every site (sublattice offsets are not
+    needed for the momenta of a Bravais torus)."""
+    pos = np.zeros((lat.n_sites, 2), np.int64)
+    for i in range(lat.N1):
+        for j in range(lat.N2):
+            for a in range(lat.M):
+                pos[lat.site(i, j, a)] = (i, j)
+    return pos
+
+
+def structure_factor_zz(lat: Lattice, kx: int, ky: int = 0):
+    """S^zz(k) = (1/N) sum_{r, r'} exp(i k.(r - r')) Sz_r Sz_r' at k = 2 pi (kx/N1, ky/N2),
+    i.e. A_k^dagger A_k for the Fourier operator A_k = N^-1/2 sum_r exp(i k.r) Sz_r
+    (FourierOperator, Q/quantum_operators/ArbitraryOperator.cu:248-285)."""
+    pos = cell_positions(lat)
+    n = lat.n_sites
+    ph = np.exp(2j * math.pi * (pos[:, 0] * kx / lat.N1 + pos[:, 1] * ky / lat.N2))
+    terms = []
+    for r in range(n):
+        for rp in range(n):
+            terms.append([(r, OP_SZ, np.conj(ph[r]) * ph[rp] / n), (rp, OP_SZ, 1.0 + 0j)])
+    return terms

@@ -4,6 +4,7 @@
 //                  JSON (no GPU); tests compare them with the oracle's restatement
 //   solve <case>   run HeisenbergXXZAbelianSymms on the GPU and print per-irrep
 //                  energies as JSON
+//   expect <case>  ground-state expectation values through ComputeStaticExpValZeroT
 //   ref <case>     run LanczosEigVals (the reference free function) on GPU-built
 //                  reference-layout arrays read from stdin (n W then ind/val/track)
 #include <complex>
@@ -127,9 +128,34 @@ static void solve(const Case &c, bool fixed, bool batched = false) {
                 ham.GetGroundStateIrrep(), ham.GetKrylovGS().size());
 }
 
+// Ground-state expectation values (ComputeStaticExpValZeroT), including the
+// reference example's Sz_0 Sz_3 (Q/examples/examples.cu:897-909).
+static void expect(const Case &c) {
+    EDOptions o;
+    HeisenbergXXZAbelianSymms<cd> ham(c.model, c.group, -1, -1, 1, 2212456, 50, 1e-5, o);
+    Operator<cd> Sz0(0, OperatorType::Sz, 1.0), Sz1(3, OperatorType::Sz, 1.0);
+    ManyBodyOperator<cd> szsz;
+    szsz.AddTerm({Sz0, Sz1});
+    ManyBodyOperator<cd> hop;  // neither Hermitian nor translation invariant
+    hop.AddTerm({Operator<cd>(0, OperatorType::Sp, cd(0.4, 0.9)), Operator<cd>(1, OperatorType::Sm, 1.0)});
+    const auto geom = c.model.GetGeomInfo();
+    const auto Ak = FullFourierOperator(Sz<cd>(0), geom);
+    const auto Amk = FullFourierOperator(Sz<cd>(0), geom, true);
+    const ManyBodyOperator<cd> szz = Amk[1][0][0] * Ak[1][0][0];  // S^zz at k = b1/N1
+    auto put = [](const char *name, cd v, bool last = false) {
+        std::printf("\"%s\": [%.17g, %.17g]%s", name, v.real(), v.imag(), last ? "" : ", ");
+    };
+    std::printf("{\"E_gs\": %.17g, \"GS_irrep\": %d, ", ham.GetGroundStateEnergy(), ham.GetGroundStateIrrep());
+    put("szsz", ham.ComputeStaticExpValZeroT(szsz));
+    put("hop", ham.ComputeStaticExpValZeroT(hop));
+    put("szz", ham.ComputeStaticExpValZeroT(szz));
+    put("H", ham.ComputeStaticExpValZeroT(c.model.GetH()), true);
+    std::printf("}\n");
+}
+
 int main(int argc, char **argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s tables|solve|solve_fixed|solve_batched <case>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s tables|solve|solve_fixed|solve_batched|expect <case>\n", argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -143,6 +169,8 @@ int main(int argc, char **argv) {
             solve(c, true);
         else if (mode == "solve_batched")
             solve(c, false, true);
+        else if (mode == "expect")
+            expect(c);
         else
             return 2;
     } catch (const std::exception &e) {

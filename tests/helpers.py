@@ -86,3 +86,35 @@ def ell_dense_nonnull(ell, orb, alpha):
     E = ell.dense()
     keep = ~orb.null_mask(alpha)
     return E[np.ix_(keep, keep)]
+
+
+def embedding(case, orb, alpha):
+    """Full-sector vectors of the sector's rows: column r = sum_g conj(chi(g)) |g.rep_r>,
+    normalised (zero for null orbits). An eigenvector x of the sector matrix is the
+    many-body state embedding @ conj(x); tests check that on H itself."""
+    n, G = case["n_sites"], orb.G
+    states = OM.sector_basis(n, orb.n_up)
+    index = {int(s): i for i, s in enumerate(states)}
+    chi = np.asarray(case["chars"])[alpha]
+    null = orb.null_mask(alpha)
+    E = np.zeros((len(states), orb.n_orb), np.complex128)
+    for r, rep in enumerate(orb.reps):
+        if null[r]:
+            continue
+        for g in range(G):
+            fl = 0 if case["flip"] is None else int(case["flip"][g])
+            E[index[oracle.apply_elem(int(rep), case["group_el"][g], fl, n)], r] += np.conj(chi[g])
+        E[:, r] /= np.linalg.norm(E[:, r])
+    return E
+
+
+def sector_ground_state(case, orb, alpha):
+    """Lowest eigenpair of the oracle's sector matrix (dense), as a reference-layout
+    vector over all orbit rows (zeros on null rows)."""
+    ell = oracle_ell(case, orb, alpha, is_complex=True)
+    keep = ~orb.null_mask(alpha)
+    M = ell.dense()[np.ix_(keep, keep)]
+    w, V = np.linalg.eigh(M)
+    x = np.zeros(orb.n_orb, np.complex128)
+    x[keep] = V[:, 0]
+    return w[0], x

@@ -123,3 +123,39 @@ def test_facade_fixed_iteration_mode():
     got = run("solve_fixed", "chain16")
     assert abs(got["E_gs"] - (-7.142296360617)) < 1e-8
     assert got["krylov_len"] == 60
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,n_up", [("chain16", 8), ("square2x4_example", 4)])
+def test_facade_ground_state_expectation_values(name, n_up):
+    """ComputeStaticExpValZeroT on the regenerated ground state against <Psi|A|Psi>
+    from dense diagonalisation of the ground-state sector (the facade's vector
+    carries the Lanczos tolerance: 1e-5 for operators that do not commute with H)."""
+    import math
+
+    from tests.helpers import embedding, sector_ground_state
+
+    print(f"[gpu-case] facade expect {name}", flush=True)
+    got = run("expect", name)
+    ref = oracle_case(name)
+    orb = oracle_orbits(ref, n_up)
+    a = got["GS_irrep"]
+    e0, x = sector_ground_state(ref, orb, a)
+    assert abs(got["E_gs"] - e0) < 1e-10 * max(1, abs(e0))
+    psi = embedding(ref, orb, a) @ np.conj(x)
+    n = ref["n_sites"]
+    n1, n2 = (16, 1) if name == "chain16" else (2, 4)
+    ph = [np.exp(2j * math.pi * (s // n2) / n1) for s in range(n)]
+    SZ, SP, SM = OM.OP_SZ, OM.OP_SP, OM.OP_SM
+    ops = {
+        "szsz": [[(0, SZ, 1.0), (3, SZ, 1.0)]],
+        "hop": [[(0, SP, 0.4 + 0.9j), (1, SM, 1.0)]],
+        "szz": [[(r, SZ, np.conj(ph[r]) / math.sqrt(n)), (rp, SZ, ph[rp] / math.sqrt(n))]
+                for r in range(n) for rp in range(n)],
+        "H": ref["terms"],
+    }
+    for key, terms in ops.items():
+        want = psi.conj() @ OM.dense_sector_hamiltonian(n, n_up, terms) @ psi
+        v = complex(*got[key])
+        tol = 1e-10 if key == "H" else 1e-5
+        assert abs(v - want) < tol * max(1, abs(want)), (key, v, want)
