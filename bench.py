@@ -518,7 +518,39 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, last, table, args.cpu_threads, B)
+    out["observables"] = ground_state_observables(ctx, lat, last, table, cfg["n"])
     print(json.dumps(out), flush=True)
+
+
+def ground_state_observables(ctx, lat, last, table, n):
+    """Expectation values in the ground state (outside the timed region): the
+    correlations <Sz_0 Sz_r> (operators that do not commute with the group: averaged
+    over the 2N images of every row), the structure factor S^zz(pi) (N^2 terms,
+    group invariant) and <H>, each timed as one qmb_expectation call."""
+    import qmbcuda_amd as q
+    from qmbcuda_amd import models as LM
+
+    b, e = last["basis"], last["ell"]
+    ev, rep, _, x = q.lanczos(e, max_iter=1000, tol=1e-13, seed=2212456, want_eigvec=True)
+    nv = float(x @ x)
+    ops = [q.Terms(n, LM.operator_table(LM.szsz(0, r))) for r in range(n)]
+    s_pi = q.Terms(n, LM.operator_table(LM.structure_factor_zz(lat, n // 2)))
+    h = q.Terms(n, table)
+    b.expectation(ops[1], x)  # warm-up
+    t0 = time.perf_counter()
+    corr = [b.expectation(t, x)[0].real / nv for t in ops]
+    t1 = time.perf_counter()
+    spi = b.expectation(s_pi, x)[0].real / nv
+    t2 = time.perf_counter()
+    eh = b.expectation(h, x)[0].real / nv
+    t3 = time.perf_counter()
+    return {"szsz_corr": [round(c, 12) for c in corr[: n // 2 + 1]],
+            "ms_per_correlation": round((t1 - t0) * 1e3 / n, 3),
+            "szz_pi": spi, "ms_szz_pi": round((t2 - t1) * 1e3, 3),
+            "energy_expectation": eh, "ms_energy": round((t3 - t2) * 1e3, 3),
+            "abs_energy_minus_e0": abs(eh - float(ev[0])),
+            "abs_sum_rule": abs(sum(corr)),
+            "abs_fourier_check": abs(spi - sum(c * (-1) ** r for r, c in enumerate(corr)))}
 
 
 if __name__ == "__main__":

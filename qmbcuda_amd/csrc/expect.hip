@@ -24,7 +24,7 @@
 // operators and real vectors.
 #include <cmath>
 #include <complex>
-#include <map>
+#include <string>
 #include <memory>
 
 #include "device_common.hpp"
@@ -138,14 +138,17 @@ __global__ __launch_bounds__(kExpThreads) void expect_rows(GroupView gv, const u
 // True when the operator commutes with every element: the images g A g^-1 (sites
 // moved by the element, S+ <-> S- and Sz -> -Sz under a spin flip) have the same
 // terms with the same summed scalars. Operators on one site keep their order.
+// A term is keyed by its (site, op) byte pairs after a stable sort by site.
 static bool commutes_with_group(const qmb_terms &t, const qmb_group &g) {
-    using Key = std::vector<int16_t>;
+    using Entry = std::pair<std::string, std::complex<double>>;
     auto canon = [&](int elem) {
-        std::map<Key, std::complex<double>> m;
+        std::vector<Entry> v;
+        v.reserve(t.n_terms);
         const int n = t.n_sites;
+        std::vector<std::pair<int, int>> ops;
         for (int h = 0; h < t.n_terms; ++h) {
             const int k = t.n_ops[h];
-            std::vector<std::pair<int, int>> ops(k);
+            ops.resize(k);
             std::complex<double> c(t.scalar[2 * h], t.scalar[2 * h + 1]);
             for (int q = 0; q < k; ++q) {
                 int ty = t.op_type[(size_t)h * t.max_ops + q];
@@ -162,29 +165,46 @@ static bool commutes_with_group(const qmb_terms &t, const qmb_group &g) {
                 ops[q] = {d, ty};
             }
             std::stable_sort(ops.begin(), ops.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
-            Key key;
-            for (const auto &o : ops) {
-                key.push_back((int16_t)o.first);
-                key.push_back((int16_t)o.second);
+            std::string key(2 * k, '\0');
+            for (int q = 0; q < k; ++q) {
+                key[2 * q] = (char)ops[q].first;
+                key[2 * q + 1] = (char)ops[q].second;
             }
-            m[key] += c;
+            v.emplace_back(std::move(key), c);
         }
-        return m;
+        std::sort(v.begin(), v.end(), [](const Entry &a, const Entry &b) { return a.first < b.first; });
+        std::vector<Entry> merged;  // equal products summed
+        for (auto &e : v) {
+            if (!merged.empty() && merged.back().first == e.first)
+                merged.back().second += e.second;
+            else
+                merged.push_back(std::move(e));
+        }
+        return merged;
     };
     const auto base = canon(-1);
     double scale = 0.0;
-    for (const auto &kv : base) scale = std::max(scale, std::abs(kv.second));
+    for (const auto &e : base) scale = std::max(scale, std::abs(e.second));
     const double tol = 1e-12 * std::max(scale, 1e-300);
-    for (int e = 0; e < g.G; ++e) {
-        const auto img = canon(e);
-        for (const auto &kv : img) {
-            auto it = base.find(kv.first);
-            const std::complex<double> b = it == base.end() ? 0.0 : it->second;
-            if (std::abs(kv.second - b) > tol) return false;
+    auto same = [&](const std::vector<Entry> &a, const std::vector<Entry> &b) {
+        size_t i = 0, j = 0;
+        while (i < a.size() || j < b.size()) {
+            if (j == b.size() || (i < a.size() && a[i].first < b[j].first)) {
+                if (std::abs(a[i].second) > tol) return false;
+                ++i;
+            } else if (i == a.size() || b[j].first < a[i].first) {
+                if (std::abs(b[j].second) > tol) return false;
+                ++j;
+            } else {
+                if (std::abs(a[i].second - b[j].second) > tol) return false;
+                ++i;
+                ++j;
+            }
         }
-        for (const auto &kv : base)
-            if (img.find(kv.first) == img.end() && std::abs(kv.second) > tol) return false;
-    }
+        return true;
+    };
+    for (int e = 0; e < g.G; ++e)
+        if (!same(base, canon(e))) return false;
     return true;
 }
 
