@@ -22,6 +22,7 @@ LAYOUT_REFERENCE, LAYOUT_COMPACT = 0, 1
 REAL, COMPLEX = 0, 1
 START_RANDOM, START_USER = 0, 1
 EXPECT_GROUP_AVERAGE = 1
+KEEP_FP64_VALUES = 16
 
 
 class BasisInfo(C.Structure):
@@ -48,6 +49,9 @@ class EllInfo(C.Structure):
         ("chunk_rows", C.c_int32),
         ("device_bytes", C.c_int64),
         ("spmv_bytes", C.c_double),
+        ("packed", C.c_int32),
+        ("n_values", C.c_int32),
+        ("format_bytes", C.c_double),
     ]
 
 

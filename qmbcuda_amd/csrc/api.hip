@@ -13,6 +13,11 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 
+DevPool *&current_pool() {
+    static thread_local DevPool *p = nullptr;
+    return p;
+}
+
 qmb_status fail(qmb_status code, const std::string &msg) {
     g_last_error = msg;
     return code;
@@ -359,6 +364,7 @@ qmb_status qmb_ctx_create(int device, qmb_ctx **out) {
             delete c;
             throw Error{QMB_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
         }
+        c->pool.stream = c->stream;
         try {
             ensure_binom(c);
         } catch (...) {

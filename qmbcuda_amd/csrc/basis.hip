@@ -197,6 +197,7 @@ extern "C" {
 
 qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep, int layout, qmb_basis **out) {
     return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
         QMB_REQUIRE(out, "qmb_basis_build: out is NULL");
         *out = nullptr;
         QMB_REQUIRE(ctx && g, "qmb_basis_build: NULL context or group");
@@ -307,6 +308,7 @@ qmb_status qmb_basis_export(const qmb_basis *b, uint64_t *reps, uint16_t *stab, 
 
 qmb_status qmb_basis_lookup(const qmb_basis *b, int64_t n, const uint64_t *states, int64_t *row, int32_t *elem) {
     return guarded([&] {
+        PoolScope pool_scope(b && b->ctx ? &b->ctx->pool : nullptr);
         QMB_REQUIRE(b && states && row && elem, "qmb_basis_lookup: NULL argument");
         QMB_REQUIRE(n >= 0, "qmb_basis_lookup: negative count");
         if (n == 0) return;

@@ -36,6 +36,8 @@ struct BInst {
     const uint16_t *row_len;
     const int32_t *col;
     const double *val;
+    const double *diag, *vtab;   // packed real values (packed = 1): diagonal, value table
+    int packed, col_bits;
     int64_t n_rows, n_chunks;
     double *P, *Y, *VB;          // w, H phi, Krylov basis [M][n*vs]
     double *alpha, *beta, *invb;  // [M], [M+1], [M+1]
@@ -76,10 +78,14 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
         const int len = r < I.n_rows ? (int)I.row_len[r] : 0;
         const int64_t p0 = base + lane;
         if (!CPLX) {
-            const double acc = sell_row_real(I.col, I.val, I.P, p0, L, len) * s;
+            const double xr = r < I.n_rows ? I.P[r] : 0.0;
+            const double acc = (I.packed ? sell_row_packed(reinterpret_cast<const uint32_t *>(I.col), I.vtab, I.P, p0,
+                                                           L, len, I.col_bits, r < I.n_rows ? I.diag[r] : 0.0, xr)
+                                         : sell_row_real(I.col, I.val, I.P, p0, L, len)) *
+                               s;
             if (r < I.n_rows) {
                 I.Y[r] = acc;
-                const double vr = I.P[r] * s;
+                const double vr = xr * s;
                 vout[r] = vr;
                 dot += vr * acc;
             }
@@ -359,6 +365,7 @@ using namespace qmb;
 extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const qmb_lanczos_opts *opts,
                                           double *evals, qmb_lanczos_report *reports) {
     return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
         QMB_REQUIRE(ctx && ells && opts && evals && n >= 1, "qmb_lanczos_batched: bad argument");
         const qmb_lanczos_opts o = *opts;
         QMB_REQUIRE(o.max_iter >= 1 && o.max_iter <= 5000, "qmb_lanczos_batched: max_iter must be in [1, 5000]");
@@ -421,6 +428,10 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             I.row_len = e->row_len.as<uint16_t>();
             I.col = e->col.as<int32_t>();
             I.val = e->val.as<double>();
+            I.diag = e->diag.as<double>();
+            I.vtab = e->vtab.as<double>();
+            I.packed = e->packed;
+            I.col_bits = e->col_bits;
             I.n_rows = e->n_rows;
             I.n_chunks = e->n_chunks;
             bufs.emplace_back(cnt * sizeof(double));
@@ -618,23 +629,49 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         QMB_HIP(hipStreamEndCapture(ctx->stream, &graph));
         QMB_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
 
-        std::vector<double> ha((size_t)n * (3 * M + 2));
+        // Pipelined host checks: chunk c+1 is queued before the host evaluates the
+        // coefficient snapshot taken after chunk c, so the tridiagonal solves of
+        // every instance overlap the GPU's next chunk. An instance that converged
+        // at chunk c is dropped from chunk c+2 on (its extra steps are discarded).
+        const size_t snap = (size_t)n * (3 * M + 2);
+        PinnedBuf hbuf(2 * snap * sizeof(double) + (size_t)n * sizeof(int));
+        double *hsnap[2] = {hbuf.as<double>(), hbuf.as<double>() + snap};
+        int *hact = reinterpret_cast<int *>(hbuf.as<double>() + 2 * snap);
+        hipEvent_t evs[2];
+        QMB_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+        QMB_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+        struct EventPair {
+            hipEvent_t *e;
+            ~EventPair() {
+                (void)hipEventDestroy(e[0]);
+                (void)hipEventDestroy(e[1]);
+            }
+        } ev_guard{evs};
         std::vector<std::vector<double>> ev(n, std::vector<double>(k_eigs, std::nan(""))),
             res(n, std::vector<double>(k_eigs, std::nan("")));
         std::vector<int> iters(n, 0), conv(n, 0), brk(n, 0);
-        int it = 0, n_active = n;
+        int launched = 0, n_active = n;
         double ms_host = 0.0;
         std::vector<double> w(k_eigs + 1), zl(k_eigs + 1);
-        while (it < M && n_active > 0) {
+        auto launch_chunk = [&](int slot) {
             QMB_HIP(hipGraphLaunch(exec, ctx->stream));
-            it += chunk;
-            QMB_HIP(hipMemcpyAsync(ha.data(), scal.p, ha.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-            QMB_HIP(hipStreamSynchronize(ctx->stream));
+            launched += chunk;
+            QMB_HIP(hipMemcpyAsync(hsnap[slot], scal.p, snap * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            QMB_HIP(hipEventRecord(evs[slot], ctx->stream));
+        };
+        launch_chunk(0);
+        int pend = 0, pend_it = launched;
+        while (true) {
+            const bool next = launched + chunk <= M && n_active > 0;
+            if (next) launch_chunk(1 - pend);
+            QMB_HIP(hipEventSynchronize(evs[pend]));
+            const int it = pend_it;
+            const double *hs = hsnap[pend];
             auto th0 = std::chrono::steady_clock::now();
             bool changed = false;
             for (int pos = 0; pos < n; ++pos) {
                 if (!active[pos]) continue;
-                const double *a = ha.data() + (size_t)pos * (3 * M + 2);
+                const double *a = hs + (size_t)pos * (3 * M + 2);
                 const double *b = a + M;
                 double tsc = 0.0;
                 for (int j = 0; j < it; ++j) tsc = std::max(tsc, std::fabs(a[j]) + b[j] + b[j + 1]);
@@ -676,9 +713,13 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                 }
             }
             ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
-            if (changed && n_active > 0)
-                QMB_HIP(hipMemcpyAsync(d_active.p, active.data(), n * sizeof(int), hipMemcpyHostToDevice,
-                                       ctx->stream));
+            if (!next) break;  // no chunk in flight: that snapshot was the last one
+            if (changed && n_active > 0) {
+                std::memcpy(hact, active.data(), n * sizeof(int));
+                QMB_HIP(hipMemcpyAsync(d_active.p, hact, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+            }
+            pend = 1 - pend;
+            pend_it = launched;
         }
         QMB_HIP(hipStreamSynchronize(ctx->stream));
         (void)hipGraphExecDestroy(exec);

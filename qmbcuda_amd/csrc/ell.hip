@@ -21,27 +21,42 @@ namespace qmb {
 
 constexpr int kEllThreads = 256;
 
+constexpr int kSetSlots = 4096;  // global set of distinct values
+constexpr int kLdsSet = 256;     // per-block set
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+__device__ inline uint32_t key_hash(unsigned long long k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    return (uint32_t)k;
+}
+
+// Inserts key into an open-addressing set; false when the set is full.
+__device__ inline bool set_insert(unsigned long long *set, int slots, unsigned long long key) {
+    uint32_t h = key_hash(key) & (uint32_t)(slots - 1);
+    for (int probe = 0; probe < slots; ++probe) {
+        const unsigned long long cur = *(volatile unsigned long long *)(set + h);
+        if (cur == key) return true;
+        if (cur == kEmptyKey) {
+            const unsigned long long old = atomicCAS(set + h, kEmptyKey, key);
+            if (old == kEmptyKey || old == key) return true;
+        }
+        h = (h + 1) & (uint32_t)(slots - 1);
+    }
+    return false;
+}
+
+// One row of the count (FILL = false) or fill pass.
 template <class S, bool FILL, bool CPLX>
-__global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
-                                                        int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
-                                                        const uint8_t *nullf, const uint64_t *bitmap,
-                                                        const uint32_t *prefix, TermsView tv, const double2 *chi,
-                                                        const double *nrm, int W, uint16_t *row_len, int *err,
-                                                        int all_rows, const int64_t *chunk_ptr, int32_t *col,
-                                                        double *val) {
-    __shared__ uint8_t perm_lds[kMaxGroup * 64];
-    __shared__ uint64_t bin[64 * kBinJ];
-    __shared__ double2 s_chi[kMaxGroup];
-    __shared__ double s_nrm[kMaxGroup + 1];
-    __shared__ GroupLds gt;
-    stage_perm(perm_lds, gv);
-    stage_group(gt, gv);
-    stage_binom(bin, binom);
-    for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
-    for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
-    __syncthreads();
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rows) return;
+__device__ inline void ell_row(const GroupView &gv, int layout, int64_t n_rows, const uint64_t *reps,
+                               const uint8_t *stab, const uint8_t *nullf, const uint64_t *bitmap,
+                               const uint32_t *prefix, const TermsView &tv, int W, uint16_t *row_len, int *err,
+                               int all_rows, const int64_t *chunk_ptr, int32_t *col, double *val,
+                               const uint8_t *perm_lds, const uint64_t *bin, const double2 *s_chi,
+                               const double *s_nrm, const GroupLds &gt, int64_t r, unsigned long long *vals,
+                               int *vfull) {
+    unsigned long long last = kEmptyKey;
     const int64_t c = r / kChunk;
     const int lane = (int)(r % kChunk);
     int64_t base = 0;
@@ -106,7 +121,15 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                             val[2 * pos] = pr * ratio;
                             val[2 * pos + 1] = pi * ratio;
                         } else {
-                            val[pos] = pr * ratio;
+                            const double v = pr * ratio;
+                            val[pos] = v;
+                            if (vals) {
+                                const unsigned long long key = (unsigned long long)__double_as_longlong(v);
+                                if (key != last) {
+                                    last = key;
+                                    if (!set_insert(vals, kLdsSet, key)) *vfull = 1;
+                                }
+                            }
                         }
                     }
                 }
@@ -143,6 +166,50 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
     }
 }
 
+template <class S, bool FILL, bool CPLX>
+__global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
+                                                        int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
+                                                        const uint8_t *nullf, const uint64_t *bitmap,
+                                                        const uint32_t *prefix, TermsView tv, const double2 *chi,
+                                                        const double *nrm, int W, uint16_t *row_len, int *err,
+                                                        int all_rows, const int64_t *chunk_ptr, int32_t *col,
+                                                        double *val, unsigned long long *vset, int *vflag) {
+    __shared__ uint8_t perm_lds[kMaxGroup * 64];
+    // distinct off-diagonal values of a real fill (vset != NULL): per-block set,
+    // merged into vset at the end of the block
+    constexpr bool kCollect = FILL && !CPLX;
+    __shared__ unsigned long long vals[kCollect ? kLdsSet : 1];
+    __shared__ int vfull;
+    if (kCollect) {
+        for (int i = threadIdx.x; i < kLdsSet; i += blockDim.x) vals[i] = kEmptyKey;
+        if (threadIdx.x == 0) vfull = 0;
+    }
+    __shared__ uint64_t bin[64 * kBinJ];
+    __shared__ double2 s_chi[kMaxGroup];
+    __shared__ double s_nrm[kMaxGroup + 1];
+    __shared__ GroupLds gt;
+    stage_perm(perm_lds, gv);
+    stage_group(gt, gv);
+    stage_binom(bin, binom);
+    for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
+    for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (kCollect && vset) {
+        if (r < n_rows)
+            ell_row<S, FILL, CPLX>(gv, layout, n_rows, reps, stab, nullf, bitmap, prefix, tv, W, row_len, err,
+                                   all_rows, chunk_ptr, col, val, perm_lds, bin, s_chi, s_nrm, gt, r, vals, &vfull);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kLdsSet; i += blockDim.x)
+            if (vals[i] != kEmptyKey && !set_insert(vset, kSetSlots, vals[i])) vflag[0] = 1;
+        if (threadIdx.x == 0 && vfull) vflag[0] = 1;
+    } else if (r < n_rows) {
+        ell_row<S, FILL, CPLX>(gv, layout, n_rows, reps, stab, nullf, bitmap, prefix, tv, W, row_len, err, all_rows,
+                               chunk_ptr, col, val, perm_lds, bin, s_chi, s_nrm, gt, r, nullptr, nullptr);
+    }
+}
+
+
 // Padding entries of the rows past n_rows in the last chunk.
 __global__ void ell_pad_tail(int64_t n_rows, int64_t n_chunks, const int64_t *chunk_ptr, int32_t *col, double *val,
                              int cplx) {
@@ -164,6 +231,146 @@ __global__ void ell_pad_tail(int64_t n_rows, int64_t n_chunks, const int64_t *ch
     }
 }
 
+// ---- packed values -----------------------------------------------------------
+// The sector matrices of these models hold a few dozen distinct values (term
+// scalars times characters times ratios of orbit norms), so a real ELL is
+// repacked after the fill: every off-diagonal slot becomes one 32-bit word
+// col | (index << col_bits) into a table of the distinct values, the diagonal
+// moves to an FP64 array. The values and the order of the products are
+// unchanged, so SpMV results are bit-identical; the matrix stream shrinks from
+// 12 to 4 bytes per entry.
+// One block: the distinct values of the set in ascending bit order, +0.0 first
+// (the padding value), into keys[0 .. n); meta = {n, ok}, ok = 0 when the set
+// overflowed or n exceeds cap.
+__global__ __launch_bounds__(1024) void ell_sort_values(const unsigned long long *vset, const int *vflag,
+                                                        unsigned long long *keys, int *meta, int cap) {
+    __shared__ unsigned long long a[kSetSlots];
+    for (int i = threadIdx.x; i < kSetSlots; i += blockDim.x) a[i] = vset[i];
+    __syncthreads();
+    for (int k = 2; k <= kSetSlots; k <<= 1) {  // bitonic sort, empty slots (all ones) last
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < kSetSlots; i += blockDim.x) {
+                const int ij = i ^ j;
+                if (ij > i) {
+                    const bool up = (i & k) == 0;
+                    const unsigned long long x = a[i], y = a[ij];
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[ij] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        int n = 1;
+        keys[0] = 0ull;
+        for (int i = 0; i < kSetSlots && a[i] != kEmptyKey; ++i) {
+            if (a[i] == 0ull) continue;
+            if (n < cap) keys[n] = a[i];
+            ++n;
+        }
+        meta[0] = n;
+        meta[1] = (vflag[0] == 0 && n <= cap) ? 1 : 0;
+    }
+}
+
+// In place, when meta[1] (ok): col[p] |= index << cb for the off-diagonal slots
+// (index by binary search in the ascending keys), slot 0 -> the row itself,
+// diag[r] = the old slot 0 value. *missing = 1 if a value is not in the table.
+__global__ __launch_bounds__(kEllThreads) void ell_pack_values(int64_t n_rows, int64_t n_lanes,
+                                                               const int64_t *chunk_ptr, const uint16_t *row_len,
+                                                               uint32_t *colw, const double *val,
+                                                               const unsigned long long *keys, const int *meta,
+                                                               int cb, double *diag, int *missing) {
+    __shared__ unsigned long long s_keys[kMaxValues];
+    if (!meta[1]) return;
+    const int n_keys = meta[0];
+    for (int i = threadIdx.x; i < n_keys; i += blockDim.x) s_keys[i] = keys[i];
+    __syncthreads();
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_lanes;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = r / kChunk;
+        const int lane = (int)(r % kChunk);
+        const int64_t base = chunk_ptr[c];
+        const int L = (int)((chunk_ptr[c + 1] - base) / kChunk);
+        const int len = r < n_rows ? (int)row_len[r] : 0;
+        if (r < n_rows) diag[r] = (len > 0 && L > 0) ? val[base + lane] : 0.0;
+        if (L > 0) colw[base + lane] = (uint32_t)(r < n_rows ? r : 0);
+        for (int k = 1; k < L; ++k) {
+            const int64_t p = base + (int64_t)k * kChunk + lane;
+            uint32_t idx = 0;
+            if (k < len) {
+                const unsigned long long key = (unsigned long long)__double_as_longlong(val[p]);
+                int lo = 0, hi = n_keys - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_keys[mid] < key)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                if (s_keys[lo] != key) *missing = 1;
+                idx = (uint32_t)lo;
+            }
+            colw[p] = colw[p] | (idx << cb);
+        }
+    }
+}
+
+// Packing state of one build: the value set filled by the fill kernel, then
+// sort + pack on the device; the host reads {n, ok, missing} once at the end.
+struct ValuePacking {
+    bool on = false;
+    int cb = 0, cap = 0;
+    DevBuf vset, flags;  // flags: overflow, n, ok, missing
+    bool prepare(qmb_ctx *ctx, const qmb_ell &e) {
+        const int64_t n = e.n_rows;
+        if (n < 2) return false;
+        cb = 1;
+        while (((int64_t)1 << cb) < n) ++cb;
+        if (cb > 31) return false;
+        cap = (int)std::min<int64_t>(kMaxValues, (int64_t)1 << (32 - cb));
+        vset.alloc((size_t)kSetSlots * 8);
+        flags.alloc(4 * sizeof(int));
+        QMB_HIP(hipMemsetAsync(vset.p, 0xFF, vset.bytes, ctx->stream));
+        QMB_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, ctx->stream));
+        on = true;
+        return true;
+    }
+    void launch(qmb_ctx *ctx, qmb_ell &e) {
+        e.vtab.alloc((size_t)kMaxValues * 8);
+        e.diag.alloc((size_t)e.n_rows * sizeof(double));
+        int *f = flags.as<int>();
+        hipLaunchKernelGGL(ell_sort_values, dim3(1), dim3(1024), 0, ctx->stream, vset.as<unsigned long long>(), f,
+                           e.vtab.as<unsigned long long>(), f + 1, cap);
+        QMB_HIP(hipGetLastError());
+        const int64_t lanes = e.n_chunks * kChunk;
+        const unsigned blocks = (unsigned)std::min<int64_t>(4096, (lanes + kEllThreads - 1) / kEllThreads);
+        hipLaunchKernelGGL(ell_pack_values, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, e.n_rows, lanes,
+                           e.chunk_ptr.as<int64_t>(), e.row_len.as<uint16_t>(), e.col.as<uint32_t>(),
+                           e.val.as<double>(), e.vtab.as<unsigned long long>(), f + 1, cb, e.diag.as<double>(),
+                           f + 3);
+        QMB_HIP(hipGetLastError());
+    }
+    // after the stream has drained
+    void finish(qmb_ell &e) {
+        int h[4];
+        QMB_HIP(hipMemcpy(h, flags.p, sizeof h, hipMemcpyDeviceToHost));
+        QMB_REQUIRE(h[3] == 0, "qmb_ell_build: internal error while packing the values");
+        if (h[2]) {
+            e.val.release();
+            e.packed = 1;
+            e.col_bits = cb;
+            e.n_values = h[1];
+        } else {
+            e.diag.release();
+            e.vtab.release();
+        }
+    }
+};
+
 }  // namespace qmb
 
 using namespace qmb;
@@ -174,7 +381,14 @@ double qmb_ell::spmv_bytes() const {
 }
 
 int64_t qmb_ell::device_bytes() const {
-    return (int64_t)(chunk_ptr.bytes + row_len.bytes + col.bytes + val.bytes);
+    return (int64_t)(chunk_ptr.bytes + row_len.bytes + col.bytes + val.bytes + diag.bytes + vtab.bytes);
+}
+
+double qmb_ell::format_bytes() const {
+    if (!packed) return spmv_bytes();
+    // one word per off-diagonal entry, the diagonal, x read once, y written, row lengths
+    const int64_t n_diag = basis && basis->layout == QMB_LAYOUT_REFERENCE ? n_rows - basis->n_null : n_rows;
+    return (double)(nnz - n_diag) * 4.0 + (double)n_rows * (8.0 + 16.0 + 2.0) + (double)n_values * 8.0;
 }
 
 extern "C" {
@@ -182,11 +396,14 @@ extern "C" {
 qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, int value_type, int max_terms,
                          qmb_ell **out) {
     return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
         QMB_REQUIRE(out, "qmb_ell_build: out is NULL");
         *out = nullptr;
         QMB_REQUIRE(ctx && b && t, "qmb_ell_build: NULL argument");
         QMB_REQUIRE(b->ctx == ctx, "qmb_ell_build: basis belongs to another context");
         QMB_REQUIRE(t->n_sites == b->n_sites, "qmb_ell_build: term table and basis disagree on n_sites");
+        const bool keep_fp64 = (value_type & QMB_KEEP_FP64_VALUES) != 0;
+        value_type &= ~QMB_KEEP_FP64_VALUES;
         QMB_REQUIRE(value_type == QMB_REAL || value_type == QMB_COMPLEX, "qmb_ell_build: bad value type");
         const qmb_group *g = b->group;
         const double *ch = &g->chars[(size_t)b->irrep * g->G * 2];
@@ -230,7 +447,7 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
                                b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), b->all_rows ? 1 : 0, nullptr, nullptr,
-                               nullptr);
+                               nullptr, nullptr, nullptr);
             QMB_HIP(hipGetLastError());
         }
         int herr = 0;
@@ -244,6 +461,8 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         e->max_row_len = max_len;
         e->col.alloc((size_t)std::max<int64_t>(e->stored, 1) * sizeof(int32_t));
         e->val.alloc((size_t)std::max<int64_t>(e->stored, 1) * (cplx ? 16 : 8));
+        ValuePacking vp;
+        if (n > 0 && !cplx && !keep_fp64) vp.prepare(ctx, *e);
         if (n > 0) {
             const bool narrow = b->n_sites <= 32;
             auto kfill = narrow ? (cplx ? ell_rows<uint32_t, true, true> : ell_rows<uint32_t, true, false>)
@@ -253,7 +472,9 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
                                b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), 0, e->chunk_ptr.as<int64_t>(),
-                               e->col.as<int32_t>(), e->val.as<double>());
+                               e->col.as<int32_t>(), e->val.as<double>(),
+                               vp.on ? vp.vset.as<unsigned long long>() : nullptr,
+                               vp.on ? vp.flags.as<int>() : nullptr);
             QMB_HIP(hipGetLastError());
             if (n % kChunk) {
                 hipLaunchKernelGGL(ell_pad_tail, dim3(1), dim3(kChunk), 0, ctx->stream, n, e->n_chunks,
@@ -261,8 +482,10 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                    cplx ? 1 : 0);
                 QMB_HIP(hipGetLastError());
             }
+            if (vp.on) vp.launch(ctx, *e);
         }
         QMB_HIP(hipStreamSynchronize(ctx->stream));
+        if (vp.on) vp.finish(*e);
         *out = e.release();
     });
 }
@@ -285,6 +508,9 @@ qmb_status qmb_ell_get_info(const qmb_ell *e, qmb_ell_info *info) {
         info->chunk_rows = kChunk;
         info->device_bytes = e->device_bytes();
         info->spmv_bytes = e->spmv_bytes();
+        info->packed = e->packed;
+        info->n_values = e->n_values;
+        info->format_bytes = e->format_bytes();
     });
 }
 
@@ -313,8 +539,33 @@ qmb_status qmb_ell_export_rows(const qmb_ell *e, int64_t row0, int64_t n_rows, u
         std::vector<int32_t> cl(len_all);
         std::vector<double> vl((size_t)len_all * vs);
         QMB_HIP(hipMemcpy(cl.data(), e->col.as<int32_t>() + base, cl.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-        QMB_HIP(hipMemcpy(vl.data(), e->val.as<double>() + (size_t)base * vs, vl.size() * sizeof(double),
-                          hipMemcpyDeviceToHost));
+        if (e->packed) {
+            // words -> columns and values; slot 0 = the diagonal
+            std::vector<double> vt(e->n_values), dg(n_rows);
+            QMB_HIP(hipMemcpy(vt.data(), e->vtab.p, vt.size() * sizeof(double), hipMemcpyDeviceToHost));
+            QMB_HIP(hipMemcpy(dg.data(), e->diag.as<double>() + row0, dg.size() * sizeof(double),
+                              hipMemcpyDeviceToHost));
+            const uint32_t mask = (1u << e->col_bits) - 1u;
+            for (int64_t i = 0; i < n_rows; ++i) {
+                const int64_t r = row0 + i;
+                const int64_t c = r / kChunk - c0, lane = r % kChunk;
+                const int64_t q0 = cp[c] - base + lane;
+                const int L = (int)((cp[c + 1] - cp[c]) / kChunk);
+                if (L > 0) {
+                    cl[q0] = (int32_t)r;
+                    vl[q0] = dg[i];
+                }
+                for (int s = 1; s < L; ++s) {
+                    const int64_t q = q0 + (int64_t)s * kChunk;
+                    const uint32_t w = (uint32_t)cl[q];
+                    cl[q] = (int32_t)(w & mask);
+                    vl[q] = vt[w >> e->col_bits];
+                }
+            }
+        } else {
+            QMB_HIP(hipMemcpy(vl.data(), e->val.as<double>() + (size_t)base * vs, vl.size() * sizeof(double),
+                              hipMemcpyDeviceToHost));
+        }
         for (int64_t i = 0; i < n_rows; ++i) {
             const int64_t r = row0 + i;
             const int64_t c = r / kChunk - c0, lane = r % kChunk;

@@ -217,6 +217,7 @@ extern "C" {
 qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a, int is_complex, const double *x,
                            int x_on_device, int flags, double *out, int *averaged) {
     return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
         QMB_REQUIRE(ctx && b && a && x && out, "qmb_expectation: NULL argument");
         QMB_REQUIRE(b->ctx == ctx, "qmb_expectation: basis belongs to another context");
         QMB_REQUIRE(a->n_sites == b->n_sites, "qmb_expectation: term table and basis disagree on n_sites");

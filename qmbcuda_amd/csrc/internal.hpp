@@ -5,7 +5,10 @@
 
 #include <atomic>
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "qmb.h"
@@ -50,22 +53,111 @@ qmb_status guarded(F &&f) {
     }
 }
 
-// ---- device buffer ----------------------------------------------------------
+// ---- device memory ------------------------------------------------------------
+// A context caches the device blocks its buffers release and hands them out
+// again (hipMalloc/hipFree cost 10-20 us each, about 40 per solve). A released
+// block is reused only once the context's stream has drained, so no queued
+// kernel and no synchronous copy can touch a block that changed hands; blocks
+// larger than 4 GiB are freed at once and the cache holds at most 16 GiB.
+struct DevPool {
+    hipStream_t stream = nullptr;
+    std::mutex m;
+    std::multimap<size_t, void *> ready;    // free blocks by size
+    std::vector<std::pair<size_t, void *>> pending;  // released while the stream may still use them
+    size_t cached = 0;
+    static constexpr size_t kMaxBlock = (size_t)4 << 30;
+    static constexpr size_t kMaxCached = (size_t)16 << 30;
+
+    void *get(size_t n, size_t &cap) {
+        {
+            std::lock_guard<std::mutex> lock(m);
+            if (!pending.empty() && hipStreamQuery(stream) == hipSuccess) {
+                for (auto &b : pending) ready.emplace(b.first, b.second);
+                pending.clear();
+            }
+            auto it = ready.lower_bound(n);
+            if (it != ready.end() && it->first <= 2 * n) {
+                void *p = it->second;
+                cap = it->first;
+                cached -= cap;
+                ready.erase(it);
+                return p;
+            }
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            trim();
+            e = hipMalloc(&p, n);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error{e == hipErrorOutOfMemory ? QMB_ERR_OOM : QMB_ERR_HIP,
+                        std::string("hipMalloc: ") + hipGetErrorString(e)};
+        }
+        cap = n;
+        return p;
+    }
+    void put(void *p, size_t cap) {
+        {
+            std::lock_guard<std::mutex> lock(m);
+            if (cap <= kMaxBlock && cached + cap <= kMaxCached) {
+                pending.emplace_back(cap, p);
+                cached += cap;
+                return;
+            }
+        }
+        (void)hipFree(p);
+    }
+    void trim() {
+        std::lock_guard<std::mutex> lock(m);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &b : ready) (void)hipFree(b.second);
+        for (auto &b : pending) (void)hipFree(b.second);
+        ready.clear();
+        pending.clear();
+        cached = 0;
+    }
+    ~DevPool() { trim(); }
+};
+
+// The pool that buffers allocated on this thread draw from: set for the
+// duration of a context's entry point (PoolScope); none -> plain hipMalloc.
+DevPool *&current_pool();
+
+struct PoolScope {
+    DevPool *prev;
+    explicit PoolScope(DevPool *p) : prev(current_pool()) { current_pool() = p; }
+    ~PoolScope() { current_pool() = prev; }
+    PoolScope(const PoolScope &) = delete;
+    PoolScope &operator=(const PoolScope &) = delete;
+};
+
 struct DevBuf {
     void *p = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;  // requested size
+    size_t cap = 0;    // block size
+    DevPool *pool = nullptr;
     DevBuf() = default;
     explicit DevBuf(size_t n) { alloc(n); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), cap(o.cap), pool(o.pool) {
+        o.p = nullptr;
+        o.bytes = o.cap = 0;
+        o.pool = nullptr;
+    }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             bytes = o.bytes;
+            cap = o.cap;
+            pool = o.pool;
             o.p = nullptr;
-            o.bytes = 0;
+            o.bytes = o.cap = 0;
+            o.pool = nullptr;
         }
         return *this;
     }
@@ -73,13 +165,38 @@ struct DevBuf {
     void alloc(size_t n) {
         release();
         if (n == 0) n = 8;
-        QMB_HIP(hipMalloc(&p, n));
+        pool = current_pool();
+        if (pool) {
+            p = pool->get(n, cap);
+        } else {
+            QMB_HIP(hipMalloc(&p, n));
+            cap = n;
+        }
         bytes = n;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (pool)
+                pool->put(p, cap);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
-        bytes = 0;
+        bytes = cap = 0;
+        pool = nullptr;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Page-locked host memory (hipHostMalloc) for copies that must not block the host.
+struct PinnedBuf {
+    void *p = nullptr;
+    explicit PinnedBuf(size_t n) { QMB_HIP(hipHostMalloc(&p, n == 0 ? 8 : n, hipHostMallocDefault)); }
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
     }
     template <class T>
     T *as() const { return static_cast<T *>(p); }
@@ -87,6 +204,7 @@ struct DevBuf {
 
 constexpr int kMaxGroup = 128;
 constexpr int kChunk = 64;  // rows per SELL chunk = one wavefront
+constexpr int kMaxValues = 1024;  // value table of a packed ELL
 
 // Device view of a symmetry group (all arrays in device memory).
 struct GroupView {
@@ -155,6 +273,7 @@ struct Ref {
 }  // namespace qmb
 
 struct qmb_ctx : qmb::Counted {
+    qmb::DevPool pool;  // declared first: destroyed after every buffer of the context
     int device = 0;
     hipStream_t stream = nullptr;
     qmb::DevBuf binom;  // uint64 [64*33] C(p, j)
@@ -164,6 +283,8 @@ struct qmb_ctx : qmb::Counted {
             (void)hipStreamSynchronize(stream);
         }
         binom.release();
+        pool.trim();
+        pool.stream = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -213,9 +334,18 @@ struct qmb_ell : qmb::Counted {
     int max_row_len = 0;
     qmb::DevBuf chunk_ptr;  // int64 [n_chunks+1]
     qmb::DevBuf row_len;    // uint16 [n_rows]
-    qmb::DevBuf col;        // int32 [stored]
-    qmb::DevBuf val;        // double or double2 [stored]
-    double spmv_bytes() const;
+    qmb::DevBuf col;        // int32 [stored]; packed: uint32 col | (value index << col_bits), slot 0 unused
+    qmb::DevBuf val;        // double or double2 [stored]; empty when packed
+    // Packed real values: the matrix has few distinct values, so an off-diagonal
+    // slot stores an index into vtab next to its column and the diagonal is kept
+    // in FP64 per row.
+    int packed = 0;
+    int col_bits = 0;
+    int n_values = 0;
+    qmb::DevBuf diag;       // double [n_rows] (packed)
+    qmb::DevBuf vtab;       // double [n_values] (packed), ascending bit patterns, vtab[0] = +0.0
+    double spmv_bytes() const;    // algorithmic bytes of one SpMV (int32 column + FP64 value per entry)
+    double format_bytes() const;  // bytes of one SpMV in the stored format
     int64_t device_bytes() const;
 };
 

@@ -33,17 +33,25 @@ constexpr int kMaxBlocks = 4096;
 // ---- K1: SELL-64 SpMV --------------------------------------------------------
 // One wavefront per 64-row chunk (row products in sell.hpp); the x gathers are
 // the only random accesses. LZ fuses the Lanczos normalisation and the a[n]
-// dot: y = s (H x), phi = s x, partial sums of phi . y.
-template <bool CPLX, bool LZ>
+// dot: y = s (H x), phi = s x, partial sums of phi . y. PACK (real only): the
+// packed value format (4 B per off-diagonal entry, value table staged in LDS).
+template <bool CPLX, bool LZ, bool PACK>
 __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_chunks,
                                                       const int64_t *__restrict__ chunk_ptr,
                                                       const uint16_t *__restrict__ row_len,
                                                       const int32_t *__restrict__ col,
                                                       const double *__restrict__ val,
+                                                      const double *__restrict__ diag,
+                                                      const double *__restrict__ vtab, int n_values, int cb,
                                                       const double *__restrict__ x, double *__restrict__ y,
                                                       const double *__restrict__ invb, double *__restrict__ vout,
                                                       double *__restrict__ partials) {
     __shared__ double lds[kWaves];
+    __shared__ double s_vt[PACK ? kMaxValues : 1];
+    if (PACK) {
+        for (int i = threadIdx.x; i < n_values; i += blockDim.x) s_vt[i] = vtab[i];
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const double s = LZ ? *invb : 1.0;
@@ -55,12 +63,20 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
         const int len = r < n_rows ? (int)row_len[r] : 0;
         const int64_t p0 = base + lane;
         if (!CPLX) {
-            double acc = sell_row_real(col, val, x, p0, L, len);
+            double acc;
+            double xr = 0.0;
+            if (PACK) {
+                xr = r < n_rows ? x[r] : 0.0;
+                const double d = r < n_rows ? diag[r] : 0.0;
+                acc = sell_row_packed(reinterpret_cast<const uint32_t *>(col), s_vt, x, p0, L, len, cb, d, xr);
+            } else {
+                acc = sell_row_real(col, val, x, p0, L, len);
+            }
             if (LZ) acc *= s;
             if (r < n_rows) {
                 y[r] = acc;
                 if (LZ) {
-                    const double vr = x[r] * s;
+                    const double vr = (PACK ? xr : x[r]) * s;
                     vout[r] = vr;
                     dot += vr * acc;
                 }
@@ -294,11 +310,13 @@ void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, boo
                  double *vout, LzWork *wk, double *alpha_out, hipEvent_t after = nullptr) {
     const unsigned blocks = grid_for(e->n_chunks, kWaves);
     const bool cplx = e->value_type == QMB_COMPLEX;
-    auto k = cplx ? (lz ? spmv_sell<true, true> : spmv_sell<true, false>)
-                  : (lz ? spmv_sell<false, true> : spmv_sell<false, false>);
+    auto k = cplx       ? (lz ? spmv_sell<true, true, false> : spmv_sell<true, false, false>)
+             : e->packed ? (lz ? spmv_sell<false, true, true> : spmv_sell<false, false, true>)
+                         : (lz ? spmv_sell<false, true, false> : spmv_sell<false, false, false>);
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, ctx->stream, e->n_rows, e->n_chunks,
                        e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
-                       e->val.as<double>(), x, y, invb, vout, wk ? wk->partials.as<double>() : nullptr);
+                       e->val.as<double>(), e->diag.as<double>(), e->vtab.as<double>(), e->n_values, e->col_bits,
+                       x, y, invb, vout, wk ? wk->partials.as<double>() : nullptr);
     QMB_HIP(hipGetLastError());
     if (after) QMB_HIP(hipEventRecord(after, ctx->stream));
     if (lz) launch_finish(ctx, wk->partials.as<double>(), blocks, alpha_out, nullptr, 0);
@@ -702,6 +720,7 @@ void qmb_lanczos_default_opts(qmb_lanczos_opts *o) {
 
 qmb_status qmb_spmv_device(qmb_ctx *ctx, const qmb_ell *e, const double *x_dev, double *y_dev) {
     return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
         QMB_REQUIRE(ctx && e && x_dev && y_dev, "qmb_spmv_device: NULL argument");
         QMB_HIP(hipSetDevice(ctx->device));
         if (e->n_rows == 0) return;
@@ -711,6 +730,7 @@ qmb_status qmb_spmv_device(qmb_ctx *ctx, const qmb_ell *e, const double *x_dev, 
 
 qmb_status qmb_spmv_host(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y) {
     return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
         QMB_REQUIRE(ctx && e && x && y, "qmb_spmv_host: NULL argument");
         QMB_HIP(hipSetDevice(ctx->device));
         const int64_t cnt = e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1);
@@ -725,6 +745,7 @@ qmb_status qmb_spmv_host(qmb_ctx *ctx, const qmb_ell *e, const double *x, double
 
 qmb_status qmb_spmv_benchmark(qmb_ctx *ctx, const qmb_ell *e, int reps, double *ms_per_spmv) {
     return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
         QMB_REQUIRE(ctx && e && ms_per_spmv && reps >= 1, "qmb_spmv_benchmark: bad argument");
         QMB_HIP(hipSetDevice(ctx->device));
         const int64_t cnt = std::max<int64_t>(1, e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1));
@@ -753,6 +774,7 @@ qmb_status qmb_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts *o
                        double *eigvec, qmb_lanczos_report *report) {
     qmb_status result = QMB_OK;
     qmb_status st = guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
         QMB_REQUIRE(ctx && e && opts && evals, "qmb_lanczos: NULL argument");
         QMB_REQUIRE(e->ctx == ctx, "qmb_lanczos: ELL belongs to another context");
         QMB_REQUIRE(e->n_rows > 0, "qmb_lanczos: empty matrix");

@@ -787,6 +787,7 @@ int qmb_mlp_num_params(void) { return kParams; }
 
 qmb_status qmb_mlp_create(qmb_ctx *ctx, uint64_t seed, qmb_mlp **out) {
     return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
         QMB_REQUIRE(ctx && out, "qmb_mlp_create: NULL argument");
         *out = nullptr;
         QMB_HIP(hipSetDevice(ctx->device));
@@ -837,6 +838,7 @@ qmb_status qmb_mlp_set_params(qmb_mlp *m, const float *theta) {
 
 qmb_status qmb_mlp_forward(qmb_mlp *m, int n, const double *delta, const double *j2, float *out) {
     return guarded([&] {
+        PoolScope pool_scope(m ? &m->ctx->pool : nullptr);
         QMB_REQUIRE(m && delta && j2 && out && n >= 1, "qmb_mlp_forward: bad argument");
         std::vector<double> coup(2 * (size_t)n);
         for (int i = 0; i < n; ++i) {
@@ -851,6 +853,7 @@ qmb_status qmb_mlp_forward(qmb_mlp *m, int n, const double *delta, const double 
 
 qmb_status qmb_mlp_add_sample(qmb_mlp *m, const qmb_basis *b, const double *ground_state, double delta, double j2) {
     return guarded([&] {
+        PoolScope pool_scope(m ? &m->ctx->pool : nullptr);
         QMB_REQUIRE(m && b && ground_state, "qmb_mlp_add_sample: NULL argument");
         QMB_REQUIRE(b->ctx == m->ctx, "qmb_mlp_add_sample: basis from another context");
         MlpSample s;
@@ -902,6 +905,7 @@ qmb_status qmb_mlp_clear_samples(qmb_mlp *m) {
 
 qmb_status qmb_mlp_loss_grad(qmb_mlp *m, double *loss, float *grad) {
     return guarded([&] {
+        PoolScope pool_scope(m ? &m->ctx->pool : nullptr);
         QMB_REQUIRE(m && loss, "qmb_mlp_loss_grad: NULL argument");
         QMB_REQUIRE(!m->samples.empty(), "qmb_mlp_loss_grad: no samples");
         *loss = loss_and_grad(m, grad != nullptr);
@@ -911,6 +915,7 @@ qmb_status qmb_mlp_loss_grad(qmb_mlp *m, double *loss, float *grad) {
 
 qmb_status qmb_mlp_train(qmb_mlp *m, int steps, double lr, double *loss_history) {
     return guarded([&] {
+        PoolScope pool_scope(m ? &m->ctx->pool : nullptr);
         QMB_REQUIRE(m && steps >= 1 && lr > 0, "qmb_mlp_train: bad argument");
         QMB_REQUIRE(!m->samples.empty(), "qmb_mlp_train: no samples");
         const float b1 = 0.9f, b2 = 0.999f;
@@ -930,6 +935,7 @@ qmb_status qmb_mlp_train(qmb_mlp *m, int steps, double lr, double *loss_history)
 
 qmb_status qmb_mlp_start_vector(qmb_mlp *m, const qmb_basis *b, double delta, double j2, double *x, int *fallback) {
     return guarded([&] {
+        PoolScope pool_scope(m ? &m->ctx->pool : nullptr);
         QMB_REQUIRE(m && b && x, "qmb_mlp_start_vector: NULL argument");
         DevBuf dx((size_t)b->n_rows * sizeof(double));
         StateWork w;
@@ -944,6 +950,7 @@ qmb_status qmb_lanczos_learned(qmb_mlp *m, const qmb_basis *b, const qmb_ell *e,
                                const qmb_lanczos_opts *opts, double *evals, double *eigvec, qmb_lanczos_report *report,
                                int *fallback) {
     qmb_status st = guarded([&] {
+        PoolScope pool_scope(m ? &m->ctx->pool : nullptr);
         QMB_REQUIRE(m && b && e && evals, "qmb_lanczos_learned: NULL argument");
         QMB_REQUIRE(e->basis == b, "qmb_lanczos_learned: the ELL was not built on this basis");
         QMB_REQUIRE(e->ctx == m->ctx, "qmb_lanczos_learned: network and ELL live on different contexts");

@@ -49,6 +49,42 @@ __device__ inline double sell_row_real(const int32_t *__restrict__ col, const do
     return acc;
 }
 
+// Packed real rows: slot 0 is the diagonal d (column = the row r itself); slot
+// k >= 1 holds w = col | (index << cb) with the value vt[index]. Same products in
+// the same order as sell_row_real on the unpacked matrix.
+__device__ inline double sell_row_packed(const uint32_t *__restrict__ w, const double *vt,
+                                         const double *__restrict__ x, int64_t p0, int L, int len, int cb,
+                                         double d, double xr) {
+    const uint32_t mask = (1u << cb) - 1u;
+    double acc = 0.0;
+    if (len > 0) acc += d * xr;
+    int k = 1;
+    for (; k + 4 <= L; k += 4) {
+        uint32_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < len) j[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(k + u) * 64);
+        double g[4], v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k + u < len) {
+                g[u] = x[j[u] & mask];
+                v[u] = vt[j[u] >> cb];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < len) acc += v[u] * g[u];
+    }
+    for (; k < L; ++k) {
+        if (k < len) {
+            const uint32_t jj = __builtin_nontemporal_load(w + p0 + (int64_t)k * 64);
+            acc += vt[jj >> cb] * x[jj & mask];
+        }
+    }
+    return acc;
+}
+
 // complex rows: val and x are interleaved (re, im) pairs
 __device__ inline double2 sell_row_cplx(const int32_t *__restrict__ col, const double *__restrict__ val,
                                         const double *__restrict__ x, int64_t p0, int L, int len) {

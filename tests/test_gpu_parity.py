@@ -96,6 +96,41 @@ def test_ell_reference_layout_bit_exact(ctx, label, mk, n_up, alphas):
         assert np.array_equal(e.spmv(x), ref.spmv(x))
 
 
+@pytest.mark.parametrize("label,mk,n_up,alphas", CASES)
+def test_packed_values_equal_fp64_values(ctx, label, mk, n_up, alphas):
+    """A real ELL is stored packed (column | value index words, FP64 diagonal):
+    its export, SpMV results and eigenvalue are bit-identical to the FP64 form."""
+    case = mk()
+    orb = oracle_orbits(case, n_up)
+    for a in alphas:
+        real = bool(np.all(np.asarray(case["chars"])[a].imag == 0))
+        for layout in (q.LAYOUT_REFERENCE, q.LAYOUT_COMPACT):
+            say(f"packed {label} irrep={a} layout={layout}")
+            g = q.Group(case["group_el"], case["chars"], case["flip"])
+            t = q.Terms(case["n_sites"], case["table"])
+            b = q.Basis(ctx, g, n_up, a, layout)
+            ep = q.Ell(ctx, b, t)
+            ef = q.Ell(ctx, b, t, keep_fp64_values=True)
+            ip, jf = ep.info(), ef.info()
+            assert jf.packed == 0
+            assert ip.packed == (1 if real else 0)
+            if real:
+                assert 1 <= ip.n_values <= 1024
+                assert ip.device_bytes < jf.device_bytes
+                assert ip.format_bytes < jf.format_bytes == jf.spmv_bytes
+            assert ip.spmv_bytes == jf.spmv_bytes and ip.nnz == jf.nnz and ip.stored == jf.stored
+            for u, v in zip(ep.export_reference(), ef.export_reference()):
+                assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
+            rng = np.random.default_rng(a + 7)
+            x = rng.normal(size=b.n_rows)
+            if not real:
+                x = x + 1j * rng.normal(size=b.n_rows)
+            assert np.array_equal(ep.spmv(x), ef.spmv(x))
+            e1, r1, _, _ = q.lanczos(ep, max_iter=300, tol=1e-13)
+            e2, r2, _, _ = q.lanczos(ef, max_iter=300, tol=1e-13)
+            assert e1[0] == e2[0] and r1.iters == r2.iters
+
+
 def test_compact_layout_is_reference_without_null_rows(ctx):
     case = chain_case(12, 1.0, 1.0, 0.3, flip=True)
     orb = oracle_orbits(case, 6)
@@ -232,6 +267,7 @@ def test_c2_full_size_majumdar_ghosh(ctx):
     e = q.Ell(ctx, b, t)
     inf = e.info()
     assert inf.max_row_len <= 57
+    assert inf.packed == 1 and inf.n_values <= 4096
     # size-independent properties: symmetric matrix (<x,Hy> == <Hx,y>)
     rng = np.random.default_rng(0)
     x, y = rng.normal(size=b.n_rows), rng.normal(size=b.n_rows)
