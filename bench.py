@@ -77,7 +77,8 @@ def gpu_step(ctx, group, table, cfg, tol):
     return dict(e0=float(ev[0]), iters=int(rep.iters), n_spmv=int(rep.n_spmv), ms_spmv=float(rep.ms_spmv),
                 basis_ms=(t1 - t0) * 1e3, ell_ms=(t2 - t1) * 1e3, lanczos_ms=(t3 - t2) * 1e3,
                 total_ms=(t3 - t0) * 1e3, n_rows=int(inf.n_rows), nnz=int(inf.nnz), stored=int(inf.stored),
-                spmv_bytes=float(inf.spmv_bytes), ell=e, basis=b)
+                spmv_bytes=float(inf.spmv_bytes), format_bytes=float(inf.format_bytes), packed=int(inf.packed),
+                ell=e, basis=b)
 
 
 def cpu_baseline(cfg, last, table, threads, spmv_bytes):
@@ -403,7 +404,10 @@ def run_c5(args):
            "roofline": {"bound": "hbm", "achieved": round(B_k1 / t_k1 / 1e9, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(B_k1 / t_k1 / 1e9 / HBM_PEAK_GBS, 4),
                         "avg_launch_us": round(t_k1 * 1e6, 2), "bytes_per_launch": B_k1,
-                        "traffic": profile_traffic("spmv_sell_square_6x6")},
+                        "traffic": profile_traffic("spmv_sell_square_6x6"),
+                        "stored_format": "packed" if inf.packed else "fp64",
+                        "format_bytes_per_launch": inf.format_bytes + 8.0 * inf.n_rows,
+                        "format_frac": round((inf.format_bytes + 8.0 * inf.n_rows) / t_k1 / 1e9 / HBM_PEAK_GBS, 4)},
            "dtype": "f64", "data": "synthetic", "n_gpus": 1, "cpu_baseline": None}
     if not args.no_learned:
         out["learned_start"] = c5_learned(ctx, lat, b, e, ev[0], rep, t3 - t2, args)
@@ -516,7 +520,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_traffic("spmv_sell_j1j2_chain_n28"),
                      "kernel": "spmv_sell<real, lanczos-fused>", "bytes_per_launch": B_k1,
-                     "avg_launch_us": round(t_k1 * 1e6, 3)},
+                     "avg_launch_us": round(t_k1 * 1e6, 3),
+                     "stored_format": "packed" if runs[0]["packed"] else "fp64",
+                     "format_bytes_per_launch": runs[0]["format_bytes"] + 8.0 * runs[0]["n_rows"],
+                     "format_frac": round((runs[0]["format_bytes"] + 8.0 * runs[0]["n_rows"]) / t_k1 / 1e9
+                                          / HBM_PEAK_GBS, 4)},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

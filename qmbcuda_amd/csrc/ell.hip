@@ -26,10 +26,8 @@ constexpr int kLdsSet = 256;     // per-block set
 constexpr unsigned long long kEmptyKey = ~0ull;
 
 __device__ inline uint32_t key_hash(unsigned long long k) {
-    k ^= k >> 33;
-    k *= 0xff51afd7ed558ccdULL;
-    k ^= k >> 33;
-    return (uint32_t)k;
+    const uint32_t h = (uint32_t)(k >> 32) * 2654435761u;  // the value bits that vary most are the high ones
+    return h ^ (h >> 15) ^ (uint32_t)k;
 }
 
 // Inserts key into an open-addressing set; false when the set is full.
