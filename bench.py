@@ -295,6 +295,61 @@ def run_c3(args):
     print(json.dumps(out), flush=True)
 
 
+def run_c2p(args):
+    """The C2 chain (N=28, J2/J1=0.5, Sz=0, k=0, flip-even) with the site reflection
+    added to the group (translations x spin flip x reflection, 112 elements): the
+    k=0 flip-even sector splits into reflection-even and -odd halves; time-to-E0 of
+    both halves against the translation x flip sector of the default workload."""
+    import qmbcuda_amd as q
+    from qmbcuda_amd import models as LM
+
+    ctx = q.Context(0)
+    lat, info = LM.chain(28, 1.0, 1.0, 0.5)
+    T = LM.translation_group(lat, spin_flip=True)
+    ge, fl, ch, pairs = LM.product_group(T, LM.reflection_group(lat))
+    gp, gt = q.Group(ge, ch, fl), q.Group(T[0], T[2], T[1])
+    table = LM.term_table(lat, info)
+
+    def solve(group, irrep):
+        t0 = time.perf_counter()
+        b = q.Basis(ctx, group, 14, irrep, q.LAYOUT_COMPACT)
+        t1 = time.perf_counter()
+        e = q.Ell(ctx, b, q.Terms(28, table))
+        t2 = time.perf_counter()
+        ev, rep, _, _ = q.lanczos(e, max_iter=1000, tol=args.tol, seed=2212456)
+        t3 = time.perf_counter()
+        return dict(e0=float(ev[0]), rows=int(b.n_rows), iters=int(rep.iters), basis_ms=(t1 - t0) * 1e3,
+                    ell_ms=(t2 - t1) * 1e3, lanczos_ms=(t3 - t2) * 1e3, total_ms=(t3 - t0) * 1e3)
+
+    halves = [pairs.index((0, j)) for j in (0, 1)]
+    for _ in range(max(1, args.warmup)):
+        solve(gt, 0)
+        for a in halves:
+            solve(gp, a)
+    reps = max(1, args.steps)
+    plain = [solve(gt, 0) for _ in range(reps)]
+    parts = [[solve(gp, a) for _ in range(reps)] for a in halves]
+
+    def mean(rs, k):
+        return round(sum(r[k] for r in rs) / len(rs), 3)
+
+    gs = min(range(2), key=lambda i: parts[i][0]["e0"])
+    out = {"metric": METRIC, "config": {"workload": "C2 chain N=28 with the site reflection: translations x spin "
+                                                    "flip x reflection (112 elements), k=0 flip-even, both parities"},
+           "value": mean(parts[gs], "total_ms"), "unit": "ms time-to-E0 (ground-state parity sector)",
+           "higher_is_better": False,
+           "plain_translation_flip_ms": mean(plain, "total_ms"),
+           "sectors": [{"reflection": ["even", "odd"][i], "rows": parts[i][0]["rows"], "e0": parts[i][0]["e0"],
+                        "iters": parts[i][0]["iters"],
+                        "breakdown_ms": {k: mean(parts[i], k) for k in ("basis_ms", "ell_ms", "lanczos_ms")}}
+                       for i in range(2)],
+           "plain": {"rows": plain[0]["rows"], "e0": plain[0]["e0"], "iters": plain[0]["iters"],
+                     "breakdown_ms": {k: mean(plain, k) for k in ("basis_ms", "ell_ms", "lanczos_ms")}},
+           "abs_dE_gs_vs_plain": abs(parts[gs][0]["e0"] - plain[0]["e0"]),
+           "dtype": "f64", "data": "synthetic", "n_gpus": 1}
+    print(json.dumps(out), flush=True)
+
+
 def run_c4(args):
     """Learned start: train the window MLP on k=0 ground states of N=20, 22, 24
     chains over an 8x8 (Delta, J2/J1) grid; evaluate on N=30, k=0 at 16 held-out
@@ -431,7 +486,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "c4", "c5"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c2p", "c3", "c4", "c5"])
     ap.add_argument("--c4-grid", type=int, default=8)
     ap.add_argument("--c4-steps", type=int, default=400)
     ap.add_argument("--c4-eval", type=int, default=16)
@@ -461,8 +516,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    if args.config in ("c3", "c4", "c5"):
-        {"c3": run_c3, "c4": run_c4, "c5": run_c5}[args.config](args)
+    if args.config in ("c2p", "c3", "c4", "c5"):
+        {"c2p": run_c2p, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.config](args)
         return
 
     import qmbcuda_amd as q

@@ -82,11 +82,17 @@ int qmb_device_count(void);
  * Replaces SymmetryClass<T>::GetGroupElemArr + the character table copy of the
  * driver (Q/symmetries/SymmetryClass.cu:26-71; Q/exact_diagonalization/
  * HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:45-61). flip may be NULL (no spin
- * flip). The element list must be closed under inversion and G <= 128. */
+ * flip). The element list must be closed under composition and G <= 128. */
 qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el,
                             const uint8_t *flip, int n_irreps, const double *chars,
                             qmb_group **out);
 qmb_status qmb_group_destroy(qmb_group *g);
+/* valid[a] = 1 when character row a is a one-dimensional representation of the
+ * group (chi(g h) = chi(g) chi(h)); qmb_group_create checks closure under
+ * composition and marks the rows. Rows of a translation group combined with a
+ * point operation are valid only at the momenta the operation fixes; bases of
+ * the other rows are refused. */
+qmb_status qmb_group_irreps_valid(const qmb_group *g, int32_t *valid);
 /* Inverse-element table (inv[g] = index of g^-1) and 1 if the fast torus
  * (cyclic-translation) path was detected, 0 for the generic permutation path. */
 qmb_status qmb_group_info(const qmb_group *g, int32_t *inv, int *torus_path);

@@ -12,6 +12,8 @@
 #include <complex>
 #include <cstdint>
 #include <functional>
+#include <map>
+#include <utility>
 #include <stdexcept>
 #include <vector>
 
@@ -70,6 +72,27 @@ inline Coords Tnm_general(const std::vector<double> &R1, const std::vector<doubl
         if (std::fabs(al1 - g.N2) < tol) al1 = 0.0;
         out[0][i] = al0 * a1[0] + al1 * a2[0];
         out[1][i] = al0 * a1[1] + al1 * a2[1];
+    }
+    return out;
+}
+
+// Coordinates folded back into the N1 x N2 cluster through their (a1, a2)
+// components, for point operations that map the cluster onto a periodic image
+// of itself (components in [0, N) up to a tolerance of 1e-6).
+inline Coords FoldIntoCluster(const std::vector<double> &R1, const std::vector<double> &R2,
+                              const LatticeGeometryInfo &g) {
+    const double a1[2] = {g.A_mat[0][0], g.A_mat[1][0]};
+    const double a2[2] = {g.A_mat[0][1], g.A_mat[1][1]};
+    const double det = a1[0] * a2[1] - a1[1] * a2[0];
+    Coords out{R1, R2};
+    for (size_t i = 0; i < R1.size(); ++i) {
+        // solve x = u a1 + v a2 for the cell coordinates (u, v)
+        double u = (R1[i] * a2[1] - R2[i] * a2[0]) / det;
+        double v = (a1[0] * R2[i] - a1[1] * R1[i]) / det;
+        u -= g.N1 * std::floor(u / g.N1 + 1e-6);
+        v -= g.N2 * std::floor(v / g.N2 + 1e-6);
+        out[0][i] = u * a1[0] + v * a2[0];
+        out[1][i] = u * a1[1] + v * a2[1];
     }
     return out;
 }
@@ -194,6 +217,77 @@ class Cn_group : public SymmetryClassC {
   protected:
     int n;
     std::vector<double> R0;
+};
+
+// Product set {a o b} (b first) of two groups acting on one lattice, e.g. the
+// translations with a reflection or a rotation about a site (new in this
+// engine; the reference uses point groups only without translations). With
+// `fold`, coordinates are folded back into that torus cluster after each
+// factor. Element ia*|B| + ib, spin flips add; characters chi_i(a) chi_j(b) for
+// the pairs (i, j) that are one-dimensional representations of the product on
+// the sites R1, R2 (a momentum the point operation does not fix gives none),
+// kept in (i, j) order; GetIrrepPairs() lists them.
+template <class T>
+class ProductGroup : public SymmetryClass<T> {
+  public:
+    ProductGroup(const SymmetryClass<T> &A, const SymmetryClass<T> &B, const std::vector<double> &R1,
+                 const std::vector<double> &R2, const LatticeGeometryInfo *fold = nullptr) {
+        const auto ea = A.GetGroupElemVec(), eb = B.GetGroupElemVec();
+        auto fa = A.GetSpinFlips(), fb = B.GetSpinFlips();
+        if (fa.empty()) fa.assign(ea.size(), 0);
+        if (fb.empty()) fb.assign(eb.size(), 0);
+        const size_t GA = ea.size(), GB = eb.size(), G = GA * GB;
+        const bool has_fold = fold != nullptr;
+        const LatticeGeometryInfo geom = has_fold ? *fold : LatticeGeometryInfo();
+        for (size_t ia = 0; ia < GA; ++ia)
+            for (size_t ib = 0; ib < GB; ++ib) {
+                const CoordMap a = ea[ia], b = eb[ib];
+                this->Group_elems.push_back([a, b, has_fold, geom](const std::vector<double> &X,
+                                                                   const std::vector<double> &Y) {
+                    Coords c = b(X, Y);
+                    if (has_fold) c = FoldIntoCluster(c[0], c[1], geom);
+                    c = a(c[0], c[1]);
+                    if (has_fold) c = FoldIntoCluster(c[0], c[1], geom);
+                    return c;
+                });
+                this->spin_flips.push_back(static_cast<uint8_t>(fa[ia] ^ fb[ib]));
+            }
+        // composition table on these sites: source tables ge[g][dest] = src
+        const size_t LS = R1.size();
+        std::vector<uint32_t> ge(G * LS);
+        this->GetGroupElemArr(R1, R2, ge.data());
+        std::map<std::pair<std::vector<uint32_t>, int>, size_t> index;
+        for (size_t e = 0; e < G; ++e)
+            index.emplace(std::make_pair(std::vector<uint32_t>(ge.begin() + e * LS, ge.begin() + (e + 1) * LS),
+                                         (int)this->spin_flips[e]),
+                          e);
+        std::vector<size_t> comp(G * G);
+        std::vector<uint32_t> pr(LS);
+        for (size_t x = 0; x < G; ++x)
+            for (size_t y = 0; y < G; ++y) {
+                for (size_t d = 0; d < LS; ++d) pr[d] = ge[y * LS + ge[x * LS + d]];
+                auto it = index.find({pr, (int)(this->spin_flips[x] ^ this->spin_flips[y])});
+                if (it == index.end()) throw std::runtime_error("ProductGroup: the product is not closed");
+                comp[x * G + y] = it->second;
+            }
+        const auto ca = A.GetCharTable(), cb = B.GetCharTable();
+        for (size_t i = 0; i < ca.size(); ++i)
+            for (size_t j = 0; j < cb.size(); ++j) {
+                std::vector<T> row(G);
+                for (size_t e = 0; e < G; ++e) row[e] = ca[i][e / GB] * cb[j][e % GB];
+                bool hom = true;
+                for (size_t x = 0; x < G && hom; ++x)
+                    for (size_t y = 0; y < G && hom; ++y)
+                        hom = std::abs(row[comp[x * G + y]] - row[x] * row[y]) < 1e-9;
+                if (!hom) continue;
+                this->char_table.push_back(row);
+                pairs.push_back({(int)i, (int)j});
+            }
+    }
+    std::vector<std::pair<int, int>> GetIrrepPairs() const { return pairs; }
+
+  private:
+    std::vector<std::pair<int, int>> pairs;
 };
 
 // Z2 spin inversion times a group (new in this engine; the reference has no

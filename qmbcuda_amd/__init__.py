@@ -79,6 +79,12 @@ class Group:
                                       _ptr(self._ch), C.byref(h)))
         self.h = h
 
+    def valid_irreps(self):
+        """1 where the character row is a one-dimensional representation of the group."""
+        v = np.zeros(self.n_irreps, np.int32)
+        _check(lib().qmb_group_irreps_valid(self.h, _ptr(v)))
+        return v.astype(bool)
+
     def info(self):
         inv = np.zeros(self.G, np.int32)
         torus = C.c_int(0)

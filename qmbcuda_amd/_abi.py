@@ -112,6 +112,7 @@ SIGNATURES = [
     ("qmb_ell_get_info", C.c_int, [_vp, C.POINTER(EllInfo)]),
     ("qmb_ell_export_reference", C.c_int, [_vp, _vp, _vp, _vp]),
     ("qmb_ell_export_rows", C.c_int, [_vp, C.c_int64, C.c_int64, _vp, _vp, _vp]),
+    ("qmb_group_irreps_valid", C.c_int, [_vp, _vp]),
     ("qmb_expectation", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp, C.POINTER(C.c_int)]),
     ("qmb_spmv_host", C.c_int, [_vp, _vp, _vp, _vp]),
     ("qmb_spmv_device", C.c_int, [_vp, _vp, _vp, _vp]),

@@ -1,9 +1,11 @@
 // Context, group and term-table handles, error reporting, device scans.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <unordered_map>
 
 #include "internal.hpp"
 
@@ -390,6 +392,41 @@ qmb_status qmb_ctx_synchronize(qmb_ctx *ctx) {
     });
 }
 
+// Closure under composition, and which character rows are one-dimensional
+// representations (chi(a b) = chi(a) chi(b) for every pair, equal characters on
+// repeated elements). Point operations combined with translations give rows
+// that are not (momenta the operation does not fix); those rows stay in the
+// table but no basis can be built for them.
+static void check_closure_and_irreps(qmb_group &g) {
+    const int n = g.n_sites, G = g.G;
+    std::unordered_map<std::string, int> index;
+    auto key = [&](const uint32_t *p, int f) {
+        std::string k((size_t)n + 1, '\0');
+        for (int s = 0; s < n; ++s) k[s] = (char)p[s];
+        k[n] = (char)f;
+        return k;
+    };
+    std::vector<int> first(G);
+    for (int e = 0; e < G; ++e) first[e] = index.emplace(key(&g.perm[(size_t)e * n], g.flip[e]), e).first->second;
+    g.irrep_ok.assign(g.n_irreps, 1);
+    auto chi = [&](int a, int e) { return std::complex<double>(g.chars[((size_t)a * G + e) * 2], g.chars[((size_t)a * G + e) * 2 + 1]); };
+    const double tol = 1e-9;
+    for (int e = 0; e < G; ++e)
+        for (int a = 0; a < g.n_irreps; ++a)
+            if (std::abs(chi(a, e) - chi(a, first[e])) > tol) g.irrep_ok[a] = 0;
+    std::vector<uint32_t> pr(n);
+    for (int x = 0; x < G; ++x)
+        for (int y = 0; y < G; ++y) {
+            const uint32_t *px = &g.perm[(size_t)x * n], *py = &g.perm[(size_t)y * n];
+            for (int s = 0; s < n; ++s) pr[s] = px[py[s]];
+            auto it = index.find(key(pr.data(), g.flip[x] ^ g.flip[y]));
+            QMB_REQUIRE(it != index.end(), "qmb_group_create: element list is not closed under composition");
+            const int z = it->second;
+            for (int a = 0; a < g.n_irreps; ++a)
+                if (g.irrep_ok[a] && std::abs(chi(a, z) - chi(a, x) * chi(a, y)) > tol) g.irrep_ok[a] = 0;
+        }
+}
+
 qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el, const uint8_t *flip,
                             int n_irreps, const double *chars, qmb_group **out) {
     return guarded([&] {
@@ -431,9 +468,17 @@ qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el, 
             }
             QMB_REQUIRE(g->inv[e] >= 0, "qmb_group_create: element list is not closed under inversion");
         }
+        check_closure_and_irreps(*g);
         detect_torus(*g);
         build_lut(*g);
         *out = g.release();
+    });
+}
+
+qmb_status qmb_group_irreps_valid(const qmb_group *g, int32_t *valid) {
+    return guarded([&] {
+        QMB_REQUIRE(g && valid, "qmb_group_irreps_valid: NULL argument");
+        for (int a = 0; a < g->n_irreps; ++a) valid[a] = g->irrep_ok[a];
     });
 }
 

@@ -204,6 +204,8 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
         QMB_REQUIRE(n_up >= 0 && n_up <= g->n_sites, "qmb_basis_build: n_up out of range");
         QMB_REQUIRE(n_up <= 32, "qmb_basis_build: at most 32 up spins (use the flipped sector)");
         QMB_REQUIRE(irrep >= 0 && irrep < g->n_irreps, "qmb_basis_build: irrep out of range");
+        QMB_REQUIRE(g->irrep_ok[irrep], "qmb_basis_build: character row " + std::to_string(irrep) +
+                                            " is not a one-dimensional representation of the group");
         QMB_REQUIRE(layout == QMB_LAYOUT_REFERENCE || layout == QMB_LAYOUT_COMPACT, "qmb_basis_build: bad layout");
         for (int e = 0; e < g->G; ++e)
             QMB_REQUIRE(!g->flip[e] || 2 * n_up == g->n_sites,

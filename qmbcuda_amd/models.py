@@ -149,6 +149,76 @@ def translation_group(lat: Lattice, spin_flip: bool = False):
     return ge2, fl, ch2
 
 
+def reflection_group(lat: Lattice):
+    """Site reflection of a chain, i -> -i (mod N): elements [identity, P], irreps
+    [even, odd]. Same return form as translation_group."""
+    n = lat.n_sites
+    assert lat.N2 == 1 and lat.M == 1, "a chain"
+    ge = np.zeros((2, n), np.uint32)
+    for s in range(n):
+        ge[0, s] = s
+        ge[1, (-s) % n] = s
+    ch = np.array([[1, 1], [1, -1]], np.complex128)
+    return ge, np.zeros(2, np.uint8), ch
+
+
+def rotation_group(lat: Lattice, order: int = 4):
+    """Rotations of an L x L square torus about site (0, 0): element k turns by
+    2 pi k/4 ((i, j) -> (-j, i) per quarter turn, mod L); order 2 keeps
+    {identity, pi}. Irrep a has chi(k) = exp(2 pi i a k / order)."""
+    L = lat.N1
+    assert lat.N2 == L and lat.M == 1 and order in (2, 4), "an L x L square torus"
+    n = lat.n_sites
+    step = 4 // order
+    ge = np.zeros((order, n), np.uint32)
+    for k in range(order):
+        for i in range(L):
+            for j in range(L):
+                x, y = i, j
+                for _ in range(k * step):
+                    x, y = (-y) % L, x
+                ge[k, lat.site(x, y)] = lat.site(i, j)
+    ch = np.array([[_root_of_unity(a * k, order) for k in range(order)] for a in range(order)], np.complex128)
+    return ge, np.zeros(order, np.uint8), ch
+
+
+def product_group(A, B):
+    """Product set {a o b} (b first) of two groups on one lattice, each given as
+    (group_el, flip, chars). Element ia*|B| + ib; characters chi_i(a) chi_j(b)
+    for every pair (i, j) that is a one-dimensional representation of the product
+    (checked on the composition table; a momentum that the point operation does
+    not fix gives none), rows in (i, j) order. Returns (group_el, flip, chars, pairs)."""
+    geA, flA, chA = (np.asarray(x) for x in A)
+    geB, flB, chB = (np.asarray(x) for x in B)
+    GA, GB = len(geA), len(geB)
+    n = geA.shape[1]
+    G = GA * GB
+    ge = np.zeros((G, n), np.uint32)
+    fl = np.zeros(G, np.uint8)
+    for ia in range(GA):
+        for ib in range(GB):
+            ge[ia * GB + ib] = geB[ib][geA[ia]]
+            fl[ia * GB + ib] = flA[ia] ^ flB[ib]
+    index = {}
+    for e in range(G):
+        index.setdefault((ge[e].tobytes(), int(fl[e])), e)
+    comp = np.zeros((G, G), np.int64)
+    for x in range(G):
+        for y in range(G):
+            key = (ge[y][ge[x]].tobytes(), int(fl[x] ^ fl[y]))
+            if key not in index:
+                raise ValueError("the product is not closed under composition")
+            comp[x, y] = index[key]
+    rows, pairs = [], []
+    for i in range(chA.shape[0]):
+        for j in range(chB.shape[0]):
+            chi = np.array([chA[i][e // GB] * chB[j][e % GB] for e in range(G)])
+            if np.allclose(chi[comp], chi[:, None] * chi[None, :], atol=1e-12):
+                rows.append(chi)
+                pairs.append((i, j))
+    return ge, fl, np.array(rows, np.complex128), pairs
+
+
 def momentum_irrep(lat: Lattice, kx: int, ky: int = 0, flip_parity: int | None = None) -> int:
     """Irrep index of momentum (2 pi kx/N1, 2 pi ky/N2) and, with spin flip, parity (0 even, 1 odd)."""
     a = (kx % lat.N1) * lat.N2 + (ky % lat.N2)

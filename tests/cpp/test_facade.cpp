@@ -74,6 +74,34 @@ static Case make_case(const std::string &name) {
         auto *f = new SpinFlipGroup<cd>(*t);
         c.owned = {t, f};
         c.group = f;
+    } else if (name == "chain12_reflection") {
+        // translations x spin flip x the site reflection x -> -x, folded into the ring
+        c.model = Heisenberg<cd>(create_square_lattice_info(12, 1), chain_info(1.0, 1.0, 0.4));
+        auto *t = new Tnm_group(c.model.GetGeomInfo());
+        auto *f = new SpinFlipGroup<cd>(*t);
+        auto *p = new SymmetryClassC({{cd(1), cd(1)}, {cd(1), cd(-1)}},
+                                     {Iden, [](const std::vector<double> &X, const std::vector<double> &Y) {
+                                          return sigma_n_arb(X, Y, 0.0);
+                                      }});
+        const auto geom = c.model.GetGeomInfo();
+        auto *pg = new ProductGroup<cd>(*f, *p, c.model.GetR1(), c.model.GetR2(), &geom);
+        c.owned = {t, f, p, pg};
+        c.group = pg;
+    } else if (name == "square4x4_c4") {
+        // translations x spin flip x C4 rotations about site 0, folded into the torus
+        HeisenbergInfo<cd> info;
+        info.Jxy_terms = {{1, 0, 0, 0}, {-1, 0, 0, 0}, {0, 1, 0, 0}, {0, -1, 0, 0}};
+        info.Jxy_couplings = {cd(0.5), cd(0.5), cd(0.5), cd(0.5)};
+        info.Jz_terms = {{1, 0, 0, 0}, {0, 1, 0, 0}};
+        info.Jz_couplings = {1.0, 1.0};
+        c.model = Heisenberg<cd>(create_square_lattice_info(4, 4), info);
+        auto *t = new Tnm_group(c.model.GetGeomInfo());
+        auto *f = new SpinFlipGroup<cd>(*t);
+        auto *r = new Cn_group(4, {0.0, 0.0});
+        const auto geom = c.model.GetGeomInfo();
+        auto *pg = new ProductGroup<cd>(*f, *r, c.model.GetR1(), c.model.GetR2(), &geom);
+        c.owned = {t, f, r, pg};
+        c.group = pg;
     } else {
         std::fprintf(stderr, "unknown case %s\n", name.c_str());
         std::exit(2);

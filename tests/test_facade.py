@@ -159,3 +159,50 @@ def test_facade_ground_state_expectation_values(name, n_up):
         v = complex(*got[key])
         tol = 1e-10 if key == "H" else 1e-5
         assert abs(v - want) < tol * max(1, abs(want)), (key, v, want)
+
+
+def product_case(name):
+    import qmbcuda_amd.models as LM
+
+    if name == "chain12_reflection":
+        lat, info = LM.chain(12, 1.0, 1.0, 0.4)
+        P = LM.reflection_group(lat)
+        n_up = 6
+    else:
+        lat, info = LM.square(4, 4, 1.0, 1.0)
+        P = LM.rotation_group(lat, 4)
+        n_up = 8
+    ge, fl, ch, pairs = LM.product_group(LM.translation_group(lat, spin_flip=True), P)
+    return dict(n_sites=lat.n_sites, table=LM.term_table(lat, info), group_el=ge, chars=ch, flip=fl), n_up
+
+
+@pytest.mark.parametrize("name", ["chain12_reflection", "square4x4_c4"])
+def test_facade_product_group_tables(name):
+    """ProductGroup (translations x spin flip x a point operation, folded into the
+    torus) gives the element arrays, flips and character rows of the Python
+    builder."""
+    got = run("tables", name)
+    ref, _ = product_case(name)
+    G = len(ref["group_el"])
+    assert np.array_equal(np.array(got["group_el"]).reshape(G, -1), ref["group_el"])
+    assert np.array_equal(np.array(got["flip"]), ref["flip"])
+    chars = np.array(got["chars"]).reshape(len(ref["chars"]), G, 2)
+    np.testing.assert_allclose(chars[..., 0] + 1j * chars[..., 1], ref["chars"], atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["chain12_reflection", "square4x4_c4"])
+def test_facade_driver_with_point_group(name):
+    print(f"[gpu-case] facade solve {name}", flush=True)
+    got = run("solve", name)
+    ref, n_up = product_case(name)
+    orb = oracle_orbits(ref, n_up)
+    E = got["E"]
+    assert len(E) == orb.n_irreps
+    for a in range(orb.n_irreps):
+        M = ell_dense_nonnull(oracle_ell(ref, orb, a, max_terms=40), orb, a)
+        if M.shape[0] == 0:
+            assert E[a] is None
+            continue
+        e0 = np.linalg.eigvalsh(M)[0]
+        assert abs(E[a] - e0) < 1e-10 * max(1, abs(e0))

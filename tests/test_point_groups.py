@@ -1,0 +1,136 @@
+"""Point operations combined with translations (reflections of a chain, C4
+rotations of a square torus): product groups whose one-dimensional
+representations split the momentum sectors that the operation fixes.
+
+Pins: the library's closure and representation checks agree with an
+independent composition table; the union of the point-group sectors' spectra
+equals the spectrum of the translation sector they split (oracle, dense); on
+the GPU the bases and ELLs are bit-exact against the oracle and the energies
+match dense diagonalisation.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import qmbcuda_amd as q
+import qmbcuda_amd.models as LM
+from oracle import model as OM
+from tests.helpers import oracle_ell
+
+SZ, SP, SM = OM.OP_SZ, OM.OP_SP, OM.OP_SM
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def chain_reflection_case(n=12, J2=0.4):
+    lat, info = LM.chain(n, 1.0, 1.0, J2)
+    T = LM.translation_group(lat, spin_flip=True)
+    ge, fl, ch, pairs = LM.product_group(T, LM.reflection_group(lat))
+    return dict(n_sites=n, lat=lat, table=LM.term_table(lat, info), base=T, group_el=ge, chars=ch, flip=fl,
+                pairs=pairs, terms=None)
+
+
+def square_c4_case(L=4):
+    lat, info = LM.square(L, L, 1.0, 1.0)
+    T = LM.translation_group(lat, spin_flip=True)
+    ge, fl, ch, pairs = LM.product_group(T, LM.rotation_group(lat, 4))
+    return dict(n_sites=L * L, lat=lat, table=LM.term_table(lat, info), base=T, group_el=ge, chars=ch, flip=fl,
+                pairs=pairs, terms=None)
+
+
+CASES = [("chain12_reflection_flip", chain_reflection_case, 6), ("square4x4_c4_flip", square_c4_case, 8)]
+
+
+def sector_spectrum(case, group_el, chars, flip, n_up, alpha):
+    orb = oracle.Orbits(case["n_sites"], n_up, group_el, chars, flip=flip)
+    keep = ~orb.null_mask(alpha)
+    if not keep.any():
+        return np.zeros(0)
+    W = OM.max_terms_rule(len(case["table"][3]), n_up, case["n_sites"]) + 4
+    real = np.all(np.asarray(chars)[alpha].imag == 0)
+    ell = oracle.Ell(orb, case["table"], alpha, W, not real)
+    return np.linalg.eigvalsh(ell.dense()[np.ix_(keep, keep)])
+
+
+@pytest.mark.parametrize("label,mk,n_up", CASES)
+def test_library_marks_the_representations(label, mk, n_up):
+    """Every pair (translation irrep, point irrep): the library's flag equals the
+    composition-table check of the Python builder."""
+    case = mk()
+    geT, flT, chT = case["base"]
+    P = LM.reflection_group(case["lat"]) if "chain" in label else LM.rotation_group(case["lat"], 4)
+    geP, flP, chP = P
+    GB = len(geP)
+    ge, fl, _, pairs = LM.product_group(case["base"], P)
+    rows = np.array([[chT[i][e // GB] * chP[j][e % GB] for e in range(len(ge))]
+                     for i in range(chT.shape[0]) for j in range(chP.shape[0])])
+    g = q.Group(ge, rows, fl)
+    valid = g.valid_irreps()
+    expect = np.zeros(len(rows), bool)
+    for (i, j) in pairs:
+        expect[i * chP.shape[0] + j] = True
+    assert np.array_equal(valid, expect)
+    assert q.Group(case["group_el"], case["chars"], case["flip"]).valid_irreps().all()
+
+
+def test_non_closed_element_list_is_refused():
+    lat, _ = LM.chain(8)
+    ge, fl, ch = LM.translation_group(lat)
+    with pytest.raises(q.QMBError):
+        q.Group(np.concatenate([ge, LM.reflection_group(lat)[0][1:]]), np.ones((1, 9)), None)
+
+
+@pytest.mark.parametrize("label,mk,n_up", CASES)
+def test_point_group_sectors_split_the_momentum_sector(label, mk, n_up):
+    """Union over the point-group irreps of the product sectors' spectra = the
+    spectrum of the translation (x spin flip) sector they refine."""
+    case = mk()
+    geT, flT, chT = case["base"]
+    by_t = {}
+    for row, (i, j) in enumerate(case["pairs"]):
+        by_t.setdefault(i, []).append(row)
+    assert len(by_t) == 4  # two fixed momenta x two flip parities
+    for i, rows in by_t.items():
+        whole = sector_spectrum(case, geT, chT, flT, n_up, i)
+        parts = np.sort(np.concatenate([sector_spectrum(case, case["group_el"], case["chars"], case["flip"], n_up, r)
+                                        for r in rows]))
+        assert len(parts) == len(whole) > 0
+        np.testing.assert_allclose(parts, whole, atol=1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("label,mk,n_up", CASES)
+def test_gpu_point_group_sectors(ctx, label, mk, n_up):
+    case = mk()
+    orb = oracle.Orbits(case["n_sites"], n_up, case["group_el"], case["chars"], flip=case["flip"])
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    t = q.Terms(case["n_sites"], case["table"])
+    for a in range(len(case["pairs"])):
+        say(f"point group {label} irrep={a}")
+        b = q.Basis(ctx, g, n_up, a, q.LAYOUT_REFERENCE)
+        reps, stab, nul = b.export()
+        assert np.array_equal(reps, orb.reps)
+        assert np.array_equal(stab.astype(np.int32), orb.stab)
+        assert np.array_equal(nul.astype(bool), orb.null_mask(a))
+        ref = oracle_ell({"table": case["table"], "chars": case["chars"]}, orb, a,
+                         max_terms=OM.max_terms_rule(len(case["table"][3]), n_up, case["n_sites"]) + 4)
+        real = not ref.is_complex
+        e = q.Ell(ctx, b, t, value_type=q.REAL if real else q.COMPLEX, max_terms=ref.W)
+        ind, val, track = e.export_reference()
+        assert np.array_equal(track, ref.track)
+        assert np.array_equal(ind, ref.ind)
+        assert np.array_equal(val.view(np.float64), ref.val)
+        keep = ~orb.null_mask(a)
+        if not keep.any():
+            continue
+        dense = np.linalg.eigvalsh(ref.dense()[np.ix_(keep, keep)])
+        bc = q.Basis(ctx, g, n_up, a, q.LAYOUT_COMPACT)
+        ec = q.Ell(ctx, bc, t, value_type=q.REAL if real else q.COMPLEX, max_terms=ref.W)
+        ev, rep, _, _ = q.lanczos(ec, max_iter=400, tol=1e-13)
+        assert rep.converged
+        assert abs(ev[0] - dense[0]) < 1e-10 * max(1.0, abs(dense[0]))
