@@ -7,6 +7,7 @@
 //   expect <case>  ground-state expectation values through ComputeStaticExpValZeroT
 //   ref <case>     run LanczosEigVals (the reference free function) on GPU-built
 //                  reference-layout arrays read from stdin (n W then ind/val/track)
+#include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstring>
@@ -74,6 +75,25 @@ static Case make_case(const std::string &name) {
         auto *f = new SpinFlipGroup<cd>(*t);
         c.owned = {t, f};
         c.group = f;
+    } else if (name == "honeycomb2x3_phase") {
+        // two sites per cell, complex hopping J/2 exp(+-i 0.37), Jz = 0.8
+        LatticeGeometryInfo g;
+        const double sq3 = std::sqrt(3.0) / 2.0;
+        g.A_mat = {{1.0, 0.5}, {0.0, sq3}};
+        g.N1 = 2;
+        g.N2 = 3;
+        g.rx_alpha = {0.0, 0.5};
+        g.ry_alpha = {0.0, sq3 / 3.0};
+        HeisenbergInfo<cd> info;
+        const cd hop = 0.5 * std::polar(1.0, 0.37);
+        info.Jxy_terms = {{0, 0, 1, 0}, {-1, 0, 1, 0}, {0, -1, 1, 0}, {0, 0, 0, 1}, {1, 0, 0, 1}, {0, 1, 0, 1}};
+        info.Jxy_couplings = {hop, hop, hop, std::conj(hop), std::conj(hop), std::conj(hop)};
+        info.Jz_terms = {{0, 0, 1, 0}, {-1, 0, 1, 0}, {0, -1, 1, 0}};
+        info.Jz_couplings = {0.8, 0.8, 0.8};
+        c.model = Heisenberg<cd>(g, info);
+        auto *t = new Tnm_group(c.model.GetGeomInfo());
+        c.owned.push_back(t);
+        c.group = t;
     } else if (name == "chain12_reflection") {
         // translations x spin flip x the site reflection x -> -x, folded into the ring
         c.model = Heisenberg<cd>(create_square_lattice_info(12, 1), chain_info(1.0, 1.0, 0.4));

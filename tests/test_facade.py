@@ -45,6 +45,11 @@ def oracle_case(name):
     elif name == "square4x4_flip":
         geom, info = OM.square_xxz_info(4, 4, 1.0, 1.0)
         flip = True
+    elif name == "honeycomb2x3_phase":
+        from tests.test_lattices import honeycomb
+
+        geom, info, _, _ = honeycomb(2, 3)
+        flip = False
     terms = OM.heisenberg_operator(geom, info)
     table = OM.conjugated_term_table(terms)
     ge, ch = OM.translation_group(geom)
@@ -54,7 +59,7 @@ def oracle_case(name):
     return dict(n_sites=geom.n_sites, terms=terms, table=table, group_el=ge, chars=ch, flip=fl)
 
 
-CASES = ["chain16", "chain12_mg_flip", "square2x4_example", "triangular3x3", "square4x4_flip"]
+CASES = ["chain16", "chain12_mg_flip", "square2x4_example", "triangular3x3", "square4x4_flip", "honeycomb2x3_phase"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -78,7 +83,7 @@ def test_facade_tables_match_oracle(name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,n_up", [("chain16", 8), ("chain12_mg_flip", 6), ("square2x4_example", 4),
-                                       ("triangular3x3", 4)])
+                                       ("triangular3x3", 4), ("honeycomb2x3_phase", 6)])
 def test_facade_driver_solves_every_irrep(name, n_up):
     print(f"[gpu-case] facade solve {name}", flush=True)
     got = run("solve", name)
