@@ -232,6 +232,28 @@ qmb_status qmb_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts *o
                        double *evals, double *krylov, double *eigvec,
                        qmb_lanczos_report *report);
 
+/* ---- on-disk formats: restartable runs (new) ---------------------------------
+ * The reference has only a commented-out raw float dump of its ELL arrays
+ * (Q/exact_diagonalization/HeisenbergHamAbelianSymms_CUDA.cu:149-162,
+ * Q/utils/print_funcs.cu:73-90). qmb_ell_save writes the device matrix (FP64 or
+ * packed) behind a versioned header tied to its basis (sector, irrep, layout,
+ * a hash of the group tables); qmb_ell_load reads it back for the same basis
+ * and checks on the device that every index stays inside the arrays.
+ * qmb_lanczos_checkpointed is qmb_lanczos that writes its exact state to `path`
+ * at the first convergence check at least `every` steps after the previous write
+ * (0: only when stopping), stops after `stop_after` steps of this call (0: no
+ * limit; returns QMB_ERR_NOT_CONVERGED with the current Ritz values), and with
+ * resume = 1 continues from `path` instead of the start vector (same ELL and
+ * options; a USER/DEVICE start vector must be passed again for want_eigvec).
+ * A resumed run ends with the same results as an uninterrupted one. The state
+ * holds a[], b[], two vectors, or every kept vector with reorthogonalisation. */
+qmb_status qmb_ell_save(const qmb_ell *e, const char *path);
+qmb_status qmb_ell_load(qmb_ctx *ctx, const qmb_basis *b, const char *path, qmb_ell **out);
+qmb_status qmb_lanczos_checkpointed(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts *opts,
+                                    const char *path, int every, int resume, int stop_after,
+                                    double *evals, double *krylov, double *eigvec,
+                                    qmb_lanczos_report *report);
+
 /* Drop-in for the reference's free function LanczosEigVals_CUDA
  * (Q/exact_diagonalization/lanczos.h:21-32): fixed NIter steps on a
  * reference-layout ELL given as HOST arrays (index_mat[nobv*max_terms],

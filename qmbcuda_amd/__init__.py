@@ -8,6 +8,7 @@ of the library, and it raises if libqmb.so is missing.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -193,6 +194,21 @@ class Ell:
     def is_complex(self) -> bool:
         return self.value_type == COMPLEX
 
+    def save(self, path):
+        """Write the device matrix to a file (qmb_ell_save)."""
+        _check(lib().qmb_ell_save(self.h, os.fsencode(str(path))))
+
+    @classmethod
+    def load(cls, ctx: Context, basis: Basis, path, terms: Terms = None):
+        """A matrix saved for this basis (qmb_ell_load)."""
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        _check(lib().qmb_ell_load(ctx.h, basis.h, os.fsencode(str(path)), C.byref(h)))
+        self.h = h
+        self.ctx, self.basis, self.terms = ctx, basis, terms
+        self.value_type = int(self.info().value_type)
+        return self
+
     def info(self) -> EllInfo:
         inf = EllInfo()
         _check(lib().qmb_ell_get_info(self.h, C.byref(inf)))
@@ -268,6 +284,28 @@ def lanczos(ell: Ell, n_eigs=1, max_iter=300, tol=1e-13, seed=2212456, reorth=No
     else:
         _check(st)
     return ev, rep, kr[: rep.iters], vec
+
+
+def lanczos_checkpointed(ell: Ell, path, every=0, resume=False, stop_after=0, n_eigs=1, max_iter=300, tol=1e-13,
+                         seed=2212456, reorth=None, check_every=10, want_eigvec=False):
+    """lanczos with a checkpoint file: written every `every` steps (at a convergence
+    check), and when the call stops after `stop_after` steps; resume=True continues
+    from it. Returns (evals, report, krylov, eigvec, converged)."""
+    if reorth is None:
+        reorth = n_eigs > 1
+    o = default_opts(max_iter=max_iter, n_eigs=n_eigs, tol=tol, seed=seed, reorth=int(reorth),
+                     check_every=check_every, want_eigvec=int(want_eigvec))
+    ev = np.zeros(n_eigs, np.float64)
+    kr = np.zeros(max_iter, np.float64)
+    n = ell.basis.n_rows
+    vec = np.zeros(n, np.complex128 if ell.is_complex else np.float64) if want_eigvec else None
+    rep = LanczosReport()
+    st = lib().qmb_lanczos_checkpointed(ell.ctx.h, ell.h, C.byref(o), os.fsencode(str(path)), int(every),
+                                        int(bool(resume)), int(stop_after), _ptr(ev), _ptr(kr), _ptr(vec),
+                                        C.byref(rep))
+    if st != 4:
+        _check(st)
+    return ev, rep, kr[: rep.iters], vec, st == 0
 
 
 def lanczos_batched(ells, n_eigs=1, max_iter=300, tol=1e-13, seed=2212456, reorth=None, check_every=10):

@@ -113,6 +113,10 @@ SIGNATURES = [
     ("qmb_ell_export_reference", C.c_int, [_vp, _vp, _vp, _vp]),
     ("qmb_ell_export_rows", C.c_int, [_vp, C.c_int64, C.c_int64, _vp, _vp, _vp]),
     ("qmb_group_irreps_valid", C.c_int, [_vp, _vp]),
+    ("qmb_ell_save", C.c_int, [_vp, C.c_char_p]),
+    ("qmb_ell_load", C.c_int, [_vp, _vp, C.c_char_p, _pp]),
+    ("qmb_lanczos_checkpointed", C.c_int, [_vp, _vp, _vp, C.c_char_p, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
+                                           _vp]),
     ("qmb_expectation", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp, C.POINTER(C.c_int)]),
     ("qmb_spmv_host", C.c_int, [_vp, _vp, _vp, _vp]),
     ("qmb_spmv_device", C.c_int, [_vp, _vp, _vp, _vp]),

@@ -469,6 +469,15 @@ qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el, 
             QMB_REQUIRE(g->inv[e] >= 0, "qmb_group_create: element list is not closed under inversion");
         }
         check_closure_and_irreps(*g);
+        uint64_t hsh = 1469598103934665603ULL;  // FNV-1a, identifies the group in saved files
+        auto mix = [&](const void *p, size_t n) {
+            const unsigned char *c = static_cast<const unsigned char *>(p);
+            for (size_t i = 0; i < n; ++i) hsh = (hsh ^ c[i]) * 1099511628211ULL;
+        };
+        mix(g->perm.data(), g->perm.size() * sizeof(uint32_t));
+        mix(g->flip.data(), g->flip.size());
+        mix(g->chars.data(), g->chars.size() * sizeof(double));
+        g->hash = hsh;
         detect_torus(*g);
         build_lut(*g);
         *out = g.release();

@@ -299,6 +299,7 @@ struct qmb_group : qmb::Counted {
     std::vector<uint64_t> keep;
     std::vector<uint8_t> small, big;
     std::vector<uint8_t> irrep_ok;  // [n_irreps]: the row is a one-dimensional representation
+    uint64_t hash = 0;              // FNV-1a of the element tables, flips and characters
     int lut = 0, lut_flip = 0;     // 1: full torus product with block width <= 8; 2: full ring
     std::vector<uint8_t> lut_tab;  // [kLutBytes]
 };

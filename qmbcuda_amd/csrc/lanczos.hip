@@ -22,6 +22,7 @@
 #include <memory>
 #include <mutex>
 
+#include "checkpoint.hpp"
 #include "internal.hpp"
 #include "reduce.hpp"
 #include "sell.hpp"
@@ -344,9 +345,33 @@ struct Timer {
     }
 };
 
+// Lanczos checkpoint: the exact state at a convergence check (steps done,
+// a[], b[], 1/b[], the next unnormalised vector, the vector(s) the recurrence
+// still needs, the orthogonality estimates and the check schedule), so a
+// resumed run continues with the same operations and ends with the same
+// results as an uninterrupted one.
+struct Checkpoint {
+    const char *path = nullptr;
+    int every = 0;       // write at the first check at least this many steps after the last write
+    int resume = 0;      // start from the file instead of the start vector
+    int stop_after = 0;  // stop (and write) at the first check at least this many steps into this call
+};
+
+struct LanczosFileHeader {
+    char magic[8];
+    uint32_t version, header_bytes;
+    int64_t n_rows, nnz;
+    int32_t value_type, reorth, max_iter, n_eigs, start, fixed_iter, check_every, pad0;
+    uint64_t seed;
+    double tol;
+    int32_t it, n_reorth, pro_second, next, it_prev, n_spmv;
+    double q_prev;
+};
+constexpr char kLanczosMagic[8] = {'Q', 'M', 'B', 'L', 'A', 'N', 'C', '1'};
+
 // The Lanczos engine shared by qmb_lanczos and the reference drop-in.
 qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o, double *evals, double *krylov,
-                       double *eigvec, qmb_lanczos_report *rep) {
+                       double *eigvec, qmb_lanczos_report *rep, const Checkpoint *ck = nullptr) {
     Timer wall;
     const bool cplx = e->value_type == QMB_COMPLEX;
     const int64_t n = e->n_rows;
@@ -497,17 +522,107 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     double ms_spmv = 0.0, ms_host = 0.0;
     int n_spmv = 0;
 
-    init_start();
     std::vector<double> ha(M), hb(M + 1);
     std::vector<double> w(k_eigs + 1), zl(k_eigs + 1), z0(M);
     int it = 0;
     int m_eff = 0;
-    bool converged = false, breakdown = false;
+    bool converged = false, breakdown = false, stopped = false;
     // Convergence is tested after `next` steps: check_every at first, then the
     // number of steps the residual's observed geometric decay predicts it needs
     // to reach the tolerance (at least 1, at most check_every).
     int next = check, it_prev = -1;
     double q_prev = 0.0;
+    auto fill_header = [&](LanczosFileHeader &h) {
+        std::memset(&h, 0, sizeof h);
+        std::memcpy(h.magic, kLanczosMagic, 8);
+        h.version = 1;
+        h.header_bytes = sizeof h;
+        h.n_rows = n;
+        h.nnz = e->nnz;
+        h.value_type = e->value_type;
+        h.reorth = rmode;
+        h.max_iter = M;
+        h.n_eigs = k_eigs;
+        h.start = o.start;
+        h.fixed_iter = o.fixed_iter;
+        h.check_every = check;
+        h.seed = o.seed;
+        h.tol = tol;
+    };
+    auto write_checkpoint = [&]() {
+        LanczosFileHeader h;
+        fill_header(h);
+        h.it = it;
+        h.n_reorth = n_reorth;
+        h.pro_second = pro_second ? 1 : 0;
+        h.next = next;
+        h.it_prev = it_prev;
+        h.n_spmv = n_spmv;
+        h.q_prev = q_prev;
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        const std::string tmp = std::string(ck->path) + ".tmp";
+        {
+            File out(tmp.c_str(), "wb");
+            out.write(&h, sizeof h);
+            write_device(out, wk.alpha(), (size_t)it * sizeof(double));
+            write_device(out, wk.beta(), (size_t)(it + 1) * sizeof(double));
+            write_device(out, wk.invb(), (size_t)(it + 1) * sizeof(double));
+            write_device(out, P.p, cnt * sizeof(double));
+            if (reorth)
+                write_device(out, VB.p, (size_t)it * cnt * sizeof(double));
+            else if (it > 0)
+                write_device(out, vcur(it - 1), cnt * sizeof(double));
+            if (rmode == 2) {
+                out.write(om_prev.data(), om_prev.size() * sizeof(double));
+                out.write(om_cur.data(), om_cur.size() * sizeof(double));
+                out.write(pa.data(), pa.size() * sizeof(double));
+                out.write(pb.data(), pb.size() * sizeof(double));
+            }
+            out.close();
+        }
+        // replace the previous checkpoint only once the new one is complete
+        QMB_REQUIRE(std::rename(tmp.c_str(), ck->path) == 0, std::string("cannot rename ") + tmp);
+    };
+    if (ck && ck->resume) {
+        File in(ck->path, "rb");
+        LanczosFileHeader h, want;
+        in.read(&h, sizeof h);
+        fill_header(want);
+        QMB_REQUIRE(std::memcmp(h.magic, kLanczosMagic, 8) == 0 && h.version == 1 && h.header_bytes == sizeof h,
+                    "qmb_lanczos_checkpointed: not a Lanczos checkpoint");
+        QMB_REQUIRE(h.n_rows == want.n_rows && h.nnz == want.nnz && h.value_type == want.value_type &&
+                        h.reorth == want.reorth && h.max_iter == want.max_iter && h.n_eigs == want.n_eigs &&
+                        h.start == want.start && h.fixed_iter == want.fixed_iter &&
+                        h.check_every == want.check_every && h.seed == want.seed && h.tol == want.tol,
+                    "qmb_lanczos_checkpointed: the checkpoint belongs to another matrix or other options");
+        QMB_REQUIRE(h.it >= 1 && h.it < M && h.next >= 1 && h.next <= check,
+                    "qmb_lanczos_checkpointed: corrupt checkpoint");
+        it = h.it;
+        n_reorth = h.n_reorth;
+        pro_second = h.pro_second != 0;
+        next = h.next;
+        it_prev = h.it_prev;
+        n_spmv = h.n_spmv;
+        q_prev = h.q_prev;
+        read_device(in, wk.alpha(), (size_t)it * sizeof(double));
+        read_device(in, wk.beta(), (size_t)(it + 1) * sizeof(double));
+        read_device(in, wk.invb(), (size_t)(it + 1) * sizeof(double));
+        read_device(in, P.p, cnt * sizeof(double));
+        if (reorth)
+            read_device(in, VB.p, (size_t)it * cnt * sizeof(double));
+        else
+            read_device(in, vcur(it - 1), cnt * sizeof(double));
+        if (rmode == 2) {
+            in.read(om_prev.data(), om_prev.size() * sizeof(double));
+            in.read(om_cur.data(), om_cur.size() * sizeof(double));
+            in.read(pa.data(), pa.size() * sizeof(double));
+            in.read(pb.data(), pb.size() * sizeof(double));
+        }
+    } else {
+        init_start();
+    }
+    const int it_start = it;
+    int last_ck = it;
     while (it < M) {
         const int win0 = it;
         const int win1 = std::min(M, it + next);
@@ -574,6 +689,17 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             q_prev = q;
             it_prev = it;
         }
+        if (ck && ck->path && it < M) {
+            const bool stop = ck->stop_after > 0 && it - it_start >= ck->stop_after;
+            if (stop || (ck->every > 0 && it - last_ck >= ck->every)) {
+                write_checkpoint();
+                last_ck = it;
+            }
+            if (stop) {
+                stopped = true;
+                break;
+            }
+        }
     }
     if (o.fixed_iter) converged = true;
     const int m = m_eff;
@@ -583,7 +709,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         for (int j = 0; j < M; ++j) krylov[j] = j < m ? z0[j] : 0.0;
     }
     double eig_res = 0.0;
-    if (o.want_eigvec) {
+    if (o.want_eigvec && !stopped) {
         // rebuild x = sum_j z0[j] phi_j (the reference's second pass, lanczos.cu:31-104)
         DevBuf X(cnt * sizeof(double));
         if (reorth) {
@@ -784,6 +910,33 @@ qmb_status qmb_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts *o
             o.max_iter = (int)std::max<int64_t>(1, e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1) + 1);
         result = run_lanczos(ctx, e, o, evals, krylov, eigvec, report);
         if (result == QMB_ERR_NOT_CONVERGED) set_error("qmb_lanczos: tolerance not met within max_iter");
+    });
+    return st != QMB_OK ? st : result;
+}
+
+qmb_status qmb_lanczos_checkpointed(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts *opts, const char *path,
+                                    int every, int resume, int stop_after, double *evals, double *krylov,
+                                    double *eigvec, qmb_lanczos_report *report) {
+    qmb_status result = QMB_OK;
+    qmb_status st = guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
+        QMB_REQUIRE(ctx && e && opts && evals && path, "qmb_lanczos_checkpointed: NULL argument");
+        QMB_REQUIRE(e->ctx == ctx, "qmb_lanczos_checkpointed: ELL belongs to another context");
+        QMB_REQUIRE(e->n_rows > 0, "qmb_lanczos_checkpointed: empty matrix");
+        QMB_REQUIRE(every >= 0 && stop_after >= 0 && (resume == 0 || resume == 1),
+                    "qmb_lanczos_checkpointed: bad checkpoint arguments");
+        check_opts(*opts);
+        qmb_lanczos_opts o = *opts;
+        if (o.max_iter > e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1) + 1 && !o.fixed_iter)
+            o.max_iter = (int)std::max<int64_t>(1, e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1) + 1);
+        Checkpoint ck;
+        ck.path = path;
+        ck.every = every;
+        ck.resume = resume;
+        ck.stop_after = stop_after;
+        result = run_lanczos(ctx, e, o, evals, krylov, eigvec, report, &ck);
+        if (result == QMB_ERR_NOT_CONVERGED)
+            set_error("qmb_lanczos_checkpointed: not converged (stopped, or max_iter reached)");
     });
     return st != QMB_OK ? st : result;
 }

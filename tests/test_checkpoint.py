@@ -1,0 +1,115 @@
+"""Restartable runs: the on-disk ELL format and the Lanczos checkpoint.
+
+A saved and reloaded matrix is the same matrix (export and SpMV bit for bit);
+files for another basis, truncated files and files whose indexes point outside
+the arrays are refused before any kernel reads them. A Lanczos run stopped and
+resumed from its checkpoint (once or many times, plain or with partial or full
+reorthogonalisation) ends with the same eigenvalues, step count and Krylov
+coefficients as the uninterrupted run.
+"""
+import struct
+import sys
+
+import numpy as np
+import pytest
+
+import qmbcuda_amd as q
+from tests.helpers import chain_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def objects(ctx, case, n_up, alpha, layout=q.LAYOUT_COMPACT, **kw):
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    t = q.Terms(case["n_sites"], case["table"])
+    b = q.Basis(ctx, g, n_up, alpha, layout)
+    return g, t, b, q.Ell(ctx, b, t, **kw)
+
+
+@pytest.mark.parametrize("alpha,layout,fp64", [(0, q.LAYOUT_COMPACT, False), (0, q.LAYOUT_REFERENCE, False),
+                                               (5, q.LAYOUT_COMPACT, False), (0, q.LAYOUT_COMPACT, True)])
+def test_ell_round_trip(ctx, tmp_path, alpha, layout, fp64):
+    case = chain_case(14, 1.0, 1.0, 0.3, flip=True)
+    say(f"ell save/load irrep={alpha} layout={layout} fp64={fp64}")
+    g, t, b, e = objects(ctx, case, 7, alpha, layout, keep_fp64_values=fp64)
+    path = tmp_path / "m.qell"
+    e.save(path)
+    e2 = q.Ell.load(ctx, b, path, t)
+    i1, i2 = e.info(), e2.info()
+    for f in ("n_rows", "nnz", "stored", "max_row_len", "max_terms", "value_type", "packed", "n_values"):
+        assert getattr(i1, f) == getattr(i2, f)
+    for u, v in zip(e.export_reference(), e2.export_reference()):
+        assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
+    x = np.random.default_rng(3).normal(size=b.n_rows)
+    if e.is_complex:
+        x = x + 1j * np.random.default_rng(4).normal(size=b.n_rows)
+    assert np.array_equal(e.spmv(x), e2.spmv(x))
+
+
+def test_ell_load_refuses_bad_files(ctx, tmp_path):
+    case = chain_case(12, 1.0, 1.0, 0.3, flip=True)
+    g, t, b, e = objects(ctx, case, 6, 0)
+    path = tmp_path / "m.qell"
+    e.save(path)
+    raw = path.read_bytes()
+    say("ell load: another basis")
+    b_other = q.Basis(ctx, g, 6, 3, q.LAYOUT_COMPACT)
+    with pytest.raises(q.QMBError):
+        q.Ell.load(ctx, b_other, path)
+    say("ell load: truncated")
+    (tmp_path / "short.qell").write_bytes(raw[: len(raw) // 2])
+    with pytest.raises(q.QMBError):
+        q.Ell.load(ctx, b, tmp_path / "short.qell")
+    say("ell load: column out of range")
+    # header (see checkpoint.hip): 8 + 4 + 4 + 6*4 + 4*8 + 4*4 + 8 bytes
+    hdr = 8 + 4 + 4 + 24 + 32 + 16 + 8
+    n_rows, n_chunks, nnz, stored = struct.unpack_from("<qqqq", raw, 8 + 4 + 4 + 24)
+    off_col = hdr + (n_chunks + 1) * 8 + n_rows * 2
+    bad = bytearray(raw)
+    # slot 1 of row 0 (chunk 0 starts at offset 0): an index far beyond n_rows
+    struct.pack_into("<I", bad, off_col + 64 * 4, (1 << 20) - 1)
+    (tmp_path / "bad.qell").write_bytes(bytes(bad))
+    with pytest.raises(q.QMBError):
+        q.Ell.load(ctx, b, tmp_path / "bad.qell")
+    # the good file still loads
+    q.Ell.load(ctx, b, path)
+
+
+@pytest.mark.parametrize("reorth,n_eigs", [(0, 1), (2, 1), (1, 3)])
+def test_lanczos_resume_equals_uninterrupted(ctx, tmp_path, reorth, n_eigs):
+    case = chain_case(20, 1.0, 1.0, 0.45, flip=True)
+    g, t, b, e = objects(ctx, case, 10, 0)
+    say(f"lanczos checkpoint reorth={reorth} n_eigs={n_eigs}")
+    kw = dict(n_eigs=n_eigs, max_iter=300, tol=1e-13, reorth=reorth, check_every=5)
+    ev0, rep0, kr0, _, ok0 = q.lanczos_checkpointed(e, tmp_path / "unused.qlz", **kw)
+    assert ok0
+    path = tmp_path / "run.qlz"
+    ev, rep, kr, _, ok = q.lanczos_checkpointed(e, path, stop_after=7, **kw)
+    assert not ok and rep.iters < rep0.iters and path.exists()
+    stops = 1
+    while not ok:
+        ev, rep, kr, _, ok = q.lanczos_checkpointed(e, path, resume=True, stop_after=9, **kw)
+        stops += 1
+        assert stops < 100
+    assert stops >= 3
+    assert np.array_equal(ev, ev0)
+    assert rep.iters == rep0.iters and rep.n_reorth == rep0.n_reorth
+    assert np.array_equal(kr, kr0)
+
+
+def test_lanczos_resume_refuses_other_options(ctx, tmp_path):
+    case = chain_case(16, 1.0, 1.0, 0.3, flip=True)
+    g, t, b, e = objects(ctx, case, 8, 0)
+    path = tmp_path / "run.qlz"
+    say("lanczos checkpoint: mismatched options")
+    q.lanczos_checkpointed(e, path, stop_after=5, max_iter=200, check_every=5)
+    with pytest.raises(q.QMBError):
+        q.lanczos_checkpointed(e, path, resume=True, max_iter=200, check_every=5, seed=7)
+    _, _, b2, e2 = objects(ctx, case, 8, 3)
+    with pytest.raises(q.QMBError):
+        q.lanczos_checkpointed(e2, path, resume=True, max_iter=200, check_every=5)
