@@ -45,16 +45,27 @@ __device__ inline bool set_insert(unsigned long long *set, int slots, unsigned l
     return false;
 }
 
-// One row of the count (FILL = false) or fill pass.
 template <class S, bool FILL, bool CPLX>
-__device__ inline void ell_row(const GroupView &gv, int layout, int64_t n_rows, const uint64_t *reps,
-                               const uint8_t *stab, const uint8_t *nullf, const uint64_t *bitmap,
-                               const uint32_t *prefix, const TermsView &tv, int W, uint16_t *row_len, int *err,
-                               int all_rows, const int64_t *chunk_ptr, int32_t *col, double *val,
-                               const uint8_t *perm_lds, const uint64_t *bin, const double2 *s_chi,
-                               const double *s_nrm, const GroupLds &gt, int64_t r, unsigned long long *vals,
-                               int *vfull) {
-    unsigned long long last = kEmptyKey;
+__global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
+                                                        int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
+                                                        const uint8_t *nullf, const uint64_t *bitmap,
+                                                        const uint32_t *prefix, TermsView tv, const double2 *chi,
+                                                        const double *nrm, int W, uint16_t *row_len, int *err,
+                                                        int all_rows, const int64_t *chunk_ptr, int32_t *col,
+                                                        double *val) {
+    __shared__ uint8_t perm_lds[kMaxGroup * 64];
+    __shared__ uint64_t bin[64 * kBinJ];
+    __shared__ double2 s_chi[kMaxGroup];
+    __shared__ double s_nrm[kMaxGroup + 1];
+    __shared__ GroupLds gt;
+    stage_perm(perm_lds, gv);
+    stage_group(gt, gv);
+    stage_binom(bin, binom);
+    for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
+    for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
     const int64_t c = r / kChunk;
     const int lane = (int)(r % kChunk);
     int64_t base = 0;
@@ -119,15 +130,7 @@ __device__ inline void ell_row(const GroupView &gv, int layout, int64_t n_rows, 
                             val[2 * pos] = pr * ratio;
                             val[2 * pos + 1] = pi * ratio;
                         } else {
-                            const double v = pr * ratio;
-                            val[pos] = v;
-                            if (vals) {
-                                const unsigned long long key = (unsigned long long)__double_as_longlong(v);
-                                if (key != last) {
-                                    last = key;
-                                    if (!set_insert(vals, kLdsSet, key)) *vfull = 1;
-                                }
-                            }
+                            val[pos] = pr * ratio;
                         }
                     }
                 }
@@ -164,50 +167,6 @@ __device__ inline void ell_row(const GroupView &gv, int layout, int64_t n_rows, 
     }
 }
 
-template <class S, bool FILL, bool CPLX>
-__global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
-                                                        int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
-                                                        const uint8_t *nullf, const uint64_t *bitmap,
-                                                        const uint32_t *prefix, TermsView tv, const double2 *chi,
-                                                        const double *nrm, int W, uint16_t *row_len, int *err,
-                                                        int all_rows, const int64_t *chunk_ptr, int32_t *col,
-                                                        double *val, unsigned long long *vset, int *vflag) {
-    __shared__ uint8_t perm_lds[kMaxGroup * 64];
-    // distinct off-diagonal values of a real fill (vset != NULL): per-block set,
-    // merged into vset at the end of the block
-    constexpr bool kCollect = FILL && !CPLX;
-    __shared__ unsigned long long vals[kCollect ? kLdsSet : 1];
-    __shared__ int vfull;
-    if (kCollect) {
-        for (int i = threadIdx.x; i < kLdsSet; i += blockDim.x) vals[i] = kEmptyKey;
-        if (threadIdx.x == 0) vfull = 0;
-    }
-    __shared__ uint64_t bin[64 * kBinJ];
-    __shared__ double2 s_chi[kMaxGroup];
-    __shared__ double s_nrm[kMaxGroup + 1];
-    __shared__ GroupLds gt;
-    stage_perm(perm_lds, gv);
-    stage_group(gt, gv);
-    stage_binom(bin, binom);
-    for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
-    for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
-    __syncthreads();
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (kCollect && vset) {
-        if (r < n_rows)
-            ell_row<S, FILL, CPLX>(gv, layout, n_rows, reps, stab, nullf, bitmap, prefix, tv, W, row_len, err,
-                                   all_rows, chunk_ptr, col, val, perm_lds, bin, s_chi, s_nrm, gt, r, vals, &vfull);
-        __syncthreads();
-        for (int i = threadIdx.x; i < kLdsSet; i += blockDim.x)
-            if (vals[i] != kEmptyKey && !set_insert(vset, kSetSlots, vals[i])) vflag[0] = 1;
-        if (threadIdx.x == 0 && vfull) vflag[0] = 1;
-    } else if (r < n_rows) {
-        ell_row<S, FILL, CPLX>(gv, layout, n_rows, reps, stab, nullf, bitmap, prefix, tv, W, row_len, err, all_rows,
-                               chunk_ptr, col, val, perm_lds, bin, s_chi, s_nrm, gt, r, nullptr, nullptr);
-    }
-}
-
-
 // Padding entries of the rows past n_rows in the last chunk.
 __global__ void ell_pad_tail(int64_t n_rows, int64_t n_chunks, const int64_t *chunk_ptr, int32_t *col, double *val,
                              int cplx) {
@@ -237,6 +196,39 @@ __global__ void ell_pad_tail(int64_t n_rows, int64_t n_chunks, const int64_t *ch
 // moves to an FP64 array. The values and the order of the products are
 // unchanged, so SpMV results are bit-identical; the matrix stream shrinks from
 // 12 to 4 bytes per entry.
+// Distinct off-diagonal values (slots 1 .. len-1 of every row): a per-block set
+// in LDS, merged into the global set once per block; vflag[0] = 1 when a set
+// fills. A separate streaming pass: inside the fill kernel, whose time goes to
+// orbit lookups, the set probes cost more than re-reading the values.
+__global__ __launch_bounds__(kEllThreads) void ell_collect_values(int64_t n_rows, const int64_t *chunk_ptr,
+                                                                  const uint16_t *row_len, const double *val,
+                                                                  unsigned long long *vset, int *vflag) {
+    __shared__ unsigned long long set[kLdsSet];
+    __shared__ int full;
+    for (int i = threadIdx.x; i < kLdsSet; i += blockDim.x) set[i] = kEmptyKey;
+    if (threadIdx.x == 0) full = 0;
+    __syncthreads();
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = r / kChunk;
+        const int lane = (int)(r % kChunk);
+        const int64_t base = chunk_ptr[c];
+        const int len = row_len[r];
+        unsigned long long last = kEmptyKey;
+        for (int k = 1; k < len; ++k) {
+            const unsigned long long key =
+                (unsigned long long)__double_as_longlong(__builtin_nontemporal_load(val + base + (int64_t)k * kChunk + lane));
+            if (key == last) continue;
+            last = key;
+            if (!set_insert(set, kLdsSet, key)) full = 1;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kLdsSet; i += blockDim.x)
+        if (set[i] != kEmptyKey && !set_insert(vset, kSetSlots, set[i])) vflag[0] = 1;
+    if (threadIdx.x == 0 && full) vflag[0] = 1;
+}
+
 // One block: the distinct values of the set in ascending bit order, +0.0 first
 // (the padding value), into keys[0 .. n); meta = {n, ok}, ok = 0 when the set
 // overflowed or n exceeds cap.
@@ -341,6 +333,11 @@ struct ValuePacking {
         e.vtab.alloc((size_t)kMaxValues * 8);
         e.diag.alloc((size_t)e.n_rows * sizeof(double));
         int *f = flags.as<int>();
+        const unsigned cblocks = (unsigned)std::min<int64_t>(4096, (e.n_rows + kEllThreads - 1) / kEllThreads);
+        hipLaunchKernelGGL(ell_collect_values, dim3(cblocks), dim3(kEllThreads), 0, ctx->stream, e.n_rows,
+                           e.chunk_ptr.as<int64_t>(), e.row_len.as<uint16_t>(), e.val.as<double>(),
+                           vset.as<unsigned long long>(), f);
+        QMB_HIP(hipGetLastError());
         hipLaunchKernelGGL(ell_sort_values, dim3(1), dim3(1024), 0, ctx->stream, vset.as<unsigned long long>(), f,
                            e.vtab.as<unsigned long long>(), f + 1, cap);
         QMB_HIP(hipGetLastError());
@@ -445,7 +442,7 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
                                b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), b->all_rows ? 1 : 0, nullptr, nullptr,
-                               nullptr, nullptr, nullptr);
+                               nullptr);
             QMB_HIP(hipGetLastError());
         }
         int herr = 0;
@@ -470,9 +467,7 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
                                b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), 0, e->chunk_ptr.as<int64_t>(),
-                               e->col.as<int32_t>(), e->val.as<double>(),
-                               vp.on ? vp.vset.as<unsigned long long>() : nullptr,
-                               vp.on ? vp.flags.as<int>() : nullptr);
+                               e->col.as<int32_t>(), e->val.as<double>());
             QMB_HIP(hipGetLastError());
             if (n % kChunk) {
                 hipLaunchKernelGGL(ell_pad_tail, dim3(1), dim3(kChunk), 0, ctx->stream, n, e->n_chunks,
