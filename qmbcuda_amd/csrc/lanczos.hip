@@ -10,9 +10,11 @@
 //   K1 spmv_sell<LZ>: y = (1/b) (H w) (the scale applied once per row after
 //      the gathers), phi_n = w/b written out, the block partials of a[n] in
 //      the same pass;
-//   K2 lz_update: w = (-a) phi_n + (y - b phi_{n-1}) and the partials of ||w||^2;
-// each followed by a one-block finish_sum that leaves a[n], b[n+1], 1/b[n+1]
-// on the device. The host reads a[], b[] every check_every steps and solves the
+//   K2 lz_update: w = (-a) phi_n + (y - b phi_{n-1}) and the partials of ||w||^2.
+// In the plain recurrence every block of K2 sums K1's partials into a[n] and
+// every block of the next K1 sums K2's into b[n+1] (same fixed order as a
+// separate one-block finish_sum, which runs only before a convergence check and
+// with reorthogonalisation). The host reads a[], b[] every check_every steps and solves the
 // tridiagonal problem in FP64 (lanczos_host.cpp). Reductions are deterministic:
 // fixed block partial slots summed in a fixed order.
 #include <algorithm>
@@ -46,16 +48,39 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                                                       const double *__restrict__ vtab, int n_values, int cb,
                                                       const double *__restrict__ x, double *__restrict__ y,
                                                       const double *__restrict__ invb, double *__restrict__ vout,
-                                                      double *__restrict__ partials) {
+                                                      double *__restrict__ partials, const double *__restrict__ bpart,
+                                                      int n_bpart, double *__restrict__ beta_out,
+                                                      double *__restrict__ invb_out) {
     __shared__ double lds[kWaves];
     __shared__ double s_vt[PACK ? kMaxValues : 1];
+    __shared__ double s_scale;
     if (PACK) {
         for (int i = threadIdx.x; i < n_values; i += blockDim.x) s_vt[i] = vtab[i];
         __syncthreads();
     }
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const double s = LZ ? *invb : 1.0;
+    double s = 1.0;
+    if (LZ) {
+        if (bpart) {
+            // the previous step's ||w||^2 partials, summed by every block in the
+            // fixed order of finish_sum: b = sqrt(sum), s = 1/b (0 if b = 0)
+            const double t = sum_partials(bpart, n_bpart, lds);
+            if (threadIdx.x == 0) {
+                const double nb = sqrt(t);
+                const double inv = nb > 0.0 ? 1.0 / nb : 0.0;
+                s_scale = inv;
+                if (blockIdx.x == 0) {
+                    *beta_out = nb;
+                    *invb_out = inv;
+                }
+            }
+            __syncthreads();
+            s = s_scale;
+        } else {
+            s = *invb;
+        }
+    }
     double dot = 0.0;
     for (int64_t c = (int64_t)blockIdx.x * kWaves + wv; c < n_chunks; c += (int64_t)gridDim.x * kWaves) {
         const int64_t r = c * 64 + lane;
@@ -144,11 +169,26 @@ __global__ __launch_bounds__(kThreads) void finish_sum(const double *__restrict_
 __global__ __launch_bounds__(kThreads) void lz_update(int64_t cnt, const double *__restrict__ y,
                                                       const double *__restrict__ v,
                                                       const double *__restrict__ vprev, double *__restrict__ w,
-                                                      const double *__restrict__ alpha_p,
+                                                      double *__restrict__ alpha_p,
                                                       const double *__restrict__ beta_p,
-                                                      double *__restrict__ partials) {
+                                                      double *__restrict__ partials,
+                                                      const double *__restrict__ apart, int n_apart) {
     __shared__ double lds[kWaves];
-    const double a = *alpha_p, b = *beta_p;
+    __shared__ double s_alpha;
+    double a;
+    if (apart) {
+        // a[n] from the SpMV's partials, summed by every block in finish_sum's order
+        const double t = sum_partials(apart, n_apart, lds);
+        if (threadIdx.x == 0) {
+            s_alpha = t;
+            if (blockIdx.x == 0) *alpha_p = t;
+        }
+        __syncthreads();
+        a = s_alpha;
+    } else {
+        a = *alpha_p;
+    }
+    const double b = *beta_p;
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
         const double t = y[i] + (-b) * vprev[i];
@@ -307,8 +347,19 @@ static void launch_finish(qmb_ctx *ctx, const double *partials, unsigned n, doub
 
 // after (optional): an event recorded between the SpMV and the a[n] finish, so
 // that profiling brackets the SpMV kernel alone.
+// fused (optional): the previous step's ||w||^2 partials (n_bpart of them) from
+// which every block derives 1/b itself, writing b and 1/b to beta_out/invb_out;
+// the a[n] finish is then left to lz_update (alpha_out unused).
+struct FusedBeta {
+    const double *bpart = nullptr;
+    int n_bpart = 0;
+    double *beta_out = nullptr, *invb_out = nullptr;
+    double *partials = nullptr;  // where this SpMV's a[n] partials go
+};
+
 void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, bool lz, const double *invb,
-                 double *vout, LzWork *wk, double *alpha_out, hipEvent_t after = nullptr) {
+                 double *vout, LzWork *wk, double *alpha_out, hipEvent_t after = nullptr,
+                 const FusedBeta *fb = nullptr) {
     const unsigned blocks = grid_for(e->n_chunks, kWaves);
     const bool cplx = e->value_type == QMB_COMPLEX;
     auto k = cplx       ? (lz ? spmv_sell<true, true, false> : spmv_sell<true, false, false>)
@@ -317,11 +368,15 @@ void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, boo
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, ctx->stream, e->n_rows, e->n_chunks,
                        e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
                        e->val.as<double>(), e->diag.as<double>(), e->vtab.as<double>(), e->n_values, e->col_bits,
-                       x, y, invb, vout, wk ? wk->partials.as<double>() : nullptr);
+                       x, y, invb, vout, fb ? fb->partials : wk ? wk->partials.as<double>() : nullptr,
+                       fb ? fb->bpart : nullptr, fb ? fb->n_bpart : 0, fb ? fb->beta_out : nullptr,
+                       fb ? fb->invb_out : nullptr);
     QMB_HIP(hipGetLastError());
     if (after) QMB_HIP(hipEventRecord(after, ctx->stream));
-    if (lz) launch_finish(ctx, wk->partials.as<double>(), blocks, alpha_out, nullptr, 0);
+    if (lz && !fb) launch_finish(ctx, wk->partials.as<double>(), blocks, alpha_out, nullptr, 0);
 }
+
+unsigned spmv_blocks(const qmb_ell *e) { return grid_for(e->n_chunks, kWaves); }
 
 static void launch_norm(qmb_ctx *ctx, int64_t cnt, const double *a, const double *b, double e, LzWork &wk,
                         double *out_norm, double *out_inv) {
@@ -426,7 +481,30 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         launch_norm(ctx, cnt, P.as<double>(), nullptr, 0.0, wk, wk.misc(), wk.invb());
     };
     const unsigned ublocks = grid_for(cnt, kThreads * 4);
+    // Plain recurrence: the a[n] and b[n+1] one-block finishes are folded into
+    // the next kernel (every block re-sums the fixed partial slots in the same
+    // order, so the values are the same bit for bit). b_ready[n]: b[n], 1/b[n]
+    // were written by an explicit finish (start vector, last step before a
+    // convergence check); otherwise K1(n) derives them from K2(n-1)'s partials.
+    const bool fuse = rmode == 0;
+    double *part1 = wk.partials.as<double>(), *part2 = wk.partials.as<double>() + kMaxBlocks;
+    const unsigned sblocks = spmv_blocks(e);
+    std::vector<char> b_ready(M + 1, 0);
+    b_ready[0] = 1;
     auto k1 = [&](int it, hipEvent_t after) {
+        if (fuse) {
+            FusedBeta fb;
+            fb.partials = part1;
+            if (!b_ready[it]) {
+                fb.bpart = part2;
+                fb.n_bpart = (int)ublocks;
+                fb.beta_out = wk.beta() + it;
+                fb.invb_out = wk.invb() + it;
+            }
+            launch_spmv(ctx, e, P.as<double>(), Y.as<double>(), true, wk.invb() + it, vcur(it), &wk,
+                        wk.alpha() + it, after, &fb);
+            return;
+        }
         launch_spmv(ctx, e, P.as<double>(), Y.as<double>(), true, wk.invb() + it, vcur(it), &wk, wk.alpha() + it,
                     after);
     };
@@ -476,9 +554,22 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         std::swap(om_cur, om_next);
         return run;
     };
-    auto k2 = [&](int it) {
+    // finish: write b[it+1], 1/b[it+1] now (needed by a convergence check)
+    auto k2 = [&](int it, bool finish) {
+        if (fuse) {
+            hipLaunchKernelGGL(lz_update, dim3(ublocks), dim3(kThreads), 0, ctx->stream, cnt, Y.as<double>(),
+                               vcur(it), vprev(it), P.as<double>(), wk.alpha() + it, wk.beta() + it, part2, part1,
+                               (int)sblocks);
+            QMB_HIP(hipGetLastError());
+            if (finish) {
+                launch_finish(ctx, part2, ublocks, wk.beta() + it + 1, wk.invb() + it + 1, 1);
+                b_ready[it + 1] = 1;
+            }
+            return;
+        }
         hipLaunchKernelGGL(lz_update, dim3(ublocks), dim3(kThreads), 0, ctx->stream, cnt, Y.as<double>(), vcur(it),
-                           vprev(it), P.as<double>(), wk.alpha() + it, wk.beta() + it, wk.partials.as<double>());
+                           vprev(it), P.as<double>(), wk.alpha() + it, wk.beta() + it, wk.partials.as<double>(),
+                           nullptr, 0);
         QMB_HIP(hipGetLastError());
         launch_finish(ctx, wk.partials.as<double>(), ublocks, wk.beta() + it + 1, wk.invb() + it + 1, 1);
         const bool run = rmode == 1 || (rmode == 2 && pro_decide(it));
@@ -607,6 +698,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         read_device(in, wk.alpha(), (size_t)it * sizeof(double));
         read_device(in, wk.beta(), (size_t)(it + 1) * sizeof(double));
         read_device(in, wk.invb(), (size_t)(it + 1) * sizeof(double));
+        b_ready[it] = 1;  // b[it], 1/b[it] come from the file
         read_device(in, P.p, cnt * sizeof(double));
         if (reorth)
             read_device(in, VB.p, (size_t)it * cnt * sizeof(double));
@@ -629,7 +721,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         for (; it < win1; ++it) {
             if (o.profile) QMB_HIP(hipEventRecord(ev[2 * (it - win0)], ctx->stream));
             k1(it, o.profile ? ev[2 * (it - win0) + 1] : nullptr);
-            k2(it);
+            k2(it, it + 1 == win1);
             ++n_spmv;
         }
         QMB_HIP(hipMemcpyAsync(ha.data(), wk.alpha(), it * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -722,12 +814,14 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             QMB_HIP(hipMemsetAsync(X.p, 0, cnt * sizeof(double), ctx->stream));
             QMB_HIP(hipMemsetAsync(Z.p, 0, cnt * sizeof(double), ctx->stream));
             init_start();
+            std::fill(b_ready.begin(), b_ready.end(), 0);
+            b_ready[0] = 1;
             for (int j = 0; j < m; ++j) {
                 k1(j, nullptr);
                 hipLaunchKernelGGL(vec_axpy, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream, cnt,
                                    z0[j], vcur(j), X.as<double>());
                 QMB_HIP(hipGetLastError());
-                k2(j);
+                k2(j, false);
             }
         }
         launch_spmv(ctx, e, X.as<double>(), Y.as<double>(), false, nullptr, nullptr, nullptr, nullptr);

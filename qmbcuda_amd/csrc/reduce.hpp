@@ -30,10 +30,21 @@ __device__ inline double block_sum(double v, double *lds) {
     return t;
 }
 
-// Deterministic sum of partials[0..n) by one block (fixed order).
+// Deterministic sum of partials[0..n) by one block (fixed order): thread t adds
+// partials t, t + blockDim, t + 2 blockDim, ... in that order; eight loads are
+// issued before they are added, so the latency of the L2 reads overlaps.
 __device__ inline double sum_partials(const double *partials, int n, double *lds) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += __builtin_nontemporal_load(partials + i);
+    const int stride = blockDim.x;
+    int i = threadIdx.x;
+    for (; i + 7 * stride < n; i += 8 * stride) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(partials + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; i < n; i += stride) s += __builtin_nontemporal_load(partials + i);
     return block_sum(s, lds);
 }
 
