@@ -65,6 +65,59 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
     const int cnt = (int)((dim - r0) < 64 ? (dim - r0) : 64);
     S x = colex_unrank<S>(r0, n_up, gv.n_sites, bin);
     uint64_t bits = 0;
+    if (gv.lut == 2) {
+        // Ring group: the images are the rotations by s = 1 .. N-1 (each one bit
+        // more than the last, computed in registers) and, with the spin flip,
+        // their complements (s = 0: the flip alone). Element indices are looked
+        // up only for images equal to x (stabiliser).
+        const int N = gv.n_sites;
+        const S full = (S)gv.full;
+        const bool hf = gv.lut_flip != 0;
+        int i = 0, st = 0;
+        S y = x;
+        uint64_t slo = 1, shi = 0;
+        auto mark = [&](int f, int sh) {
+            const int g = gt.lut[512 + f * 64 + sh];
+            if (g < 64)
+                slo |= 1ULL << g;
+            else
+                shi |= 1ULL << (g - 64);
+        };
+        while (i < cnt) {
+            bool done = false, rep = false;
+            if (st >= N) {
+                done = true;
+                rep = true;
+            } else {
+                if (st > 0) {
+                    y = (S)(((y << 1) | (y >> (N - 1))) & full);
+                    if (y < x)
+                        done = true;
+                    else if (y == x)
+                        mark(0, st);
+                }
+                if (!done && hf) {
+                    const S yf = (S)(y ^ full);
+                    if (yf < x)
+                        done = true;
+                    else if (yf == x)
+                        mark(1, st);
+                }
+                ++st;
+            }
+            if (done) {
+                if (rep && (!compact || compatible(slo, shi, triv_lo, triv_hi))) bits |= 1ULL << i;
+                ++i;
+                if (i < cnt) x = next_comb<S>(x);
+                y = x;
+                st = 0;
+                slo = 1;
+                shi = 0;
+            }
+        }
+        bitmap[w] = bits;
+        return;
+    }
     // One flat loop over (state, element) steps per lane: a lane moves to its
     // next state as soon as an image is smaller (most states exit after a few
     // images), so lanes never wait for a neighbour's full orbit scan.
