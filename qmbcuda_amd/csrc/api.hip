@@ -291,6 +291,68 @@ static bool build_ring_lut(qmb_group &g) {
     return true;
 }
 
+// A group that is not a ring itself but holds every rotation of the ring (with
+// or without the spin flip) as a subgroup H, e.g. translations x spin flip x a
+// reflection: G = H c_0 + H c_1 + ... (c_0 = identity, at most kMaxCosets
+// cosets). The orbit minimum is then the smallest of the ring minima of the
+// images c_i x (lut = 3); cos_tab names the element (rotation s, flip f) o c_i.
+static void build_coset_ring(qmb_group &g) {
+    const int N = g.n_sites, G = g.G;
+    if (g.torus || N < 8 || N > 56) return;
+    std::vector<int> gid(128, -1), in_h(G, 0);
+    for (int e = 0; e < G; ++e) {
+        const uint32_t *p = &g.perm[(size_t)e * N];
+        const int sh = (int)p[0];
+        bool rot = true;
+        for (int q = 0; q < N && rot; ++q) rot = (int)p[q] == (q + sh) % N;
+        if (!rot) continue;
+        const int i = (g.flip[e] ? 64 : 0) + sh;
+        if (gid[i] >= 0) return;  // repeated element
+        gid[i] = e;
+        in_h[e] = 1;
+    }
+    int hf = 0;
+    for (int e = 0; e < G; ++e) hf |= (in_h[e] && g.flip[e]) ? 1 : 0;
+    int nh = 0;
+    for (int f = 0; f <= hf; ++f)
+        for (int sh = 0; sh < N; ++sh) {
+            if (gid[f * 64 + sh] < 0) return;
+            ++nh;
+        }
+    if (nh == G || G % nh || G / nh > kMaxCosets || gid[0] != 0) return;
+    // cosets H c: c = the first element not yet covered
+    std::vector<int> covered(G, 0), reps;
+    for (int e = 0; e < G; ++e) {
+        if (covered[e]) continue;
+        reps.push_back(e);
+        for (int h = 0; h < G; ++h)
+            if (in_h[h]) covered[g.comp[(size_t)h * G + e]] = 1;
+    }
+    if ((int)reps.size() * nh != G) return;
+    // a coset holding the site reversal s -> N-1-s (no flip) is represented by
+    // it: its image is one bit reversal on the device
+    g.cos_rev = 0;
+    for (size_t c = 1; c < reps.size(); ++c)
+        for (int h = 0; h < G; ++h) {
+            if (!in_h[h]) continue;
+            const int e = g.comp[(size_t)h * G + reps[c]];
+            bool rev = !g.flip[e];
+            for (int q = 0; q < N && rev; ++q) rev = (int)g.perm[(size_t)e * N + q] == N - 1 - q;
+            if (rev) {
+                reps[c] = e;
+                g.cos_rev |= 1 << c;
+                break;
+            }
+        }
+    g.cos_tab.assign(kCosBytes, 0);
+    for (size_t c = 0; c < reps.size(); ++c)
+        for (int i = 0; i < 128; ++i)
+            if (gid[i] >= 0) g.cos_tab[c * 128 + i] = g.comp[(size_t)gid[i] * G + reps[c]];
+    g.n_cos = (int)reps.size();
+    g.lut = 3;
+    g.lut_flip = hf;
+}
+
 static void build_lut(qmb_group &g) {
     g.lut = 0;
     g.lut_flip = 0;
@@ -415,6 +477,7 @@ static void check_closure_and_irreps(qmb_group &g) {
         for (int a = 0; a < g.n_irreps; ++a)
             if (std::abs(chi(a, e) - chi(a, first[e])) > tol) g.irrep_ok[a] = 0;
     std::vector<uint32_t> pr(n);
+    g.comp.assign((size_t)G * G, 0);
     for (int x = 0; x < G; ++x)
         for (int y = 0; y < G; ++y) {
             const uint32_t *px = &g.perm[(size_t)x * n], *py = &g.perm[(size_t)y * n];
@@ -422,6 +485,7 @@ static void check_closure_and_irreps(qmb_group &g) {
             auto it = index.find(key(pr.data(), g.flip[x] ^ g.flip[y]));
             QMB_REQUIRE(it != index.end(), "qmb_group_create: element list is not closed under composition");
             const int z = it->second;
+            g.comp[(size_t)x * G + y] = (uint8_t)z;
             for (int a = 0; a < g.n_irreps; ++a)
                 if (g.irrep_ok[a] && std::abs(chi(a, z) - chi(a, x) * chi(a, y)) > tol) g.irrep_ok[a] = 0;
         }
@@ -480,6 +544,7 @@ qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el, 
         g->hash = hsh;
         detect_torus(*g);
         build_lut(*g);
+        if (!g->lut) build_coset_ring(*g);
         *out = g.release();
     });
 }

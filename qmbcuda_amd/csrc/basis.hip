@@ -35,7 +35,10 @@ qmb::GroupView qmb_basis::view() const {
     v.inv = g_inv.as<uint8_t>();
     v.lut = group->lut;
     v.lut_flip = group->lut_flip;
-    v.lut_tab = group->lut ? g_lut.as<uint8_t>() : nullptr;
+    v.lut_tab = (group->lut == 1 || group->lut == 2) ? g_lut.as<uint8_t>() : nullptr;
+    v.n_cos = group->lut == 3 ? group->n_cos : 0;
+    v.cos_rev = group->lut == 3 ? group->cos_rev : 0;
+    v.cos_tab = group->lut == 3 ? g_cos.as<uint8_t>() : nullptr;
     return v;
 }
 
@@ -112,6 +115,61 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
                 y = x;
                 st = 0;
                 slo = 1;
+                shi = 0;
+            }
+        }
+        bitmap[w] = bits;
+        return;
+    }
+    if (gv.lut == 3) {
+        // Ring rotations times cosets: per coset c_i one permutation image
+        // y = c_i x, then its rotations one bit at a time, as above.
+        const int N = gv.n_sites;
+        const S full = (S)gv.full;
+        const bool hf = gv.lut_flip != 0;
+        int i = 0, ci = 0, st = 0;
+        S y = x;
+        uint64_t slo = 0, shi = 0;
+        auto mark = [&](int g) {
+            if (g < 64)
+                slo |= 1ULL << g;
+            else
+                shi |= 1ULL << (g - 64);
+        };
+        while (i < cnt) {
+            bool done = false, rep = false;
+            if (ci >= gv.n_cos) {
+                done = true;
+                rep = true;
+            } else {
+                const uint8_t *ct = gt.cos + ci * 128;
+                if (st == 0)
+                    y = ci ? coset_image<S>(x, ci, gv, gt, perm_lds) : x;
+                else
+                    y = rotl_ring<S>(y, 1, N, full);
+                if (y < x)
+                    done = true;
+                else if (y == x)
+                    mark(ct[st]);
+                if (!done && hf) {
+                    const S yf = (S)(y ^ full);
+                    if (yf < x)
+                        done = true;
+                    else if (yf == x)
+                        mark(ct[64 + st]);
+                }
+                if (++st == N) {
+                    st = 0;
+                    ++ci;
+                }
+            }
+            if (done) {
+                if (rep && (!compact || compatible(slo, shi, triv_lo, triv_hi))) bits |= 1ULL << i;
+                ++i;
+                if (i < cnt) x = next_comb<S>(x);
+                ci = 0;
+                st = 0;
+                slo = 0;
                 shi = 0;
             }
         }
@@ -225,9 +283,13 @@ static void upload_group(qmb_basis *b, const qmb_group *g) {
     QMB_HIP(hipMemcpy(b->g_flip.p, g->flip.data(), G, hipMemcpyHostToDevice));
     QMB_HIP(hipMemcpy(b->g_inv.p, inv8.data(), G, hipMemcpyHostToDevice));
     QMB_HIP(hipMemcpy(b->g_perm.p, perm8.data(), perm8.size(), hipMemcpyHostToDevice));
-    if (g->lut) {
+    if (g->lut == 1 || g->lut == 2) {
         b->g_lut.alloc(kLutBytes);
         QMB_HIP(hipMemcpy(b->g_lut.p, g->lut_tab.data(), kLutBytes, hipMemcpyHostToDevice));
+    }
+    if (g->lut == 3) {
+        b->g_cos.alloc(kCosBytes);
+        QMB_HIP(hipMemcpy(b->g_cos.p, g->cos_tab.data(), kCosBytes, hipMemcpyHostToDevice));
     }
 }
 

@@ -76,6 +76,7 @@ struct GroupLds {
     uint8_t flip[kMaxGroup];
     uint8_t inv[kMaxGroup];
     uint8_t lut[kLutBytes];
+    uint8_t cos[kCosBytes];
 };
 
 __device__ inline void stage_group(GroupLds &t, const GroupView &g) {
@@ -86,8 +87,10 @@ __device__ inline void stage_group(GroupLds &t, const GroupView &g) {
         t.flip[i] = g.flip[i];
         t.inv[i] = g.inv[i];
     }
-    if (g.lut)
+    if (g.lut == 1 || g.lut == 2)
         for (int i = threadIdx.x; i < kLutBytes; i += blockDim.x) t.lut[i] = g.lut_tab[i];
+    if (g.lut == 3)
+        for (int i = threadIdx.x; i < kCosBytes; i += blockDim.x) t.cos[i] = g.cos_tab[i];
 }
 
 // apply_elem with the element index varying across lanes (tables from LDS).
@@ -157,11 +160,67 @@ __device__ inline S next_comb(S x) {
     return (S)((((r ^ x) >> 2) >> ctz_s(x)) | r);
 }
 
+// Image of x under coset representative c_i (i >= 1) of a lut-3 group: one bit
+// reversal when c_i is the site reversal, else the generic permutation.
+template <class S>
+__device__ inline S reverse_sites(S x, int n);
+template <>
+__device__ inline uint32_t reverse_sites<uint32_t>(uint32_t x, int n) {
+    return __builtin_bitreverse32(x) >> (32 - n);
+}
+template <>
+__device__ inline uint64_t reverse_sites<uint64_t>(uint64_t x, int n) {
+    return __builtin_bitreverse64(x) >> (64 - n);
+}
+template <class S>
+__device__ inline S coset_image(S x, int ci, const GroupView &gv, const GroupLds &t, const uint8_t *perm_lds) {
+    if ((gv.cos_rev >> ci) & 1) return reverse_sites<S>(x, gv.n_sites);
+    return apply_elem_div<S>(x, t.cos[ci * 128], gv, t, perm_lds);
+}
+
+// Left rotation of an n-site ring state by s (0 <= s < n).
+template <class S>
+__device__ inline S rotl_ring(S y, int s, int n, S full) {
+    return s ? (S)(((y << s) | (y >> (n - s))) & full) : y;
+}
+
+// Stabiliser of x for a group of ring rotations times cosets (gv.lut == 3):
+// the images of (rotation s, flip f) o c_i are the rotations of c_i x.
+template <class S>
+__device__ inline void stabiliser_coset(S x, const GroupView &gv, const uint8_t *perm_lds, uint64_t &stab_lo,
+                                        uint64_t &stab_hi) {
+    const int N = gv.n_sites;
+    const S full = (S)gv.full;
+    stab_lo = 0;
+    stab_hi = 0;
+    for (int ci = 0; ci < gv.n_cos; ++ci) {
+        const uint8_t *ct = gv.cos_tab + ci * 128;
+        S y = !ci                        ? x
+              : ((gv.cos_rev >> ci) & 1) ? reverse_sites<S>(x, N)
+                                         : apply_elem<S>(x, ct[0], gv, perm_lds);
+        for (int s = 0; s < N; ++s) {
+            if (s) y = rotl_ring<S>(y, 1, N, full);
+            for (int f = 0; f < (gv.lut_flip ? 2 : 1); ++f) {
+                if ((f ? (S)(y ^ full) : y) != x) continue;
+                const int g = ct[f * 64 + s];
+                if (g < 64)
+                    stab_lo |= 1ULL << g;
+                else
+                    stab_hi |= 1ULL << (g - 64);
+            }
+        }
+    }
+}
+
 // Representative test. Returns true when x is the smallest integer of its
 // orbit; on success stab_lo/hi hold the stabiliser (elements with image x).
 template <class S>
 __device__ inline bool is_representative(S x, const GroupView &gv, const uint8_t *perm_lds, uint64_t &stab_lo,
                                          uint64_t &stab_hi) {
+    if (gv.lut == 3) {
+        stabiliser_coset<S>(x, gv, perm_lds, stab_lo, stab_hi);
+        return true;  // callers pass representatives only
+    }
     stab_lo = 0;
     stab_hi = 0;
     for (int g = 0; g < gv.G; ++g) {
@@ -286,6 +345,79 @@ __device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, 
             if (bi < 0 || y < best || (y == best && ig < bi)) {
                 best = y;
                 bi = ig;
+            }
+        }
+    }
+    gm = bi;
+    return best;
+}
+
+// Window scan of the ring orbit minimum: the smallest top byte over the
+// rotations of y (and their complements when hf) and the window positions p
+// that reach it (c0: plain, c1: complemented). The top 8 bits of rotation s
+// are the circular window of y starting at site p = (N - 8 - s) mod N.
+template <class S>
+__device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint64_t &c1) {
+    const uint64_t yy = (uint64_t)y | ((uint64_t)y << N);  // 8 <= N <= 56
+    unsigned tmin = 0x1FFu;
+    c0 = 0;
+    c1 = 0;
+    for (int p = 0; p < N; ++p) {
+        const unsigned w = (unsigned)(yy >> p) & 0xFFu;
+        if (w < tmin) {
+            tmin = w;
+            c0 = 0;
+            c1 = 0;
+        }
+        if (w == tmin) c0 |= 1ull << p;
+        if (hf) {
+            const unsigned wf = w ^ 0xFFu;
+            if (wf < tmin) {
+                tmin = wf;
+                c0 = 0;
+                c1 = 0;
+            }
+            if (wf == tmin) c1 |= 1ull << p;
+        }
+    }
+    return tmin;
+}
+
+// orbit_min for ring rotations times cosets (gv.lut == 3): the orbit is the
+// union of the ring orbits of y_i = c_i x. Each coset's window scan gives its
+// smallest top byte; only cosets reaching the smallest so far take full images
+// of their candidate rotations. Same result and same gm as orbit_min.
+template <class S>
+__device__ inline S orbit_min_coset(S x, const GroupView &gv, const GroupLds &t, const uint8_t *perm_lds,
+                                    int &gm) {
+    const int N = gv.n_sites;
+    const S full = (S)gv.full;
+    const bool hf = gv.lut_flip != 0;
+    S best = 0;
+    int bi = -1;
+    unsigned tbest = 0x1FFu;
+    for (int ci = 0; ci < gv.n_cos; ++ci) {
+        const uint8_t *ct = t.cos + ci * 128;
+        const S y = ci ? coset_image<S>(x, ci, gv, t, perm_lds) : x;
+        uint64_t c0, c1;
+        const unsigned tmin = ring_windows<S>(y, N, hf, c0, c1);
+        if (tmin > tbest) continue;
+        tbest = tmin;
+        for (int f = 0; f < (hf ? 2 : 1); ++f) {
+            uint64_t c = f ? c1 : c0;
+            while (c) {
+                const int p = __builtin_ctzll(c);
+                c &= c - 1;
+                int s = N - 8 - p;
+                if (s < 0) s += N;
+                const int g = ct[f * 64 + s];
+                S z = rotl_ring<S>(y, s, N, full);
+                if (f) z ^= full;
+                const int ig = t.inv[g];
+                if (bi < 0 || z < best || (z == best && ig < bi)) {
+                    best = z;
+                    bi = ig;
+                }
             }
         }
     }

@@ -114,6 +114,7 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                 int gm;
                 const S rep = gv.lut == 1   ? orbit_min_lut<S>(tgt, gv, gt, gm)
                               : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm)
+                              : gv.lut == 3 ? orbit_min_coset<S>(tgt, gv, gt, perm_lds, gm)
                                             : orbit_min<S>(tgt, gv, perm_lds, gm);
                 const int64_t r2 = row_of_rep<S>(rep, bin, bitmap, prefix);
                 if (r2 < 0) continue;

@@ -221,11 +221,18 @@ struct GroupView {
     const uint8_t *inv;    // [G]
     int lut;               // 1: the group is every (block shift, in-block shift[, flip]) of a torus;
                            // 2: every rotation[, flip] of a ring
+                           // 3: every rotation[, flip] of a ring times n_cos coset representatives
     int lut_flip;          //    with a spin-flip half
     const uint8_t *lut_tab;  // [kLutBytes] rotation minima and element index table (see device_common.hpp)
+    int n_cos;
+    int cos_rev;             // bit i: c_i is the site reversal s -> N-1-s
+    const uint8_t *cos_tab;  // lut 3: [kCosBytes] element index of (rotation s, flip f) o c_i at
+                             // i * 128 + f * 64 + s (c_i first; entry i * 128 is c_i itself)
 };
 
 constexpr int kLutBytes = 256 + 256 + 2 * 8 * 8;
+constexpr int kMaxCosets = 8;
+constexpr int kCosBytes = kMaxCosets * 128;
 
 // Handles are reference counted: a child (basis, ELL, network) holds its
 // parents (context, group, basis), so the C ABI's destroy calls may come in any
@@ -300,8 +307,12 @@ struct qmb_group : qmb::Counted {
     std::vector<uint8_t> small, big;
     std::vector<uint8_t> irrep_ok;  // [n_irreps]: the row is a one-dimensional representation
     uint64_t hash = 0;              // FNV-1a of the element tables, flips and characters
-    int lut = 0, lut_flip = 0;     // 1: full torus product with block width <= 8; 2: full ring
+    int lut = 0, lut_flip = 0;     // 1: full torus product with block width <= 8; 2: full ring;
+                                   // 3: a full ring subgroup H times coset representatives
     std::vector<uint8_t> lut_tab;  // [kLutBytes]
+    std::vector<uint8_t> comp;     // [G*G]: index of a o b (b applied first)
+    int n_cos = 0, cos_rev = 0;    // lut 3: cosets of the ring subgroup and their element table
+    std::vector<uint8_t> cos_tab;  // [kCosBytes]
 };
 
 struct qmb_terms {
@@ -321,7 +332,7 @@ struct qmb_basis : qmb::Counted {
     qmb::DevBuf stab;    // uint8 [n_rows]: stabiliser size
     qmb::DevBuf nullf;   // uint8 [n_rows]: 1 if the orbit is null for the irrep
     // device copies of the group tables
-    qmb::DevBuf g_keep, g_small, g_big, g_flip, g_perm, g_inv, g_lut;
+    qmb::DevBuf g_keep, g_small, g_big, g_flip, g_perm, g_inv, g_lut, g_cos;
     qmb::GroupView view() const;
     uint64_t triv_lo = 0, triv_hi = 0;  // elements with character 1 for the irrep
     bool all_rows = false;              // every element has character 1: no null orbits
