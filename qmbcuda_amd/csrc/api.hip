@@ -68,17 +68,9 @@ struct PopcountSrc {
     __device__ int64_t operator()(int64_t i) const { return __popcll(w[i]); }
 };
 
-struct ChunkSizeSrc {
-    const uint16_t *row_len;
-    int64_t n_rows;
-    // value = 64 * (max row length in the chunk)
-    __device__ int64_t operator()(int64_t c) const {
-        int m = 0;
-        const int64_t r0 = c * kChunk;
-        const int64_t r1 = r0 + kChunk < n_rows ? r0 + kChunk : n_rows;
-        for (int64_t r = r0; r < r1; ++r) m = m > row_len[r] ? m : (int)row_len[r];
-        return (int64_t)kChunk * m;
-    }
+struct ArraySrc {
+    const int64_t *v;
+    __device__ int64_t operator()(int64_t i) const { return v[i]; }
 };
 
 template <class Src>
@@ -93,8 +85,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_tile_sums(Src src, int64_t 
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
-// Single-block exclusive scan of the tile sums (in place); writes the grand total at [n].
-__global__ __launch_bounds__(kScanThreads) void scan_tile_offsets(int64_t *tile_sums, int64_t n) {
+// Single-block exclusive scan of the tile sums (in place); writes the grand total
+// at [n] and, when total_out is set, there too.
+__global__ __launch_bounds__(kScanThreads) void scan_tile_offsets(int64_t *tile_sums, int64_t n,
+                                                                  int64_t *total_out) {
     __shared__ int64_t lds[kScanThreads];
     int64_t carry = 0;
     for (int64_t base = 0; base < n; base += kScanThreads) {
@@ -105,7 +99,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_tile_offsets(int64_t *tile_
         if (i < n) tile_sums[i] = carry + ex;
         carry += total;
     }
-    if (threadIdx.x == 0) tile_sums[n] = carry;
+    if (threadIdx.x == 0) {
+        tile_sums[n] = carry;
+        if (total_out) *total_out = carry;
+    }
 }
 
 template <class Src, class Out>
@@ -127,35 +124,46 @@ __global__ __launch_bounds__(kScanThreads) void scan_write(Src src, int64_t n, c
     }
 }
 
+// Queues the three scan phases on the context's stream; returns the device
+// address of the grand total (inside sums).
 template <class Src, class Out>
-static int64_t run_scan(qmb_ctx *ctx, Src src, int64_t n, Out *out) {
+static const int64_t *launch_scan(qmb_ctx *ctx, Src src, int64_t n, Out *out, DevBuf &sums, int64_t *total_out) {
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    DevBuf sums((size_t)(tiles + 1) * sizeof(int64_t));
+    sums.alloc((size_t)(tiles + 1) * sizeof(int64_t));
     if (tiles > 0) {
         hipLaunchKernelGGL(scan_tile_sums<Src>, dim3((unsigned)tiles), dim3(kScanThreads), 0, ctx->stream, src, n,
                            sums.as<int64_t>());
         QMB_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(scan_tile_offsets, dim3(1), dim3(kScanThreads), 0, ctx->stream, sums.as<int64_t>(), tiles);
+    hipLaunchKernelGGL(scan_tile_offsets, dim3(1), dim3(kScanThreads), 0, ctx->stream, sums.as<int64_t>(), tiles,
+                       total_out);
     QMB_HIP(hipGetLastError());
     if (tiles > 0) {
         hipLaunchKernelGGL((scan_write<Src, Out>), dim3((unsigned)tiles), dim3(kScanThreads), 0, ctx->stream, src, n,
                            sums.as<int64_t>(), out);
         QMB_HIP(hipGetLastError());
     }
-    int64_t total = 0;
-    QMB_HIP(hipMemcpyAsync(&total, sums.as<int64_t>() + tiles, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    return sums.as<int64_t>() + tiles;
+}
+
+template <class Src, class Out>
+static int64_t run_scan(qmb_ctx *ctx, Src src, int64_t n, Out *out) {
+    DevBuf sums;
+    const int64_t *d_total = launch_scan(ctx, src, n, out, sums, nullptr);
+    int64_t *st = ctx->host_stage<int64_t>(sizeof(int64_t));
+    QMB_HIP(hipMemcpyAsync(st, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     QMB_HIP(hipStreamSynchronize(ctx->stream));
-    return total;
+    return st[0];
 }
 
 int64_t scan_popcount_u32(qmb_ctx *ctx, const uint64_t *words, uint32_t *prefix, int64_t n) {
     return run_scan(ctx, PopcountSrc{words}, n, prefix);
 }
 
-// Row-length statistics per chunk: max and sum, one wavefront per chunk.
-__global__ __launch_bounds__(256) void chunk_stats(const uint16_t *row_len, int64_t n_rows, int64_t n_chunks,
-                                                   int64_t *nnz_partial, int *max_partial) {
+// Chunk widths (64 x the longest row of each 64-row chunk), one wavefront per
+// chunk, and per-block row-length sums and maxima.
+__global__ __launch_bounds__(256) void chunk_widths(const uint16_t *row_len, int64_t n_rows, int64_t n_chunks,
+                                                    int64_t *width, int64_t *nnz_partial, int *max_partial) {
     __shared__ int64_t s_sum[4];
     __shared__ int s_max[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -164,13 +172,16 @@ __global__ __launch_bounds__(256) void chunk_stats(const uint16_t *row_len, int6
     for (int64_t c = (int64_t)blockIdx.x * 4 + wv; c < n_chunks; c += (int64_t)gridDim.x * 4) {
         const int64_t r = c * kChunk + lane;
         const int l = r < n_rows ? (int)row_len[r] : 0;
-        sum += l;
-        mx = mx > l ? mx : l;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        sum += __shfl_down(sum, off, 64);
-        const int o = __shfl_down(mx, off, 64);
-        mx = mx > o ? mx : o;
+        int64_t cs = l;
+        int cm = l;
+        for (int off = 32; off > 0; off >>= 1) {
+            cs += __shfl_xor(cs, off, 64);
+            const int o = __shfl_xor(cm, off, 64);
+            cm = cm > o ? cm : o;
+        }
+        if (lane == 0) width[c] = (int64_t)kChunk * cm;
+        sum += cs;
+        mx = mx > cm ? mx : cm;
     }
     if (lane == 0) {
         s_sum[wv] = sum;
@@ -189,29 +200,48 @@ __global__ __launch_bounds__(256) void chunk_stats(const uint16_t *row_len, int6
     }
 }
 
-int64_t scan_chunks_i64(qmb_ctx *ctx, const uint16_t *row_len, int64_t n_rows, int64_t *chunk_ptr,
-                        int64_t n_chunks, int64_t *nnz_out, int *max_len_out) {
-    const int64_t total = run_scan(ctx, ChunkSizeSrc{row_len, n_rows}, n_chunks, chunk_ptr);
-    QMB_HIP(hipMemcpyAsync(chunk_ptr + n_chunks, &total, sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
-    const int blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n_chunks + 3) / 4));
-    DevBuf np((size_t)blocks * sizeof(int64_t)), mp((size_t)blocks * sizeof(int));
-    hipLaunchKernelGGL(chunk_stats, dim3(blocks), dim3(256), 0, ctx->stream, row_len, n_rows, n_chunks,
-                       np.as<int64_t>(), mp.as<int>());
-    QMB_HIP(hipGetLastError());
-    std::vector<int64_t> hn(blocks);
-    std::vector<int> hm(blocks);
-    QMB_HIP(hipMemcpyAsync(hn.data(), np.p, blocks * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    QMB_HIP(hipMemcpyAsync(hm.data(), mp.p, blocks * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    QMB_HIP(hipStreamSynchronize(ctx->stream));
-    int64_t nnz = 0;
-    int mx = 0;
-    for (int i = 0; i < blocks; ++i) {
-        nnz += hn[i];
-        mx = std::max(mx, hm[i]);
+// {stored, nnz, max row length, error flag} in one word group for one read-back.
+__global__ __launch_bounds__(64) void chunk_stats_final(const int64_t *nnz_partial, const int *max_partial,
+                                                        int blocks, const int64_t *total, const int *err,
+                                                        int64_t *out) {
+    int64_t t = 0;
+    int m = 0;
+    for (int i = threadIdx.x; i < blocks; i += 64) {
+        t += nnz_partial[i];
+        m = m > max_partial[i] ? m : max_partial[i];
     }
-    *nnz_out = nnz;
-    *max_len_out = mx;
-    return total;
+    for (int off = 32; off > 0; off >>= 1) {
+        t += __shfl_xor(t, off, 64);
+        const int o = __shfl_xor(m, off, 64);
+        m = m > o ? m : o;
+    }
+    if (threadIdx.x != 0) return;
+    out[0] = *total;
+    out[1] = t;
+    out[2] = m;
+    out[3] = err ? *err : 0;
+}
+
+int64_t scan_chunks_i64(qmb_ctx *ctx, const uint16_t *row_len, int64_t n_rows, int64_t *chunk_ptr,
+                        int64_t n_chunks, int64_t *nnz_out, int *max_len_out, const int *d_err, int *err_out) {
+    const int blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n_chunks + 3) / 4));
+    DevBuf wid((size_t)std::max<int64_t>(n_chunks, 1) * sizeof(int64_t)), np((size_t)blocks * sizeof(int64_t)),
+        mp((size_t)blocks * sizeof(int)), stats(4 * sizeof(int64_t)), sums;
+    hipLaunchKernelGGL(chunk_widths, dim3(blocks), dim3(256), 0, ctx->stream, row_len, n_rows, n_chunks,
+                       wid.as<int64_t>(), np.as<int64_t>(), mp.as<int>());
+    QMB_HIP(hipGetLastError());
+    const int64_t *d_total = launch_scan(ctx, ArraySrc{wid.as<int64_t>()}, n_chunks, chunk_ptr, sums,
+                                         chunk_ptr + n_chunks);
+    hipLaunchKernelGGL(chunk_stats_final, dim3(1), dim3(64), 0, ctx->stream, np.as<int64_t>(), mp.as<int>(),
+                       blocks, d_total, d_err, stats.as<int64_t>());
+    QMB_HIP(hipGetLastError());
+    int64_t *h = ctx->host_stage<int64_t>(4 * sizeof(int64_t));
+    QMB_HIP(hipMemcpyAsync(h, stats.p, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    QMB_HIP(hipStreamSynchronize(ctx->stream));
+    *nnz_out = h[1];
+    *max_len_out = (int)h[2];
+    if (err_out) *err_out = (int)h[3];
+    return h[0];
 }
 
 // ---- group analysis ---------------------------------------------------------

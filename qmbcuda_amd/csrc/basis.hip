@@ -16,6 +16,7 @@
 // The bitmap + offsets also serve as the rank -> row dictionary of the ELL
 // build (replacing the C(N,k)*2 orbit_indices table).
 #include <cmath>
+#include <cstring>
 #include <memory>
 
 #include "device_common.hpp"
@@ -27,18 +28,19 @@ qmb::GroupView qmb_basis::view() const {
     v.torus = group->torus;
     v.block = group->block;
     v.full = group->n_sites == 64 ? ~0ULL : ((1ULL << group->n_sites) - 1ULL);
-    v.keep = g_keep.as<uint64_t>();
-    v.small = g_small.as<uint8_t>();
-    v.big = g_big.as<uint8_t>();
-    v.flip = g_flip.as<uint8_t>();
-    v.perm = g_perm.as<uint8_t>();
-    v.inv = g_inv.as<uint8_t>();
+    const uint8_t *t = g_tab.as<uint8_t>();
+    v.keep = reinterpret_cast<const uint64_t *>(t);
+    v.small = t + o_small;
+    v.big = t + o_big;
+    v.flip = t + o_flip;
+    v.perm = t + o_perm;
+    v.inv = t + o_inv;
     v.lut = group->lut;
     v.lut_flip = group->lut_flip;
-    v.lut_tab = (group->lut == 1 || group->lut == 2) ? g_lut.as<uint8_t>() : nullptr;
+    v.lut_tab = (group->lut == 1 || group->lut == 2) ? t + o_lut : nullptr;
     v.n_cos = group->lut == 3 ? group->n_cos : 0;
     v.cos_rev = group->lut == 3 ? group->cos_rev : 0;
-    v.cos_tab = group->lut == 3 ? g_cos.as<uint8_t>() : nullptr;
+    v.cos_tab = group->lut == 3 ? t + o_cos : nullptr;
     return v;
 }
 
@@ -267,30 +269,29 @@ __global__ __launch_bounds__(kBasisThreads) void basis_lookup(GroupView gv, cons
 }
 
 static void upload_group(qmb_basis *b, const qmb_group *g) {
-    const int G = g->G;
-    b->g_keep.alloc(G * sizeof(uint64_t));
-    b->g_small.alloc(G);
-    b->g_big.alloc(G);
-    b->g_flip.alloc(G);
-    b->g_inv.alloc(G);
-    b->g_perm.alloc((size_t)G * g->n_sites);
-    std::vector<uint8_t> inv8(G), perm8((size_t)G * g->n_sites);
-    for (int e = 0; e < G; ++e) inv8[e] = (uint8_t)g->inv[e];
-    for (size_t i = 0; i < perm8.size(); ++i) perm8[i] = (uint8_t)g->perm[i];
-    QMB_HIP(hipMemcpy(b->g_keep.p, g->keep.data(), G * sizeof(uint64_t), hipMemcpyHostToDevice));
-    QMB_HIP(hipMemcpy(b->g_small.p, g->small.data(), G, hipMemcpyHostToDevice));
-    QMB_HIP(hipMemcpy(b->g_big.p, g->big.data(), G, hipMemcpyHostToDevice));
-    QMB_HIP(hipMemcpy(b->g_flip.p, g->flip.data(), G, hipMemcpyHostToDevice));
-    QMB_HIP(hipMemcpy(b->g_inv.p, inv8.data(), G, hipMemcpyHostToDevice));
-    QMB_HIP(hipMemcpy(b->g_perm.p, perm8.data(), perm8.size(), hipMemcpyHostToDevice));
-    if (g->lut == 1 || g->lut == 2) {
-        b->g_lut.alloc(kLutBytes);
-        QMB_HIP(hipMemcpy(b->g_lut.p, g->lut_tab.data(), kLutBytes, hipMemcpyHostToDevice));
+    const size_t G = (size_t)g->G, n = (size_t)g->n_sites;
+    auto up16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    b->o_small = up16(G * 8);
+    b->o_big = b->o_small + G;
+    b->o_flip = b->o_big + G;
+    b->o_inv = b->o_flip + G;
+    b->o_perm = up16(b->o_inv + G);
+    b->o_lut = up16(b->o_perm + G * n);
+    b->o_cos = b->o_lut + kLutBytes;
+    const size_t total = up16(b->o_cos + kCosBytes);
+    std::vector<uint8_t> h(total, 0);
+    std::memcpy(h.data(), g->keep.data(), G * 8);
+    for (size_t e = 0; e < G; ++e) {
+        h[b->o_small + e] = g->small[e];
+        h[b->o_big + e] = g->big[e];
+        h[b->o_flip + e] = g->flip[e];
+        h[b->o_inv + e] = (uint8_t)g->inv[e];
     }
-    if (g->lut == 3) {
-        b->g_cos.alloc(kCosBytes);
-        QMB_HIP(hipMemcpy(b->g_cos.p, g->cos_tab.data(), kCosBytes, hipMemcpyHostToDevice));
-    }
+    for (size_t i = 0; i < G * n; ++i) h[b->o_perm + i] = (uint8_t)g->perm[i];
+    if (g->lut == 1 || g->lut == 2) std::memcpy(h.data() + b->o_lut, g->lut_tab.data(), kLutBytes);
+    if (g->lut == 3) std::memcpy(h.data() + b->o_cos, g->cos_tab.data(), kCosBytes);
+    b->g_tab.alloc(total);
+    QMB_HIP(hipMemcpy(b->g_tab.p, h.data(), total, hipMemcpyHostToDevice));
 }
 
 static uint64_t binom_host(int n, int k) {

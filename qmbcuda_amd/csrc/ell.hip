@@ -349,11 +349,13 @@ struct ValuePacking {
                            e.val.as<double>(), e.vtab.as<unsigned long long>(), f + 1, cb, e.diag.as<double>(),
                            f + 3);
         QMB_HIP(hipGetLastError());
+        QMB_HIP(hipMemcpyAsync(ctx->host_stage<int>(4 * sizeof(int)), flags.p, 4 * sizeof(int),
+                               hipMemcpyDeviceToHost, ctx->stream));
     }
-    // after the stream has drained
-    void finish(qmb_ell &e) {
-        int h[4];
-        QMB_HIP(hipMemcpy(h, flags.p, sizeof h, hipMemcpyDeviceToHost));
+    // after the stream has drained (the flags were copied into the context's
+    // staging buffer at the end of launch)
+    void finish(qmb_ctx *ctx, qmb_ell &e) {
+        const int *h = ctx->host_stage<int>(4 * sizeof(int));
         QMB_REQUIRE(h[3] == 0, "qmb_ell_build: internal error while packing the values");
         if (h[2]) {
             e.val.release();
@@ -421,12 +423,9 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         e->n_rows = b->n_rows;
         e->n_chunks = (b->n_rows + kChunk - 1) / kChunk;
         // term table and irrep data on device
-        const DeviceTerms dt(*t);
+        const DeviceTerms dt(*t, ch, g->G);
         const TermsView tv = dt.view;
-        const std::vector<double> nrm = orbit_norm_table(g->G);
-        DevBuf d_chi((size_t)g->G * 2 * sizeof(double)), d_nrm(nrm.size() * sizeof(double)), d_err(sizeof(int));
-        QMB_HIP(hipMemcpy(d_chi.p, ch, (size_t)g->G * 2 * sizeof(double), hipMemcpyHostToDevice));
-        QMB_HIP(hipMemcpy(d_nrm.p, nrm.data(), nrm.size() * sizeof(double), hipMemcpyHostToDevice));
+        DevBuf d_err(sizeof(int));
         QMB_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), ctx->stream));
         const GroupView gv = b->view();
         const int64_t n = b->n_rows;
@@ -441,19 +440,17 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
             hipLaunchKernelGGL(kcount, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
                                ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
-                               b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
+                               b->prefix.as<uint32_t>(), tv, dt.chi, dt.nrm, W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), b->all_rows ? 1 : 0, nullptr, nullptr,
                                nullptr);
             QMB_HIP(hipGetLastError());
         }
-        int herr = 0;
-        QMB_HIP(hipMemcpyAsync(&herr, d_err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        QMB_HIP(hipStreamSynchronize(ctx->stream));
-        QMB_REQUIRE(herr == 0, "qmb_ell_build: a row needs more than max_terms = " + std::to_string(W) + " slots");
-        int max_len = 0;
+        // the offsets are scanned before the overflow flag is read (one synchronisation)
+        int herr = 0, max_len = 0;
         e->stored = n > 0 ? scan_chunks_i64(ctx, e->row_len.as<uint16_t>(), n, e->chunk_ptr.as<int64_t>(),
-                                            e->n_chunks, &e->nnz, &max_len)
+                                            e->n_chunks, &e->nnz, &max_len, d_err.as<int>(), &herr)
                           : 0;
+        QMB_REQUIRE(herr == 0, "qmb_ell_build: a row needs more than max_terms = " + std::to_string(W) + " slots");
         e->max_row_len = max_len;
         e->col.alloc((size_t)std::max<int64_t>(e->stored, 1) * sizeof(int32_t));
         e->val.alloc((size_t)std::max<int64_t>(e->stored, 1) * (cplx ? 16 : 8));
@@ -466,7 +463,7 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
             hipLaunchKernelGGL(kfill, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
                                ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
-                               b->prefix.as<uint32_t>(), tv, d_chi.as<double2>(), d_nrm.as<double>(), W,
+                               b->prefix.as<uint32_t>(), tv, dt.chi, dt.nrm, W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), 0, e->chunk_ptr.as<int64_t>(),
                                e->col.as<int32_t>(), e->val.as<double>());
             QMB_HIP(hipGetLastError());
@@ -479,7 +476,7 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
             if (vp.on) vp.launch(ctx, *e);
         }
         QMB_HIP(hipStreamSynchronize(ctx->stream));
-        if (vp.on) vp.finish(*e);
+        if (vp.on) vp.finish(ctx, *e);
         *out = e.release();
     });
 }

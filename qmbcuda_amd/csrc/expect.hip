@@ -244,11 +244,7 @@ qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a,
             QMB_HIP(hipMemcpyAsync(d_x.p, x, xbytes, hipMemcpyHostToDevice, ctx->stream));
             xd = d_x.as<double>();
         }
-        const DeviceTerms dt(*a);
-        const std::vector<double> nrm = orbit_norm_table(g->G);
-        DevBuf d_chi((size_t)g->G * 2 * sizeof(double)), d_nrm(nrm.size() * sizeof(double));
-        QMB_HIP(hipMemcpy(d_chi.p, ch, (size_t)g->G * 2 * sizeof(double), hipMemcpyHostToDevice));
-        QMB_HIP(hipMemcpy(d_nrm.p, nrm.data(), nrm.size() * sizeof(double), hipMemcpyHostToDevice));
+        const DeviceTerms dt(*a, ch, g->G);
         const int blocks = (int)std::min<int64_t>(kExpMaxBlocks, (n + kExpThreads - 1) / kExpThreads);
         DevBuf d_part((size_t)blocks * 2 * sizeof(double));
         const bool narrow = b->n_sites <= 32;
@@ -264,8 +260,7 @@ qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a,
                            : (avg ? expect_rows<uint64_t, false, true> : expect_rows<uint64_t, false, false>);
         hipLaunchKernelGGL(k, dim3(blocks), dim3(kExpThreads), 0, ctx->stream, b->view(), ctx->binom.as<uint64_t>(),
                            b->layout, n, b->reps.as<uint64_t>(), b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(),
-                           b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), dt.view, d_chi.as<double2>(),
-                           d_nrm.as<double>(), xd, d_part.as<double>());
+                           b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), dt.view, dt.chi, dt.nrm, xd, d_part.as<double>());
         QMB_HIP(hipGetLastError());
         std::vector<double> part((size_t)blocks * 2);
         QMB_HIP(hipMemcpyAsync(part.data(), d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,

@@ -3,9 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -284,12 +286,25 @@ struct qmb_ctx : qmb::Counted {
     int device = 0;
     hipStream_t stream = nullptr;
     qmb::DevBuf binom;  // uint64 [64*33] C(p, j)
+    // pinned host staging for small per-call read-backs (grows, kept)
+    std::unique_ptr<qmb::PinnedBuf> stage;
+    size_t stage_bytes = 0;
+    template <class T>
+    T *host_stage(size_t bytes) {
+        if (bytes > stage_bytes) {
+            stage.reset();
+            stage_bytes = std::max<size_t>(bytes, 64 * 1024);
+            stage = std::make_unique<qmb::PinnedBuf>(stage_bytes);
+        }
+        return stage->as<T>();
+    }
     ~qmb_ctx() {
         if (stream) {
             (void)hipSetDevice(device);
             (void)hipStreamSynchronize(stream);
         }
         binom.release();
+        stage.reset();
         pool.trim();
         pool.stream = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
@@ -332,7 +347,10 @@ struct qmb_basis : qmb::Counted {
     qmb::DevBuf stab;    // uint8 [n_rows]: stabiliser size
     qmb::DevBuf nullf;   // uint8 [n_rows]: 1 if the orbit is null for the irrep
     // device copies of the group tables
-    qmb::DevBuf g_keep, g_small, g_big, g_flip, g_perm, g_inv, g_lut, g_cos;
+    // one device block: keep (8 B each), then small, big, flip, inv (1 B each),
+    // perm (G * n_sites bytes), the lookup table and the coset table
+    qmb::DevBuf g_tab;
+    size_t o_small = 0, o_big = 0, o_flip = 0, o_inv = 0, o_perm = 0, o_lut = 0, o_cos = 0;
     qmb::GroupView view() const;
     uint64_t triv_lo = 0, triv_hi = 0;  // elements with character 1 for the irrep
     bool all_rows = false;              // every element has character 1: no null orbits
@@ -366,8 +384,12 @@ namespace qmb {
 
 // Exclusive scans on device (3-phase). Values come from a functor.
 int64_t scan_popcount_u32(qmb_ctx *ctx, const uint64_t *words, uint32_t *prefix, int64_t n);
+// Chunk offsets of a SELL-64 layout from its row lengths (chunk_ptr[n_chunks] =
+// stored total, returned), with the entry count and longest row; d_err (if set)
+// is read back in the same synchronisation into err_out.
 int64_t scan_chunks_i64(qmb_ctx *ctx, const uint16_t *row_len, int64_t n_rows, int64_t *chunk_ptr,
-                        int64_t n_chunks, int64_t *nnz_out, int *max_len_out);
+                        int64_t n_chunks, int64_t *nnz_out, int *max_len_out, const int *d_err = nullptr,
+                        int *err_out = nullptr);
 
 void ensure_binom(qmb_ctx *ctx);
 

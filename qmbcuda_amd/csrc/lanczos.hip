@@ -529,9 +529,12 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     int n_reorth = 0;
     auto pro_decide = [&](int it) -> bool {
         // a[it], b[it+1] of this step (one small read per step)
-        QMB_HIP(hipMemcpyAsync(&pa[it], wk.alpha() + it, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        QMB_HIP(hipMemcpyAsync(&pb[it + 1], wk.beta() + it + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        double *st = ctx->host_stage<double>(2 * sizeof(double));
+        QMB_HIP(hipMemcpyAsync(st, wk.alpha() + it, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipMemcpyAsync(st + 1, wk.beta() + it + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         QMB_HIP(hipStreamSynchronize(ctx->stream));
+        pa[it] = st[0];
+        pb[it + 1] = st[1];
         const int j = it;
         const double bj1 = pb[j + 1];
         if (!(bj1 > 0.0)) return false;  // breakdown: nothing to orthogonalise
@@ -597,19 +600,29 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             }
             if (rmode == 2) {
                 // the orthogonalised vector has a new norm b[it+1]
-                QMB_HIP(hipMemcpyAsync(&pb[it + 1], wk.beta() + it + 1, sizeof(double), hipMemcpyDeviceToHost,
-                                       ctx->stream));
+                double *st = ctx->host_stage<double>(sizeof(double));
+                QMB_HIP(hipMemcpyAsync(st, wk.beta() + it + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
                 QMB_HIP(hipStreamSynchronize(ctx->stream));
+                pb[it + 1] = st[0];
             }
         }
     };
 
-    // profiling events (one pair per SpMV in a check window)
+    // profiling events: one pair per SpMV, two check windows' worth (the first
+    // step of a window may be queued while the previous window is evaluated)
     std::vector<hipEvent_t> ev;
     if (o.profile) {
-        ev.resize(2 * (size_t)check);
+        ev.resize(4 * (size_t)check);
         for (auto &x : ev) QMB_HIP(hipEventCreate(&x));
     }
+    struct EventHolder {
+        hipEvent_t e = nullptr;
+        ~EventHolder() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } copy_done;
+    QMB_HIP(hipEventCreateWithFlags(&copy_done.e, hipEventDisableTiming));
+    const hipEvent_t ev_copy = copy_done.e;
     double ms_spmv = 0.0, ms_host = 0.0;
     int n_spmv = 0;
 
@@ -715,25 +728,47 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     }
     const int it_start = it;
     int last_ck = it;
+    // Plain recurrence without checkpoints: while the host reads and solves the
+    // tridiagonal matrix at a check, the next step is already queued (with an
+    // explicit b finish, which gives the same values as the fused one), so the
+    // device does not idle. Not after a check the decay prediction scheduled:
+    // that one usually ends the run, and its queued step would be wasted.
+    const bool can_queue_ahead = fuse && !o.fixed_iter && !(ck && ck->path);
+    bool predicted = false;
+    int it_q = it;  // steps queued so far
+    int wpar = 0;   // event half of the current window
+    auto queue_step = [&](int step, int slot, int half, bool finish) {
+        hipEvent_t *e2 = o.profile ? &ev[(size_t)half * 2 * check + 2 * slot] : nullptr;
+        if (e2) QMB_HIP(hipEventRecord(e2[0], ctx->stream));
+        k1(step, e2 ? e2[1] : nullptr);
+        k2(step, finish);
+        it_q = step + 1;
+    };
     while (it < M) {
         const int win0 = it;
         const int win1 = std::min(M, it + next);
-        for (; it < win1; ++it) {
-            if (o.profile) QMB_HIP(hipEventRecord(ev[2 * (it - win0)], ctx->stream));
-            k1(it, o.profile ? ev[2 * (it - win0) + 1] : nullptr);
-            k2(it, it + 1 == win1);
-            ++n_spmv;
-        }
-        QMB_HIP(hipMemcpyAsync(ha.data(), wk.alpha(), it * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        QMB_HIP(hipMemcpyAsync(hb.data(), wk.beta(), (it + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        for (; it < win1; ++it)
+            if (it >= it_q) queue_step(it, it - win0, wpar, it + 1 == win1);
+        n_spmv += win1 - win0;
+        // a[0..M) and b[0..it] are one span of the coefficient array: one copy
+        // into pinned memory
+        const size_t span = (size_t)M + it + 1;
+        double *st = ctx->host_stage<double>(span * sizeof(double));
+        QMB_HIP(hipMemcpyAsync(st, wk.alpha(), span * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipEventRecord(ev_copy, ctx->stream));
+        if (can_queue_ahead && !predicted && it < M) queue_step(it, 0, wpar ^ 1, true);
+        QMB_HIP(hipEventSynchronize(ev_copy));
+        std::copy(st, st + it, ha.begin());
+        std::copy(st + M, st + M + it + 1, hb.begin());
         if (o.profile) {
             for (int q = 0; q < it - win0; ++q) {
                 float t = 0.f;
-                QMB_HIP(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
+                const size_t i0 = (size_t)wpar * 2 * check + 2 * q;
+                QMB_HIP(hipEventElapsedTime(&t, ev[i0], ev[i0 + 1]));
                 ms_spmv += t;
             }
         }
+        wpar ^= 1;
         Timer th;
         // breakdown: an (almost) zero b[j] ends the Krylov space at j
         double tsc = 0.0;
@@ -778,6 +813,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
                 const double steps = std::ceil(std::log(q) / rate);
                 if (steps < (double)check) next = std::max(1, (int)steps);
             }
+            predicted = next < check;
             q_prev = q;
             it_prev = it;
         }
@@ -793,6 +829,9 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             }
         }
     }
+    // a step queued ahead of the last check is not part of the result; let it
+    // drain before the buffers and events go
+    if (it_q > it) QMB_HIP(hipStreamSynchronize(ctx->stream));
     if (o.fixed_iter) converged = true;
     const int m = m_eff;
     const int kk = std::min(k_eigs, m);
