@@ -300,36 +300,19 @@ __device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, i
     return best;
 }
 
-// orbit_min for a ring group (gv.lut == 2): the top 8 bits of rotation s of x
-// are the circular window of x starting at site p = (N - 8 - s) mod N
-// (complemented under flip). Pass 1 finds the smallest window and the lane's
-// candidate positions (bit masks); pass 2 takes full images of the candidates
-// only. Same result and same gm as orbit_min.
+// orbit_min for a ring group (gv.lut == 2): ring_windows finds the smallest
+// top byte over the rotations (and complements) and the lane's candidate
+// positions; full images are taken of the candidates only. Same result and
+// same gm as orbit_min.
+template <class S>
+__device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint64_t &c1);
+
 template <class S>
 __device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, int &gm) {
-    const int N = gv.n_sites;  // 8 <= N <= 56: the doubled word holds bits p..p+7 for every p < N
-    const uint64_t xx = (uint64_t)x | ((uint64_t)x << N);
+    const int N = gv.n_sites;
     const bool hf = gv.lut_flip != 0;
-    unsigned tmin = 0x1FFu;
-    uint64_t c0 = 0, c1 = 0;
-    for (int p = 0; p < N; ++p) {
-        const unsigned w = (unsigned)(xx >> p) & 0xFFu;
-        if (w < tmin) {
-            tmin = w;
-            c0 = 0;
-            c1 = 0;
-        }
-        if (w == tmin) c0 |= 1ull << p;
-        if (hf) {
-            const unsigned wf = w ^ 0xFFu;
-            if (wf < tmin) {
-                tmin = wf;
-                c0 = 0;
-                c1 = 0;
-            }
-            if (wf == tmin) c1 |= 1ull << p;
-        }
-    }
+    uint64_t c0, c1;
+    ring_windows<S>(x, N, hf, c0, c1);
     S best = 0;
     int bi = -1;
     for (int f = 0; f < (hf ? 2 : 1); ++f) {
@@ -355,31 +338,28 @@ __device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, 
 // Window scan of the ring orbit minimum: the smallest top byte over the
 // rotations of y (and their complements when hf) and the window positions p
 // that reach it (c0: plain, c1: complemented). The top 8 bits of rotation s
-// are the circular window of y starting at site p = (N - 8 - s) mod N.
+// are the circular window of y starting at site p = (N - 8 - s) mod N, whose
+// bit i is bit p + i of the doubled word. All windows are compared at once,
+// bit-sliced from the top bit down: at bit i the candidates whose window bit
+// is 0 (complemented: 1) survive if there are any, else bit i of the minimum
+// is 1 and every candidate stays.
 template <class S>
 __device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint64_t &c1) {
-    const uint64_t yy = (uint64_t)y | ((uint64_t)y << N);  // 8 <= N <= 56
-    unsigned tmin = 0x1FFu;
-    c0 = 0;
-    c1 = 0;
-    for (int p = 0; p < N; ++p) {
-        const unsigned w = (unsigned)(yy >> p) & 0xFFu;
-        if (w < tmin) {
-            tmin = w;
-            c0 = 0;
-            c1 = 0;
-        }
-        if (w == tmin) c0 |= 1ull << p;
-        if (hf) {
-            const unsigned wf = w ^ 0xFFu;
-            if (wf < tmin) {
-                tmin = wf;
-                c0 = 0;
-                c1 = 0;
-            }
-            if (wf == tmin) c1 |= 1ull << p;
-        }
+    const uint64_t yy = (uint64_t)y | ((uint64_t)y << N);  // 8 <= N <= 56: bits p + i < 64
+    const uint64_t all = (1ull << N) - 1ull;
+    uint64_t a = all, b = hf ? all : 0ull;
+    unsigned tmin = 0;
+#pragma unroll
+    for (int i = 7; i >= 0; --i) {
+        const uint64_t bi = yy >> i;
+        const uint64_t z0 = a & ~bi, z1 = b & bi;
+        const bool any = (z0 | z1) != 0ull;
+        a = any ? z0 : a;
+        b = any ? z1 : b;
+        tmin |= any ? 0u : (1u << i);
     }
+    c0 = a;
+    c1 = b;
     return tmin;
 }
 
