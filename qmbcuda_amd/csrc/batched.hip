@@ -169,12 +169,19 @@ __global__ __launch_bounds__(kThreads) void bk_update(const BInst *__restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = tb;
 }
 
+// Reorthogonalisation launch shape: blockIdx.y = 0..kJY-1 splits the Krylov
+// index, and a block takes kJB vectors per pass over its range (w read once,
+// one combined reduction for the kJB dots).
+constexpr int kJY = 8;
+constexpr int kJB = 4;
+
 // part2[(j*nbv + block)*2 + {0,1}] = <V_j, w> over the block's range, j <= it.
+// Per dot: thread-strided sums, wave shuffles, then the waves in order.
 template <bool CPLX>
 __global__ __launch_bounds__(kThreads) void bk_rdots(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
                                                      const int *__restrict__ active, const int *__restrict__ it_p,
                                                      double *__restrict__ part2, int nbv, int gated) {
-    __shared__ double lds[kWaves];
+    __shared__ double lds[kWaves * kJB * 2];
     const BTask t = tasks[blockIdx.x];
     if (!active[t.inst]) return;
     const BInst I = insts[t.inst];
@@ -182,52 +189,84 @@ __global__ __launch_bounds__(kThreads) void bk_rdots(const BInst *__restrict__ i
     const int it = *it_p;
     const int vs = CPLX ? 2 : 1;
     const int64_t cnt = I.n_rows * vs;
-    for (int j = 0; j <= it; ++j) {
-        const double *vj = I.VB + (size_t)j * cnt;
-        double sr = 0.0, si = 0.0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j0 = (int)blockIdx.y * kJB; j0 <= it; j0 += kJY * kJB) {
+        const int nj = min(kJB, it + 1 - j0);
+        double sr[kJB], si[kJB];
+#pragma unroll
+        for (int u = 0; u < kJB; ++u) sr[u] = si[u] = 0.0;
         if (CPLX) {
             for (int64_t i = t.lo / 2 + threadIdx.x; i < t.hi / 2; i += blockDim.x) {
-                const double vr = vj[2 * i], vi = vj[2 * i + 1], wr = I.P[2 * i], wi = I.P[2 * i + 1];
-                sr += vr * wr + vi * wi;
-                si += vr * wi - vi * wr;
+                const double wr = I.P[2 * i], wi = I.P[2 * i + 1];
+#pragma unroll
+                for (int u = 0; u < kJB; ++u) {
+                    if (u < nj) {
+                        const double *vj = I.VB + (size_t)(j0 + u) * cnt;
+                        const double vr = vj[2 * i], vi = vj[2 * i + 1];
+                        sr[u] += vr * wr + vi * wi;
+                        si[u] += vr * wi - vi * wr;
+                    }
+                }
             }
         } else {
-            for (int64_t i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) sr += vj[i] * I.P[i];
+            for (int64_t i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
+                const double w = I.P[i];
+#pragma unroll
+                for (int u = 0; u < kJB; ++u)
+                    if (u < nj) sr[u] += I.VB[(size_t)(j0 + u) * cnt + i] * w;
+            }
         }
-        const double tr = block_sum(sr, lds);
-        const double ti = CPLX ? block_sum(si, lds) : 0.0;
-        if (threadIdx.x == 0) {
-            part2[((size_t)j * nbv + blockIdx.x) * 2] = tr;
-            part2[((size_t)j * nbv + blockIdx.x) * 2 + 1] = ti;
+#pragma unroll
+        for (int u = 0; u < kJB; ++u) {
+            sr[u] = wave_sum(sr[u]);
+            if (CPLX) si[u] = wave_sum(si[u]);
         }
+        if (lane == 0)
+#pragma unroll
+            for (int u = 0; u < kJB; ++u) {
+                lds[(wv * kJB + u) * 2] = sr[u];
+                lds[(wv * kJB + u) * 2 + 1] = CPLX ? si[u] : 0.0;
+            }
+        __syncthreads();
+        if ((int)threadIdx.x < 2 * nj) {
+            const int u = (int)threadIdx.x >> 1, c = (int)threadIdx.x & 1;
+            double tot = 0.0;
+            for (int w = 0; w < kWaves; ++w) tot += lds[(w * kJB + u) * 2 + c];
+            part2[((size_t)(j0 + u) * nbv + blockIdx.x) * 2 + c] = tot;
+        }
+        __syncthreads();
     }
 }
 
-// coef_j = sum of the instance's block partials, fixed order; one block per instance.
+// coef_j = sum of the instance's block partials in a fixed order (lane-strided
+// sums, then wave shuffles): one wavefront per Krylov index j, gridDim.y blocks
+// of kWaves wavefronts per instance.
 __global__ __launch_bounds__(kThreads) void bk_rreduce(const BInst *__restrict__ insts, const int *__restrict__ active,
                                                        const int *__restrict__ it_p, const double *__restrict__ part2,
                                                        int nbv, int first_inst, int gated) {
-    __shared__ double lds[kWaves];
     const int inst = first_inst + blockIdx.x;
     if (!active[inst]) return;
     const BInst I = insts[inst];
     if (I.pro[0] == 0.0 || (gated && *I.gate == 0.0)) return;
     const int it = *it_p;
-    for (int j = 0; j <= it; ++j) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = (int)blockIdx.y * kWaves + wv; j <= it; j += (int)gridDim.y * kWaves) {
+        const double *pj = part2 + ((size_t)j * nbv + I.vec_first) * 2;
         double sr = 0.0, si = 0.0;
-        for (int b = threadIdx.x; b < I.vec_nblk; b += blockDim.x) {
-            sr += part2[((size_t)j * nbv + I.vec_first + b) * 2];
-            si += part2[((size_t)j * nbv + I.vec_first + b) * 2 + 1];
+        for (int b = lane; b < I.vec_nblk; b += 64) {
+            sr += pj[2 * b];
+            si += pj[2 * b + 1];
         }
-        const double tr = block_sum(sr, lds);
-        const double ti = block_sum(si, lds);
-        if (threadIdx.x == 0) {
-            I.coef[2 * j] = tr;
-            I.coef[2 * j + 1] = ti;
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (lane == 0) {
+            I.coef[2 * j] = sr;
+            I.coef[2 * j + 1] = si;
         }
     }
 }
 
+// w -= sum_j coef_j V_j (j in order; four vectors loaded per step), ||w||^2 partials.
 template <bool CPLX>
 __global__ __launch_bounds__(kThreads) void bk_rapply(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
                                                       const int *__restrict__ active, const int *__restrict__ it_p,
@@ -244,7 +283,22 @@ __global__ __launch_bounds__(kThreads) void bk_rapply(const BInst *__restrict__ 
     if (CPLX) {
         for (int64_t i = t.lo / 2 + threadIdx.x; i < t.hi / 2; i += blockDim.x) {
             double wr = I.P[2 * i], wi = I.P[2 * i + 1];
-            for (int j = 0; j <= it; ++j) {
+            int j = 0;
+            for (; j + 4 <= it + 1; j += 4) {
+                double vr[4], vi[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    vr[u] = I.VB[(size_t)(j + u) * cnt + 2 * i];
+                    vi[u] = I.VB[(size_t)(j + u) * cnt + 2 * i + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double cr = I.coef[2 * (j + u)], ci = I.coef[2 * (j + u) + 1];
+                    wr -= cr * vr[u] - ci * vi[u];
+                    wi -= cr * vi[u] + ci * vr[u];
+                }
+            }
+            for (; j <= it; ++j) {
                 const double cr = I.coef[2 * j], ci = I.coef[2 * j + 1];
                 const double vr = I.VB[(size_t)j * cnt + 2 * i], vi = I.VB[(size_t)j * cnt + 2 * i + 1];
                 wr -= cr * vr - ci * vi;
@@ -257,7 +311,15 @@ __global__ __launch_bounds__(kThreads) void bk_rapply(const BInst *__restrict__ 
     } else {
         for (int64_t i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
             double wr = I.P[i];
-            for (int j = 0; j <= it; ++j) wr -= I.coef[2 * j] * I.VB[(size_t)j * cnt + i];
+            int j = 0;
+            for (; j + 4 <= it + 1; j += 4) {
+                double v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = I.VB[(size_t)(j + u) * cnt + i];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) wr -= I.coef[2 * (j + u)] * v[u];
+            }
+            for (; j <= it; ++j) wr -= I.coef[2 * j] * I.VB[(size_t)j * cnt + i];
             I.P[i] = wr;
             acc += wr * wr;
         }
@@ -603,10 +665,10 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                                        d_it.as<int>(), f.first_inst, M);
                 if (reorth) {
                     for (int pass = 0; pass < 2; ++pass) {
-                        hipLaunchKernelGGL(f.cplx ? bk_rdots<true> : bk_rdots<false>, dim3(nvb), dim3(kThreads), 0,
+                        hipLaunchKernelGGL(f.cplx ? bk_rdots<true> : bk_rdots<false>, dim3(nvb, kJY), dim3(kThreads), 0,
                                            s, d_inst.as<BInst>(), f.d_vec_tasks.as<BTask>(), d_active.as<int>(),
                                            d_it.as<int>(), part2.as<double>(), (int)nvb, pass);
-                        hipLaunchKernelGGL(bk_rreduce, dim3((unsigned)f.members.size()), dim3(kThreads), 0, s,
+                        hipLaunchKernelGGL(bk_rreduce, dim3((unsigned)f.members.size(), 4), dim3(kThreads), 0, s,
                                            d_inst.as<BInst>(), d_active.as<int>(), d_it.as<int>(), part2.as<double>(),
                                            (int)nvb, f.first_inst, pass);
                         hipLaunchKernelGGL(f.cplx ? bk_rapply<true> : bk_rapply<false>, dim3(nvb), dim3(kThreads), 0,
