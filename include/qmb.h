@@ -150,15 +150,18 @@ typedef struct {
     int64_t device_bytes;
     double spmv_bytes;    /* algorithmic bytes of one SpMV (int32 column + FP64 value per entry) */
     int32_t packed;       /* 1: packed values (below) */
-    int32_t n_values;     /* distinct values in the table of a packed ELL */
+    int32_t n_values;     /* distinct values in the table(s) of a packed ELL (complex: real + imaginary parts) */
     double format_bytes;  /* bytes of one SpMV in the stored format */
 } qmb_ell_info;
 
 /* A REAL matrix is stored packed when its distinct values fit a table of at most
  * 1024 entries and the index bits left above the column bits (32 - ceil(log2 n_rows)):
  * every off-diagonal slot is one 32-bit word col | (index << col_bits), the
- * diagonal an FP64 array. Values and products are unchanged (SpMV results are
- * bit-identical to the unpacked form); the matrix stream is 4 instead of 12
+ * diagonal an FP64 array. A COMPLEX matrix likewise when its distinct real parts
+ * and its distinct imaginary parts each fit 1024 entries and both indexes fit
+ * beside the column: col | (re index << col_bits) | (im index << (col_bits +
+ * re bits)). Values and products are unchanged (SpMV results are bit-identical
+ * to the unpacked form); the matrix stream is 4 instead of 12 (20 complex)
  * bytes per entry. value_type | QMB_KEEP_FP64_VALUES keeps the unpacked form. */
 enum { QMB_KEEP_FP64_VALUES = 16 };
 qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t,

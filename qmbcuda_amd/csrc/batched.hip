@@ -36,8 +36,8 @@ struct BInst {
     const uint16_t *row_len;
     const int32_t *col;
     const double *val;
-    const double *diag, *vtab;   // packed real values (packed = 1): diagonal, value table
-    int packed, col_bits;
+    const double *diag, *vtab;   // packed values (packed = 1): diagonal, value table(s)
+    int packed, col_bits, re_bits;
     int64_t n_rows, n_chunks;
     double *P, *Y, *VB;          // w, H phi, Krylov basis [M][n*vs]
     double *alpha, *beta, *invb;  // [M], [M+1], [M+1]
@@ -90,7 +90,14 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
                 dot += vr * acc;
             }
         } else {
-            double2 a = sell_row_cplx(I.col, I.val, I.P, p0, L, len);
+            const double2 xr = r < I.n_rows ? reinterpret_cast<const double2 *>(I.P)[r] : make_double2(0.0, 0.0);
+            double2 a = I.packed
+                            ? sell_row_cplx_packed(reinterpret_cast<const uint32_t *>(I.col), I.vtab,
+                                                   I.vtab + kMaxValues, I.P, p0, L, len, I.col_bits, I.re_bits,
+                                                   r < I.n_rows ? reinterpret_cast<const double2 *>(I.diag)[r]
+                                                                : make_double2(0.0, 0.0),
+                                                   xr)
+                            : sell_row_cplx(I.col, I.val, I.P, p0, L, len);
             a.x *= s;
             a.y *= s;
             if (r < I.n_rows) {
@@ -493,6 +500,7 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             I.diag = e->diag.as<double>();
             I.vtab = e->vtab.as<double>();
             I.packed = e->packed;
+            I.re_bits = e->re_bits;
             I.col_bits = e->col_bits;
             I.n_rows = e->n_rows;
             I.n_chunks = e->n_chunks;

@@ -8,7 +8,9 @@
 //
 // Layout (little-endian): EllFileHeader, then chunk_ptr (int64, n_chunks + 1),
 // row_len (uint16, n_rows), col (uint32, stored), then either val (double,
-// stored x 1 or 2) or, packed, diag (double, n_rows) and vtab (double, n_values).
+// stored x 1 or 2) or, packed, diag (double, n_rows x 1 or 2), the value table
+// (double, n_values) and, complex, the imaginary-part table (double,
+// n_values_im).
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -28,6 +30,7 @@ struct EllFileHeader {
     int64_t n_rows, n_chunks, nnz, stored;
     int32_t n_sites, n_up, irrep, layout;
     uint64_t group_hash;
+    int32_t re_bits, n_values_im;  // packed complex values
 };
 
 constexpr char kEllMagic[8] = {'Q', 'M', 'B', 'E', 'L', 'L', '0', '1'};
@@ -82,7 +85,8 @@ void read_device(File &in, void *dev, size_t bytes) {
 // inside the table). err = 1 on the first violation.
 __global__ __launch_bounds__(256) void ell_validate(int64_t n_rows, int64_t n_chunks, const int64_t *chunk_ptr,
                                                      const uint16_t *row_len, const uint32_t *col, int64_t stored,
-                                                     int packed, int cb, int n_values, int W, int *err) {
+                                                     int packed, int cb, int n_values, int cplx, int rb,
+                                                     int n_values_im, int W, int *err) {
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_chunks * 64;
          r += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = r / 64;
@@ -101,7 +105,12 @@ __global__ __launch_bounds__(256) void ell_validate(int64_t n_rows, int64_t n_ch
         const uint32_t mask = packed ? ((1u << cb) - 1u) : 0xFFFFFFFFu;
         for (int k = packed ? 1 : 0; k < len; ++k) {
             const uint32_t w = col[base + (int64_t)k * 64 + lane];
-            if ((int64_t)(w & mask) >= n_rows || (packed && (int)(w >> cb) >= n_values)) *err = 1;
+            if ((int64_t)(w & mask) >= n_rows) *err = 1;
+            if (packed && !cplx && (int)(w >> cb) >= n_values) *err = 1;
+            if (packed && cplx &&
+                ((int)((w >> cb) & ((1u << rb) - 1u)) >= n_values ||
+                 (int)((uint64_t)w >> (cb + rb)) >= n_values_im))
+                *err = 1;
         }
     }
 }
@@ -120,7 +129,7 @@ qmb_status qmb_ell_save(const qmb_ell *e, const char *path) {
         const qmb_basis *b = e->basis;
         EllFileHeader h{};
         std::memcpy(h.magic, kEllMagic, 8);
-        h.version = 1;
+        h.version = 2;
         h.header_bytes = sizeof(EllFileHeader);
         h.value_type = e->value_type;
         h.packed = e->packed;
@@ -137,6 +146,8 @@ qmb_status qmb_ell_save(const qmb_ell *e, const char *path) {
         h.irrep = b->irrep;
         h.layout = b->layout;
         h.group_hash = b->group->hash;
+        h.re_bits = e->re_bits;
+        h.n_values_im = e->n_values_im;
         File out(path, "wb");
         out.write(&h, sizeof h);
         const int vs = e->value_type == QMB_COMPLEX ? 2 : 1;
@@ -144,8 +155,9 @@ qmb_status qmb_ell_save(const qmb_ell *e, const char *path) {
         write_device(out, e->row_len.p, (size_t)e->n_rows * sizeof(uint16_t));
         write_device(out, e->col.p, (size_t)e->stored * sizeof(int32_t));
         if (e->packed) {
-            write_device(out, e->diag.p, (size_t)e->n_rows * sizeof(double));
+            write_device(out, e->diag.p, (size_t)e->n_rows * vs * sizeof(double));
             write_device(out, e->vtab.p, (size_t)e->n_values * sizeof(double));
+            if (vs == 2) write_device(out, e->vtab.as<double>() + kMaxValues, (size_t)e->n_values_im * sizeof(double));
         } else {
             write_device(out, e->val.p, (size_t)e->stored * vs * sizeof(double));
         }
@@ -165,12 +177,13 @@ qmb_status qmb_ell_load(qmb_ctx *ctx, const qmb_basis *b, const char *path, qmb_
         EllFileHeader h{};
         in.read(&h, sizeof h);
         QMB_REQUIRE(std::memcmp(h.magic, kEllMagic, 8) == 0, "qmb_ell_load: not an ELL file");
-        QMB_REQUIRE(h.version == 1 && h.header_bytes == sizeof(EllFileHeader), "qmb_ell_load: unknown version");
+        QMB_REQUIRE(h.version == 2 && h.header_bytes == sizeof(EllFileHeader), "qmb_ell_load: unknown version");
         QMB_REQUIRE(h.n_rows == b->n_rows && h.n_sites == b->n_sites && h.n_up == b->n_up && h.irrep == b->irrep &&
                         h.layout == b->layout && h.group_hash == b->group->hash,
                     "qmb_ell_load: the file was built for another basis");
         QMB_REQUIRE((h.value_type == QMB_REAL || h.value_type == QMB_COMPLEX) && h.n_chunks == (h.n_rows + 63) / 64 &&
-                        h.stored >= 0 && h.n_values >= 0 && h.n_values <= kMaxValues,
+                        h.stored >= 0 && h.n_values >= 0 && h.n_values <= kMaxValues && h.n_values_im >= 0 &&
+                        h.n_values_im <= kMaxValues && h.re_bits >= 0 && h.re_bits <= 31,
                     "qmb_ell_load: corrupt header");
         auto e = std::make_unique<qmb_ell>();
         e->ctx = ctx;
@@ -185,6 +198,8 @@ qmb_status qmb_ell_load(qmb_ctx *ctx, const qmb_basis *b, const char *path, qmb_
         e->packed = h.packed;
         e->col_bits = h.col_bits;
         e->n_values = h.n_values;
+        e->re_bits = h.re_bits;
+        e->n_values_im = h.n_values_im;
         const int vs = h.value_type == QMB_COMPLEX ? 2 : 1;
         e->chunk_ptr.alloc((size_t)(h.n_chunks + 1) * sizeof(int64_t));
         e->row_len.alloc((size_t)std::max<int64_t>(h.n_rows, 1) * sizeof(uint16_t));
@@ -193,10 +208,11 @@ qmb_status qmb_ell_load(qmb_ctx *ctx, const qmb_basis *b, const char *path, qmb_
         read_device(in, e->row_len.p, (size_t)h.n_rows * sizeof(uint16_t));
         read_device(in, e->col.p, (size_t)h.stored * sizeof(int32_t));
         if (h.packed) {
-            e->diag.alloc((size_t)std::max<int64_t>(h.n_rows, 1) * sizeof(double));
-            e->vtab.alloc((size_t)kMaxValues * sizeof(double));
-            read_device(in, e->diag.p, (size_t)h.n_rows * sizeof(double));
+            e->diag.alloc((size_t)std::max<int64_t>(h.n_rows, 1) * vs * sizeof(double));
+            e->vtab.alloc((size_t)kMaxValues * vs * sizeof(double));
+            read_device(in, e->diag.p, (size_t)h.n_rows * vs * sizeof(double));
             read_device(in, e->vtab.p, (size_t)h.n_values * sizeof(double));
+            if (vs == 2) read_device(in, e->vtab.as<double>() + kMaxValues, (size_t)h.n_values_im * sizeof(double));
         } else {
             e->val.alloc((size_t)std::max<int64_t>(h.stored, 1) * vs * sizeof(double));
             read_device(in, e->val.p, (size_t)h.stored * vs * sizeof(double));
@@ -207,8 +223,12 @@ qmb_status qmb_ell_load(qmb_ctx *ctx, const qmb_basis *b, const char *path, qmb_
         QMB_HIP(hipMemcpy(&first, e->chunk_ptr.p, sizeof first, hipMemcpyDeviceToHost));
         QMB_HIP(hipMemcpy(&last, e->chunk_ptr.as<int64_t>() + h.n_chunks, sizeof last, hipMemcpyDeviceToHost));
         QMB_REQUIRE(first == 0 && last == h.stored, "qmb_ell_load: corrupt chunk offsets");
+        const bool cplx = h.value_type == QMB_COMPLEX;
         QMB_REQUIRE(!h.packed || (h.col_bits >= 1 && h.col_bits <= 31 && ((int64_t)1 << h.col_bits) >= h.n_rows &&
-                                  h.n_values >= 1 && h.n_values <= ((int64_t)1 << (32 - h.col_bits))),
+                                  h.n_values >= 1 &&
+                                  (cplx ? (h.n_values_im >= 1 && h.re_bits == index_bits(h.n_values) &&
+                                           h.col_bits + h.re_bits + index_bits(h.n_values_im) <= 32)
+                                        : h.n_values <= ((int64_t)1 << (32 - h.col_bits)))),
                     "qmb_ell_load: corrupt packing parameters");
         DevBuf err(sizeof(int));
         QMB_HIP(hipMemsetAsync(err.p, 0, sizeof(int), ctx->stream));
@@ -216,7 +236,8 @@ qmb_status qmb_ell_load(qmb_ctx *ctx, const qmb_basis *b, const char *path, qmb_
             const unsigned blocks = (unsigned)std::min<int64_t>(4096, (h.n_chunks * 64 + 255) / 256);
             hipLaunchKernelGGL(ell_validate, dim3(blocks), dim3(256), 0, ctx->stream, h.n_rows, h.n_chunks,
                                e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<uint32_t>(),
-                               h.stored, h.packed, h.col_bits, h.n_values, h.max_terms, err.as<int>());
+                               h.stored, h.packed, h.col_bits, h.n_values, cplx ? 1 : 0, h.re_bits, h.n_values_im,
+                               h.max_terms, err.as<int>());
             QMB_HIP(hipGetLastError());
         }
         int herr = 0;

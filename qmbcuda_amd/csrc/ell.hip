@@ -201,12 +201,17 @@ __global__ void ell_pad_tail(int64_t n_rows, int64_t n_chunks, const int64_t *ch
 // in LDS, merged into the global set once per block; vflag[0] = 1 when a set
 // fills. A separate streaming pass: inside the fill kernel, whose time goes to
 // orbit lookups, the set probes cost more than re-reading the values.
+// Complex values: the real and the imaginary parts go to two sets (vset and
+// vset + kSetSlots).
+template <bool CPLX>
 __global__ __launch_bounds__(kEllThreads) void ell_collect_values(int64_t n_rows, const int64_t *chunk_ptr,
                                                                   const uint16_t *row_len, const double *val,
                                                                   unsigned long long *vset, int *vflag) {
-    __shared__ unsigned long long set[kLdsSet];
+    constexpr int kParts = CPLX ? 2 : 1;
+    __shared__ unsigned long long set[kParts][kLdsSet];
     __shared__ int full;
-    for (int i = threadIdx.x; i < kLdsSet; i += blockDim.x) set[i] = kEmptyKey;
+    for (int i = threadIdx.x; i < kLdsSet; i += blockDim.x)
+        for (int q = 0; q < kParts; ++q) set[q][i] = kEmptyKey;
     if (threadIdx.x == 0) full = 0;
     __syncthreads();
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows;
@@ -215,18 +220,23 @@ __global__ __launch_bounds__(kEllThreads) void ell_collect_values(int64_t n_rows
         const int lane = (int)(r % kChunk);
         const int64_t base = chunk_ptr[c];
         const int len = row_len[r];
-        unsigned long long last = kEmptyKey;
+        unsigned long long last[kParts];
+        for (int q = 0; q < kParts; ++q) last[q] = kEmptyKey;
         for (int k = 1; k < len; ++k) {
-            const unsigned long long key =
-                (unsigned long long)__double_as_longlong(__builtin_nontemporal_load(val + base + (int64_t)k * kChunk + lane));
-            if (key == last) continue;
-            last = key;
-            if (!set_insert(set, kLdsSet, key)) full = 1;
+            const int64_t p = base + (int64_t)k * kChunk + lane;
+            for (int q = 0; q < kParts; ++q) {
+                const unsigned long long key = (unsigned long long)__double_as_longlong(
+                    __builtin_nontemporal_load(val + (CPLX ? 2 * p + q : p)));
+                if (key == last[q]) continue;
+                last[q] = key;
+                if (!set_insert(set[q], kLdsSet, key)) full = 1;
+            }
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kLdsSet; i += blockDim.x)
-        if (set[i] != kEmptyKey && !set_insert(vset, kSetSlots, set[i])) vflag[0] = 1;
+        for (int q = 0; q < kParts; ++q)
+            if (set[q][i] != kEmptyKey && !set_insert(vset + q * kSetSlots, kSetSlots, set[q][i])) vflag[0] = 1;
     if (threadIdx.x == 0 && full) vflag[0] = 1;
 }
 
@@ -310,58 +320,134 @@ __global__ __launch_bounds__(kEllThreads) void ell_pack_values(int64_t n_rows, i
     }
 }
 
-// Packing state of one build: the value set filled by the fill kernel, then
-// sort + pack on the device; the host reads {n, ok, missing} once at the end.
+__device__ inline uint32_t key_index(const unsigned long long *keys, int n, unsigned long long key, int *missing) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (keys[lo] != key) *missing = 1;
+    return (uint32_t)lo;
+}
+
+// Complex values, in place when both tables fit (meta = {overflow, n_re, ok_re,
+// missing, n_im, ok_im}) and the two indexes fit beside the column:
+// col[p] |= (re index << cb) | (im index << (cb + re bits)); diag = the old slot 0.
+__global__ __launch_bounds__(kEllThreads) void ell_pack_values_cplx(int64_t n_rows, int64_t n_lanes,
+                                                                    const int64_t *chunk_ptr,
+                                                                    const uint16_t *row_len, uint32_t *colw,
+                                                                    const double *val,
+                                                                    const unsigned long long *keys, const int *meta,
+                                                                    int cb, double *diag, int *missing) {
+    __shared__ unsigned long long s_re[kMaxValues], s_im[kMaxValues];
+    const int n_re = meta[1], n_im = meta[4];
+    if (!meta[2] || !meta[5]) return;
+    const int rb = index_bits(n_re), sh2 = cb + rb;
+    if (sh2 + index_bits(n_im) > 32) return;
+    for (int i = threadIdx.x; i < n_re; i += blockDim.x) s_re[i] = keys[i];
+    for (int i = threadIdx.x; i < n_im; i += blockDim.x) s_im[i] = keys[kMaxValues + i];
+    __syncthreads();
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_lanes;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = r / kChunk;
+        const int lane = (int)(r % kChunk);
+        const int64_t base = chunk_ptr[c];
+        const int L = (int)((chunk_ptr[c + 1] - base) / kChunk);
+        const int len = r < n_rows ? (int)row_len[r] : 0;
+        if (r < n_rows) {
+            const bool has = len > 0 && L > 0;
+            diag[2 * r] = has ? val[2 * (base + lane)] : 0.0;
+            diag[2 * r + 1] = has ? val[2 * (base + lane) + 1] : 0.0;
+        }
+        if (L > 0) colw[base + lane] = (uint32_t)(r < n_rows ? r : 0);
+        for (int k = 1; k < L; ++k) {
+            const int64_t p = base + (int64_t)k * kChunk + lane;
+            uint64_t ir = 0, ii = 0;
+            if (k < len) {
+                ir = key_index(s_re, n_re, (unsigned long long)__double_as_longlong(val[2 * p]), missing);
+                ii = key_index(s_im, n_im, (unsigned long long)__double_as_longlong(val[2 * p + 1]), missing);
+            }
+            colw[p] = colw[p] | (uint32_t)((ir << cb) | (ii << sh2));
+        }
+    }
+}
+
+// Packing state of one build: the distinct values collected after the fill,
+// then sort + pack on the device; the host reads the flags once at the end.
+// flags: {overflow, n, ok, missing} (real) or {overflow, n_re, ok_re, missing,
+// n_im, ok_im} (complex).
 struct ValuePacking {
-    bool on = false;
+    bool on = false, cplx = false;
     int cb = 0, cap = 0;
-    DevBuf vset, flags;  // flags: overflow, n, ok, missing
+    DevBuf vset, flags;
     bool prepare(qmb_ctx *ctx, const qmb_ell &e) {
         const int64_t n = e.n_rows;
         if (n < 2) return false;
+        cplx = e.value_type == QMB_COMPLEX;
         cb = 1;
         while (((int64_t)1 << cb) < n) ++cb;
         if (cb > 31) return false;
-        cap = (int)std::min<int64_t>(kMaxValues, (int64_t)1 << (32 - cb));
-        vset.alloc((size_t)kSetSlots * 8);
-        flags.alloc(4 * sizeof(int));
+        cap = cplx ? kMaxValues : (int)std::min<int64_t>(kMaxValues, (int64_t)1 << (32 - cb));
+        const int parts = cplx ? 2 : 1;
+        vset.alloc((size_t)parts * kSetSlots * 8);
+        flags.alloc(8 * sizeof(int));
         QMB_HIP(hipMemsetAsync(vset.p, 0xFF, vset.bytes, ctx->stream));
         QMB_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, ctx->stream));
         on = true;
         return true;
     }
     void launch(qmb_ctx *ctx, qmb_ell &e) {
-        e.vtab.alloc((size_t)kMaxValues * 8);
-        e.diag.alloc((size_t)e.n_rows * sizeof(double));
+        const int parts = cplx ? 2 : 1;
+        e.vtab.alloc((size_t)parts * kMaxValues * 8);
+        e.diag.alloc((size_t)e.n_rows * parts * sizeof(double));
         int *f = flags.as<int>();
+        unsigned long long *keys = e.vtab.as<unsigned long long>();
         const unsigned cblocks = (unsigned)std::min<int64_t>(4096, (e.n_rows + kEllThreads - 1) / kEllThreads);
-        hipLaunchKernelGGL(ell_collect_values, dim3(cblocks), dim3(kEllThreads), 0, ctx->stream, e.n_rows,
-                           e.chunk_ptr.as<int64_t>(), e.row_len.as<uint16_t>(), e.val.as<double>(),
-                           vset.as<unsigned long long>(), f);
+        hipLaunchKernelGGL(cplx ? ell_collect_values<true> : ell_collect_values<false>, dim3(cblocks),
+                           dim3(kEllThreads), 0, ctx->stream, e.n_rows, e.chunk_ptr.as<int64_t>(),
+                           e.row_len.as<uint16_t>(), e.val.as<double>(), vset.as<unsigned long long>(), f);
         QMB_HIP(hipGetLastError());
         hipLaunchKernelGGL(ell_sort_values, dim3(1), dim3(1024), 0, ctx->stream, vset.as<unsigned long long>(), f,
-                           e.vtab.as<unsigned long long>(), f + 1, cap);
+                           keys, f + 1, cap);
         QMB_HIP(hipGetLastError());
+        if (cplx) {
+            hipLaunchKernelGGL(ell_sort_values, dim3(1), dim3(1024), 0, ctx->stream,
+                               vset.as<unsigned long long>() + kSetSlots, f, keys + kMaxValues, f + 4, cap);
+            QMB_HIP(hipGetLastError());
+        }
         const int64_t lanes = e.n_chunks * kChunk;
         const unsigned blocks = (unsigned)std::min<int64_t>(4096, (lanes + kEllThreads - 1) / kEllThreads);
-        hipLaunchKernelGGL(ell_pack_values, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, e.n_rows, lanes,
-                           e.chunk_ptr.as<int64_t>(), e.row_len.as<uint16_t>(), e.col.as<uint32_t>(),
-                           e.val.as<double>(), e.vtab.as<unsigned long long>(), f + 1, cb, e.diag.as<double>(),
-                           f + 3);
+        if (cplx)
+            hipLaunchKernelGGL(ell_pack_values_cplx, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, e.n_rows,
+                               lanes, e.chunk_ptr.as<int64_t>(), e.row_len.as<uint16_t>(), e.col.as<uint32_t>(),
+                               e.val.as<double>(), keys, f, cb, e.diag.as<double>(), f + 3);
+        else
+            hipLaunchKernelGGL(ell_pack_values, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, e.n_rows, lanes,
+                               e.chunk_ptr.as<int64_t>(), e.row_len.as<uint16_t>(), e.col.as<uint32_t>(),
+                               e.val.as<double>(), keys, f + 1, cb, e.diag.as<double>(), f + 3);
         QMB_HIP(hipGetLastError());
-        QMB_HIP(hipMemcpyAsync(ctx->host_stage<int>(4 * sizeof(int)), flags.p, 4 * sizeof(int),
+        QMB_HIP(hipMemcpyAsync(ctx->host_stage<int>(8 * sizeof(int)), flags.p, 8 * sizeof(int),
                                hipMemcpyDeviceToHost, ctx->stream));
     }
     // after the stream has drained (the flags were copied into the context's
     // staging buffer at the end of launch)
     void finish(qmb_ctx *ctx, qmb_ell &e) {
-        const int *h = ctx->host_stage<int>(4 * sizeof(int));
-        QMB_REQUIRE(h[3] == 0, "qmb_ell_build: internal error while packing the values");
-        if (h[2]) {
+        const int *h = ctx->host_stage<int>(8 * sizeof(int));
+        bool ok = h[2] != 0;
+        if (cplx) ok = ok && h[5] != 0 && cb + index_bits(h[1]) + index_bits(h[4]) <= 32;
+        QMB_REQUIRE(!ok || h[3] == 0, "qmb_ell_build: internal error while packing the values");
+        if (ok) {
             e.val.release();
             e.packed = 1;
             e.col_bits = cb;
             e.n_values = h[1];
+            if (cplx) {
+                e.re_bits = index_bits(h[1]);
+                e.n_values_im = h[4];
+            }
         } else {
             e.diag.release();
             e.vtab.release();
@@ -386,7 +472,9 @@ double qmb_ell::format_bytes() const {
     if (!packed) return spmv_bytes();
     // one word per off-diagonal entry, the diagonal, x read once, y written, row lengths
     const int64_t n_diag = basis && basis->layout == QMB_LAYOUT_REFERENCE ? n_rows - basis->n_null : n_rows;
-    return (double)(nnz - n_diag) * 4.0 + (double)n_rows * (8.0 + 16.0 + 2.0) + (double)n_values * 8.0;
+    const double sv = value_type == QMB_COMPLEX ? 16.0 : 8.0;
+    return (double)(nnz - n_diag) * 4.0 + (double)n_rows * (3.0 * sv + 2.0) +
+           (double)(n_values + n_values_im) * 8.0;
 }
 
 extern "C" {
@@ -455,7 +543,7 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         e->col.alloc((size_t)std::max<int64_t>(e->stored, 1) * sizeof(int32_t));
         e->val.alloc((size_t)std::max<int64_t>(e->stored, 1) * (cplx ? 16 : 8));
         ValuePacking vp;
-        if (n > 0 && !cplx && !keep_fp64) vp.prepare(ctx, *e);
+        if (n > 0 && !keep_fp64) vp.prepare(ctx, *e);
         if (n > 0) {
             const bool narrow = b->n_sites <= 32;
             auto kfill = narrow ? (cplx ? ell_rows<uint32_t, true, true> : ell_rows<uint32_t, true, false>)
@@ -500,7 +588,7 @@ qmb_status qmb_ell_get_info(const qmb_ell *e, qmb_ell_info *info) {
         info->device_bytes = e->device_bytes();
         info->spmv_bytes = e->spmv_bytes();
         info->packed = e->packed;
-        info->n_values = e->n_values;
+        info->n_values = e->n_values + e->n_values_im;
         info->format_bytes = e->format_bytes();
     });
 }
@@ -532,11 +620,12 @@ qmb_status qmb_ell_export_rows(const qmb_ell *e, int64_t row0, int64_t n_rows, u
         QMB_HIP(hipMemcpy(cl.data(), e->col.as<int32_t>() + base, cl.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
         if (e->packed) {
             // words -> columns and values; slot 0 = the diagonal
-            std::vector<double> vt(e->n_values), dg(n_rows);
+            std::vector<double> vt((size_t)kMaxValues * vs), dg((size_t)n_rows * vs);
             QMB_HIP(hipMemcpy(vt.data(), e->vtab.p, vt.size() * sizeof(double), hipMemcpyDeviceToHost));
-            QMB_HIP(hipMemcpy(dg.data(), e->diag.as<double>() + row0, dg.size() * sizeof(double),
+            QMB_HIP(hipMemcpy(dg.data(), e->diag.as<double>() + (size_t)row0 * vs, dg.size() * sizeof(double),
                               hipMemcpyDeviceToHost));
-            const uint32_t mask = (1u << e->col_bits) - 1u;
+            const uint32_t mask = (1u << e->col_bits) - 1u, rmask = (1u << e->re_bits) - 1u;
+            const int sh2 = e->col_bits + e->re_bits;
             for (int64_t i = 0; i < n_rows; ++i) {
                 const int64_t r = row0 + i;
                 const int64_t c = r / kChunk - c0, lane = r % kChunk;
@@ -544,13 +633,18 @@ qmb_status qmb_ell_export_rows(const qmb_ell *e, int64_t row0, int64_t n_rows, u
                 const int L = (int)((cp[c + 1] - cp[c]) / kChunk);
                 if (L > 0) {
                     cl[q0] = (int32_t)r;
-                    vl[q0] = dg[i];
+                    for (int q = 0; q < vs; ++q) vl[(size_t)q0 * vs + q] = dg[(size_t)i * vs + q];
                 }
                 for (int s = 1; s < L; ++s) {
                     const int64_t q = q0 + (int64_t)s * kChunk;
                     const uint32_t w = (uint32_t)cl[q];
                     cl[q] = (int32_t)(w & mask);
-                    vl[q] = vt[w >> e->col_bits];
+                    if (vs == 1) {
+                        vl[q] = vt[w >> e->col_bits];
+                    } else {
+                        vl[2 * q] = vt[(w >> e->col_bits) & rmask];
+                        vl[2 * q + 1] = vt[kMaxValues + (uint32_t)((uint64_t)w >> sh2)];
+                    }
                 }
             }
         } else {

@@ -206,7 +206,14 @@ struct PinnedBuf {
 
 constexpr int kMaxGroup = 128;
 constexpr int kChunk = 64;  // rows per SELL chunk = one wavefront
-constexpr int kMaxValues = 1024;  // value table of a packed ELL
+constexpr int kMaxValues = 1024;  // value table of a packed ELL (per part for complex values)
+
+// smallest b with 2^b >= n (n >= 1)
+__host__ __device__ inline int index_bits(int n) {
+    int b = 0;
+    while ((1 << b) < n) ++b;
+    return b;
+}
 
 // Device view of a symmetry group (all arrays in device memory).
 struct GroupView {
@@ -367,14 +374,18 @@ struct qmb_ell : qmb::Counted {
     qmb::DevBuf row_len;    // uint16 [n_rows]
     qmb::DevBuf col;        // int32 [stored]; packed: uint32 col | (value index << col_bits), slot 0 unused
     qmb::DevBuf val;        // double or double2 [stored]; empty when packed
-    // Packed real values: the matrix has few distinct values, so an off-diagonal
-    // slot stores an index into vtab next to its column and the diagonal is kept
-    // in FP64 per row.
+    // Packed values: the matrix has few distinct values, so an off-diagonal slot
+    // stores value indexes next to its column and the diagonal is kept in FP64
+    // per row. Real: col | (index << col_bits) into vtab[0 .. n_values). Complex:
+    // col | (re index << col_bits) | (im index << (col_bits + re_bits)) into
+    // vtab[0 .. n_values) (real parts) and vtab[kMaxValues ..] (n_values_im
+    // imaginary parts).
     int packed = 0;
     int col_bits = 0;
     int n_values = 0;
-    qmb::DevBuf diag;       // double [n_rows] (packed)
-    qmb::DevBuf vtab;       // double [n_values] (packed), ascending bit patterns, vtab[0] = +0.0
+    int re_bits = 0, n_values_im = 0;  // complex only
+    qmb::DevBuf diag;       // double [n_rows] or double2 [n_rows] (packed)
+    qmb::DevBuf vtab;       // double [kMaxValues] (x 2 complex), ascending bit patterns, entry 0 = +0.0
     double spmv_bytes() const;    // algorithmic bytes of one SpMV (int32 column + FP64 value per entry)
     double format_bytes() const;  // bytes of one SpMV in the stored format
     int64_t device_bytes() const;

@@ -36,8 +36,9 @@ constexpr int kMaxBlocks = 4096;
 // ---- K1: SELL-64 SpMV --------------------------------------------------------
 // One wavefront per 64-row chunk (row products in sell.hpp); the x gathers are
 // the only random accesses. LZ fuses the Lanczos normalisation and the a[n]
-// dot: y = s (H x), phi = s x, partial sums of phi . y. PACK (real only): the
-// packed value format (4 B per off-diagonal entry, value table staged in LDS).
+// dot: y = s (H x), phi = s x, partial sums of phi . y. PACK: the packed value
+// format (4 B per off-diagonal entry, value tables staged in LDS; complex: real
+// parts at [0, n_values), imaginary parts at kMaxValues + [0, n_values_im)).
 template <bool CPLX, bool LZ, bool PACK>
 __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_chunks,
                                                       const int64_t *__restrict__ chunk_ptr,
@@ -46,16 +47,19 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ vtab, int n_values, int cb,
+                                                      int n_values_im, int rb,
                                                       const double *__restrict__ x, double *__restrict__ y,
                                                       const double *__restrict__ invb, double *__restrict__ vout,
                                                       double *__restrict__ partials, const double *__restrict__ bpart,
                                                       int n_bpart, double *__restrict__ beta_out,
                                                       double *__restrict__ invb_out) {
     __shared__ double lds[kWaves];
-    __shared__ double s_vt[PACK ? kMaxValues : 1];
+    __shared__ double s_vt[PACK ? (CPLX ? 2 : 1) * kMaxValues : 1];
     __shared__ double s_scale;
     if (PACK) {
         for (int i = threadIdx.x; i < n_values; i += blockDim.x) s_vt[i] = vtab[i];
+        if (CPLX)
+            for (int i = threadIdx.x; i < n_values_im; i += blockDim.x) s_vt[kMaxValues + i] = vtab[kMaxValues + i];
         __syncthreads();
     }
     const int lane = threadIdx.x & 63;
@@ -108,7 +112,15 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                 }
             }
         } else {
-            double2 a = sell_row_cplx(col, val, x, p0, L, len);
+            double2 a;
+            if (PACK) {
+                const double2 xr = r < n_rows ? reinterpret_cast<const double2 *>(x)[r] : make_double2(0.0, 0.0);
+                const double2 d = r < n_rows ? reinterpret_cast<const double2 *>(diag)[r] : make_double2(0.0, 0.0);
+                a = sell_row_cplx_packed(reinterpret_cast<const uint32_t *>(col), s_vt, s_vt + kMaxValues, x, p0, L,
+                                         len, cb, rb, d, xr);
+            } else {
+                a = sell_row_cplx(col, val, x, p0, L, len);
+            }
             if (LZ) {
                 a.x *= s;
                 a.y *= s;
@@ -362,13 +374,14 @@ void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, boo
                  const FusedBeta *fb = nullptr) {
     const unsigned blocks = grid_for(e->n_chunks, kWaves);
     const bool cplx = e->value_type == QMB_COMPLEX;
-    auto k = cplx       ? (lz ? spmv_sell<true, true, false> : spmv_sell<true, false, false>)
-             : e->packed ? (lz ? spmv_sell<false, true, true> : spmv_sell<false, false, true>)
-                         : (lz ? spmv_sell<false, true, false> : spmv_sell<false, false, false>);
+    auto k = cplx ? (e->packed ? (lz ? spmv_sell<true, true, true> : spmv_sell<true, false, true>)
+                               : (lz ? spmv_sell<true, true, false> : spmv_sell<true, false, false>))
+                  : (e->packed ? (lz ? spmv_sell<false, true, true> : spmv_sell<false, false, true>)
+                               : (lz ? spmv_sell<false, true, false> : spmv_sell<false, false, false>));
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, ctx->stream, e->n_rows, e->n_chunks,
                        e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
                        e->val.as<double>(), e->diag.as<double>(), e->vtab.as<double>(), e->n_values, e->col_bits,
-                       x, y, invb, vout, fb ? fb->partials : wk ? wk->partials.as<double>() : nullptr,
+                       e->n_values_im, e->re_bits, x, y, invb, vout, fb ? fb->partials : wk ? wk->partials.as<double>() : nullptr,
                        fb ? fb->bpart : nullptr, fb ? fb->n_bpart : 0, fb ? fb->beta_out : nullptr,
                        fb ? fb->invb_out : nullptr);
     QMB_HIP(hipGetLastError());

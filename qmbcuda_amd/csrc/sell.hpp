@@ -128,4 +128,54 @@ __device__ inline double2 sell_row_cplx(const int32_t *__restrict__ col, const d
     return make_double2(ar, ai);
 }
 
+// Packed complex rows: slot 0 is the diagonal d; slot k >= 1 holds
+// w = col | (re index << cb) | (im index << (cb + rb)) with the value
+// (vre[re index], vim[im index]). Same products in the same order as
+// sell_row_cplx on the unpacked matrix.
+__device__ inline double2 sell_row_cplx_packed(const uint32_t *__restrict__ w, const double *vre, const double *vim,
+                                               const double *__restrict__ x, int64_t p0, int L, int len, int cb,
+                                               int rb, double2 d, double2 xr) {
+    const double2 *x2 = reinterpret_cast<const double2 *>(x);
+    const uint32_t mask = (1u << cb) - 1u, rmask = (1u << rb) - 1u;
+    const int sh2 = cb + rb;
+    double ar = 0.0, ai = 0.0;
+    if (len > 0) {
+        ar += d.x * xr.x - d.y * xr.y;
+        ai += d.x * xr.y + d.y * xr.x;
+    }
+    int k = 1;
+    for (; k + 4 <= L; k += 4) {
+        uint32_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < len) j[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(k + u) * 64);
+        double2 g[4], v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k + u < len) {
+                g[u] = x2[j[u] & mask];
+                v[u].x = vre[(j[u] >> cb) & rmask];
+                v[u].y = vim[(uint32_t)((uint64_t)j[u] >> sh2)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k + u < len) {
+                ar += v[u].x * g[u].x - v[u].y * g[u].y;
+                ai += v[u].x * g[u].y + v[u].y * g[u].x;
+            }
+        }
+    }
+    for (; k < L; ++k) {
+        if (k < len) {
+            const uint32_t jj = __builtin_nontemporal_load(w + p0 + (int64_t)k * 64);
+            const double vr = vre[(jj >> cb) & rmask], vi = vim[(uint32_t)((uint64_t)jj >> sh2)];
+            const double2 gg = x2[jj & mask];
+            ar += vr * gg.x - vi * gg.y;
+            ai += vr * gg.y + vi * gg.x;
+        }
+    }
+    return make_double2(ar, ai);
+}
+
 }  // namespace qmb

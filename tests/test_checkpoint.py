@@ -66,8 +66,8 @@ def test_ell_load_refuses_bad_files(ctx, tmp_path):
     with pytest.raises(q.QMBError):
         q.Ell.load(ctx, b, tmp_path / "short.qell")
     say("ell load: column out of range")
-    # header (see checkpoint.hip): 8 + 4 + 4 + 6*4 + 4*8 + 4*4 + 8 bytes
-    hdr = 8 + 4 + 4 + 24 + 32 + 16 + 8
+    # header (see checkpoint.hip): 8 + 4 + 4 + 6*4 + 4*8 + 4*4 + 8 + 2*4 bytes
+    hdr = 8 + 4 + 4 + 24 + 32 + 16 + 8 + 8
     n_rows, n_chunks, nnz, stored = struct.unpack_from("<qqqq", raw, 8 + 4 + 4 + 24)
     off_col = hdr + (n_chunks + 1) * 8 + n_rows * 2
     bad = bytearray(raw)

@@ -98,8 +98,9 @@ def test_ell_reference_layout_bit_exact(ctx, label, mk, n_up, alphas):
 
 @pytest.mark.parametrize("label,mk,n_up,alphas", CASES)
 def test_packed_values_equal_fp64_values(ctx, label, mk, n_up, alphas):
-    """A real ELL is stored packed (column | value index words, FP64 diagonal):
-    its export, SpMV results and eigenvalue are bit-identical to the FP64 form."""
+    """A real or complex ELL is stored packed (column | value index words, FP64
+    diagonal): its export, SpMV results and eigenvalue are bit-identical to the
+    FP64 form."""
     case = mk()
     orb = oracle_orbits(case, n_up)
     for a in alphas:
@@ -113,11 +114,10 @@ def test_packed_values_equal_fp64_values(ctx, label, mk, n_up, alphas):
             ef = q.Ell(ctx, b, t, keep_fp64_values=True)
             ip, jf = ep.info(), ef.info()
             assert jf.packed == 0
-            assert ip.packed == (1 if real else 0)
-            if real:
-                assert 1 <= ip.n_values <= 1024
-                assert ip.device_bytes < jf.device_bytes
-                assert ip.format_bytes < jf.format_bytes == jf.spmv_bytes
+            assert ip.packed == 1
+            assert 1 <= ip.n_values <= (1024 if real else 2048)
+            assert ip.device_bytes < jf.device_bytes
+            assert ip.format_bytes < jf.format_bytes == jf.spmv_bytes
             assert ip.spmv_bytes == jf.spmv_bytes and ip.nnz == jf.nnz and ip.stored == jf.stored
             for u, v in zip(ep.export_reference(), ef.export_reference()):
                 assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
