@@ -63,9 +63,10 @@ __device__ inline T block_exclusive_scan(T v, T *lds, T &total) {
     return incl - v;
 }
 
+// bit words at stride 2 (the rank dictionary's records)
 struct PopcountSrc {
     const uint64_t *w;
-    __device__ int64_t operator()(int64_t i) const { return __popcll(w[i]); }
+    __device__ int64_t operator()(int64_t i) const { return __popcll(w[2 * i]); }
 };
 
 struct ArraySrc {
@@ -105,9 +106,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_tile_offsets(int64_t *tile_
     }
 }
 
+// out[i * stride + offset] = exclusive prefix of element i
 template <class Src, class Out>
 __global__ __launch_bounds__(kScanThreads) void scan_write(Src src, int64_t n, const int64_t *tile_off,
-                                                           Out *out) {
+                                                           Out *out, int stride, int offset) {
     __shared__ int64_t lds[kScanThreads];
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     int64_t vals[kScanItems];
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_write(Src src, int64_t n, c
     int64_t total;
     int64_t run = block_exclusive_scan<int64_t>(s, lds, total) + tile_off[blockIdx.x];
     for (int i = 0; i < kScanItems; ++i) {
-        if (base + i < n) out[base + i] = (Out)run;
+        if (base + i < n) out[(base + i) * stride + offset] = (Out)run;
         run += vals[i];
     }
 }
@@ -127,7 +129,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_write(Src src, int64_t n, c
 // Queues the three scan phases on the context's stream; returns the device
 // address of the grand total (inside sums).
 template <class Src, class Out>
-static const int64_t *launch_scan(qmb_ctx *ctx, Src src, int64_t n, Out *out, DevBuf &sums, int64_t *total_out) {
+static const int64_t *launch_scan(qmb_ctx *ctx, Src src, int64_t n, Out *out, DevBuf &sums, int64_t *total_out,
+                                  int stride = 1, int offset = 0) {
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
     sums.alloc((size_t)(tiles + 1) * sizeof(int64_t));
     if (tiles > 0) {
@@ -140,24 +143,24 @@ static const int64_t *launch_scan(qmb_ctx *ctx, Src src, int64_t n, Out *out, De
     QMB_HIP(hipGetLastError());
     if (tiles > 0) {
         hipLaunchKernelGGL((scan_write<Src, Out>), dim3((unsigned)tiles), dim3(kScanThreads), 0, ctx->stream, src, n,
-                           sums.as<int64_t>(), out);
+                           sums.as<int64_t>(), out, stride, offset);
         QMB_HIP(hipGetLastError());
     }
     return sums.as<int64_t>() + tiles;
 }
 
 template <class Src, class Out>
-static int64_t run_scan(qmb_ctx *ctx, Src src, int64_t n, Out *out) {
+static int64_t run_scan(qmb_ctx *ctx, Src src, int64_t n, Out *out, int stride = 1, int offset = 0) {
     DevBuf sums;
-    const int64_t *d_total = launch_scan(ctx, src, n, out, sums, nullptr);
+    const int64_t *d_total = launch_scan(ctx, src, n, out, sums, nullptr, stride, offset);
     int64_t *st = ctx->host_stage<int64_t>(sizeof(int64_t));
     QMB_HIP(hipMemcpyAsync(st, d_total, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     QMB_HIP(hipStreamSynchronize(ctx->stream));
     return st[0];
 }
 
-int64_t scan_popcount_u32(qmb_ctx *ctx, const uint64_t *words, uint32_t *prefix, int64_t n) {
-    return run_scan(ctx, PopcountSrc{words}, n, prefix);
+int64_t scan_popcount_dict(qmb_ctx *ctx, uint64_t *dict, int64_t n_words) {
+    return run_scan(ctx, PopcountSrc{dict}, n_words, dict, 2, 1);
 }
 
 // Chunk widths (64 x the longest row of each 64-row chunk), one wavefront per

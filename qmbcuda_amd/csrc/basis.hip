@@ -9,12 +9,14 @@
 //   mark    : one thread per 64 consecutive sector ranks; unrank the first,
 //             step with the next-combination trick, keep a state when it is
 //             the minimum of its orbit (and, in the compact layout, when the
-//             irrep's character is 1 on its stabiliser) -> one bitmap word;
-//   scan    : exclusive popcount scan of the words -> row offsets;
+//             irrep's character is 1 on its stabiliser) -> one bit word;
+//   scan    : exclusive popcount scan of the words -> row offsets, stored
+//             beside each word (16-byte records);
 //   compact : rewrite the kept states in ascending order with their
 //             stabiliser size and null flag.
-// The bitmap + offsets also serve as the rank -> row dictionary of the ELL
-// build (replacing the C(N,k)*2 orbit_indices table).
+// The records also serve as the rank -> row dictionary of the ELL build
+// (replacing the C(N,k)*2 orbit_indices table): one random 16-byte read per
+// lookup.
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -56,7 +58,7 @@ template <class S>
 __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const uint64_t *binom, int n_up,
                                                             int64_t dim, int64_t n_words, int compact,
                                                             uint64_t triv_lo, uint64_t triv_hi,
-                                                            uint64_t *bitmap) {
+                                                            uint64_t *dict) {
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
     __shared__ GroupLds gt;
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
                 shi = 0;
             }
         }
-        bitmap[w] = bits;
+        dict[2 * w] = bits;
         return;
     }
     if (gv.lut == 3) {
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
                 shi = 0;
             }
         }
-        bitmap[w] = bits;
+        dict[2 * w] = bits;
         return;
     }
     // One flat loop over (state, element) steps per lane: a lane moves to its
@@ -211,10 +213,10 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
             shi = 0;
         }
     }
-    bitmap[w] = bits;
+    dict[2 * w] = bits;
 }
 
-// One thread per row: the row's bitmap word is the last word whose row offset
+// One thread per row: the row's bit word is the last word whose row offset
 // is <= row (binary search over the offsets), its bit the (row - offset)-th
 // set bit of that word. Every thread does the same work (one unrank, one
 // stabiliser count) and the writes are coalesced.
@@ -222,7 +224,7 @@ template <class S>
 __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, const uint64_t *binom, int n_up,
                                                                int64_t n_rows, int64_t n_words,
                                                                uint64_t triv_lo, uint64_t triv_hi,
-                                                               const uint64_t *bitmap, const uint32_t *prefix,
+                                                               const uint64_t *dict,
                                                                uint64_t *reps, uint8_t *stab, uint8_t *nullf) {
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
@@ -231,16 +233,16 @@ __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, con
     __syncthreads();
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rows) return;
-    int64_t lo = 0, hi = n_words - 1;  // prefix[0] = 0 <= r
+    int64_t lo = 0, hi = n_words - 1;  // the first word has 0 rows before it
     while (lo < hi) {
         const int64_t mid = (lo + hi + 1) >> 1;
-        if ((int64_t)prefix[mid] <= r)
+        if ((int64_t)dict[2 * mid + 1] <= r)
             lo = mid;
         else
             hi = mid - 1;
     }
-    uint64_t bits = bitmap[lo];
-    for (int64_t k = r - (int64_t)prefix[lo]; k > 0; --k) bits &= bits - 1;
+    uint64_t bits = dict[2 * lo];
+    for (int64_t k = r - (int64_t)dict[2 * lo + 1]; k > 0; --k) bits &= bits - 1;
     const int i = __builtin_ctzll(bits);
     const S x = colex_unrank<S>(lo * 64 + i, n_up, gv.n_sites, bin);
     uint64_t slo, shi;
@@ -252,8 +254,8 @@ __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, con
 
 template <class S>
 __global__ __launch_bounds__(kBasisThreads) void basis_lookup(GroupView gv, const uint64_t *binom, int64_t n,
-                                                              const uint64_t *states, const uint64_t *bitmap,
-                                                              const uint32_t *prefix, int64_t *rows,
+                                                              const uint64_t *states, const uint64_t *dict,
+                                                              int64_t *rows,
                                                               int32_t *elems) {
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(kBasisThreads) void basis_lookup(GroupView gv, cons
     if (i >= n) return;
     int gm;
     const S rep = orbit_min<S>((S)states[i], gv, perm_lds, gm);
-    rows[i] = row_of_rep<S>(rep, bin, bitmap, prefix);
+    rows[i] = row_of_rep<S>(rep, bin, dict);
     elems[i] = gm;
 }
 
@@ -327,7 +329,7 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
             QMB_REQUIRE(!g->flip[e] || 2 * n_up == g->n_sites,
                         "qmb_basis_build: spin flip needs n_up == n_sites/2");
         const uint64_t dim = binom_host(g->n_sites, n_up);
-        QMB_REQUIRE(dim > 0 && dim <= (1ULL << 38), "qmb_basis_build: sector too large for the rank bitmap");
+        QMB_REQUIRE(dim > 0 && dim <= (1ULL << 38), "qmb_basis_build: sector too large for the rank dictionary");
         QMB_HIP(hipSetDevice(ctx->device));
         auto b = std::make_unique<qmb_basis>();
         b->ctx = ctx;
@@ -352,15 +354,14 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
         }
         upload_group(b.get(), g);
         const GroupView gv = b->view();
-        b->bitmap.alloc((size_t)b->n_words * sizeof(uint64_t));
-        b->prefix.alloc((size_t)b->n_words * sizeof(uint32_t));
+        b->dict.alloc((size_t)b->n_words * 2 * sizeof(uint64_t));
         const unsigned blocks = (unsigned)((b->n_words + kBasisThreads - 1) / kBasisThreads);
         const bool narrow = g->n_sites <= 32;
         hipLaunchKernelGGL(narrow ? basis_mark<uint32_t> : basis_mark<uint64_t>, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, gv,
                            ctx->binom.as<uint64_t>(), n_up, b->sector_dim, b->n_words,
-                           layout == QMB_LAYOUT_COMPACT ? 1 : 0, b->triv_lo, b->triv_hi, b->bitmap.as<uint64_t>());
+                           layout == QMB_LAYOUT_COMPACT ? 1 : 0, b->triv_lo, b->triv_hi, b->dict.as<uint64_t>());
         QMB_HIP(hipGetLastError());
-        const int64_t rows = scan_popcount_u32(ctx, b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), b->n_words);
+        const int64_t rows = scan_popcount_dict(ctx, b->dict.as<uint64_t>(), b->n_words);
         QMB_REQUIRE(rows < (int64_t)INT32_MAX, "qmb_basis_build: more than 2^31 rows (int32 columns)");
         b->n_rows = rows;
         b->reps.alloc((size_t)std::max<int64_t>(rows, 1) * sizeof(uint64_t));
@@ -370,7 +371,7 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
             const unsigned rblocks = (unsigned)((rows + kBasisThreads - 1) / kBasisThreads);
             hipLaunchKernelGGL(narrow ? basis_compact<uint32_t> : basis_compact<uint64_t>, dim3(rblocks), dim3(kBasisThreads), 0, ctx->stream, gv,
                                ctx->binom.as<uint64_t>(), n_up, rows, b->n_words, b->triv_lo, b->triv_hi,
-                               b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), b->reps.as<uint64_t>(),
+                               b->dict.as<uint64_t>(), b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>());
             QMB_HIP(hipGetLastError());
         }
@@ -402,7 +403,7 @@ qmb_status qmb_basis_get_info(const qmb_basis *b, qmb_basis_info *info) {
         info->n_up = b->n_up;
         info->irrep = b->irrep;
         info->layout = b->layout;
-        info->device_bytes = (int64_t)(b->bitmap.bytes + b->prefix.bytes + b->reps.bytes + b->stab.bytes +
+        info->device_bytes = (int64_t)(b->dict.bytes + b->reps.bytes + b->stab.bytes +
                                        b->nullf.bytes);
     });
 }
@@ -440,8 +441,8 @@ qmb_status qmb_basis_lookup(const qmb_basis *b, int64_t n, const uint64_t *state
         QMB_HIP(hipMemcpyAsync(ds.p, states, n * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
         const unsigned blocks = (unsigned)((n + kBasisThreads - 1) / kBasisThreads);
         hipLaunchKernelGGL(b->n_sites <= 32 ? basis_lookup<uint32_t> : basis_lookup<uint64_t>, dim3(blocks), dim3(kBasisThreads), 0, ctx->stream, b->view(),
-                           ctx->binom.as<uint64_t>(), n, ds.as<uint64_t>(), b->bitmap.as<uint64_t>(),
-                           b->prefix.as<uint32_t>(), dr.as<int64_t>(), de.as<int32_t>());
+                           ctx->binom.as<uint64_t>(), n, ds.as<uint64_t>(), b->dict.as<uint64_t>(),
+                           dr.as<int64_t>(), de.as<int32_t>());
         QMB_HIP(hipGetLastError());
         QMB_HIP(hipMemcpyAsync(row, dr.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
         QMB_HIP(hipMemcpyAsync(elem, de.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));

@@ -405,16 +405,18 @@ __device__ inline S orbit_min_coset(S x, const GroupView &gv, const GroupLds &t,
     return best;
 }
 
-// Row of a representative through the basis bitmap (rank -> word -> prefix +
-// popcount). Returns -1 when the representative is not a row.
+// Row of a representative through the rank dictionary (rank -> record -> rows
+// before the word + popcount of the lower bits). Returns -1 when the
+// representative is not a row.
 template <class S>
-__device__ inline int64_t row_of_rep(S rep, const uint64_t *bin_lds, const uint64_t *bitmap, const uint32_t *prefix) {
+__device__ inline int64_t row_of_rep(S rep, const uint64_t *bin_lds, const uint64_t *dict) {
     const int64_t rk = colex_rank<S>(rep, bin_lds);
-    const uint64_t w = bitmap[rk >> 6];
+    const ulonglong2 rec = reinterpret_cast<const ulonglong2 *>(dict)[rk >> 6];
+    const uint64_t w = rec.x;
     const int b = (int)(rk & 63);
     if (!((w >> b) & 1ULL)) return -1;
     const uint64_t below = b ? (w & ((1ULL << b) - 1ULL)) : 0ULL;
-    return (int64_t)prefix[rk >> 6] + __popcll(below);
+    return (int64_t)rec.y + __popcll(below);
 }
 
 }  // namespace qmb

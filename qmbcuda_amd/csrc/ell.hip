@@ -48,8 +48,8 @@ __device__ inline bool set_insert(unsigned long long *set, int slots, unsigned l
 template <class S, bool FILL, bool CPLX>
 __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
                                                         int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
-                                                        const uint8_t *nullf, const uint64_t *bitmap,
-                                                        const uint32_t *prefix, TermsView tv, const double2 *chi,
+                                                        const uint8_t *nullf, const uint64_t *dict,
+                                                        TermsView tv, const double2 *chi,
                                                         const double *nrm, int W, uint16_t *row_len, int *err,
                                                         int all_rows, const int64_t *chunk_ptr, int32_t *col,
                                                         double *val) {
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                               : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm)
                               : gv.lut == 3 ? orbit_min_coset<S>(tgt, gv, gt, perm_lds, gm)
                                             : orbit_min<S>(tgt, gv, perm_lds, gm);
-                const int64_t r2 = row_of_rep<S>(rep, bin, bitmap, prefix);
+                const int64_t r2 = row_of_rep<S>(rep, bin, dict);
                 if (r2 < 0) continue;
                 if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
                 if (FILL) {
@@ -527,8 +527,8 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                  : (cplx ? ell_rows<uint64_t, false, true> : ell_rows<uint64_t, false, false>);
             hipLaunchKernelGGL(kcount, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
                                ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
-                               b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
-                               b->prefix.as<uint32_t>(), tv, dt.chi, dt.nrm, W,
+                               b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->dict.as<uint64_t>(),
+                               tv, dt.chi, dt.nrm, W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), b->all_rows ? 1 : 0, nullptr, nullptr,
                                nullptr);
             QMB_HIP(hipGetLastError());
@@ -550,8 +550,8 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                 : (cplx ? ell_rows<uint64_t, true, true> : ell_rows<uint64_t, true, false>);
             hipLaunchKernelGGL(kfill, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
                                ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
-                               b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->bitmap.as<uint64_t>(),
-                               b->prefix.as<uint32_t>(), tv, dt.chi, dt.nrm, W,
+                               b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->dict.as<uint64_t>(),
+                               tv, dt.chi, dt.nrm, W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), 0, e->chunk_ptr.as<int64_t>(),
                                e->col.as<int32_t>(), e->val.as<double>());
             QMB_HIP(hipGetLastError());

@@ -39,8 +39,8 @@ constexpr int kExpMaxBlocks = 4096;
 template <class S, bool CPLX, bool AVG>
 __global__ __launch_bounds__(kExpThreads) void expect_rows(GroupView gv, const uint64_t *binom, int layout,
                                                            int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
-                                                           const uint8_t *nullf, const uint64_t *bitmap,
-                                                           const uint32_t *prefix, TermsView tv, const double2 *chi,
+                                                           const uint8_t *nullf, const uint64_t *dict,
+                                                           TermsView tv, const double2 *chi,
                                                            const double *nrm, const double *x, double *partials) {
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kExpThreads) void expect_rows(GroupView gv, const u
                                    : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm)
                                    : gv.lut == 3 ? orbit_min_coset<S>(tgt, gv, gt, perm_lds, gm)
                                                  : orbit_min<S>(tgt, gv, perm_lds, gm);
-                    const int64_t r2 = row_of_rep<S>(rep2, bin, bitmap, prefix);
+                    const int64_t r2 = row_of_rep<S>(rep2, bin, dict);
                     if (r2 < 0) continue;
                     if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
                     const double ratio = s_nrm[stab[r2]] / n_src;
@@ -249,7 +249,7 @@ qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a,
         DevBuf d_part((size_t)blocks * 2 * sizeof(double));
         const bool narrow = b->n_sites <= 32;
         using K = void (*)(GroupView, const uint64_t *, int, int64_t, const uint64_t *, const uint8_t *,
-                           const uint8_t *, const uint64_t *, const uint32_t *, TermsView, const double2 *,
+                           const uint8_t *, const uint64_t *, TermsView, const double2 *,
                            const double *, const double *, double *);
         K k;
         if (narrow)
@@ -260,7 +260,7 @@ qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a,
                            : (avg ? expect_rows<uint64_t, false, true> : expect_rows<uint64_t, false, false>);
         hipLaunchKernelGGL(k, dim3(blocks), dim3(kExpThreads), 0, ctx->stream, b->view(), ctx->binom.as<uint64_t>(),
                            b->layout, n, b->reps.as<uint64_t>(), b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(),
-                           b->bitmap.as<uint64_t>(), b->prefix.as<uint32_t>(), dt.view, dt.chi, dt.nrm, xd, d_part.as<double>());
+                           b->dict.as<uint64_t>(), dt.view, dt.chi, dt.nrm, xd, d_part.as<double>());
         QMB_HIP(hipGetLastError());
         std::vector<double> part((size_t)blocks * 2);
         QMB_HIP(hipMemcpyAsync(part.data(), d_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,

@@ -348,8 +348,10 @@ struct qmb_basis : qmb::Counted {
     qmb::Ref<const qmb_group> group;
     int n_sites = 0, n_up = 0, irrep = 0, layout = 0;
     int64_t sector_dim = 0, n_words = 0, n_rows = 0, n_null = 0;
-    qmb::DevBuf bitmap;  // uint64 [n_words]: bit = representative kept as a row
-    qmb::DevBuf prefix;  // uint32 [n_words]: rows before the word
+    // rank -> row dictionary, one 16-byte record per 64 sector ranks:
+    // dict[2w] = bit i set when rank 64w + i is a kept representative,
+    // dict[2w + 1] = rows before the word (one random access per lookup)
+    qmb::DevBuf dict;
     qmb::DevBuf reps;    // uint64 [n_rows]
     qmb::DevBuf stab;    // uint8 [n_rows]: stabiliser size
     qmb::DevBuf nullf;   // uint8 [n_rows]: 1 if the orbit is null for the irrep
@@ -394,7 +396,7 @@ struct qmb_ell : qmb::Counted {
 namespace qmb {
 
 // Exclusive scans on device (3-phase). Values come from a functor.
-int64_t scan_popcount_u32(qmb_ctx *ctx, const uint64_t *words, uint32_t *prefix, int64_t n);
+int64_t scan_popcount_dict(qmb_ctx *ctx, uint64_t *dict, int64_t n_words);
 // Chunk offsets of a SELL-64 layout from its row lengths (chunk_ptr[n_chunks] =
 // stored total, returned), with the entry count and longest row; d_err (if set)
 // is read back in the same synchronisation into err_out.
