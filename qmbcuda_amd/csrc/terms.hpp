@@ -8,44 +8,83 @@
 #include <cstring>
 #include <vector>
 
-#include "internal.hpp"
+#include "device_common.hpp"
 
 namespace qmb {
 
-struct TermsView {
-    int n_terms, max_ops;
-    const int8_t *op;      // [n_terms*max_ops]
-    const int8_t *site;    // [n_terms*max_ops]
-    const double2 *coef;   // [n_terms]
-    const int8_t *n_ops;   // [n_terms]
+// One term compiled on the host from its operator list (applied left to right,
+// the reference's inner loop, utils.cu:133-161): the term acts on state x iff
+// (x & req) == req_val; the image is (x & ~set) | set_val; its scalar is
+// coef * 0.5^(number of Sz) times the signs of the Sz factors whose site value
+// is known at that point (folded into coef), times -1 for every Sz site that
+// still carries its original bit and has it 0 (the parity of ~x & sz). The
+// factors are 1, +-0.5, so the products are exact in any order: the values
+// equal the sequential ones bit for bit. A term that can never act has
+// req = 1, req_val = 2.
+struct TermC {
+    uint64_t req, req_val, set, set_val, sz;
+    double2 coef;
+    uint64_t pad;
 };
 
-// Conjugated operators of term h applied left to right to src (the reference's
-// inner loop, utils.cu:133-161). Returns false when the term annihilates src.
+struct TermsView {
+    int n_terms;
+    const TermC *t;  // [n_terms]
+};
+
+// The term h on src. Returns false when the term annihilates src.
 template <class S>
 __device__ inline bool apply_term(const TermsView &tv, int h, S src, S &tgt, double &cr, double &ci) {
-    tgt = src;
-    const double2 c0 = tv.coef[h];
-    cr = c0.x;
-    ci = c0.y;
-    const int nk = tv.n_ops[h];
-    for (int k = 0; k < nk; ++k) {
-        const int s = tv.site[h * tv.max_ops + k];
-        const int op = tv.op[h * tv.max_ops + k];
-        const int spin = (int)((tgt >> s) & (S)1);
-        if (op == QMB_OP_SM) {
-            if (!spin) return false;
-            tgt &= (S)~((S)1 << s);
-        } else if (op == QMB_OP_SP) {
-            if (spin) return false;
-            tgt |= (S)1 << s;
+    const TermC &t = tv.t[h];
+    if ((src & (S)t.req) != (S)t.req_val) return false;
+    tgt = (S)((src & (S)~(S)t.set) | (S)t.set_val);
+    const bool neg = (popc_s((S)(~src & (S)t.sz)) & 1) != 0;
+    cr = neg ? -t.coef.x : t.coef.x;
+    ci = neg ? -t.coef.y : t.coef.y;
+    return true;
+}
+
+// Host: the compiled form of term h of a table.
+inline TermC compile_term(const qmb_terms &tt, int h) {
+    TermC c{};
+    uint64_t known = 0, kval = 0;  // sites whose current value is fixed, and that value
+    double cr = tt.scalar[2 * h], ci = tt.scalar[2 * h + 1];
+    bool never = false;
+    for (int k = 0; k < tt.n_ops[h]; ++k) {
+        const int s = tt.site[h * tt.max_ops + k];
+        const int op = tt.op_type[h * tt.max_ops + k];
+        const uint64_t b = 1ULL << s;
+        if (op == QMB_OP_SM || op == QMB_OP_SP) {
+            const uint64_t need = op == QMB_OP_SM ? b : 0;  // S- needs an up spin, S+ a down spin
+            if (known & b) {
+                if ((kval & b) != need) never = true;
+            } else {
+                c.req |= b;
+                c.req_val |= need;
+            }
+            known |= b;
+            kval = (kval & ~b) | (op == QMB_OP_SM ? 0 : b);
         } else {
-            const double f = -1.0 + 2.0 * (double)spin;
-            cr = cr * 0.5 * f;
-            ci = ci * 0.5 * f;
+            cr *= 0.5;
+            ci *= 0.5;
+            if (known & b) {
+                if (!(kval & b)) {
+                    cr = -cr;
+                    ci = -ci;
+                }
+            } else {
+                c.sz ^= b;  // sign from the original bit (applied twice: no sign)
+            }
         }
     }
-    return true;
+    if (never) {
+        c.req = 1;
+        c.req_val = 2;
+    }
+    c.set = known;
+    c.set_val = kval;
+    c.coef = make_double2(cr, ci);
+    return c;
 }
 
 // Orbit norms n(m) = sqrt(G m)/G for stabiliser sizes m = 0..G (entry 0 unused).
@@ -55,32 +94,22 @@ inline std::vector<double> orbit_norm_table(int G) {
     return nrm;
 }
 
-// A term table uploaded to the device (int8 operators and sites, double2
-// scalars) together with the irrep's character row and the orbit norm table of
-// its group: one host blob, one copy.
+// A term table uploaded to the device (compiled terms) together with the
+// irrep's character row and the orbit norm table of its group: one host blob,
+// one copy.
 struct DeviceTerms {
     DevBuf blob;
     TermsView view{};
     const double2 *chi = nullptr;  // [G]
     const double *nrm = nullptr;   // [G + 1]
     DeviceTerms(const qmb_terms &t, const double *chi_row, int G) {
-        const int nt = t.n_terms, mo = t.max_ops;
-        const size_t n_coef = (size_t)std::max(1, nt), n_ops8 = (size_t)std::max(1, nt * mo);
+        const int nt = t.n_terms;
         auto up16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
-        const size_t o_coef = 0, o_chi = up16(o_coef + n_coef * 16), o_nrm = up16(o_chi + (size_t)G * 16),
-                     o_op = up16(o_nrm + (size_t)(G + 1) * 8), o_site = up16(o_op + n_ops8),
-                     o_nops = up16(o_site + n_ops8), total = up16(o_nops + n_coef);
+        const size_t o_terms = 0, o_chi = up16((size_t)std::max(1, nt) * sizeof(TermC)),
+                     o_nrm = up16(o_chi + (size_t)G * 16), total = up16(o_nrm + (size_t)(G + 1) * 8);
         std::vector<unsigned char> h(total, 0);
-        double *coef = reinterpret_cast<double *>(h.data() + o_coef);
-        for (int k = 0; k < nt; ++k) {
-            coef[2 * k] = t.scalar[2 * k];
-            coef[2 * k + 1] = t.scalar[2 * k + 1];
-            h[o_nops + k] = (unsigned char)(int8_t)t.n_ops[k];
-        }
-        for (int i = 0; i < nt * mo; ++i) {
-            h[o_op + i] = (unsigned char)(int8_t)t.op_type[i];
-            h[o_site + i] = (unsigned char)(int8_t)(t.site[i] < 0 ? 0 : t.site[i]);
-        }
+        TermC *tc = reinterpret_cast<TermC *>(h.data() + o_terms);
+        for (int k = 0; k < nt; ++k) tc[k] = compile_term(t, k);
         if (chi_row) std::memcpy(h.data() + o_chi, chi_row, (size_t)G * 16);
         const std::vector<double> nt_tab = orbit_norm_table(G);
         std::memcpy(h.data() + o_nrm, nt_tab.data(), nt_tab.size() * 8);
@@ -89,12 +118,7 @@ struct DeviceTerms {
         char *d = static_cast<char *>(blob.p);
         chi = reinterpret_cast<const double2 *>(d + o_chi);
         nrm = reinterpret_cast<const double *>(d + o_nrm);
-        view = TermsView{nt,
-                         mo,
-                         reinterpret_cast<const int8_t *>(d + o_op),
-                         reinterpret_cast<const int8_t *>(d + o_site),
-                         reinterpret_cast<const double2 *>(d + o_coef),
-                         reinterpret_cast<const int8_t *>(d + o_nops)};
+        view = TermsView{nt, reinterpret_cast<const TermC *>(d + o_terms)};
     }
 };
 
