@@ -125,6 +125,41 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
         dict[2 * w] = bits;
         return;
     }
+    if (gv.lut == 1) {
+        // Full torus product, two passes. Pass 1 keeps the states whose top block
+        // equals the smallest in-block rotation of any block (complemented too
+        // under flip): a necessary condition, cheap and the same work for every
+        // lane. Pass 2 takes each lane's survivors in turn (stepping its state
+        // forward) and decides them from the candidate images alone.
+        const int B = gv.block, nb = gv.n_sites / B;
+        const unsigned mB = (1u << B) - 1u;
+        const int topsh = (nb - 1) * B;
+        const bool hf = gv.lut_flip != 0;
+        uint64_t surv = 0;
+        S y = x;
+        for (int i = 0; i < cnt; ++i) {
+            unsigned tmin = 0xFFu;
+            for (int j = 0; j < nb; ++j) {
+                const unsigned v = (unsigned)(y >> (j * B)) & mB;
+                tmin = min(tmin, (unsigned)gt.lut[v]);
+                if (hf) tmin = min(tmin, (unsigned)gt.lut[v ^ mB]);
+            }
+            if (((unsigned)(y >> topsh) & mB) == tmin) surv |= 1ULL << i;
+            if (i + 1 < cnt) y = next_comb<S>(y);
+        }
+        y = x;
+        int at = 0;
+        while (surv) {
+            const int i = __builtin_ctzll(surv);
+            surv &= surv - 1;
+            for (; at < i; ++at) y = next_comb<S>(y);
+            uint64_t slo, shi;
+            if (rep_stab_lut<S>(y, gv, gt, slo, shi) && (!compact || compatible(slo, shi, triv_lo, triv_hi)))
+                bits |= 1ULL << i;
+        }
+        dict[2 * w] = bits;
+        return;
+    }
     if (gv.lut == 3) {
         // Ring rotations times cosets: per coset c_i one permutation image
         // y = c_i x, then its rotations one bit at a time, as above.

@@ -300,6 +300,44 @@ __device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, i
     return best;
 }
 
+// Representative test for a full torus product (gv.lut == 1): x is the orbit
+// minimum iff no image is smaller. Only the images whose top block reaches the
+// smallest rotated block tmin can be <= x (the others have a larger top block),
+// so those candidates decide it, and the ones equal to x are the stabiliser.
+// The caller has checked that x's own top block is tmin.
+template <class S>
+__device__ inline bool rep_stab_lut(S x, const GroupView &gv, const GroupLds &t, uint64_t &stab_lo,
+                                    uint64_t &stab_hi) {
+    const int B = gv.block, nb = gv.n_sites / B;
+    const unsigned mB = (1u << B) - 1u;
+    const unsigned tmin = (unsigned)(x >> ((nb - 1) * B)) & mB;
+    stab_lo = 0;
+    stab_hi = 0;
+    for (int j = 0; j < nb; ++j) {
+        const unsigned v0 = (unsigned)(x >> (j * B)) & mB;
+        const int a = nb - 1 - j;
+        for (int f = 0; f < (gv.lut_flip ? 2 : 1); ++f) {
+            const unsigned v = f ? (v0 ^ mB) : v0;
+            if (t.lut[v] != tmin) continue;
+            unsigned sm = t.lut[256 + v];
+            while (sm) {
+                const int sh = __builtin_ctz(sm);
+                sm &= sm - 1;
+                const int g = t.lut[512 + (f * 8 + a) * 8 + sh];
+                const S y = apply_elem_div<S>(x, g, gv, t, nullptr);
+                if (y < x) return false;
+                if (y == x) {
+                    if (g < 64)
+                        stab_lo |= 1ULL << g;
+                    else
+                        stab_hi |= 1ULL << (g - 64);
+                }
+            }
+        }
+    }
+    return true;
+}
+
 // orbit_min for a ring group (gv.lut == 2): ring_windows finds the smallest
 // top byte over the rotations (and complements) and the lane's candidate
 // positions; full images are taken of the candidates only. Same result and
