@@ -263,8 +263,10 @@ __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, con
                                                                uint64_t *reps, uint8_t *stab, uint8_t *nullf) {
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ uint64_t bin[64 * kBinJ];
+    __shared__ GroupLds gt;
     stage_perm(perm_lds, gv);
     stage_binom(bin, binom);
+    stage_group(gt, gv);
     __syncthreads();
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rows) return;
@@ -281,7 +283,10 @@ __global__ __launch_bounds__(kBasisThreads) void basis_compact(GroupView gv, con
     const int i = __builtin_ctzll(bits);
     const S x = colex_unrank<S>(lo * 64 + i, n_up, gv.n_sites, bin);
     uint64_t slo, shi;
-    is_representative<S>(x, gv, perm_lds, slo, shi);
+    if (gv.lut == 1)
+        rep_stab_lut<S>(x, gv, gt, slo, shi);  // x is a representative: the candidates hold its stabiliser
+    else
+        is_representative<S>(x, gv, perm_lds, slo, shi);
     reps[r] = (uint64_t)x;
     stab[r] = (uint8_t)(__popcll(slo) + __popcll(shi));
     nullf[r] = compatible(slo, shi, triv_lo, triv_hi) ? 0 : 1;
