@@ -58,12 +58,25 @@ __device__ inline double sell_row_packed(const uint32_t *__restrict__ w, const d
     const uint32_t mask = (1u << cb) - 1u;
     double acc = 0.0;
     if (len > 0) acc += d * xr;
-    int k = 1;
-    for (; k + 4 <= L; k += 4) {
-        uint32_t j[4];
+    // groups of four slots from slot 1; the next group's words are loaded
+    // before this group's gathers, so the two latencies overlap
+    const int n4 = (L - 1) / 4;
+    uint32_t jn[4] = {0u, 0u, 0u, 0u};
+    if (n4 > 0) {
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (k + u < len) j[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(k + u) * 64);
+            if (1 + u < len) jn[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(1 + u) * 64);
+    }
+    int k = 1;
+    for (int q = 0; q < n4; ++q, k += 4) {
+        uint32_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = jn[u];
+        if (q + 1 < n4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k + 4 + u < len) jn[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(k + 4 + u) * 64);
+        }
         double g[4], v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
