@@ -18,18 +18,31 @@ namespace qmb {
 __device__ inline double sell_row_real(const int32_t *__restrict__ col, const double *__restrict__ val,
                                        const double *__restrict__ x, int64_t p0, int L, int len) {
     double acc = 0.0;
+    // next group's columns and values loaded before this group's gathers
+    const int n4 = L / 4;
+    int32_t jn[4] = {0, 0, 0, 0};
+    double vn[4] = {0.0, 0.0, 0.0, 0.0};
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k0 + u < len) {
+                const int64_t p = p0 + (int64_t)(k0 + u) * 64;
+                jn[u] = __builtin_nontemporal_load(col + p);
+                vn[u] = __builtin_nontemporal_load(val + p);
+            }
+        }
+    };
+    if (n4 > 0) load(0);
     int k = 0;
-    for (; k + 4 <= L; k += 4) {
+    for (int q = 0; q < n4; ++q, k += 4) {
         int32_t j[4];
         double v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            if (k + u < len) {
-                const int64_t p = p0 + (int64_t)(k + u) * 64;
-                j[u] = __builtin_nontemporal_load(col + p);
-                v[u] = __builtin_nontemporal_load(val + p);
-            }
+            j[u] = jn[u];
+            v[u] = vn[u];
         }
+        if (q + 1 < n4) load(k + 4);
         double g[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -156,12 +169,24 @@ __device__ inline double2 sell_row_cplx_packed(const uint32_t *__restrict__ w, c
         ar += d.x * xr.x - d.y * xr.y;
         ai += d.x * xr.y + d.y * xr.x;
     }
-    int k = 1;
-    for (; k + 4 <= L; k += 4) {
-        uint32_t j[4];
+    // next group's words loaded before this group's gathers (as in sell_row_packed)
+    const int n4 = (L - 1) / 4;
+    uint32_t jn[4] = {0u, 0u, 0u, 0u};
+    if (n4 > 0) {
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (k + u < len) j[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(k + u) * 64);
+            if (1 + u < len) jn[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(1 + u) * 64);
+    }
+    int k = 1;
+    for (int q = 0; q < n4; ++q, k += 4) {
+        uint32_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = jn[u];
+        if (q + 1 < n4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k + 4 + u < len) jn[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(k + 4 + u) * 64);
+        }
         double2 g[4], v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
