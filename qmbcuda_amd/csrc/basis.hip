@@ -304,8 +304,8 @@ __global__ __launch_bounds__(kBasisThreads) void basis_lookup(GroupView gv, cons
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int gm;
-    const S rep = orbit_min<S>((S)states[i], gv, perm_lds, gm);
+    int gm, mult;
+    const S rep = orbit_min<S>((S)states[i], gv, perm_lds, gm, mult);
     rows[i] = row_of_rep<S>(rep, bin, dict);
     elems[i] = gm;
 }

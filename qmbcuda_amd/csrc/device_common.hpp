@@ -238,19 +238,24 @@ __device__ inline bool is_representative(S x, const GroupView &gv, const uint8_t
 
 // Orbit minimum of a state and the canonical element gm = the smallest g with
 // image_g(min) == x (the reference's orbit_indices entry, utils.cu:66-80, made
-// deterministic).
+// deterministic). mult = the number of elements mapping x to the minimum, i.e.
+// the stabiliser size of the minimum (those elements form a coset of x's
+// stabiliser, which has the same size as the minimum's).
 template <class S>
-__device__ inline S orbit_min(S x, const GroupView &gv, const uint8_t *perm_lds, int &gm) {
+__device__ inline S orbit_min(S x, const GroupView &gv, const uint8_t *perm_lds, int &gm, int &mult) {
     S best = x;
     int bi = gv.inv[0];
+    mult = 1;
     for (int g = 1; g < gv.G; ++g) {
         const S y = apply_elem<S>(x, g, gv, perm_lds);
         const int ig = gv.inv[g];
         if (y < best) {
             best = y;
             bi = ig;
-        } else if (y == best && ig < bi) {
-            bi = ig;
+            mult = 1;
+        } else if (y == best) {
+            ++mult;
+            if (ig < bi) bi = ig;
         }
     }
     gm = bi;
@@ -264,7 +269,7 @@ __device__ inline S orbit_min(S x, const GroupView &gv, const uint8_t *perm_lds,
 // block tmin over all (j, s, f), and only the elements reaching tmin need full
 // images. Same result and same gm as orbit_min.
 template <class S>
-__device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, int &gm) {
+__device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, int &gm, int &mult) {
     const int B = gv.block, nb = gv.n_sites / B;
     const unsigned mB = (1u << B) - 1u;
     const int nf = gv.lut_flip ? 2 : 1;
@@ -276,6 +281,7 @@ __device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, i
     }
     S best = 0;
     int bi = -1;
+    mult = 0;
     for (int j = 0; j < nb; ++j) {
         const unsigned v0 = (unsigned)(x >> (j * B)) & mB;
         const int a = nb - 1 - j;
@@ -289,9 +295,13 @@ __device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, i
                 const int g = t.lut[512 + (f * 8 + a) * 8 + sh];
                 const S y = apply_elem_div<S>(x, g, gv, t, nullptr);
                 const int ig = t.inv[g];
-                if (bi < 0 || y < best || (y == best && ig < bi)) {
+                if (bi < 0 || y < best) {
                     best = y;
                     bi = ig;
+                    mult = 1;
+                } else if (y == best) {
+                    ++mult;
+                    if (ig < bi) bi = ig;
                 }
             }
         }
@@ -346,13 +356,14 @@ template <class S>
 __device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint64_t &c1);
 
 template <class S>
-__device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, int &gm) {
+__device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, int &gm, int &mult) {
     const int N = gv.n_sites;
     const bool hf = gv.lut_flip != 0;
     uint64_t c0, c1;
     ring_windows<S>(x, N, hf, c0, c1);
     S best = 0;
     int bi = -1;
+    mult = 0;
     for (int f = 0; f < (hf ? 2 : 1); ++f) {
         uint64_t c = f ? c1 : c0;
         while (c) {
@@ -363,9 +374,13 @@ __device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, 
             const int g = t.lut[512 + f * 64 + s];
             const S y = apply_elem_div<S>(x, g, gv, t, nullptr);
             const int ig = t.inv[g];
-            if (bi < 0 || y < best || (y == best && ig < bi)) {
+            if (bi < 0 || y < best) {
                 best = y;
                 bi = ig;
+                mult = 1;
+            } else if (y == best) {
+                ++mult;
+                if (ig < bi) bi = ig;
             }
         }
     }
@@ -407,12 +422,13 @@ __device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint6
 // of their candidate rotations. Same result and same gm as orbit_min.
 template <class S>
 __device__ inline S orbit_min_coset(S x, const GroupView &gv, const GroupLds &t, const uint8_t *perm_lds,
-                                    int &gm) {
+                                    int &gm, int &mult) {
     const int N = gv.n_sites;
     const S full = (S)gv.full;
     const bool hf = gv.lut_flip != 0;
     S best = 0;
     int bi = -1;
+    mult = 0;
     unsigned tbest = 0x1FFu;
     for (int ci = 0; ci < gv.n_cos; ++ci) {
         const uint8_t *ct = t.cos + ci * 128;
@@ -432,9 +448,13 @@ __device__ inline S orbit_min_coset(S x, const GroupView &gv, const GroupLds &t,
                 S z = rotl_ring<S>(y, s, N, full);
                 if (f) z ^= full;
                 const int ig = t.inv[g];
-                if (bi < 0 || z < best || (z == best && ig < bi)) {
+                if (bi < 0 || z < best) {
                     best = z;
                     bi = ig;
+                    mult = 1;
+                } else if (z == best) {
+                    ++mult;
+                    if (ig < bi) bi = ig;
                 }
             }
         }

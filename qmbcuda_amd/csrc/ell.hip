@@ -79,7 +79,8 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
     double dr = 0.0, di = 0.0;
     if (!null_row) {
         const S src = (S)reps[r];
-        const double n_src = s_nrm[stab[r]];
+        const int m_src = stab[r];
+        const double n_src = s_nrm[m_src];
         // Terms are taken 64 at a time. Pass A applies the operators (cheap bit
         // tests, wave-uniform term index), adds diagonal terms in term order and
         // marks the lane's off-diagonal candidates; pass B pops each lane's
@@ -111,17 +112,19 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                 double cr, ci;
                 S tgt;
                 apply_term<S>(tv, h, src, tgt, cr, ci);
-                int gm;
-                const S rep = gv.lut == 1   ? orbit_min_lut<S>(tgt, gv, gt, gm)
-                              : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm)
-                              : gv.lut == 3 ? orbit_min_coset<S>(tgt, gv, gt, perm_lds, gm)
-                                            : orbit_min<S>(tgt, gv, perm_lds, gm);
+                int gm, mult;
+                const S rep = gv.lut == 1   ? orbit_min_lut<S>(tgt, gv, gt, gm, mult)
+                              : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm, mult)
+                              : gv.lut == 3 ? orbit_min_coset<S>(tgt, gv, gt, perm_lds, gm, mult)
+                                            : orbit_min<S>(tgt, gv, perm_lds, gm, mult);
                 const int64_t r2 = row_of_rep<S>(rep, bin, dict);
                 if (r2 < 0) continue;
                 if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
                 if (FILL) {
                     if (slot < L) {
-                        const double ratio = s_nrm[stab[r2]] / n_src;
+                        // mult = the target row's stabiliser size (no read of stab[r2]);
+                        // equal sizes give the ratio 1 exactly
+                        const double ratio = mult == m_src ? 1.0 : s_nrm[mult] / n_src;
                         const double2 x = s_chi[gm];
                         const double pr = cr * x.x - ci * x.y;
                         const double pi = cr * x.y + ci * x.x;

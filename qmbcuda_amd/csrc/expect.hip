@@ -96,15 +96,15 @@ __global__ __launch_bounds__(kExpThreads) void expect_rows(GroupView gv, const u
                     double cr, ci;
                     S tgt;
                     apply_term<S>(tv, h, u, tgt, cr, ci);
-                    int gm;
-                    const S rep2 = gv.lut == 1   ? orbit_min_lut<S>(tgt, gv, gt, gm)
-                                   : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm)
-                                   : gv.lut == 3 ? orbit_min_coset<S>(tgt, gv, gt, perm_lds, gm)
-                                                 : orbit_min<S>(tgt, gv, perm_lds, gm);
+                    int gm, mult;
+                    const S rep2 = gv.lut == 1   ? orbit_min_lut<S>(tgt, gv, gt, gm, mult)
+                                   : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm, mult)
+                                   : gv.lut == 3 ? orbit_min_coset<S>(tgt, gv, gt, perm_lds, gm, mult)
+                                                 : orbit_min<S>(tgt, gv, perm_lds, gm, mult);
                     const int64_t r2 = row_of_rep<S>(rep2, bin, dict);
                     if (r2 < 0) continue;
                     if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
-                    const double ratio = s_nrm[stab[r2]] / n_src;
+                    const double ratio = s_nrm[mult] / n_src;  // mult = the target row's stabiliser size
                     const double2 cg = s_chi[gm];
                     // f = c * conj(chi(j)) * chi(gm) * ratio
                     const double qr = cj.x * cg.x - cj.y * cg.y, qi = cj.x * cg.y + cj.y * cg.x;
