@@ -71,10 +71,13 @@ def gpu_step(ctx, group, table, cfg, tol):
     terms = q.Terms(cfg["n"], table)
     e = q.Ell(ctx, b, terms)
     t2 = time.perf_counter()
-    ev, rep, _, _ = q.lanczos(e, max_iter=1000, tol=tol, seed=2212456, profile=True)
+    # HIP events around every 8th fused SpMV (the kernel's duration for the roofline;
+    # events around every launch would add ~0.4 ms to the solve)
+    ev, rep, _, _ = q.lanczos(e, max_iter=1000, tol=tol, seed=2212456, profile=8)
     t3 = time.perf_counter()
     inf = e.info()
     return dict(e0=float(ev[0]), iters=int(rep.iters), n_spmv=int(rep.n_spmv), ms_spmv=float(rep.ms_spmv),
+                n_spmv_timed=int(rep.n_spmv_timed),
                 basis_ms=(t1 - t0) * 1e3, ell_ms=(t2 - t1) * 1e3, lanczos_ms=(t3 - t2) * 1e3,
                 total_ms=(t3 - t0) * 1e3, n_rows=int(inf.n_rows), nnz=int(inf.nnz), stored=int(inf.stored),
                 spmv_bytes=float(inf.spmv_bytes), format_bytes=float(inf.format_bytes), packed=int(inf.packed),
@@ -448,7 +451,7 @@ def run_c5(args):
     t3 = time.perf_counter()
     inf = e.info()
     B_k1 = inf.spmv_bytes + 8.0 * inf.n_rows
-    t_k1 = rep.ms_spmv / rep.n_spmv * 1e-3
+    t_k1 = rep.ms_spmv / rep.n_spmv_timed * 1e-3
     out = {"metric": METRIC, "config": {"workload": "C5: 6x6 square Heisenberg Sz=0 k=(0,0) flip-even"},
            "value": round((t3 - t0) * 1e3, 1), "unit": "ms time-to-E0 (basis + ELL + Lanczos)",
            "higher_is_better": False, "e0": ev[0], "e0_per_site": ev[0] / 36, "iters": rep.iters,
@@ -544,7 +547,7 @@ def main():
     value = n_spmv * B / wall / 1e9
     # dominant kernel: fused Lanczos SpMV (K1); it also writes phi_n (8 B/row)
     B_k1 = B + 8.0 * runs[0]["n_rows"]
-    t_k1 = sum(r["ms_spmv"] for r in runs) / n_spmv * 1e-3
+    t_k1 = sum(r["ms_spmv"] for r in runs) / sum(r["n_spmv_timed"] for r in runs) * 1e-3
     achieved = B_k1 / t_k1 / 1e9
     e0 = runs[-1]["e0"]
     out = {

@@ -205,7 +205,7 @@ typedef struct {
     int32_t fixed_iter;  /* 1: run exactly max_iter steps, no test (reference semantics) */
     int32_t start;       /* QMB_START_RANDOM, QMB_START_USER or QMB_START_DEVICE */
     int32_t want_eigvec; /* 1: rebuild the ground-state vector (second pass) */
-    int32_t profile;     /* 1: time every SpMV with HIP events */
+    int32_t profile;     /* k > 0: time every k-th SpMV (steps 0, k, 2k, ...) with HIP events */
     double tol;          /* eigenvalue tolerance, relative to max(1,|E|) (0 -> 1e-13) */
     uint64_t seed;       /* start-vector seed (reference default 2212456) */
     const double *start_vec; /* start vector: host (USER) or device (DEVICE) pointer, n_rows
@@ -218,11 +218,11 @@ typedef struct {
     int32_t breakdown;   /* 1 if an invariant subspace was hit (exact result) */
     int32_t n_spmv;
     int32_t n_reorth;    /* steps that reorthogonalised */
-    int32_t reserved;
+    int32_t n_spmv_timed; /* SpMVs whose kernel time is in ms_spmv (profile) */
     double residual[8];  /* Ritz residuals of the returned eigenvalues */
     double eig_residual; /* ||H x - E x|| of the rebuilt vector (want_eigvec) */
     double ms_total;     /* wall time of the call */
-    double ms_spmv;      /* summed SpMV kernel time (profile) */
+    double ms_spmv;      /* summed kernel time of the timed SpMVs (profile) */
     double ms_host;      /* host tridiagonal work */
 } qmb_lanczos_report;
 

@@ -262,6 +262,8 @@ def default_opts(**kw) -> LanczosOpts:
 def lanczos(ell: Ell, n_eigs=1, max_iter=300, tol=1e-13, seed=2212456, reorth=None, check_every=10,
             fixed_iter=False, start_vec=None, want_eigvec=False, profile=False, allow_unconverged=False):
     """Lowest eigenvalue(s) of an ELL Hamiltonian on the GPU.
+    profile: False (off), True (time every SpMV) or k (every k-th) with HIP events;
+    report.ms_spmv sums the timed launches, report.n_spmv_timed counts them.
     Returns (evals, report, krylov, eigvec)."""
     if reorth is None:
         reorth = n_eigs > 1

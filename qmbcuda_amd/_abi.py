@@ -78,7 +78,7 @@ class LanczosReport(C.Structure):
         ("breakdown", C.c_int32),
         ("n_spmv", C.c_int32),
         ("n_reorth", C.c_int32),
-        ("reserved", C.c_int32),
+        ("n_spmv_timed", C.c_int32),
         ("residual", C.c_double * 8),
         ("eig_residual", C.c_double),
         ("ms_total", C.c_double),

@@ -637,7 +637,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     QMB_HIP(hipEventCreateWithFlags(&copy_done.e, hipEventDisableTiming));
     const hipEvent_t ev_copy = copy_done.e;
     double ms_spmv = 0.0, ms_host = 0.0;
-    int n_spmv = 0;
+    int n_spmv = 0, n_timed = 0;
 
     std::vector<double> ha(M), hb(M + 1);
     std::vector<double> w(k_eigs + 1), zl(k_eigs + 1), z0(M);
@@ -750,8 +750,10 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     bool predicted = false;
     int it_q = it;  // steps queued so far
     int wpar = 0;   // event half of the current window
+    // profile = k: steps 0, k, 2k, ... are timed
+    auto timed = [&](int step) { return o.profile > 0 && step % o.profile == 0; };
     auto queue_step = [&](int step, int slot, int half, bool finish) {
-        hipEvent_t *e2 = o.profile ? &ev[(size_t)half * 2 * check + 2 * slot] : nullptr;
+        hipEvent_t *e2 = timed(step) ? &ev[(size_t)half * 2 * check + 2 * slot] : nullptr;
         if (e2) QMB_HIP(hipEventRecord(e2[0], ctx->stream));
         k1(step, e2 ? e2[1] : nullptr);
         k2(step, finish);
@@ -775,10 +777,12 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         std::copy(st + M, st + M + it + 1, hb.begin());
         if (o.profile) {
             for (int q = 0; q < it - win0; ++q) {
+                if (!timed(win0 + q)) continue;
                 float t = 0.f;
                 const size_t i0 = (size_t)wpar * 2 * check + 2 * q;
                 QMB_HIP(hipEventElapsedTime(&t, ev[i0], ev[i0 + 1]));
                 ms_spmv += t;
+                ++n_timed;
             }
         }
         wpar ^= 1;
@@ -897,6 +901,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         rep->eig_residual = eig_res;
         rep->ms_total = wall.ms();
         rep->ms_spmv = ms_spmv;
+        rep->n_spmv_timed = n_timed;
         rep->ms_host = ms_host;
     }
     return converged ? QMB_OK : QMB_ERR_NOT_CONVERGED;

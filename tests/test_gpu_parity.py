@@ -276,6 +276,11 @@ def test_c2_full_size_majumdar_ghosh(ctx):
     ev, rep, _, _ = q.lanczos(e, max_iter=500, tol=1e-13)
     assert rep.converged
     assert abs(ev[0] + 10.5) < 1e-10 * 10.5
+    assert rep.n_spmv_timed == 0 and rep.ms_spmv == 0.0
+    # sampled SpMV timing (steps 0, 3, 6, ...) does not change the run
+    ev3, rep3, _, _ = q.lanczos(e, max_iter=500, tol=1e-13, profile=3)
+    assert ev3[0] == ev[0] and rep3.iters == rep.iters
+    assert rep3.n_spmv_timed == (rep3.iters + 2) // 3 and rep3.ms_spmv > 0.0
     # the representatives are the orbit minima: spot-check with the lookup
     reps, stab, _ = b.export()
     rows, el = b.lookup(reps[:: 997])
