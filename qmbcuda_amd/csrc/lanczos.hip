@@ -64,63 +64,86 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
     }
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    double s = 1.0;
-    if (LZ) {
-        if (bpart) {
-            // the previous step's ||w||^2 partials, summed by every block in the
-            // fixed order of finish_sum: b = sqrt(sum), s = 1/b (0 if b = 0)
-            const double t = sum_partials(bpart, n_bpart, lds);
-            if (threadIdx.x == 0) {
-                const double nb = sqrt(t);
-                const double inv = nb > 0.0 ? 1.0 / nb : 0.0;
-                s_scale = inv;
-                if (blockIdx.x == 0) {
-                    *beta_out = nb;
-                    *invb_out = inv;
-                }
-            }
-            __syncthreads();
-            s = s_scale;
-        } else {
-            s = *invb;
+    // Fused b[n]: the previous step's ||w||^2 partials, summed by every block in
+    // the fixed order of finish_sum (thread t adds partials t, t + blockDim, ...
+    // in order, then the block sum): b = sqrt(sum), s = 1/b (0 if b = 0). The
+    // first kPre partials of each thread are loaded at once, and s is formed
+    // after the block's first chunk product (which does not need it), so the
+    // sum's latency overlaps the SpMV.
+    constexpr int kPre = 4;
+    const bool fused = LZ && bpart != nullptr;
+    double pv[kPre];
+    if (fused) {
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) {
+            const int i = (int)threadIdx.x + u * (int)blockDim.x;
+            pv[u] = i < n_bpart ? __builtin_nontemporal_load(bpart + i) : 0.0;
         }
     }
+    double s = (LZ && !fused) ? *invb : 1.0;
+    auto form_scale = [&]() {
+        double t = 0.0;
+#pragma unroll
+        for (int u = 0; u < kPre; ++u)
+            if ((int)threadIdx.x + u * (int)blockDim.x < n_bpart) t += pv[u];
+        for (int i = (int)threadIdx.x + kPre * (int)blockDim.x; i < n_bpart; i += blockDim.x)
+            t += __builtin_nontemporal_load(bpart + i);
+        t = block_sum(t, lds);
+        if (threadIdx.x == 0) {
+            const double nb = sqrt(t);
+            const double inv = nb > 0.0 ? 1.0 / nb : 0.0;
+            s_scale = inv;
+            if (blockIdx.x == 0) {
+                *beta_out = nb;
+                *invb_out = inv;
+            }
+        }
+        __syncthreads();
+        s = s_scale;
+    };
     double dot = 0.0;
-    for (int64_t c = (int64_t)blockIdx.x * kWaves + wv; c < n_chunks; c += (int64_t)gridDim.x * kWaves) {
+    // row product of chunk c (unscaled) and its output with the scale s
+    auto product = [&](int64_t c, double2 &acc, double2 &xr) {
         const int64_t r = c * 64 + lane;
         const int64_t base = chunk_ptr[c];
         const int L = (int)((chunk_ptr[c + 1] - base) >> 6);
         const int len = r < n_rows ? (int)row_len[r] : 0;
         const int64_t p0 = base + lane;
         if (!CPLX) {
-            double acc;
-            double xr = 0.0;
             if (PACK) {
-                xr = r < n_rows ? x[r] : 0.0;
+                xr.x = r < n_rows ? x[r] : 0.0;
                 const double d = r < n_rows ? diag[r] : 0.0;
-                acc = sell_row_packed(reinterpret_cast<const uint32_t *>(col), s_vt, x, p0, L, len, cb, d, xr);
+                acc.x = sell_row_packed(reinterpret_cast<const uint32_t *>(col), s_vt, x, p0, L, len, cb, d, xr.x);
             } else {
-                acc = sell_row_real(col, val, x, p0, L, len);
+                acc.x = sell_row_real(col, val, x, p0, L, len);
+                if (LZ) xr.x = r < n_rows ? x[r] : 0.0;
             }
-            if (LZ) acc *= s;
+        } else {
+            xr = r < n_rows ? reinterpret_cast<const double2 *>(x)[r] : make_double2(0.0, 0.0);
+            if (PACK) {
+                const double2 d = r < n_rows ? reinterpret_cast<const double2 *>(diag)[r] : make_double2(0.0, 0.0);
+                acc = sell_row_cplx_packed(reinterpret_cast<const uint32_t *>(col), s_vt, s_vt + kMaxValues, x, p0,
+                                           L, len, cb, rb, d, xr);
+            } else {
+                acc = sell_row_cplx(col, val, x, p0, L, len);
+            }
+        }
+    };
+    auto emit = [&](int64_t c, double2 acc, double2 xr) {
+        const int64_t r = c * 64 + lane;
+        if (!CPLX) {
+            double a = acc.x;
+            if (LZ) a *= s;
             if (r < n_rows) {
-                y[r] = acc;
+                y[r] = a;
                 if (LZ) {
-                    const double vr = (PACK ? xr : x[r]) * s;
+                    const double vr = xr.x * s;
                     vout[r] = vr;
-                    dot += vr * acc;
+                    dot += vr * a;
                 }
             }
         } else {
-            double2 a;
-            if (PACK) {
-                const double2 xr = r < n_rows ? reinterpret_cast<const double2 *>(x)[r] : make_double2(0.0, 0.0);
-                const double2 d = r < n_rows ? reinterpret_cast<const double2 *>(diag)[r] : make_double2(0.0, 0.0);
-                a = sell_row_cplx_packed(reinterpret_cast<const uint32_t *>(col), s_vt, s_vt + kMaxValues, x, p0, L,
-                                         len, cb, rb, d, xr);
-            } else {
-                a = sell_row_cplx(col, val, x, p0, L, len);
-            }
+            double2 a = acc;
             if (LZ) {
                 a.x *= s;
                 a.y *= s;
@@ -128,7 +151,7 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
             if (r < n_rows) {
                 reinterpret_cast<double2 *>(y)[r] = a;
                 if (LZ) {
-                    double2 vr = reinterpret_cast<const double2 *>(x)[r];
+                    double2 vr = xr;
                     vr.x = vr.x * s;
                     vr.y = vr.y * s;
                     reinterpret_cast<double2 *>(vout)[r] = vr;
@@ -136,6 +159,22 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                 }
             }
         }
+    };
+    const int64_t cstride = (int64_t)gridDim.x * kWaves;
+    int64_t c = (int64_t)blockIdx.x * kWaves + wv;
+    if (fused) {
+        // every thread joins the block sum, with or without a chunk
+        double2 acc0 = make_double2(0.0, 0.0), xr0 = make_double2(0.0, 0.0);
+        const bool have = c < n_chunks;
+        if (have) product(c, acc0, xr0);
+        form_scale();
+        if (have) emit(c, acc0, xr0);
+        c += cstride;
+    }
+    for (; c < n_chunks; c += cstride) {
+        double2 acc = make_double2(0.0, 0.0), xr = make_double2(0.0, 0.0);
+        product(c, acc, xr);
+        emit(c, acc, xr);
     }
     if (LZ) {
         const double t = block_sum(dot, lds);
