@@ -245,38 +245,34 @@ __global__ __launch_bounds__(kEllThreads) void ell_collect_values(int64_t n_rows
 
 // One block: the distinct values of the set in ascending bit order, +0.0 first
 // (the padding value), into keys[0 .. n); meta = {n, ok}, ok = 0 when the set
-// overflowed or n exceeds cap.
+// overflowed or n exceeds cap. The non-zero keys are gathered into LDS, then
+// each goes to 1 + (number of smaller keys): the keys are distinct, so the
+// ranks are a permutation.
 __global__ __launch_bounds__(1024) void ell_sort_values(const unsigned long long *vset, const int *vflag,
                                                         unsigned long long *keys, int *meta, int cap) {
     __shared__ unsigned long long a[kSetSlots];
-    for (int i = threadIdx.x; i < kSetSlots; i += blockDim.x) a[i] = vset[i];
+    __shared__ int m;
+    if (threadIdx.x == 0) m = 0;
     __syncthreads();
-    for (int k = 2; k <= kSetSlots; k <<= 1) {  // bitonic sort, empty slots (all ones) last
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < kSetSlots; i += blockDim.x) {
-                const int ij = i ^ j;
-                if (ij > i) {
-                    const bool up = (i & k) == 0;
-                    const unsigned long long x = a[i], y = a[ij];
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[ij] = x;
-                    }
-                }
-            }
-            __syncthreads();
+    for (int i = threadIdx.x; i < kSetSlots; i += blockDim.x) {
+        const unsigned long long k = vset[i];
+        if (k != kEmptyKey && k != 0ull) a[atomicAdd(&m, 1)] = k;
+    }
+    __syncthreads();
+    const int n = m + 1;
+    const bool ok = vflag[0] == 0 && n <= cap;
+    if (ok) {
+        for (int i = threadIdx.x; i < m; i += blockDim.x) {
+            const unsigned long long k = a[i];
+            int rank = 0;
+            for (int j = 0; j < m; ++j) rank += a[j] < k ? 1 : 0;
+            keys[1 + rank] = k;
         }
     }
     if (threadIdx.x == 0) {
-        int n = 1;
         keys[0] = 0ull;
-        for (int i = 0; i < kSetSlots && a[i] != kEmptyKey; ++i) {
-            if (a[i] == 0ull) continue;
-            if (n < cap) keys[n] = a[i];
-            ++n;
-        }
         meta[0] = n;
-        meta[1] = (vflag[0] == 0 && n <= cap) ? 1 : 0;
+        meta[1] = ok ? 1 : 0;
     }
 }
 
@@ -292,6 +288,7 @@ __global__ __launch_bounds__(kEllThreads) void ell_pack_values(int64_t n_rows, i
     if (!meta[1]) return;
     const int n_keys = meta[0];
     for (int i = threadIdx.x; i < n_keys; i += blockDim.x) s_keys[i] = keys[i];
+    const int top = n_keys > 1 ? 1 << (31 - __clz(n_keys - 1)) : 0;  // largest power of two <= n_keys - 1
     __syncthreads();
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_lanes;
          r += (int64_t)gridDim.x * blockDim.x) {
@@ -302,23 +299,33 @@ __global__ __launch_bounds__(kEllThreads) void ell_pack_values(int64_t n_rows, i
         const int len = r < n_rows ? (int)row_len[r] : 0;
         if (r < n_rows) diag[r] = (len > 0 && L > 0) ? val[base + lane] : 0.0;
         if (L > 0) colw[base + lane] = (uint32_t)(r < n_rows ? r : 0);
-        for (int k = 1; k < L; ++k) {
-            const int64_t p = base + (int64_t)k * kChunk + lane;
-            uint32_t idx = 0;
-            if (k < len) {
-                const unsigned long long key = (unsigned long long)__double_as_longlong(val[p]);
-                int lo = 0, hi = n_keys - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s_keys[mid] < key)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                if (s_keys[lo] != key) *missing = 1;
-                idx = (uint32_t)lo;
+        // four slots at a time, their searches interleaved: the last index whose
+        // key is <= the value (keys[0] = 0 is <= every bit pattern), by a
+        // branch-free descent over powers of two
+        for (int k = 1; k < L; k += 4) {
+            unsigned long long key[4];
+            uint32_t w[4];
+            int pos[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t p = base + (int64_t)(k + u) * kChunk + lane;
+                w[u] = k + u < L ? colw[p] : 0u;
+                key[u] = k + u < len ? (unsigned long long)__double_as_longlong(val[p]) : 0ull;
+                pos[u] = 0;
             }
-            colw[p] = colw[p] | (idx << cb);
+            for (int step = top; step > 0; step >>= 1) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = pos[u] + step;
+                    if (q < n_keys && s_keys[q] <= key[u]) pos[u] = q;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k + u >= L) continue;
+                if (k + u < len && s_keys[pos[u]] != key[u]) *missing = 1;
+                colw[base + (int64_t)(k + u) * kChunk + lane] = w[u] | ((uint32_t)pos[u] << cb);
+            }
         }
     }
 }
