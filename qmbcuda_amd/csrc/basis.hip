@@ -73,54 +73,59 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
     S x = colex_unrank<S>(r0, n_up, gv.n_sites, bin);
     uint64_t bits = 0;
     if (gv.lut == 2) {
-        // Ring group: the images are the rotations by s = 1 .. N-1 (each one bit
-        // more than the last, computed in registers) and, with the spin flip,
-        // their complements (s = 0: the flip alone). Element indices are looked
-        // up only for images equal to x (stabiliser).
+        // Ring group, two passes. Pass 1 keeps the states that are not above
+        // their rotations by 1, 2 and 3 sites (nor, with the spin flip, above
+        // their complements or theirs): a necessary condition, branch-free.
+        // Pass 2 takes each lane's survivors in turn: the rotations by
+        // s = 1 .. N-1 one bit at a time (and their complements), exit at the
+        // first smaller image; element indices are looked up only for images
+        // equal to the state (stabiliser).
         const int N = gv.n_sites;
         const S full = (S)gv.full;
         const bool hf = gv.lut_flip != 0;
-        int i = 0, st = 0;
+        auto rotl = [&](S v, int k) { return (S)(((v << k) | (v >> (N - k))) & full); };
+        uint64_t surv = 0;
         S y = x;
-        uint64_t slo = 1, shi = 0;
-        auto mark = [&](int f, int sh) {
-            const int g = gt.lut[512 + f * 64 + sh];
-            if (g < 64)
-                slo |= 1ULL << g;
-            else
-                shi |= 1ULL << (g - 64);
-        };
-        while (i < cnt) {
-            bool done = false, rep = false;
-            if (st >= N) {
-                done = true;
-                rep = true;
-            } else {
+        for (int i = 0; i < cnt; ++i) {
+            const S a1 = rotl(y, 1), a2 = rotl(y, 2), a3 = rotl(y, 3);
+            bool ok = a1 >= y && a2 >= y && a3 >= y;
+            if (hf) ok = ok && (S)(y ^ full) >= y && (S)(a1 ^ full) >= y && (S)(a2 ^ full) >= y && (S)(a3 ^ full) >= y;
+            if (ok) surv |= 1ULL << i;
+            if (i + 1 < cnt) y = next_comb<S>(y);
+        }
+        y = x;
+        int at = 0;
+        while (surv) {
+            const int i = __builtin_ctzll(surv);
+            surv &= surv - 1;
+            for (; at < i; ++at) y = next_comb<S>(y);
+            uint64_t slo = 1, shi = 0;  // element 0 (identity) fixes y
+            auto mark = [&](int f, int sh) {
+                const int g = gt.lut[512 + f * 64 + sh];
+                if (g < 64)
+                    slo |= 1ULL << g;
+                else
+                    shi |= 1ULL << (g - 64);
+            };
+            bool rep = true;
+            S z = y;
+            for (int st = 0; st < N && rep; ++st) {
                 if (st > 0) {
-                    y = (S)(((y << 1) | (y >> (N - 1))) & full);
-                    if (y < x)
-                        done = true;
-                    else if (y == x)
+                    z = rotl(z, 1);
+                    if (z < y)
+                        rep = false;
+                    else if (z == y)
                         mark(0, st);
                 }
-                if (!done && hf) {
-                    const S yf = (S)(y ^ full);
-                    if (yf < x)
-                        done = true;
-                    else if (yf == x)
+                if (rep && hf) {
+                    const S zf = (S)(z ^ full);
+                    if (zf < y)
+                        rep = false;
+                    else if (zf == y)
                         mark(1, st);
                 }
-                ++st;
             }
-            if (done) {
-                if (rep && (!compact || compatible(slo, shi, triv_lo, triv_hi))) bits |= 1ULL << i;
-                ++i;
-                if (i < cnt) x = next_comb<S>(x);
-                y = x;
-                st = 0;
-                slo = 1;
-                shi = 0;
-            }
+            if (rep && (!compact || compatible(slo, shi, triv_lo, triv_hi))) bits |= 1ULL << i;
         }
         dict[2 * w] = bits;
         return;
