@@ -134,3 +134,72 @@ def test_gpu_point_group_sectors(ctx, label, mk, n_up):
         ev, rep, _, _ = q.lanczos(ec, max_iter=400, tol=1e-13)
         assert rep.converged
         assert abs(ev[0] - dense[0]) < 1e-10 * max(1.0, abs(dense[0]))
+
+
+def affine_ring_case(n=10, u=3, delta=0.7):
+    """Sites s -> u^k s + t (mod n), k = 0..3 (3 has order 4 mod 10), t = 0..n-1:
+    the rotations times four cosets (k = 0..3). The k = 2 coset holds the site
+    reversal (u^2 = -1 mod n); the k = 1, 3 cosets hold none, so their images
+    take the generic permutation path of the ring x coset orbit minimum. The
+    Hamiltonian couples every pair equally (XXZ, all to all), so every
+    permutation leaves it invariant. Irreps: trivial and (-1)^k."""
+    els, ks = [], []
+    for k in range(4):
+        m = pow(u, k, n)
+        for t in range(n):
+            els.append([(m * s + t) % n for s in range(n)])
+            ks.append(k)
+    chars = np.array([[1.0] * len(els), [(-1.0) ** k for k in ks]], dtype=np.complex128)
+    ops, sites, scal = [], [], []
+    for i in range(n):
+        for j in range(i + 1, n):
+            ops += [SM, SP]
+            sites += [i, j]
+            scal.append(0.5)
+            ops += [SP, SM]
+            sites += [i, j]
+            scal.append(0.5)
+            ops += [SZ, SZ]
+            sites += [i, j]
+            scal.append(delta)
+    table = (np.array(ops, np.int32), np.array(sites, np.int32), np.array(scal, np.complex128),
+             np.full(len(scal), 2, np.int32), 2)
+    return dict(n_sites=n, group_el=np.array(els, np.uint32), chars=chars, flip=None, table=table)
+
+
+def test_affine_ring_group_is_closed():
+    case = affine_ring_case()
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    assert g.valid_irreps().all()
+
+
+@pytest.mark.gpu
+def test_gpu_affine_ring_cosets(ctx):
+    """Ring rotations times generic coset representatives: bases and ELLs
+    bit-exact against the oracle, energies against dense diagonalisation."""
+    case = affine_ring_case()
+    n_up = 5
+    orb = oracle.Orbits(case["n_sites"], n_up, case["group_el"], case["chars"], flip=None)
+    g = q.Group(case["group_el"], case["chars"], None)
+    t = q.Terms(case["n_sites"], case["table"])
+    for a in range(2):
+        say(f"affine ring n=10 irrep={a}")
+        b = q.Basis(ctx, g, n_up, a, q.LAYOUT_REFERENCE)
+        reps, stab, nul = b.export()
+        assert np.array_equal(reps, orb.reps)
+        assert np.array_equal(stab.astype(np.int32), orb.stab)
+        assert np.array_equal(nul.astype(bool), orb.null_mask(a))
+        W = OM.max_terms_rule(len(case["table"][3]), n_up, case["n_sites"]) + 4
+        ref = oracle_ell({"table": case["table"], "chars": case["chars"]}, orb, a, max_terms=W)
+        e = q.Ell(ctx, b, t, value_type=q.REAL, max_terms=ref.W)
+        ind, val, track = e.export_reference()
+        assert np.array_equal(track, ref.track)
+        assert np.array_equal(ind, ref.ind)
+        assert np.array_equal(val.view(np.float64), ref.val)
+        keep = ~orb.null_mask(a)
+        if keep.any():
+            dense = np.linalg.eigvalsh(ref.dense()[np.ix_(keep, keep)])
+            bc = q.Basis(ctx, g, n_up, a, q.LAYOUT_COMPACT)
+            ec = q.Ell(ctx, bc, t, value_type=q.REAL, max_terms=ref.W)
+            ev, rep, _, _ = q.lanczos(ec, max_iter=300, tol=1e-13)
+            assert rep.converged and abs(ev[0] - dense[0]) < 1e-10 * max(1.0, abs(dense[0]))
