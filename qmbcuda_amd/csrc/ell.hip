@@ -58,12 +58,17 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
     __shared__ double2 s_chi[kMaxGroup];
     __shared__ double s_nrm[kMaxGroup + 1];
     __shared__ GroupLds gt;
-    stage_perm(perm_lds, gv);
-    stage_group(gt, gv);
-    stage_binom(bin, binom);
-    for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
-    for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
-    __syncthreads();
+    // the count pass over a sector without null orbits needs no orbit lookups
+    // (row length = 1 + the acting off-diagonal terms): no tables to stage
+    const bool lookups = FILL || !all_rows;
+    if (lookups) {
+        stage_perm(perm_lds, gv);
+        stage_group(gt, gv);
+        stage_binom(bin, binom);
+        for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
+        for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
+        __syncthreads();
+    }
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rows) return;
     const int64_t c = r / kChunk;
@@ -79,8 +84,8 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
     double dr = 0.0, di = 0.0;
     if (!null_row) {
         const S src = (S)reps[r];
-        const int m_src = stab[r];
-        const double n_src = s_nrm[m_src];
+        const int m_src = FILL ? (int)stab[r] : 0;
+        const double n_src = FILL ? s_nrm[m_src] : 0.0;
         // Terms are taken 64 at a time. Pass A applies the operators (cheap bit
         // tests, wave-uniform term index), adds diagonal terms in term order and
         // marks the lane's off-diagonal candidates; pass B pops each lane's
