@@ -16,7 +16,9 @@
  *    group, an ELL its basis, a network its context and sample bases), so
  *    *_destroy calls may come in any order; host arrays belong to the caller;
  *  - a handle is used by one host thread at a time; every handle made from a
- *    context runs on that context's HIP stream;
+ *    context runs on that context's HIP stream, and a call may return while its
+ *    last kernels still run there (every call that reads their results is
+ *    ordered after them on the stream or synchronises it first);
  *  - states are uint64 (at most 64 sites); Lanczos runs in FP64.
  *
  * Conventions kept from the reference (bit-exact):

@@ -420,8 +420,10 @@ qmb_status qmb_basis_build(qmb_ctx *ctx, const qmb_group *g, int n_up, int irrep
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>());
             QMB_HIP(hipGetLastError());
         }
-        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        // the compact layout returns with the compaction still queued (the ELL
+        // build and every reader order after it on the stream)
         if (layout == QMB_LAYOUT_REFERENCE && rows > 0) {
+            QMB_HIP(hipStreamSynchronize(ctx->stream));
             std::vector<uint8_t> nf(rows);
             QMB_HIP(hipMemcpy(nf.data(), b->nullf.p, rows, hipMemcpyDeviceToHost));
             int64_t nn = 0;
