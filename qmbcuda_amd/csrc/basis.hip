@@ -166,56 +166,57 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
         return;
     }
     if (gv.lut == 3) {
-        // Ring rotations times cosets: per coset c_i one permutation image
-        // y = c_i x, then its rotations one bit at a time, as above.
+        // Ring rotations times cosets, two passes as for the ring: the same
+        // branch-free filter (rotations by 1-3 sites and complements, images
+        // under the ring subgroup), then each lane's survivors: per coset c_i
+        // one image y = c_i x, then its rotations one bit at a time, exit at
+        // the first smaller image.
         const int N = gv.n_sites;
         const S full = (S)gv.full;
         const bool hf = gv.lut_flip != 0;
-        int i = 0, ci = 0, st = 0;
+        auto rotl = [&](S v, int k) { return (S)(((v << k) | (v >> (N - k))) & full); };
+        uint64_t surv = 0;
         S y = x;
-        uint64_t slo = 0, shi = 0;
-        auto mark = [&](int g) {
-            if (g < 64)
-                slo |= 1ULL << g;
-            else
-                shi |= 1ULL << (g - 64);
-        };
-        while (i < cnt) {
-            bool done = false, rep = false;
-            if (ci >= gv.n_cos) {
-                done = true;
-                rep = true;
-            } else {
-                const uint8_t *ct = gt.cos + ci * 128;
-                if (st == 0)
-                    y = ci ? coset_image<S>(x, ci, gv, gt, perm_lds) : x;
+        for (int i = 0; i < cnt; ++i) {
+            const S a1 = rotl(y, 1), a2 = rotl(y, 2), a3 = rotl(y, 3);
+            bool ok = a1 >= y && a2 >= y && a3 >= y;
+            if (hf) ok = ok && (S)(y ^ full) >= y && (S)(a1 ^ full) >= y && (S)(a2 ^ full) >= y && (S)(a3 ^ full) >= y;
+            if (ok) surv |= 1ULL << i;
+            if (i + 1 < cnt) y = next_comb<S>(y);
+        }
+        y = x;
+        int at = 0;
+        while (surv) {
+            const int i = __builtin_ctzll(surv);
+            surv &= surv - 1;
+            for (; at < i; ++at) y = next_comb<S>(y);
+            uint64_t slo = 0, shi = 0;
+            auto mark = [&](int g) {
+                if (g < 64)
+                    slo |= 1ULL << g;
                 else
-                    y = rotl_ring<S>(y, 1, N, full);
-                if (y < x)
-                    done = true;
-                else if (y == x)
-                    mark(ct[st]);
-                if (!done && hf) {
-                    const S yf = (S)(y ^ full);
-                    if (yf < x)
-                        done = true;
-                    else if (yf == x)
-                        mark(ct[64 + st]);
-                }
-                if (++st == N) {
-                    st = 0;
-                    ++ci;
+                    shi |= 1ULL << (g - 64);
+            };
+            bool rep = true;
+            for (int ci = 0; ci < gv.n_cos && rep; ++ci) {
+                const uint8_t *ct = gt.cos + ci * 128;
+                S z = ci ? coset_image<S>(y, ci, gv, gt, perm_lds) : y;
+                for (int st = 0; st < N && rep; ++st) {
+                    if (st > 0) z = rotl(z, 1);
+                    if (z < y)
+                        rep = false;
+                    else if (z == y)
+                        mark(ct[st]);
+                    if (rep && hf) {
+                        const S zf = (S)(z ^ full);
+                        if (zf < y)
+                            rep = false;
+                        else if (zf == y)
+                            mark(ct[64 + st]);
+                    }
                 }
             }
-            if (done) {
-                if (rep && (!compact || compatible(slo, shi, triv_lo, triv_hi))) bits |= 1ULL << i;
-                ++i;
-                if (i < cnt) x = next_comb<S>(x);
-                ci = 0;
-                st = 0;
-                slo = 0;
-                shi = 0;
-            }
+            if (rep && (!compact || compatible(slo, shi, triv_lo, triv_hi))) bits |= 1ULL << i;
         }
         dict[2 * w] = bits;
         return;
