@@ -789,6 +789,9 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     bool predicted = false;
     int it_q = it;  // steps queued so far
     int wpar = 0;   // event half of the current window
+    // the staging buffer at its largest size up front: it is never reallocated
+    // while a step is queued
+    ctx->host_stage<double>((size_t)(2 * M + 1) * sizeof(double));
     // profile = k: steps 0, k, 2k, ... are timed
     auto timed = [&](int step) { return o.profile > 0 && step % o.profile == 0; };
     auto queue_step = [&](int step, int slot, int half, bool finish) {
