@@ -30,6 +30,9 @@ qmb::GroupView qmb_basis::view() const {
     v.torus = group->torus;
     v.block = group->block;
     v.full = group->n_sites == 64 ? ~0ULL : ((1ULL << group->n_sites) - 1ULL);
+    v.rep_unit = 0;
+    if (group->torus)
+        for (int b0 = 0; b0 < group->n_sites; b0 += group->block) v.rep_unit |= 1ULL << b0;
     const uint8_t *t = g_tab.as<uint8_t>();
     v.keep = reinterpret_cast<const uint64_t *>(t);
     v.small = t + o_small;

@@ -221,6 +221,27 @@ __device__ inline bool is_representative(S x, const GroupView &gv, const uint8_t
         stabiliser_coset<S>(x, gv, perm_lds, stab_lo, stab_hi);
         return true;  // callers pass representatives only
     }
+    if (gv.lut == 2) {
+        // ring: the rotations one bit at a time (and their complements); the
+        // element table is read only for images equal to x
+        const int N = gv.n_sites;
+        const S full = (S)gv.full;
+        stab_lo = 0;
+        stab_hi = 0;
+        S y = x;
+        for (int st = 0; st < N; ++st) {
+            if (st) y = rotl_ring<S>(y, 1, N, full);
+            for (int f = 0; f < (gv.lut_flip ? 2 : 1); ++f) {
+                if ((f ? (S)(y ^ full) : y) != x) continue;
+                const int g = gv.lut_tab[512 + f * 64 + st];
+                if (g < 64)
+                    stab_lo |= 1ULL << g;
+                else
+                    stab_hi |= 1ULL << (g - 64);
+            }
+        }
+        return true;  // callers pass representatives only
+    }
     stab_lo = 0;
     stab_hi = 0;
     for (int g = 0; g < gv.G; ++g) {
@@ -270,8 +291,9 @@ __device__ inline S orbit_min(S x, const GroupView &gv, const uint8_t *perm_lds,
 // images. Same result and same gm as orbit_min.
 template <class S>
 __device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, int &gm, int &mult) {
-    const int B = gv.block, nb = gv.n_sites / B;
+    const int B = gv.block, N = gv.n_sites, nb = N / B;
     const unsigned mB = (1u << B) - 1u;
+    const S full = (S)gv.full;
     const int nf = gv.lut_flip ? 2 : 1;
     unsigned tmin = 0xFFu;
     for (int j = 0; j < nb; ++j) {
@@ -289,18 +311,27 @@ __device__ inline S orbit_min_lut(S x, const GroupView &gv, const GroupLds &t, i
             const unsigned v = f ? (v0 ^ mB) : v0;
             if (t.lut[v] != tmin) continue;
             unsigned sm = t.lut[256 + v];
+            const int big = a * B;
             while (sm) {
                 const int sh = __builtin_ctz(sm);
                 sm &= sm - 1;
-                const int g = t.lut[512 + (f * 8 + a) * 8 + sh];
-                const S y = apply_elem_div<S>(x, g, gv, t, nullptr);
-                const int ig = t.inv[g];
+                // the element (in-block shift sh, block shift a, flip f) applied
+                // directly (as apply_elem): its tables are read only for a new
+                // or tied minimum
+                S y = x;
+                if (sh) {
+                    const S k = (S)((S)gv.rep_unit * (S)((1u << (B - sh)) - 1u));
+                    y = (S)(((y & k) << sh) | ((y & (S)~k & full) >> (B - sh)));
+                }
+                if (big) y = (S)(((y << big) | (y >> (N - big))) & full);
+                if (f) y ^= full;
                 if (bi < 0 || y < best) {
                     best = y;
-                    bi = ig;
+                    bi = t.inv[t.lut[512 + (f * 8 + a) * 8 + sh]];
                     mult = 1;
                 } else if (y == best) {
                     ++mult;
+                    const int ig = t.inv[t.lut[512 + (f * 8 + a) * 8 + sh]];
                     if (ig < bi) bi = ig;
                 }
             }

@@ -222,6 +222,7 @@ struct GroupView {
     int torus;       // 1: cyclic-translation fast path, 0: generic permutation
     int block;       // torus block width B
     uint64_t full;   // mask of n_sites bits
+    uint64_t rep_unit;  // torus: bit 0 of every block (sum of 1 << (b * block))
     const uint64_t *keep;  // [G] torus in-block no-wrap mask
     const uint8_t *small;  // [G] in-block rotation
     const uint8_t *big;    // [G] whole-word rotation
