@@ -389,6 +389,7 @@ __device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint6
 template <class S>
 __device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, int &gm, int &mult) {
     const int N = gv.n_sites;
+    const S full = (S)gv.full;
     const bool hf = gv.lut_flip != 0;
     uint64_t c0, c1;
     ring_windows<S>(x, N, hf, c0, c1);
@@ -402,15 +403,17 @@ __device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, 
             c &= c - 1;
             int s = N - 8 - p;
             if (s < 0) s += N;
-            const int g = t.lut[512 + f * 64 + s];
-            const S y = apply_elem_div<S>(x, g, gv, t, nullptr);
-            const int ig = t.inv[g];
+            // rotation by s (complemented under f) in registers; the element
+            // tables are read only for a new or tied minimum
+            S y = rotl_ring<S>(x, s, N, full);
+            if (f) y ^= full;
             if (bi < 0 || y < best) {
                 best = y;
-                bi = ig;
+                bi = t.inv[t.lut[512 + f * 64 + s]];
                 mult = 1;
             } else if (y == best) {
                 ++mult;
+                const int ig = t.inv[t.lut[512 + f * 64 + s]];
                 if (ig < bi) bi = ig;
             }
         }
