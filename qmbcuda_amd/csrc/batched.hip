@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
         const int64_t p0 = base + lane;
         if (!CPLX) {
             const double xr = r < I.n_rows ? I.P[r] : 0.0;
-            const double acc = (I.packed ? sell_row_packed(reinterpret_cast<const uint32_t *>(I.col), I.vtab, I.P, p0,
+            const double acc = (I.packed ? sell_row_packed<true>(reinterpret_cast<const uint32_t *>(I.col), I.vtab, I.P, p0,
                                                            L, len, I.col_bits, r < I.n_rows ? I.diag[r] : 0.0, xr)
                                          : sell_row_real(I.col, I.val, I.P, p0, L, len)) *
                                s;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
         } else {
             const double2 xr = r < I.n_rows ? reinterpret_cast<const double2 *>(I.P)[r] : make_double2(0.0, 0.0);
             double2 a = I.packed
-                            ? sell_row_cplx_packed(reinterpret_cast<const uint32_t *>(I.col), I.vtab,
+                            ? sell_row_cplx_packed<true>(reinterpret_cast<const uint32_t *>(I.col), I.vtab,
                                                    I.vtab + kMaxValues, I.P, p0, L, len, I.col_bits, I.re_bits,
                                                    r < I.n_rows ? reinterpret_cast<const double2 *>(I.diag)[r]
                                                                 : make_double2(0.0, 0.0),

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ vtab, int n_values, int cb,
-                                                      int n_values_im, int rb,
+                                                      int n_values_im, int rb, int stream,
                                                       const double *__restrict__ x, double *__restrict__ y,
                                                       const double *__restrict__ invb, double *__restrict__ vout,
                                                       double *__restrict__ partials, const double *__restrict__ bpart,
@@ -113,7 +113,9 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
             if (PACK) {
                 xr.x = r < n_rows ? x[r] : 0.0;
                 const double d = r < n_rows ? diag[r] : 0.0;
-                acc.x = sell_row_packed(reinterpret_cast<const uint32_t *>(col), s_vt, x, p0, L, len, cb, d, xr.x);
+                const uint32_t *wd = reinterpret_cast<const uint32_t *>(col);
+                acc.x = stream ? sell_row_packed<true>(wd, s_vt, x, p0, L, len, cb, d, xr.x)
+                               : sell_row_packed<false>(wd, s_vt, x, p0, L, len, cb, d, xr.x);
             } else {
                 acc.x = sell_row_real(col, val, x, p0, L, len);
                 if (LZ) xr.x = r < n_rows ? x[r] : 0.0;
@@ -122,8 +124,9 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
             xr = r < n_rows ? reinterpret_cast<const double2 *>(x)[r] : make_double2(0.0, 0.0);
             if (PACK) {
                 const double2 d = r < n_rows ? reinterpret_cast<const double2 *>(diag)[r] : make_double2(0.0, 0.0);
-                acc = sell_row_cplx_packed(reinterpret_cast<const uint32_t *>(col), s_vt, s_vt + kMaxValues, x, p0,
-                                           L, len, cb, rb, d, xr);
+                const uint32_t *wd = reinterpret_cast<const uint32_t *>(col);
+                acc = stream ? sell_row_cplx_packed<true>(wd, s_vt, s_vt + kMaxValues, x, p0, L, len, cb, rb, d, xr)
+                             : sell_row_cplx_packed<false>(wd, s_vt, s_vt + kMaxValues, x, p0, L, len, cb, rb, d, xr);
             } else {
                 acc = sell_row_cplx(col, val, x, p0, L, len);
             }
@@ -408,10 +411,17 @@ struct FusedBeta {
     double *partials = nullptr;  // where this SpMV's a[n] partials go
 };
 
+// MI355X's memory-side cache (MALL) holds 256 MB: a packed matrix that fits it
+// beside the Lanczos vectors is read with plain loads, so its lines stay cached
+// from one SpMV to the next; a larger one is streamed (nontemporal loads).
+constexpr double kMallBytes = 256.0 * 1024 * 1024;
+
 void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, bool lz, const double *invb,
                  double *vout, LzWork *wk, double *alpha_out, hipEvent_t after = nullptr,
                  const FusedBeta *fb = nullptr) {
     const unsigned blocks = grid_for(e->n_chunks, kWaves);
+    const double vec_bytes = (e->value_type == QMB_COMPLEX ? 16.0 : 8.0) * (double)e->n_rows;
+    const int stream = e->format_bytes() + 8.0 * vec_bytes > 0.75 * kMallBytes ? 1 : 0;
     const bool cplx = e->value_type == QMB_COMPLEX;
     auto k = cplx ? (e->packed ? (lz ? spmv_sell<true, true, true> : spmv_sell<true, false, true>)
                                : (lz ? spmv_sell<true, true, false> : spmv_sell<true, false, false>))
@@ -420,7 +430,7 @@ void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, boo
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, ctx->stream, e->n_rows, e->n_chunks,
                        e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
                        e->val.as<double>(), e->diag.as<double>(), e->vtab.as<double>(), e->n_values, e->col_bits,
-                       e->n_values_im, e->re_bits, x, y, invb, vout, fb ? fb->partials : wk ? wk->partials.as<double>() : nullptr,
+                       e->n_values_im, e->re_bits, stream, x, y, invb, vout, fb ? fb->partials : wk ? wk->partials.as<double>() : nullptr,
                        fb ? fb->bpart : nullptr, fb ? fb->n_bpart : 0, fb ? fb->beta_out : nullptr,
                        fb ? fb->invb_out : nullptr);
     QMB_HIP(hipGetLastError());

@@ -14,6 +14,16 @@
 
 namespace qmb {
 
+// Matrix-stream load: nontemporal (streamed past the caches) when NT, else
+// plain (the lines may stay in L2 and the MALL for the next SpMV).
+template <bool NT, class T>
+__device__ inline T stream_load(const T *p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 // sum over slots k < len of val[p0 + 64k] * x[col[p0 + 64k]] (L = chunk slots)
 __device__ inline double sell_row_real(const int32_t *__restrict__ col, const double *__restrict__ val,
                                        const double *__restrict__ x, int64_t p0, int L, int len) {
@@ -65,6 +75,7 @@ __device__ inline double sell_row_real(const int32_t *__restrict__ col, const do
 // Packed real rows: slot 0 is the diagonal d (column = the row r itself); slot
 // k >= 1 holds w = col | (index << cb) with the value vt[index]. Same products in
 // the same order as sell_row_real on the unpacked matrix.
+template <bool NT>
 __device__ inline double sell_row_packed(const uint32_t *__restrict__ w, const double *vt,
                                          const double *__restrict__ x, int64_t p0, int L, int len, int cb,
                                          double d, double xr) {
@@ -78,7 +89,7 @@ __device__ inline double sell_row_packed(const uint32_t *__restrict__ w, const d
     if (n4 > 0) {
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (1 + u < len) jn[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(1 + u) * 64);
+            if (1 + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(1 + u) * 64);
     }
     int k = 1;
     for (int q = 0; q < n4; ++q, k += 4) {
@@ -88,7 +99,7 @@ __device__ inline double sell_row_packed(const uint32_t *__restrict__ w, const d
         if (q + 1 < n4) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (k + 4 + u < len) jn[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(k + 4 + u) * 64);
+                if (k + 4 + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(k + 4 + u) * 64);
         }
         double g[4], v[4];
 #pragma unroll
@@ -104,7 +115,7 @@ __device__ inline double sell_row_packed(const uint32_t *__restrict__ w, const d
     }
     for (; k < L; ++k) {
         if (k < len) {
-            const uint32_t jj = __builtin_nontemporal_load(w + p0 + (int64_t)k * 64);
+            const uint32_t jj = stream_load<NT>(w + p0 + (int64_t)k * 64);
             acc += vt[jj >> cb] * x[jj & mask];
         }
     }
@@ -158,6 +169,7 @@ __device__ inline double2 sell_row_cplx(const int32_t *__restrict__ col, const d
 // w = col | (re index << cb) | (im index << (cb + rb)) with the value
 // (vre[re index], vim[im index]). Same products in the same order as
 // sell_row_cplx on the unpacked matrix.
+template <bool NT>
 __device__ inline double2 sell_row_cplx_packed(const uint32_t *__restrict__ w, const double *vre, const double *vim,
                                                const double *__restrict__ x, int64_t p0, int L, int len, int cb,
                                                int rb, double2 d, double2 xr) {
@@ -175,7 +187,7 @@ __device__ inline double2 sell_row_cplx_packed(const uint32_t *__restrict__ w, c
     if (n4 > 0) {
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (1 + u < len) jn[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(1 + u) * 64);
+            if (1 + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(1 + u) * 64);
     }
     int k = 1;
     for (int q = 0; q < n4; ++q, k += 4) {
@@ -185,7 +197,7 @@ __device__ inline double2 sell_row_cplx_packed(const uint32_t *__restrict__ w, c
         if (q + 1 < n4) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (k + 4 + u < len) jn[u] = __builtin_nontemporal_load(w + p0 + (int64_t)(k + 4 + u) * 64);
+                if (k + 4 + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(k + 4 + u) * 64);
         }
         double2 g[4], v[4];
 #pragma unroll
@@ -206,7 +218,7 @@ __device__ inline double2 sell_row_cplx_packed(const uint32_t *__restrict__ w, c
     }
     for (; k < L; ++k) {
         if (k < len) {
-            const uint32_t jj = __builtin_nontemporal_load(w + p0 + (int64_t)k * 64);
+            const uint32_t jj = stream_load<NT>(w + p0 + (int64_t)k * 64);
             const double vr = vre[(jj >> cb) & rmask], vi = vim[(uint32_t)((uint64_t)jj >> sh2)];
             const double2 gg = x2[jj & mask];
             ar += vr * gg.x - vi * gg.y;
