@@ -233,8 +233,7 @@ __global__ __launch_bounds__(kEllThreads) void ell_collect_values(int64_t n_rows
         for (int k = 1; k < len; ++k) {
             const int64_t p = base + (int64_t)k * kChunk + lane;
             for (int q = 0; q < kParts; ++q) {
-                const unsigned long long key = (unsigned long long)__double_as_longlong(
-                    __builtin_nontemporal_load(val + (CPLX ? 2 * p + q : p)));
+                const unsigned long long key = (unsigned long long)__double_as_longlong(val[CPLX ? 2 * p + q : p]);
                 if (key == last[q]) continue;
                 last[q] = key;
                 if (!set_insert(set[q], kLdsSet, key)) full = 1;

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
 #pragma unroll
         for (int u = 0; u < kPre; ++u) {
             const int i = (int)threadIdx.x + u * (int)blockDim.x;
-            pv[u] = i < n_bpart ? __builtin_nontemporal_load(bpart + i) : 0.0;
+            pv[u] = i < n_bpart ? bpart[i] : 0.0;
         }
     }
     double s = (LZ && !fused) ? *invb : 1.0;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
         for (int u = 0; u < kPre; ++u)
             if ((int)threadIdx.x + u * (int)blockDim.x < n_bpart) t += pv[u];
         for (int i = (int)threadIdx.x + kPre * (int)blockDim.x; i < n_bpart; i += blockDim.x)
-            t += __builtin_nontemporal_load(bpart + i);
+            t += bpart[i];
         t = block_sum(t, lds);
         if (threadIdx.x == 0) {
             const double nb = sqrt(t);

@@ -40,11 +40,11 @@ __device__ inline double sum_partials(const double *partials, int n, double *lds
     for (; i + 7 * stride < n; i += 8 * stride) {
         double v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(partials + i + u * stride);
+        for (int u = 0; u < 8; ++u) v[u] = partials[i + u * stride];
 #pragma unroll
         for (int u = 0; u < 8; ++u) s += v[u];
     }
-    for (; i < n; i += stride) s += __builtin_nontemporal_load(partials + i);
+    for (; i < n; i += stride) s += partials[i];
     return block_sum(s, lds);
 }
 
