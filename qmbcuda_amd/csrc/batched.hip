@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
             const double xr = r < I.n_rows ? I.P[r] : 0.0;
             const double acc = (I.packed ? sell_row_packed<true>(reinterpret_cast<const uint32_t *>(I.col), I.vtab, I.P, p0,
                                                            L, len, I.col_bits, r < I.n_rows ? I.diag[r] : 0.0, xr)
-                                         : sell_row_real(I.col, I.val, I.P, p0, L, len)) *
+                                         : sell_row_real<true>(I.col, I.val, I.P, p0, L, len)) *
                                s;
             if (r < I.n_rows) {
                 I.Y[r] = acc;
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
                                                    r < I.n_rows ? reinterpret_cast<const double2 *>(I.diag)[r]
                                                                 : make_double2(0.0, 0.0),
                                                    xr)
-                            : sell_row_cplx(I.col, I.val, I.P, p0, L, len);
+                            : sell_row_cplx<true>(I.col, I.val, I.P, p0, L, len);
             a.x *= s;
             a.y *= s;
             if (r < I.n_rows) {

@@ -117,7 +117,8 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                 acc.x = stream ? sell_row_packed<true>(wd, s_vt, x, p0, L, len, cb, d, xr.x)
                                : sell_row_packed<false>(wd, s_vt, x, p0, L, len, cb, d, xr.x);
             } else {
-                acc.x = sell_row_real(col, val, x, p0, L, len);
+                acc.x = stream ? sell_row_real<true>(col, val, x, p0, L, len)
+                               : sell_row_real<false>(col, val, x, p0, L, len);
                 if (LZ) xr.x = r < n_rows ? x[r] : 0.0;
             }
         } else {
@@ -128,7 +129,8 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                 acc = stream ? sell_row_cplx_packed<true>(wd, s_vt, s_vt + kMaxValues, x, p0, L, len, cb, rb, d, xr)
                              : sell_row_cplx_packed<false>(wd, s_vt, s_vt + kMaxValues, x, p0, L, len, cb, rb, d, xr);
             } else {
-                acc = sell_row_cplx(col, val, x, p0, L, len);
+                acc = stream ? sell_row_cplx<true>(col, val, x, p0, L, len)
+                             : sell_row_cplx<false>(col, val, x, p0, L, len);
             }
         }
     };

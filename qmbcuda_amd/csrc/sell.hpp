@@ -25,6 +25,7 @@ __device__ inline T stream_load(const T *p) {
 }
 
 // sum over slots k < len of val[p0 + 64k] * x[col[p0 + 64k]] (L = chunk slots)
+template <bool NT>
 __device__ inline double sell_row_real(const int32_t *__restrict__ col, const double *__restrict__ val,
                                        const double *__restrict__ x, int64_t p0, int L, int len) {
     double acc = 0.0;
@@ -37,8 +38,8 @@ __device__ inline double sell_row_real(const int32_t *__restrict__ col, const do
         for (int u = 0; u < 4; ++u) {
             if (k0 + u < len) {
                 const int64_t p = p0 + (int64_t)(k0 + u) * 64;
-                jn[u] = __builtin_nontemporal_load(col + p);
-                vn[u] = __builtin_nontemporal_load(val + p);
+                jn[u] = stream_load<NT>(col + p);
+                vn[u] = stream_load<NT>(val + p);
             }
         }
     };
@@ -64,8 +65,8 @@ __device__ inline double sell_row_real(const int32_t *__restrict__ col, const do
     for (; k < L; ++k) {
         if (k < len) {
             const int64_t p = p0 + (int64_t)k * 64;
-            const int32_t jj = __builtin_nontemporal_load(col + p);
-            const double vv = __builtin_nontemporal_load(val + p);
+            const int32_t jj = stream_load<NT>(col + p);
+            const double vv = stream_load<NT>(val + p);
             acc += vv * x[jj];
         }
     }
@@ -123,6 +124,7 @@ __device__ inline double sell_row_packed(const uint32_t *__restrict__ w, const d
 }
 
 // complex rows: val and x are interleaved (re, im) pairs
+template <bool NT>
 __device__ inline double2 sell_row_cplx(const int32_t *__restrict__ col, const double *__restrict__ val,
                                         const double *__restrict__ x, int64_t p0, int L, int len) {
     const double2 *val2 = reinterpret_cast<const double2 *>(val);
@@ -136,7 +138,7 @@ __device__ inline double2 sell_row_cplx(const int32_t *__restrict__ col, const d
         for (int u = 0; u < 4; ++u) {
             if (k + u < len) {
                 const int64_t p = p0 + (int64_t)(k + u) * 64;
-                j[u] = __builtin_nontemporal_load(col + p);
+                j[u] = stream_load<NT>(col + p);
                 v[u] = val2[p];
             }
         }
@@ -155,7 +157,7 @@ __device__ inline double2 sell_row_cplx(const int32_t *__restrict__ col, const d
     for (; k < L; ++k) {
         if (k < len) {
             const int64_t p = p0 + (int64_t)k * 64;
-            const int32_t jj = __builtin_nontemporal_load(col + p);
+            const int32_t jj = stream_load<NT>(col + p);
             const double2 vv = val2[p];
             const double2 gg = x2[jj];
             ar += vv.x * gg.x - vv.y * gg.y;
