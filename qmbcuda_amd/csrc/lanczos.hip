@@ -39,7 +39,7 @@ constexpr int kMaxBlocks = 4096;
 // dot: y = s (H x), phi = s x, partial sums of phi . y. PACK: the packed value
 // format (4 B per off-diagonal entry, value tables staged in LDS; complex: real
 // parts at [0, n_values), imaginary parts at kMaxValues + [0, n_values_im)).
-template <bool CPLX, bool LZ, bool PACK>
+template <bool CPLX, bool LZ, bool PACK, bool NT>
 __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_chunks,
                                                       const int64_t *__restrict__ chunk_ptr,
                                                       const uint16_t *__restrict__ row_len,
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ vtab, int n_values, int cb,
-                                                      int n_values_im, int rb, int stream,
+                                                      int n_values_im, int rb,
                                                       const double *__restrict__ x, double *__restrict__ y,
                                                       const double *__restrict__ invb, double *__restrict__ vout,
                                                       double *__restrict__ partials, const double *__restrict__ bpart,
@@ -114,11 +114,9 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
                 xr.x = r < n_rows ? x[r] : 0.0;
                 const double d = r < n_rows ? diag[r] : 0.0;
                 const uint32_t *wd = reinterpret_cast<const uint32_t *>(col);
-                acc.x = stream ? sell_row_packed<true>(wd, s_vt, x, p0, L, len, cb, d, xr.x)
-                               : sell_row_packed<false>(wd, s_vt, x, p0, L, len, cb, d, xr.x);
+                acc.x = sell_row_packed<NT>(wd, s_vt, x, p0, L, len, cb, d, xr.x);
             } else {
-                acc.x = stream ? sell_row_real<true>(col, val, x, p0, L, len)
-                               : sell_row_real<false>(col, val, x, p0, L, len);
+                acc.x = sell_row_real<NT>(col, val, x, p0, L, len);
                 if (LZ) xr.x = r < n_rows ? x[r] : 0.0;
             }
         } else {
@@ -126,11 +124,9 @@ __global__ __launch_bounds__(kThreads) void spmv_sell(int64_t n_rows, int64_t n_
             if (PACK) {
                 const double2 d = r < n_rows ? reinterpret_cast<const double2 *>(diag)[r] : make_double2(0.0, 0.0);
                 const uint32_t *wd = reinterpret_cast<const uint32_t *>(col);
-                acc = stream ? sell_row_cplx_packed<true>(wd, s_vt, s_vt + kMaxValues, x, p0, L, len, cb, rb, d, xr)
-                             : sell_row_cplx_packed<false>(wd, s_vt, s_vt + kMaxValues, x, p0, L, len, cb, rb, d, xr);
+                acc = sell_row_cplx_packed<NT>(wd, s_vt, s_vt + kMaxValues, x, p0, L, len, cb, rb, d, xr);
             } else {
-                acc = stream ? sell_row_cplx<true>(col, val, x, p0, L, len)
-                             : sell_row_cplx<false>(col, val, x, p0, L, len);
+                acc = sell_row_cplx<NT>(col, val, x, p0, L, len);
             }
         }
     };
@@ -423,16 +419,24 @@ void launch_spmv(qmb_ctx *ctx, const qmb_ell *e, const double *x, double *y, boo
                  const FusedBeta *fb = nullptr) {
     const unsigned blocks = grid_for(e->n_chunks, kWaves);
     const double vec_bytes = (e->value_type == QMB_COMPLEX ? 16.0 : 8.0) * (double)e->n_rows;
-    const int stream = e->format_bytes() + 8.0 * vec_bytes > 0.75 * kMallBytes ? 1 : 0;
+    const bool stream = e->format_bytes() + 8.0 * vec_bytes > 0.75 * kMallBytes;
     const bool cplx = e->value_type == QMB_COMPLEX;
-    auto k = cplx ? (e->packed ? (lz ? spmv_sell<true, true, true> : spmv_sell<true, false, true>)
-                               : (lz ? spmv_sell<true, true, false> : spmv_sell<true, false, false>))
-                  : (e->packed ? (lz ? spmv_sell<false, true, true> : spmv_sell<false, false, true>)
-                               : (lz ? spmv_sell<false, true, false> : spmv_sell<false, false, false>));
+    using K = decltype(&spmv_sell<false, false, false, false>);
+    // [complex][packed][lz][stream]
+    static const K table[2][2][2][2] = {
+        {{{spmv_sell<false, false, false, false>, spmv_sell<false, false, false, true>},
+          {spmv_sell<false, true, false, false>, spmv_sell<false, true, false, true>}},
+         {{spmv_sell<false, false, true, false>, spmv_sell<false, false, true, true>},
+          {spmv_sell<false, true, true, false>, spmv_sell<false, true, true, true>}}},
+        {{{spmv_sell<true, false, false, false>, spmv_sell<true, false, false, true>},
+          {spmv_sell<true, true, false, false>, spmv_sell<true, true, false, true>}},
+         {{spmv_sell<true, false, true, false>, spmv_sell<true, false, true, true>},
+          {spmv_sell<true, true, true, false>, spmv_sell<true, true, true, true>}}}};
+    const K k = table[cplx ? 1 : 0][e->packed ? 1 : 0][lz ? 1 : 0][stream ? 1 : 0];
     hipLaunchKernelGGL(k, dim3(blocks), dim3(kThreads), 0, ctx->stream, e->n_rows, e->n_chunks,
                        e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
                        e->val.as<double>(), e->diag.as<double>(), e->vtab.as<double>(), e->n_values, e->col_bits,
-                       e->n_values_im, e->re_bits, stream, x, y, invb, vout, fb ? fb->partials : wk ? wk->partials.as<double>() : nullptr,
+                       e->n_values_im, e->re_bits, x, y, invb, vout, fb ? fb->partials : wk ? wk->partials.as<double>() : nullptr,
                        fb ? fb->bpart : nullptr, fb ? fb->n_bpart : 0, fb ? fb->beta_out : nullptr,
                        fb ? fb->invb_out : nullptr);
     QMB_HIP(hipGetLastError());

@@ -83,35 +83,39 @@ __device__ inline double sell_row_packed(const uint32_t *__restrict__ w, const d
     const uint32_t mask = (1u << cb) - 1u;
     double acc = 0.0;
     if (len > 0) acc += d * xr;
-    // groups of four slots from slot 1; the next group's words are loaded
+    // groups of U slots from slot 1 (8 when the matrix is streamed from HBM: more
+    // loads in flight; 4 when it is cached); the next group's words are loaded
     // before this group's gathers, so the two latencies overlap
-    const int n4 = (L - 1) / 4;
-    uint32_t jn[4] = {0u, 0u, 0u, 0u};
+    constexpr int U = NT ? 8 : 4;
+    const int n4 = (L - 1) / U;
+    uint32_t jn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = 0u;
     if (n4 > 0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < U; ++u)
             if (1 + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(1 + u) * 64);
     }
     int k = 1;
-    for (int q = 0; q < n4; ++q, k += 4) {
-        uint32_t j[4];
+    for (int q = 0; q < n4; ++q, k += U) {
+        uint32_t j[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) j[u] = jn[u];
+        for (int u = 0; u < U; ++u) j[u] = jn[u];
         if (q + 1 < n4) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (k + 4 + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(k + 4 + u) * 64);
+            for (int u = 0; u < U; ++u)
+                if (k + U + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(k + U + u) * 64);
         }
-        double g[4], v[4];
+        double g[U], v[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             if (k + u < len) {
                 g[u] = x[j[u] & mask];
                 v[u] = vt[j[u] >> cb];
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < U; ++u)
             if (k + u < len) acc += v[u] * g[u];
     }
     for (; k < L; ++k) {
