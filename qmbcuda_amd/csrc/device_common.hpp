@@ -384,7 +384,7 @@ __device__ inline bool rep_stab_lut(S x, const GroupView &gv, const GroupLds &t,
 // positions; full images are taken of the candidates only. Same result and
 // same gm as orbit_min.
 template <class S>
-__device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint64_t &c1);
+__host__ __device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint64_t &c1);
 
 template <class S>
 __device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, int &gm, int &mult) {
@@ -431,7 +431,7 @@ __device__ inline S orbit_min_ring(S x, const GroupView &gv, const GroupLds &t, 
 // is 0 (complemented: 1) survive if there are any, else bit i of the minimum
 // is 1 and every candidate stays.
 template <class S>
-__device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint64_t &c1) {
+__host__ __device__ inline unsigned ring_windows(S y, int N, bool hf, uint64_t &c0, uint64_t &c1) {
     const uint64_t yy = (uint64_t)y | ((uint64_t)y << N);  // 8 <= N <= 56: bits p + i < 64
     const uint64_t all = (1ull << N) - 1ull;
     uint64_t a = all, b = hf ? all : 0ull;

@@ -1,0 +1,28 @@
+"""Host checks of device-code building blocks, built by __graft_entry__.build():
+the term compiler (compiled masks and scalars equal the operator-by-operator
+application bit for bit, tests/cpp/test_terms) and the bit-sliced ring window
+scan of the ring orbit minimum (equal to a plain scan of every rotation's
+window, tests/cpp/test_ring_windows)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def run(name):
+    exe = os.path.join(ROOT, "tests", "cpp", name)
+    if not os.path.exists(exe):
+        pytest.fail(f"tests/cpp/{name} is not built (run __graft_entry__.build())")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "mismatches 0" in out.stdout
+
+
+def test_compiled_terms_equal_sequential_application():
+    run("test_terms")
+
+
+def test_bit_sliced_ring_windows_equal_plain_scan():
+    run("test_ring_windows")
