@@ -227,6 +227,19 @@ __global__ __launch_bounds__(kThreads) void lz_update(int64_t cnt, const double 
                                                       const double *__restrict__ apart, int n_apart) {
     __shared__ double lds[kWaves];
     __shared__ double s_alpha;
+    // The thread's first U elements are loaded before the a[n] sum, so their
+    // latency overlaps it; the sum over the elements keeps index order.
+    constexpr int U = 4;
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double ry[U], rv[U], rp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t i = i0 + u * st;
+        ry[u] = i < cnt ? y[i] : 0.0;
+        rv[u] = i < cnt ? v[i] : 0.0;
+        rp[u] = i < cnt ? vprev[i] : 0.0;
+    }
     double a;
     if (apart) {
         // a[n] from the SpMV's partials, summed by every block in finish_sum's order
@@ -242,7 +255,17 @@ __global__ __launch_bounds__(kThreads) void lz_update(int64_t cnt, const double 
     }
     const double b = *beta_p;
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t i = i0 + u * st;
+        if (i < cnt) {
+            const double t = ry[u] + (-b) * rp[u];
+            const double wi = (-a) * rv[u] + t;
+            w[i] = wi;
+            acc += wi * wi;
+        }
+    }
+    for (int64_t i = i0 + U * st; i < cnt; i += st) {
         const double t = y[i] + (-b) * vprev[i];
         const double wi = (-a) * v[i] + t;
         w[i] = wi;
