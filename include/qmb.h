@@ -272,6 +272,21 @@ double qmb_lanczos_eigvals_ref(double *E_tmp, int NStates, int NIter, int nobv, 
                                int exportKrylovVec, int checkEigState, int is_complex,
                                qmb_status *status);
 
+/* The same drop-in with the reference's ownership: index_mat, vals_mat and
+ * track_vec are DEVICE pointers of the default context's GPU (device 0), borrowed
+ * like LanczosEigVals_CUDA borrows the driver's thrust::device_vector data
+ * (Q/exact_diagonalization/lanczos.h:21-32; call site
+ * HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:194-197). The matrix is converted to
+ * the SELL-64 layout on the device (no copy through host memory); a track entry
+ * outside [0, max_terms] or a column >= nobv is QMB_ERR_INVALID, and so is a
+ * pointer that is not device memory. E_tmp and eig_vec_Krylov_out are host
+ * pointers, as in the reference. */
+double qmb_lanczos_eigvals_ref_device(double *E_tmp, int NStates, int NIter, int nobv, int max_terms,
+                                      long long seed, const uint32_t *index_mat, const double *vals_mat,
+                                      const int16_t *track_vec, double *eig_vec_Krylov_out,
+                                      int exportKrylovVec, int checkEigState, int is_complex,
+                                      qmb_status *status);
+
 /* ---- expectation values ------------------------------------------------------
  * Replaces the device part of ComputeStaticExpValZeroT
  * (Q/exact_diagonalization/HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:421-601): the kernel
@@ -348,9 +363,13 @@ qmb_status qmb_mlp_start_vector(qmb_mlp *m, const qmb_basis *b, double delta, do
  * stream and only two norms are read back. The start is x/||x|| + 0.1 xi/||xi||
  * with xi_r = 2u_r - 1 from the SplitMix64 stream of opts->seed (the noise keeps
  * every eigenvector in reach when x misses the ground state). opts may be NULL
- * (defaults); its start fields are overridden. *fallback (optional) = 1 when
- * the learned vector was unusable (zero or not finite) and the seeded random
- * start was used instead. eigvec as in qmb_lanczos. */
+ * (defaults); its start fields are overridden. The Lanczos residual judges the
+ * learned start, and the seeded random start is the fallback: *fallback
+ * (optional) = 0 when the learned run is the result; 1 when the learned vector
+ * was unusable (zero or not finite) and only the random-start run was made; 2
+ * when the learned run's Ritz residual missed the tolerance within max_iter
+ * (QMB_ERR_NOT_CONVERGED) and the random-start run, with the same options,
+ * is the result. eigvec as in qmb_lanczos. */
 qmb_status qmb_lanczos_learned(qmb_mlp *m, const qmb_basis *b, const qmb_ell *e, double delta, double j2,
                                const qmb_lanczos_opts *opts, double *evals, double *eigvec,
                                qmb_lanczos_report *report, int *fallback);

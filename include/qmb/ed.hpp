@@ -399,4 +399,25 @@ double LanczosEigVals(double *E_tmp, int NStates, int NIter, int nobv, int max_t
     return r;
 }
 
+// LanczosEigVals_CUDA (lanczos.h:21-32) with the reference's ownership: the ELL
+// arrays are borrowed DEVICE pointers (of device 0), converted on the device; E_tmp
+// and eig_vec_Krylov_out are host pointers. T = double or std::complex<double>.
+template <class T>
+double LanczosEigVals_CUDA(double *E_tmp, int NStates, const int NIter, int nobv, int max_terms, long long seed,
+                           const uint32_t *index_mat, const T *vals_mat, const int16_t *track_vec,
+                           double *eig_vec_Krylov_out, bool exportKrylovVec, bool checkEigState) {
+    qmb_status st = QMB_OK;
+    const double r = qmb_lanczos_eigvals_ref_device(E_tmp, NStates, NIter, nobv, max_terms, seed, index_mat,
+                                                    reinterpret_cast<const double *>(vals_mat), track_vec,
+                                                    eig_vec_Krylov_out, exportKrylovVec ? 1 : 0,
+                                                    checkEigState ? 1 : 0,
+                                                    std::is_same_v<T, std::complex<double>> ? 1 : 0, &st);
+    check(st, "LanczosEigVals_CUDA");
+    return r;
+}
+
+// The reference's driver class name (HeisenbergGenericXXZ_AbelianSymms_CUDA.h:27-50).
+template <class T>
+using HeisenbergXXZAbelianSymms_CUDA = HeisenbergXXZAbelianSymms<T>;
+
 }  // namespace qmb

@@ -7,8 +7,8 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstdio>
 #include <utility>
+#include <stdexcept>
 #include <vector>
 
 #include "operators.hpp"
@@ -138,10 +138,9 @@ class Heisenberg : public SpinHamiltonian<T> {
         const int M = static_cast<int>(g.rx_alpha.size()), N1 = g.N1, N2 = g.N2;
         this->H = ManyBodyOperator<T>();
         const bool intra = heisenberg_info.intra_c_bool, inter = heisenberg_info.inter_c_bool;
-        if (!intra && !inter) {
-            std::printf("ERROR: BOTH INTER AND INTRA-CLUSTER BOOLEANS ZERO, ABORTING");
-            return;
-        }
+        if (!intra && !inter)
+            throw std::invalid_argument(
+                "Heisenberg::CreateHamiltonian: intra_c_bool and inter_c_bool are both false, so no bond is kept");
         const std::vector<std::vector<int>> *rows[2] = {&heisenberg_info.Jxy_terms, &heisenberg_info.Jz_terms};
         for (int uc = 0; uc < N1 * N2; ++uc) {
             const int j1 = uc / N2, j2 = uc % N2;

@@ -69,6 +69,10 @@ def lib():
     L.orc_ell_build.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, _f64p, c.c_int, c.c_int, c.c_int,
                                 _i32p, _i32p, _f64p, _i32p, _u64p, c.c_int64, _i64p, _f64p, _i64p,
                                 c.c_double, c.c_int, c.c_int, _u32p, _f64p, _i16p]
+    L.orc_ell_rows_from_reps.restype = c.c_int
+    L.orc_ell_rows_from_reps.argtypes = [c.c_int, c.c_int, _u32p, c.c_void_p, c.c_int, _f64p, c.c_int, c.c_int,
+                                         c.c_int, _i32p, _i32p, _f64p, _i32p, _u64p, c.c_int64, _i64p, c.c_int64,
+                                         c.c_double, c.c_int, c.c_int, _u32p, _f64p, _i16p]
     L.orc_expectation.restype = None
     L.orc_expectation.argtypes = [c.c_int, c.c_int, _u32p, c.c_void_p, c.c_int, _f64p, c.c_int, c.c_int, c.c_int,
                                   _i32p, _i32p, _f64p, _i32p, _u64p, c.c_int64, _i64p, _f64p, _i64p, c.c_double,
@@ -269,6 +273,39 @@ class Ell:
         r = lib().orc_lanczos_eigvec(self.n_rows, self.W, self.ind, self.val, self.track, int(self.is_complex),
                                      len(kr), seed, st, kr, e0, x.view(np.float64))
         return r, x
+
+
+def ell_rows_from_reps(n_sites, group_el, flip, chars, term_table, alpha, reps, rows, max_terms, is_complex,
+                       tol_norm=1e-5):
+    """Rows `rows` of the sector matrix whose rows are the ascending representatives
+    `reps` (orc_ell_rows_from_reps): orbit data by brute force over the group
+    images, targets by binary search in `reps`. Same layout as Ell / export_rows
+    (ind[n*W], val, track). Raises if a listed representative is not an orbit
+    minimum or a target's orbit is missing from the list."""
+    op, site, scalar, n_ops, h_col = term_table
+    G = group_el.shape[0]
+    rows = np.ascontiguousarray(rows, np.int64)
+    n, W = len(rows), int(max_terms)
+    ind = np.zeros(n * W, np.uint32)
+    val = np.zeros(n * W * (2 if is_complex else 1), np.float64)
+    track = np.zeros(n, np.int16)
+    fl = None if flip is None else np.ascontiguousarray(flip, np.uint8)
+    sc = np.ascontiguousarray(np.asarray(scalar, np.complex128).view(np.float64))
+    reps = np.ascontiguousarray(reps, np.uint64)
+    rc = lib().orc_ell_rows_from_reps(int(n_sites), G, np.ascontiguousarray(group_el, np.uint32).reshape(-1),
+                                      None if fl is None else fl.ctypes.data, chars.shape[0], _chars_f64(chars),
+                                      int(alpha), len(n_ops), h_col, np.ascontiguousarray(op, np.int32),
+                                      np.ascontiguousarray(site, np.int32), sc, np.ascontiguousarray(n_ops, np.int32),
+                                      reps, len(reps), rows, n, tol_norm, W, int(is_complex), ind, val, track)
+    if rc == 1:
+        raise ValueError("a row needs more slots than max_terms")
+    if rc == 2:
+        raise ValueError("a listed representative is not the minimum of its orbit")
+    if rc == 3:
+        raise ValueError("a target's orbit is missing from the representative list")
+    if rc:
+        raise ValueError(f"orc_ell_rows_from_reps failed ({rc})")
+    return ind, (val.view(np.complex128) if is_complex else val), track
 
 
 def expectation(orbits: Orbits, term_table, alpha, x):

@@ -18,6 +18,10 @@
  *   orc_ell_build        spectral completeness: the union of all sector spectra
  *                        equals the dense full-sector spectrum (numpy, N <= 14);
  *                        Hermiticity after merging duplicate columns
+ *   orc_ell_rows_from_reps  equal, row for row and bit for bit, to orc_ell_build on
+ *                        every orbit of small sectors (chains with flip, 4x4, all
+ *                        irreps); a representative removed from the list is
+ *                        reported missing
  *   orc_spmv             dense matvec of the exported matrix (numpy)
  *   orc_expectation      <Psi|A|Psi> computed densely in the full Sz sector from
  *                        the operator algebra (numpy), with Psi embedded through
@@ -297,6 +301,105 @@ void orc_closed_form_norms(int64_t n_orb, int G, int n_irreps, const int32_t *st
  * Complex products are written out as (ar*br - ai*bi, ar*bi + ai*br) so the
  * HIP kernel can reproduce every bit. Returns 0, or 1 if a row needs more
  * than W slots (the reference would write past the row). */
+typedef struct {
+    int n_irreps, alpha, n_terms, max_ops, W, is_complex;
+    const double *ch; /* characters of irrep alpha, (re, im) per element */
+    const int32_t *op_type, *site, *n_ops;
+    const double *scalar;
+    double tol_norm;
+} row_spec;
+
+/* Target state -> its orbit's row o, the element gm with image_gm(rep_o) = target,
+ * and the norm of that orbit for the irrep. Returns 1 to use the entry, 0 to skip
+ * it (null orbit), a negative code on an inconsistency. */
+typedef int (*target_fn)(void *lk, uint64_t tgt, int64_t *o, int64_t *gm, double *on);
+
+/* One row of BuildHamiltonianXXZForAbelianGroup (utils.cu:115-181): slot 0 is
+ * the diagonal (column = the row), then the off-diagonal targets in term order.
+ * ind/val/track point at the row. Returns 0, 1 on overflow of W, or the lookup's
+ * negative code. */
+static int ell_row(const row_spec *r, uint64_t src, double cur, int64_t id, target_fn look, void *lk,
+                   uint32_t *ind, double *val, int16_t *track) {
+    const int W = r->W;
+    ind[0] = (uint32_t)id;
+    int slot = 1;
+    if (cur < r->tol_norm) return 0;
+    double dr = 0.0, di = 0.0;
+    for (int hi = 0; hi < r->n_terms; hi++) {
+        uint64_t tgt = src;
+        double cr = r->scalar[2 * hi], ci = r->scalar[2 * hi + 1];
+        int dead = 0;
+        for (int k = 0; k < r->n_ops[hi]; k++) {
+            int s = r->site[hi * r->max_ops + k];
+            int op = r->op_type[hi * r->max_ops + k];
+            int spin = (int)((tgt >> s) & 1ULL);
+            if (op == 2) { /* Sm */
+                if (!spin) { dead = 1; break; }
+                tgt &= ~(1ULL << s);
+            } else if (op == 1) { /* Sp */
+                if (spin) { dead = 1; break; }
+                tgt |= 1ULL << s;
+            } else if (op == 0) { /* Sz */
+                double f = -1.0 + 2.0 * (double)spin;
+                cr = cr * 0.5 * f;
+                ci = ci * 0.5 * f;
+            } else {
+                dead = 1;
+                break;
+            }
+        }
+        if (dead) continue;
+        if (tgt == src) {
+            dr += cr;
+            di += ci;
+            continue;
+        }
+        int64_t o, gm;
+        double on;
+        int use = look(lk, tgt, &o, &gm, &on);
+        if (use < 0) return use;
+        if (!use) continue;
+        if (slot >= W) return 1;
+        double ratio = on / cur;
+        double xr = r->ch[2 * gm], xi = r->ch[2 * gm + 1];
+        double pr = cr * xr - ci * xi;
+        double pi = cr * xi + ci * xr;
+        ind[slot] = (uint32_t)o;
+        if (r->is_complex) {
+            val[2 * slot] = pr * ratio;
+            val[2 * slot + 1] = pi * ratio;
+        } else {
+            val[slot] = pr * ratio;
+        }
+        slot++;
+    }
+    if (r->is_complex) {
+        val[0] = dr;
+        val[1] = di;
+    } else {
+        val[0] = dr;
+    }
+    *track = (int16_t)slot;
+    return 0;
+}
+
+typedef struct {
+    const int64_t *orbit_idx;
+    const double *norms;
+    int n_irreps, alpha;
+    double tol_norm;
+} table_lookup;
+
+/* ComputeOrbitIndices table (utils.cu:66-80): rank -> (orbit, g). */
+static int lookup_table(void *lk, uint64_t tgt, int64_t *o, int64_t *gm, double *on) {
+    const table_lookup *t = (const table_lookup *)lk;
+    int64_t rk = orc_rank_colex(tgt);
+    *o = t->orbit_idx[2 * rk];
+    *gm = t->orbit_idx[2 * rk + 1];
+    *on = t->norms[*o * t->n_irreps + t->alpha];
+    return *on >= t->tol_norm;
+}
+
 int orc_ell_build(int n_sites, int n_up, int G, int n_irreps, const double *chars,
                   int alpha, int n_terms, int max_ops, const int32_t *op_type,
                   const int32_t *site, const double *scalar, const int32_t *n_ops,
@@ -308,81 +411,130 @@ int orc_ell_build(int n_sites, int n_up, int G, int n_irreps, const double *char
     (void)n_up;
     const int W = max_terms;
     const int vs = is_complex ? 2 : 1;
-    const double *ch = chars + (size_t)alpha * G * 2;
+    row_spec r = {n_irreps, alpha, n_terms, max_ops, W, is_complex, chars + (size_t)alpha * G * 2,
+                  op_type, site, n_ops, scalar, tol_norm};
+    table_lookup lk = {orbit_idx, norms, n_irreps, alpha, tol_norm};
     int overflow = 0;
     memset(ind, 0, sizeof(uint32_t) * (size_t)n_orb * W);
     memset(val, 0, sizeof(double) * (size_t)n_orb * W * vs);
     memset(track, 0, sizeof(int16_t) * (size_t)n_orb);
 #pragma omp parallel for schedule(dynamic, 256) reduction(| : overflow)
     for (int64_t id = 0; id < n_orb; id++) {
-        ind[(size_t)W * id] = (uint32_t)id;
-        int slot = 1;
-        double cur = norms[id * n_irreps + alpha];
-        if (cur < tol_norm) continue;
-        double dr = 0.0, di = 0.0;
-        uint64_t src = bas[rep_rank[id]];
-        for (int hi = 0; hi < n_terms; hi++) {
-            uint64_t tgt = src;
-            double cr = scalar[2 * hi], ci = scalar[2 * hi + 1];
-            int dead = 0;
-            for (int k = 0; k < n_ops[hi]; k++) {
-                int s = site[hi * max_ops + k];
-                int op = op_type[hi * max_ops + k];
-                int spin = (int)((tgt >> s) & 1ULL);
-                if (op == 2) { /* Sm */
-                    if (!spin) { dead = 1; break; }
-                    tgt &= ~(1ULL << s);
-                } else if (op == 1) { /* Sp */
-                    if (spin) { dead = 1; break; }
-                    tgt |= 1ULL << s;
-                } else if (op == 0) { /* Sz */
-                    double f = -1.0 + 2.0 * (double)spin;
-                    cr = cr * 0.5 * f;
-                    ci = ci * 0.5 * f;
-                } else {
-                    dead = 1;
-                    break;
-                }
-            }
-            if (dead) continue;
-            if (tgt == src) {
-                dr += cr;
-                di += ci;
-                continue;
-            }
-            int64_t t = orc_rank_colex(tgt);
-            int64_t o = orbit_idx[2 * t];
-            int64_t gm = orbit_idx[2 * t + 1];
-            double on = norms[o * n_irreps + alpha];
-            if (on < tol_norm) continue;
-            if (slot >= W) {
-                overflow = 1;
-                break;
-            }
-            double ratio = on / cur;
-            double xr = ch[2 * gm], xi = ch[2 * gm + 1];
-            double pr = cr * xr - ci * xi;
-            double pi = cr * xi + ci * xr;
-            size_t w = (size_t)W * id + slot;
-            ind[w] = (uint32_t)o;
-            if (is_complex) {
-                val[2 * w] = pr * ratio;
-                val[2 * w + 1] = pi * ratio;
-            } else {
-                val[w] = pr * ratio;
-            }
-            slot++;
-        }
-        size_t w0 = (size_t)W * id;
-        if (is_complex) {
-            val[2 * w0] = dr;
-            val[2 * w0 + 1] = di;
-        } else {
-            val[w0] = dr;
-        }
-        track[id] = (int16_t)slot;
+        int rc = ell_row(&r, bas[rep_rank[id]], norms[id * n_irreps + alpha], id, lookup_table, &lk,
+                         ind + (size_t)W * id, val + (size_t)W * id * vs, track + id);
+        overflow |= (rc != 0);
     }
     return overflow;
+}
+
+/* Orbit data of one state by brute force over the G images (SetUpSRS +
+ * SolveSRS_min + SetUpSRS_from_indices + ComputeSRSNorms for a single orbit,
+ * utils.cu:39-64,716-805): the orbit minimum, its stabiliser size, the
+ * closed-form norm (0 when the double-sum norm is below tol_norm), and the
+ * smallest element mapping the minimum onto the state. imgs holds G words. */
+typedef struct {
+    int n_sites, G, n_irreps, alpha;
+    const uint32_t *perm;
+    const uint8_t *flip;
+    const double *ch;
+    double tol_norm;
+    const uint64_t *reps; /* ascending representatives of the rows */
+    int64_t n_reps;
+} brute_lookup;
+
+static uint64_t brute_orbit(const brute_lookup *b, uint64_t s, uint64_t *imgs, int *stab, double *norm,
+                            int64_t *g_to_s) {
+    const int G = b->G;
+    uint64_t mn = ~0ULL;
+    for (int g = 0; g < G; g++) {
+        uint64_t im = orc_apply(s, b->perm + (size_t)g * b->n_sites, b->flip ? b->flip[g] : 0, b->n_sites);
+        if (im < mn) mn = im;
+    }
+    for (int g = 0; g < G; g++)
+        imgs[g] = orc_apply(mn, b->perm + (size_t)g * b->n_sites, b->flip ? b->flip[g] : 0, b->n_sites);
+    int m = 0;
+    *g_to_s = -1;
+    for (int g = 0; g < G; g++) {
+        m += (imgs[g] == mn);
+        if (*g_to_s < 0 && imgs[g] == s) *g_to_s = g;
+    }
+    double sr = 0.0;
+    for (int i = 0; i < G; i++)
+        for (int j = 0; j < G; j++) {
+            if (imgs[i] != imgs[j]) continue;
+            double ar = b->ch[2 * i], ai = -b->ch[2 * i + 1];
+            double br = b->ch[2 * j], bi = b->ch[2 * j + 1];
+            sr += ar * br - ai * bi;
+        }
+    double ns = sqrt(fabs(sr)) / (double)G;
+    *stab = m;
+    *norm = (ns < b->tol_norm) ? 0.0 : sqrt((double)G * (double)m) / (double)G;
+    return mn;
+}
+
+static int64_t find_rep(const brute_lookup *b, uint64_t v) {
+    int64_t lo = 0, hi = b->n_reps;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (b->reps[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < b->n_reps && b->reps[lo] == v) ? lo : -1;
+}
+
+static int lookup_brute(void *lk, uint64_t tgt, int64_t *o, int64_t *gm, double *on) {
+    const brute_lookup *b = (const brute_lookup *)lk;
+    uint64_t imgs[256];
+    int m;
+    uint64_t mn = brute_orbit(b, tgt, imgs, &m, on, gm);
+    *o = find_rep(b, mn);
+    if (*o < 0) return (*on >= b->tol_norm) ? -3 : 0; /* a compatible orbit missing from the list */
+    return *on >= b->tol_norm;
+}
+
+/* Rows of the sector matrix for given row indices into an ascending list of
+ * representatives (a list too large for the full reference tables, e.g. a
+ * 36-site sector): each row is built as in orc_ell_build, with the orbit data of
+ * the row and of every target found by brute force over the G images and the
+ * target's row by binary search in the list (rows of null orbits are not in a
+ * compact list and are skipped like the reference's null targets). Returns 0,
+ * 1 on overflow of W, 2 if a listed representative is not its orbit's minimum,
+ * 3 if a target's compatible orbit is missing from the list. */
+int orc_ell_rows_from_reps(int n_sites, int G, const uint32_t *perm, const uint8_t *flip, int n_irreps,
+                           const double *chars, int alpha, int n_terms, int max_ops, const int32_t *op_type,
+                           const int32_t *site, const double *scalar, const int32_t *n_ops,
+                           const uint64_t *reps, int64_t n_reps, const int64_t *rows, int64_t n_sample,
+                           double tol_norm, int max_terms, int is_complex, uint32_t *ind, double *val,
+                           int16_t *track) {
+    if (G > 256) return -1;
+    const int W = max_terms;
+    const int vs = is_complex ? 2 : 1;
+    row_spec r = {n_irreps, alpha, n_terms, max_ops, W, is_complex, chars + (size_t)alpha * G * 2,
+                  op_type, site, n_ops, scalar, tol_norm};
+    brute_lookup b = {n_sites, G, n_irreps, alpha, perm, flip, chars + (size_t)alpha * G * 2, tol_norm,
+                      reps, n_reps};
+    int err = 0;
+    memset(ind, 0, sizeof(uint32_t) * (size_t)n_sample * W);
+    memset(val, 0, sizeof(double) * (size_t)n_sample * W * vs);
+    memset(track, 0, sizeof(int16_t) * (size_t)n_sample);
+#pragma omp parallel for schedule(dynamic, 16) reduction(max : err)
+    for (int64_t i = 0; i < n_sample; i++) {
+        int64_t id = rows[i];
+        uint64_t src = reps[id];
+        uint64_t imgs[256];
+        int m;
+        double cur;
+        int64_t g0;
+        if (brute_orbit(&b, src, imgs, &m, &cur, &g0) != src) {
+            err = err > 2 ? err : 2;
+            continue;
+        }
+        int rc = ell_row(&r, src, cur, id, lookup_brute, &b, ind + (size_t)W * i, val + (size_t)W * i * vs,
+                         track + i);
+        int e = rc < 0 ? -rc : rc;
+        err = err > e ? err : e;
+    }
+    return err;
 }
 
 /* <Psi|A|Psi> for the sector vector x (reference layout rows), the restatement of

@@ -58,6 +58,13 @@ int orc_ell_build(int n_sites, int n_up, int G, int n_irreps, const double *char
                   int max_terms, int is_complex, uint32_t *ind, double *val,
                   int16_t *track);
 
+int orc_ell_rows_from_reps(int n_sites, int G, const uint32_t *perm, const uint8_t *flip, int n_irreps,
+                           const double *chars, int alpha, int n_terms, int max_ops, const int32_t *op_type,
+                           const int32_t *site, const double *scalar, const int32_t *n_ops,
+                           const uint64_t *reps, int64_t n_reps, const int64_t *rows, int64_t n_sample,
+                           double tol_norm, int max_terms, int is_complex, uint32_t *ind, double *val,
+                           int16_t *track);
+
 void orc_expectation(int n_sites, int G, const uint32_t *group_el, const uint8_t *flip,
                      int n_irreps, const double *chars, int alpha, int n_terms, int max_ops,
                      const int32_t *op_type, const int32_t *site, const double *scalar,

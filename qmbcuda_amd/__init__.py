@@ -386,9 +386,12 @@ class Mlp:
         return x, bool(fb.value)
 
     def lanczos(self, basis: Basis, ell: "Ell", delta: float, j2: float, n_eigs=1, max_iter=300, tol=1e-13,
-                seed=2212456, check_every=10, want_eigvec=False, profile=False, reorth=None):
+                seed=2212456, check_every=10, want_eigvec=False, profile=False, reorth=None, allow_unconverged=False):
         """Lanczos from the learned start vector kept on the device
-        (qmb_lanczos_learned). Returns (evals, report, fallback, eigvec)."""
+        (qmb_lanczos_learned). Returns (evals, report, fallback, eigvec); fallback
+        0 = learned run, 1 = unusable learned vector (random start only), 2 = the
+        learned run missed the tolerance and the random-start run is the result.
+        allow_unconverged: return instead of raising when neither run converged."""
         if reorth is None:
             reorth = n_eigs > 1
         o = default_opts(max_iter=max_iter, n_eigs=n_eigs, tol=tol, seed=seed, reorth=int(reorth),
@@ -397,9 +400,11 @@ class Mlp:
         vec = np.zeros(basis.n_rows, np.float64) if want_eigvec else None
         rep = LanczosReport()
         fb = C.c_int(0)
-        _check(lib().qmb_lanczos_learned(self.h, basis.h, ell.h, delta, j2, C.byref(o), _ptr(ev), _ptr(vec),
-                                         C.byref(rep), C.byref(fb)))
-        return ev, rep, bool(fb.value), vec
+        st = lib().qmb_lanczos_learned(self.h, basis.h, ell.h, delta, j2, C.byref(o), _ptr(ev), _ptr(vec),
+                                       C.byref(rep), C.byref(fb))
+        if not (st == 4 and allow_unconverged):
+            _check(st)
+        return ev, rep, int(fb.value), vec
 
     def __del__(self):
         try:
@@ -418,5 +423,18 @@ def eigvals_ref(E_out, NIter, nobv, max_terms, seed, ind, val, track, is_complex
     r = lib().qmb_lanczos_eigvals_ref(C.byref(E), 1, NIter, nobv, max_terms, seed, _ptr(np.ascontiguousarray(ind)),
                                       _ptr(v), _ptr(np.ascontiguousarray(track)), _ptr(kr), 1, int(check),
                                       int(is_complex), C.byref(st))
+    _check(st.value)
+    return E.value, r, kr
+
+
+def eigvals_ref_device(NIter, nobv, max_terms, seed, ind_dev, val_dev, track_dev, is_complex, check=True):
+    """LanczosEigVals_CUDA with device pointers (qmb_lanczos_eigvals_ref_device):
+    ind_dev, val_dev, track_dev are device addresses (ints) of the reference-layout
+    arrays (uint32 [nobv*W], float64 [nobv*W] (x2 complex), int16 [nobv])."""
+    E = C.c_double(0.0)
+    kr = np.zeros(NIter, np.float64)
+    st = C.c_int(0)
+    r = lib().qmb_lanczos_eigvals_ref_device(C.byref(E), 1, NIter, nobv, max_terms, seed, ind_dev, val_dev, track_dev,
+                                             _ptr(kr), 1, int(check), int(is_complex), C.byref(st))
     _check(st.value)
     return E.value, r, kr

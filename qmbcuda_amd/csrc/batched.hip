@@ -39,7 +39,7 @@ struct BInst {
     const double *diag, *vtab;   // packed values (packed = 1): diagonal, value table(s)
     int packed, col_bits, re_bits;
     int64_t n_rows, n_chunks;
-    double *P, *Y, *VB;          // w, H phi, Krylov basis [M][n*vs]
+    double *P, *Y, *VB;          // w, H phi, Krylov basis [M][n*vs] (two slots, it & 1, without reorthogonalisation)
     double *alpha, *beta, *invb;  // [M], [M+1], [M+1]
     double *coef;                 // [M*2] reorthogonalisation coefficients
     double *gate;                 // [1] second Gram-Schmidt pass wanted (1) or not (0)
@@ -47,7 +47,11 @@ struct BInst {
     double *omega;                // [3][M+2] orthogonality estimates (partial reorthogonalisation)
     int spmv_first, spmv_nblk;
     int vec_first, vec_nblk;
+    int vb_ring;                  // 1: VB holds only phi_it and phi_{it-1} (plain recurrence)
 };
+
+// Slot of phi_it in an instance's Krylov storage.
+__device__ inline size_t vb_slot(const BInst &I, int it) { return (size_t)(I.vb_ring ? (it & 1) : it); }
 
 struct BTask {
     int inst;
@@ -68,7 +72,7 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
     const int it = *it_p;
     const double s = I.invb[it];
     const int vs = CPLX ? 2 : 1;
-    double *vout = I.VB + (size_t)it * I.n_rows * vs;
+    double *vout = I.VB + vb_slot(I, it) * I.n_rows * vs;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double dot = 0.0;
     for (int64_t c = t.lo + wv; c < t.hi; c += kWaves) {
@@ -163,8 +167,8 @@ __global__ __launch_bounds__(kThreads) void bk_update(const BInst *__restrict__ 
     const int it = *it_p;
     const int64_t cnt = I.n_rows * vs;
     const double a = I.alpha[it], b = I.beta[it];
-    const double *v = I.VB + (size_t)it * cnt;
-    const double *vp = it > 0 ? I.VB + (size_t)(it - 1) * cnt : nullptr;
+    const double *v = I.VB + vb_slot(I, it) * cnt;
+    const double *vp = it > 0 ? I.VB + vb_slot(I, it - 1) * cnt : nullptr;
     double acc = 0.0;
     for (int64_t i = t.lo + threadIdx.x; i < t.hi; i += blockDim.x) {
         const double tt = vp ? I.Y[i] + (-b) * vp[i] : I.Y[i] + (-b) * 0.0;
@@ -508,7 +512,9 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             I.P = bufs.back().as<double>();
             bufs.emplace_back(cnt * sizeof(double));
             I.Y = bufs.back().as<double>();
-            bufs.emplace_back((size_t)M * cnt * sizeof(double));
+            // every phi_n is kept only when it is reorthogonalised against
+            I.vb_ring = reorth ? 0 : 1;
+            bufs.emplace_back((size_t)(reorth ? M : 2) * cnt * sizeof(double));
             I.VB = bufs.back().as<double>();
             double *sc = scal.as<double>() + (size_t)pos * (3 * M + 2);
             I.alpha = sc;

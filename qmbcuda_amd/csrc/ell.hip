@@ -649,7 +649,9 @@ qmb_status qmb_ell_export_rows(const qmb_ell *e, int64_t row0, int64_t n_rows, u
                     cl[q0] = (int32_t)r;
                     for (int q = 0; q < vs; ++q) vl[(size_t)q0 * vs + q] = dg[(size_t)i * vs + q];
                 }
-                for (int s = 1; s < L; ++s) {
+                // only the row's own slots: padding words past its length carry no value index
+                const int len = std::min<int>(rl[i], L);
+                for (int s = 1; s < len; ++s) {
                     const int64_t q = q0 + (int64_t)s * kChunk;
                     const uint32_t w = (uint32_t)cl[q];
                     cl[q] = (int32_t)(w & mask);
@@ -669,7 +671,7 @@ qmb_status qmb_ell_export_rows(const qmb_ell *e, int64_t row0, int64_t n_rows, u
             const int64_t r = row0 + i;
             const int64_t c = r / kChunk - c0, lane = r % kChunk;
             ind[(size_t)i * W] = (uint32_t)r;
-            const int len = rl[i];
+            const int len = std::min<int>(rl[i], (int)((cp[c + 1] - cp[c]) / kChunk));
             track[i] = (int16_t)len;
             for (int s = 0; s < len; ++s) {
                 const int64_t pos = cp[c] - base + (int64_t)s * kChunk + lane;

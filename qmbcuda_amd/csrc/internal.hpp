@@ -101,6 +101,10 @@ struct DevPool {
         cap = n;
         return p;
     }
+    size_t cached_bytes() {
+        std::lock_guard<std::mutex> lock(m);
+        return cached;
+    }
     void put(void *p, size_t cap) {
         {
             std::lock_guard<std::mutex> lock(m);

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -531,6 +532,24 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     const int check = o.fixed_iter ? M : (o.check_every > 0 ? o.check_every : 10);
     const double tol = o.tol > 0 ? o.tol : 1e-13;
     QMB_HIP(hipSetDevice(ctx->device));
+    const unsigned rblocks = grid_for(n, kThreads * 4);
+    {
+        // the plan's vectors, refused up front when they cannot fit in device memory:
+        // w, H phi, zero, then phi_{n-1}, phi_n or every kept phi_n with its coefficients
+        const double need = (double)cnt * sizeof(double) * (3.0 + (reorth ? (double)M : 2.0)) +
+                            (reorth ? (double)M * rblocks * 2 * sizeof(double) : 0.0);
+        size_t free_b = 0, total_b = 0;
+        QMB_HIP(hipMemGetInfo(&free_b, &total_b));
+        const double avail = (double)free_b + (double)ctx->pool.cached_bytes();
+        if (need > avail) {
+            char msg[256];
+            std::snprintf(msg, sizeof msg,
+                          "qmb_lanczos: the plan needs %.1f GB of device memory (%d kept Lanczos vectors of %.3f GB); "
+                          "%.1f GB are free",
+                          need / 1e9, reorth ? M : 2, (double)cnt * sizeof(double) / 1e9, avail / 1e9);
+            throw Error{QMB_ERR_OOM, msg};
+        }
+    }
     LzWork wk;
     wk.init(ctx, M);
     DevBuf P(cnt * sizeof(double)), Y(cnt * sizeof(double)), Z(cnt * sizeof(double));
@@ -543,7 +562,6 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     }
     QMB_HIP(hipMemsetAsync(Z.p, 0, cnt * sizeof(double), ctx->stream));
     DevBuf rpart, rcoef;
-    const unsigned rblocks = grid_for(n, kThreads * 4);
     if (reorth) {
         rpart.alloc((size_t)M * rblocks * 2 * sizeof(double));
         rcoef.alloc((size_t)M * 2 * sizeof(double));
@@ -1044,6 +1062,113 @@ qmb_ell *ell_from_reference(qmb_ctx *ctx, int64_t n, int W, const uint32_t *ind,
     return e.release();
 }
 
+// Reference-layout DEVICE arrays -> SELL-64 device ELL, on the device: row
+// lengths from track (checked to lie in [0, W]), chunk offsets by the device
+// scan, then one thread per row copies its slots (checked columns) into the
+// slot-major chunks. The caller's arrays are only read.
+__global__ __launch_bounds__(256) void ref_row_len(const int16_t *__restrict__ track, int64_t n, int W,
+                                                   uint16_t *__restrict__ row_len, int *__restrict__ err) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int t = track[r];
+        const bool ok = t >= 0 && t <= W;
+        row_len[r] = ok ? (uint16_t)t : (uint16_t)0;
+        if (!ok) *err = 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void ref_fill(const uint32_t *__restrict__ ind, const double *__restrict__ val,
+                                                int64_t n, int W, int vs, const int64_t *__restrict__ chunk_ptr,
+                                                const uint16_t *__restrict__ row_len, int32_t *__restrict__ col,
+                                                double *__restrict__ vout, int *__restrict__ err) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = r / kChunk, lane = r % kChunk;
+        const int64_t base = chunk_ptr[c];
+        const int L = (int)((chunk_ptr[c + 1] - base) / kChunk);
+        const int len = row_len[r];
+        for (int s = 0; s < L; ++s) {
+            const int64_t pos = base + (int64_t)s * kChunk + lane;
+            int32_t j = (int32_t)r;
+            if (s < len) {
+                const uint32_t jj = ind[(size_t)r * W + s];
+                if ((int64_t)jj < n)
+                    j = (int32_t)jj;
+                else
+                    *err = 2;
+            }
+            col[pos] = j;
+            for (int q = 0; q < vs; ++q) vout[(size_t)pos * vs + q] = s < len ? val[((size_t)r * W + s) * vs + q] : 0.0;
+        }
+    }
+}
+
+qmb_ell *ell_from_reference_device(qmb_ctx *ctx, int64_t n, int W, const uint32_t *ind, const double *val,
+                                   const int16_t *track, bool cplx) {
+    auto e = std::make_unique<qmb_ell>();
+    e->ctx = ctx;
+    e->value_type = cplx ? QMB_COMPLEX : QMB_REAL;
+    e->max_terms = W;
+    e->n_rows = n;
+    e->n_chunks = (n + kChunk - 1) / kChunk;
+    const int vs = cplx ? 2 : 1;
+    e->row_len.alloc((size_t)n * sizeof(uint16_t));
+    e->chunk_ptr.alloc((size_t)(e->n_chunks + 1) * sizeof(int64_t));
+    DevBuf err(sizeof(int));
+    const int zero = 0;
+    QMB_HIP(hipMemcpyAsync(err.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    const int blocks = (int)std::min<int64_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(ref_row_len, dim3(blocks), dim3(256), 0, ctx->stream, track, n, W, e->row_len.as<uint16_t>(),
+                       err.as<int>());
+    QMB_HIP(hipGetLastError());
+    int herr = 0, mx = 0;
+    int64_t nnz = 0;
+    e->stored = scan_chunks_i64(ctx, e->row_len.as<uint16_t>(), n, e->chunk_ptr.as<int64_t>(), e->n_chunks, &nnz,
+                                &mx, err.as<int>(), &herr);
+    QMB_REQUIRE(herr == 0, "reference ELL: track out of range [0, max_terms]");
+    e->nnz = nnz;
+    e->max_row_len = mx;
+    e->col.alloc((size_t)std::max<int64_t>(e->stored, 1) * sizeof(int32_t));
+    e->val.alloc((size_t)std::max<int64_t>(e->stored, 1) * vs * sizeof(double));
+    hipLaunchKernelGGL(ref_fill, dim3(blocks), dim3(256), 0, ctx->stream, ind, val, n, W, vs,
+                       e->chunk_ptr.as<int64_t>(), e->row_len.as<uint16_t>(), e->col.as<int32_t>(),
+                       e->val.as<double>(), err.as<int>());
+    QMB_HIP(hipGetLastError());
+    QMB_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    QMB_HIP(hipStreamSynchronize(ctx->stream));
+    QMB_REQUIRE(herr == 0, "reference ELL: column out of range");
+    return e.release();
+}
+
+// The caller's pointer must be device memory of the context's device.
+void require_device_ptr(const qmb_ctx *ctx, const void *p, const char *what) {
+    hipPointerAttribute_t a{};
+    const hipError_t r = hipPointerGetAttributes(&a, p);
+    if (r != hipSuccess) (void)hipGetLastError();
+    QMB_REQUIRE(r == hipSuccess && (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) &&
+                    a.device == ctx->device,
+                std::string("qmb_lanczos_eigvals_ref_device: ") + what + " is not device memory of the context's GPU");
+}
+
+// Fixed NIter steps from the seeded start on a reference-layout ELL: the
+// reference free function's semantics (lanczos.cu:128-317).
+double eigvals_ref_run(qmb_ctx *ctx, qmb_ell *e, double *E_tmp, int NIter, int nobv, long long seed,
+                       double *eig_vec_Krylov_out, int exportKrylovVec, int checkEigState) {
+    qmb_lanczos_opts o;
+    qmb_lanczos_default_opts(&o);
+    o.max_iter = NIter;
+    o.fixed_iter = 1;
+    o.seed = (uint64_t)seed;
+    o.want_eigvec = checkEigState ? 1 : 0;
+    std::vector<double> kr(NIter);
+    qmb_lanczos_report rep;
+    double ev = 0.0;
+    qmb_status r = run_lanczos(ctx, e, o, &ev, kr.data(), nullptr, &rep);
+    if (r != QMB_OK && r != QMB_ERR_NOT_CONVERGED) throw Error{r, qmb_last_error()};
+    *E_tmp = ev;
+    if (exportKrylovVec && eig_vec_Krylov_out)
+        for (int i = 0; i < NIter; ++i) eig_vec_Krylov_out[i] = kr[i];
+    return checkEigState ? rep.eig_residual / (double)nobv : 0.0;
+}
+
 qmb_ctx *default_ctx() {
     static std::once_flag once;
     static qmb_ctx *ctx = nullptr;
@@ -1186,21 +1311,32 @@ double qmb_lanczos_eigvals_ref(double *E_tmp, int NStates, int NIter, int nobv, 
         QMB_HIP(hipSetDevice(ctx->device));
         std::unique_ptr<qmb_ell> e(
             ell_from_reference(ctx, nobv, max_terms, index_mat, vals_mat, track_vec, is_complex != 0));
-        qmb_lanczos_opts o;
-        qmb_lanczos_default_opts(&o);
-        o.max_iter = NIter;
-        o.fixed_iter = 1;
-        o.seed = (uint64_t)seed;
-        o.want_eigvec = checkEigState ? 1 : 0;
-        std::vector<double> kr(NIter);
-        qmb_lanczos_report rep;
-        double ev = 0.0;
-        qmb_status r = run_lanczos(ctx, e.get(), o, &ev, kr.data(), nullptr, &rep);
-        if (r != QMB_OK && r != QMB_ERR_NOT_CONVERGED) throw Error{r, qmb_last_error()};
-        *E_tmp = ev;
-        if (exportKrylovVec && eig_vec_Krylov_out)
-            for (int i = 0; i < NIter; ++i) eig_vec_Krylov_out[i] = kr[i];
-        if (checkEigState) ret = rep.eig_residual / (double)nobv;
+        ret = eigvals_ref_run(ctx, e.get(), E_tmp, NIter, nobv, seed, eig_vec_Krylov_out, exportKrylovVec,
+                              checkEigState);
+    });
+    if (status) *status = st;
+    return st == QMB_OK ? ret : std::nan("");
+}
+
+double qmb_lanczos_eigvals_ref_device(double *E_tmp, int NStates, int NIter, int nobv, int max_terms, long long seed,
+                                      const uint32_t *index_mat, const double *vals_mat, const int16_t *track_vec,
+                                      double *eig_vec_Krylov_out, int exportKrylovVec, int checkEigState,
+                                      int is_complex, qmb_status *status) {
+    double ret = 0.0;
+    qmb_status st = guarded([&] {
+        QMB_REQUIRE(E_tmp && index_mat && vals_mat && track_vec, "qmb_lanczos_eigvals_ref_device: NULL argument");
+        QMB_REQUIRE(NStates == 1, "qmb_lanczos_eigvals_ref_device: NStates must be 1 (as in the reference)");
+        QMB_REQUIRE(NIter >= 1 && nobv >= 1 && max_terms >= 1 && max_terms <= 65535,
+                    "qmb_lanczos_eigvals_ref_device: bad sizes");
+        qmb_ctx *ctx = default_ctx();
+        QMB_HIP(hipSetDevice(ctx->device));
+        require_device_ptr(ctx, index_mat, "index_mat");
+        require_device_ptr(ctx, vals_mat, "vals_mat");
+        require_device_ptr(ctx, track_vec, "track_vec");
+        std::unique_ptr<qmb_ell> e(
+            ell_from_reference_device(ctx, nobv, max_terms, index_mat, vals_mat, track_vec, is_complex != 0));
+        ret = eigvals_ref_run(ctx, e.get(), E_tmp, NIter, nobv, seed, eig_vec_Krylov_out, exportKrylovVec,
+                              checkEigState);
     });
     if (status) *status = st;
     return st == QMB_OK ? ret : std::nan("");

@@ -987,7 +987,16 @@ qmb_status qmb_lanczos_learned(qmb_mlp *m, const qmb_basis *b, const qmb_ell *e,
         o.start = QMB_START_RANDOM;
         o.start_vec = nullptr;
     }
-    return qmb_lanczos(m->ctx, e, &o, evals, nullptr, eigvec, report);
+    st = qmb_lanczos(m->ctx, e, &o, evals, nullptr, eigvec, report);
+    if (ok && st == QMB_ERR_NOT_CONVERGED) {
+        // the Ritz residual of the learned run missed the tolerance within max_iter:
+        // the seeded random start is the fallback and its run is the result
+        if (fallback) *fallback = 2;
+        o.start = QMB_START_RANDOM;
+        o.start_vec = nullptr;
+        st = qmb_lanczos(m->ctx, e, &o, evals, nullptr, eigvec, report);
+    }
+    return st;
 }
 
 }  // extern "C"

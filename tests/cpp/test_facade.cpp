@@ -5,14 +5,18 @@
 //   solve <case>   run HeisenbergXXZAbelianSymms on the GPU and print per-irrep
 //                  energies as JSON
 //   expect <case>  ground-state expectation values through ComputeStaticExpValZeroT
-//   ref <case>     run LanczosEigVals (the reference free function) on GPU-built
-//                  reference-layout arrays read from stdin (n W then ind/val/track)
+//   ref_cuda <case>  the reference free function on reference-layout arrays of
+//                  irrep 0 (built through the C ABI): LanczosEigVals_CUDA on device
+//                  copies (borrowed device pointers) and LanczosEigVals on the host
+//                  arrays, plus a solve through the HeisenbergXXZAbelianSymms_CUDA name
 #include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <hip/hip_runtime_api.h>
 
 #include "qmb/qmb.hpp"
 
@@ -201,9 +205,85 @@ static void expect(const Case &c) {
     std::printf("}\n");
 }
 
+// The reference's free function on its own row-major arrays (irrep 0, reference
+// layout): the driver's row width, device copies for LanczosEigVals_CUDA.
+static void ref_cuda(const Case &c) {
+    const int LS = c.model.GetLS();
+    const auto chars = c.group->GetCharTable();
+    const int G = static_cast<int>(c.group->GetGroupElemVec().size()), NIr = static_cast<int>(chars.size());
+    std::vector<uint32_t> ge((size_t)G * LS);
+    c.group->GetGroupElemArr(c.model.GetR1(), c.model.GetR2(), ge.data());
+    auto fl = c.group->GetSpinFlips();
+    if (fl.empty()) fl.assign(G, 0);
+    std::vector<double> ch((size_t)NIr * G * 2);
+    for (int a = 0; a < NIr; ++a)
+        for (int g = 0; g < G; ++g) {
+            ch[((size_t)a * G + g) * 2] = chars[a][g].real();
+            ch[((size_t)a * G + g) * 2 + 1] = chars[a][g].imag();
+        }
+    const auto ov = BuildOperatorVecs(c.model.GetH());
+    std::vector<double> sc((size_t)ov.H_size * 2);
+    for (int h = 0; h < ov.H_size; ++h) {
+        sc[2 * h] = ov.H_scalar_table[h].real();
+        sc[2 * h + 1] = ov.H_scalar_table[h].imag();
+    }
+    const int n_up = LS / 2, W = ov.H_size * n_up / LS + 1;
+    qmb_ctx *ctx = nullptr;
+    qmb_group *grp = nullptr;
+    qmb_terms *terms = nullptr;
+    qmb_basis *b = nullptr;
+    qmb_ell *e = nullptr;
+    check(qmb_ctx_create(0, &ctx), "qmb_ctx_create");
+    check(qmb_group_create(LS, G, ge.data(), fl.data(), NIr, ch.data(), &grp), "qmb_group_create");
+    check(qmb_terms_create(LS, ov.H_size, ov.H_col_size, ov.H_decode_table.data(), ov.H_site_table.data(), sc.data(),
+                           ov.H_NOTerms.data(), &terms),
+          "qmb_terms_create");
+    check(qmb_basis_build(ctx, grp, n_up, 0, QMB_LAYOUT_REFERENCE, &b), "qmb_basis_build");
+    check(qmb_ell_build(ctx, b, terms, QMB_COMPLEX, W, &e), "qmb_ell_build");
+    qmb_ell_info inf;
+    check(qmb_ell_get_info(e, &inf), "qmb_ell_get_info");
+    const int n = (int)inf.n_rows;
+    std::vector<uint32_t> ind((size_t)n * W);
+    std::vector<cd> val((size_t)n * W);
+    std::vector<int16_t> track(n);
+    check(qmb_ell_export_reference(e, ind.data(), reinterpret_cast<double *>(val.data()), track.data()),
+          "qmb_ell_export_reference");
+    void *d_ind = nullptr, *d_val = nullptr, *d_tr = nullptr;
+    if (hipMalloc(&d_ind, ind.size() * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&d_val, val.size() * sizeof(cd)) != hipSuccess ||
+        hipMalloc(&d_tr, track.size() * sizeof(int16_t)) != hipSuccess ||
+        hipMemcpy(d_ind, ind.data(), ind.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_val, val.data(), val.size() * sizeof(cd), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_tr, track.data(), track.size() * sizeof(int16_t), hipMemcpyHostToDevice) != hipSuccess)
+        throw std::runtime_error("device copies of the reference arrays failed");
+    const int NIter = 60;
+    double E_dev = 0.0, E_host = 0.0;
+    std::vector<double> kr_dev(NIter), kr_host(NIter);
+    const double r_dev = LanczosEigVals_CUDA<cd>(&E_dev, 1, NIter, n, W, 2212456, static_cast<const uint32_t *>(d_ind),
+                                                 static_cast<const cd *>(d_val), static_cast<const int16_t *>(d_tr),
+                                                 kr_dev.data(), true, true);
+    const double r_host = LanczosEigVals<cd>(&E_host, 1, NIter, n, W, 2212456, ind.data(), val.data(), track.data(),
+                                             kr_host.data(), true, true);
+    (void)hipFree(d_ind);
+    (void)hipFree(d_val);
+    (void)hipFree(d_tr);
+    qmb_ell_destroy(e);
+    qmb_basis_destroy(b);
+    qmb_terms_destroy(terms);
+    qmb_group_destroy(grp);
+    qmb_ctx_destroy(ctx);
+    // the driver under the reference's class name
+    HeisenbergXXZAbelianSymms_CUDA<cd> ham(c.model, c.group);
+    bool same_kr = true;
+    for (int i = 0; i < NIter; ++i) same_kr = same_kr && kr_dev[i] == kr_host[i];
+    std::printf("{\"n\": %d, \"W\": %d, \"E_dev\": %.17g, \"E_host\": %.17g, \"res_dev\": %.17g, "
+                "\"res_host\": %.17g, \"same_krylov\": %s, \"E_driver_irrep0\": %.17g}\n",
+                n, W, E_dev, E_host, r_dev, r_host, same_kr ? "true" : "false", ham.get_E_vecs()[0][0]);
+}
+
 int main(int argc, char **argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s tables|solve|solve_fixed|solve_batched|expect <case>\n", argv[0]);
+        std::fprintf(stderr, "usage: %s tables|solve|solve_fixed|solve_batched|expect|ref_cuda <case>\n", argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -219,6 +299,8 @@ int main(int argc, char **argv) {
             solve(c, false, true);
         else if (mode == "expect")
             expect(c);
+        else if (mode == "ref_cuda")
+            ref_cuda(c);
         else
             return 2;
     } catch (const std::exception &e) {

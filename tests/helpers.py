@@ -118,3 +118,42 @@ def sector_ground_state(case, orb, alpha):
     x = np.zeros(orb.n_orb, np.complex128)
     x[keep] = V[:, 0]
     return w[0], x
+
+
+def burnside_count(case, n_up):
+    """Number of orbits of the Sz sector under the group by Burnside's lemma,
+    (1/G) sum_g |Fix(g)|, with the fixed states of each element counted from its
+    cycles: without flip a fixed state is constant on every cycle (all down or
+    all up); with flip (image = complement of the permuted state) it alternates
+    along every cycle, so every cycle must be even and gives two states with
+    half of its sites up."""
+    ge, fl = case["group_el"], case["flip"]
+    n = case["n_sites"]
+    total = 0
+    for g in range(ge.shape[0]):
+        perm = [int(v) for v in ge[g]]
+        seen = [False] * n
+        poly = {0: 1}  # number of fixed states by up-spin count
+        flip = fl is not None and fl[g]
+        for s in range(n):
+            if seen[s]:
+                continue
+            c, t = 0, s
+            while not seen[t]:
+                seen[t] = True
+                t = perm[t]
+                c += 1
+            if not flip:
+                opts = ((0, 1), (c, 1))
+            elif c % 2 == 0:
+                opts = ((c // 2, 2),)
+            else:
+                opts = ()
+            nxt = {}
+            for u, m in poly.items():
+                for du, w in opts:
+                    nxt[u + du] = nxt.get(u + du, 0) + m * w
+            poly = nxt
+        total += poly.get(n_up, 0)
+    assert total % ge.shape[0] == 0
+    return total // ge.shape[0]

@@ -5,7 +5,9 @@ files for another basis, truncated files and files whose indexes point outside
 the arrays are refused before any kernel reads them. A Lanczos run stopped and
 resumed from its checkpoint (once or many times, plain or with partial or full
 reorthogonalisation) ends with the same eigenvalues, step count and Krylov
-coefficients as the uninterrupted run.
+coefficients as the uninterrupted run. Independently of libqmb, a saved file
+read by the oracle's numpy reader is the oracle's own matrix bit for bit, and
+resumed runs reach the oracle's converged and the dense eigenvalues.
 """
 import struct
 import sys
@@ -14,7 +16,8 @@ import numpy as np
 import pytest
 
 import qmbcuda_amd as q
-from tests.helpers import chain_case
+from oracle.qell import read_qell
+from tests.helpers import chain_case, ell_dense_nonnull, oracle_ell, oracle_orbits, square_case
 
 pytestmark = pytest.mark.gpu
 
@@ -113,3 +116,100 @@ def test_lanczos_resume_refuses_other_options(ctx, tmp_path):
     _, _, b2, e2 = objects(ctx, case, 8, 3)
     with pytest.raises(q.QMBError):
         q.lanczos_checkpointed(e2, path, resume=True, max_iter=200, check_every=5)
+
+
+@pytest.mark.parametrize("label,mk,n_up,alpha,layout,fp64", [
+    ("chain14_flip", lambda: chain_case(14, 1.0, 1.0, 0.3, flip=True), 7, 0, q.LAYOUT_COMPACT, False),
+    ("chain14_flip", lambda: chain_case(14, 1.0, 1.0, 0.3, flip=True), 7, 0, q.LAYOUT_REFERENCE, False),
+    ("chain14_flip", lambda: chain_case(14, 1.0, 1.0, 0.3, flip=True), 7, 5, q.LAYOUT_COMPACT, False),
+    ("chain14_flip", lambda: chain_case(14, 1.0, 1.0, 0.3, flip=True), 7, 5, q.LAYOUT_REFERENCE, True),
+    ("square4x4_flip", lambda: square_case(4, 4, 1.0, 0.8, flip=True), 8, 0, q.LAYOUT_COMPACT, False),
+])
+def test_saved_file_is_the_oracle_matrix(ctx, tmp_path, label, mk, n_up, alpha, layout, fp64):
+    """A saved matrix read back by the independent numpy reader (oracle/qell.py,
+    no libqmb) equals the oracle's own reference-layout build bit for bit."""
+    case = mk()
+    say(f"qell vs oracle {label} irrep={alpha} layout={layout} fp64={fp64}")
+    orb = oracle_orbits(case, n_up)
+    ref = oracle_ell(case, orb, alpha)
+    g, t, b, e = objects(ctx, case, n_up, alpha, layout, keep_fp64_values=fp64, max_terms=ref.W)
+    path = tmp_path / "m.qell"
+    e.save(path)
+    hd, ind, val, track = read_qell(path)
+    assert hd["n_sites"] == case["n_sites"] and hd["n_up"] == n_up and hd["irrep"] == alpha
+    assert hd["packed"] == (0 if fp64 else 1)
+    W = ref.W
+    if layout == q.LAYOUT_REFERENCE:
+        assert np.array_equal(track, ref.track)
+        assert np.array_equal(ind, ref.ind)
+        assert np.array_equal(val, ref.values)
+    else:
+        keep = ~orb.null_mask(alpha)
+        cidx = np.cumsum(keep) - 1
+        rows = np.flatnonzero(keep)
+        assert hd["n_rows"] == len(rows)
+        for i, r in enumerate(rows):
+            n = int(ref.track[r])
+            assert track[i] == n
+            assert ind[i * W] == i
+            assert np.array_equal(ind[i * W + 1:i * W + n], cidx[ref.ind[r * W + 1:r * W + n]])
+            assert np.array_equal(val[i * W:i * W + n], ref.values[r * W:r * W + n])
+
+
+def test_padding_words_are_never_read(ctx, tmp_path):
+    """A corrupt padding word (a slot past its row's length inside the chunk
+    width) is accepted by the loader and changes nothing: neither the SpMV nor
+    the export reads slots past a row's length."""
+    case = chain_case(12, 1.0, 1.0, 0.3, flip=True)
+    g, t, b, e = objects(ctx, case, 6, 0)
+    path = tmp_path / "m.qell"
+    e.save(path)
+    raw = path.read_bytes()
+    hd, _, _, _ = read_qell(path)
+    n, nc = hd["n_rows"], hd["n_chunks"]
+    hdr = 104
+    cp = np.frombuffer(raw[hdr:hdr + (nc + 1) * 8], "<i8")
+    rl = np.frombuffer(raw[hdr + (nc + 1) * 8:hdr + (nc + 1) * 8 + n * 2], "<u2")
+    off_col = hdr + (nc + 1) * 8 + n * 2
+    # the first row shorter than its chunk's width
+    pad = None
+    for r in range(n):
+        L = (cp[r // 64 + 1] - cp[r // 64]) // 64
+        if rl[r] < L:
+            pad = cp[r // 64] + 64 * int(rl[r]) + r % 64
+            break
+    assert pad is not None
+    say("ell load: corrupt padding word")
+    bad = bytearray(raw)
+    struct.pack_into("<I", bad, off_col + 4 * int(pad), 0xFFFFFFFF)
+    (tmp_path / "pad.qell").write_bytes(bytes(bad))
+    e2 = q.Ell.load(ctx, b, tmp_path / "pad.qell", t)
+    for u, v in zip(e.export_reference(), e2.export_reference()):
+        assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
+    x = np.random.default_rng(3).normal(size=b.n_rows)
+    assert np.array_equal(e.spmv(x), e2.spmv(x))
+
+
+@pytest.mark.parametrize("reorth", [0, 2])
+def test_resumed_run_matches_oracle_and_dense(ctx, tmp_path, reorth):
+    """A run stopped three times and resumed from its checkpoint reaches the
+    oracle's converged Lanczos eigenvalue and the dense ground state (1e-10)."""
+    case = chain_case(16, 1.0, 0.8, 0.35, flip=True)
+    orb = oracle_orbits(case, 8)
+    ref = oracle_ell(case, orb, 0)
+    dense = np.linalg.eigvalsh(ell_dense_nonnull(ref, orb, 0))[0]
+    st, e0_or, _, _, _ = ref.lanczos_converge(400, 2212456, tol=1e-13, check_every=5)
+    assert st == 0
+    g, t, b, e = objects(ctx, case, 8, 0)
+    say(f"resumed checkpointed run vs oracle reorth={reorth}")
+    kw = dict(max_iter=300, tol=1e-13, reorth=reorth, check_every=5)
+    path = tmp_path / "run.qlz"
+    ev, rep, _, _, ok = q.lanczos_checkpointed(e, path, stop_after=6, **kw)
+    stops = 0
+    while not ok:
+        ev, rep, _, _, ok = q.lanczos_checkpointed(e, path, resume=True, stop_after=6, **kw)
+        stops += 1
+        assert stops < 100
+    assert stops >= 2 and rep.converged
+    assert abs(ev[0] - e0_or) < 1e-10 * abs(e0_or)
+    assert abs(ev[0] - dense) < 1e-10 * abs(dense)

@@ -211,3 +211,18 @@ def test_facade_driver_with_point_group(name):
             continue
         e0 = np.linalg.eigvalsh(M)[0]
         assert abs(E[a] - e0) < 1e-10 * max(1, abs(e0))
+
+
+@pytest.mark.gpu
+def test_facade_reference_free_function_on_device_arrays():
+    """LanczosEigVals_CUDA with borrowed device pointers equals LanczosEigVals on
+    the same host arrays bit for bit, both reach the dense ground state of the
+    irrep, and the driver answers to the reference's class name."""
+    print("[gpu-case] facade ref_cuda chain16", flush=True)
+    got = run("ref_cuda", "chain16")
+    assert got["E_dev"] == got["E_host"] and got["res_dev"] == got["res_host"] and got["same_krylov"]
+    ref = oracle_case("chain16")
+    orb = oracle_orbits(ref, 8)
+    e0 = np.linalg.eigvalsh(ell_dense_nonnull(oracle_ell(ref, orb, 0), orb, 0))[0]
+    assert abs(got["E_dev"] - e0) < 1e-9 * abs(e0)
+    assert abs(got["E_driver_irrep0"] - e0) < 1e-10 * abs(e0)

@@ -346,3 +346,40 @@ def test_reference_signature_drop_in(ctx):
     assert abs(E - w[0]) < 1e-11
     res_or, _ = ref.eigvec_residual(2212456, z[:, 0] * np.sign(z[0, 0]), w[0])
     assert abs(r - res_or / ref.n_rows) < 1e-8
+
+
+def test_reference_signature_device_pointers(ctx):
+    """qmb_lanczos_eigvals_ref_device: the reference's free function with its ELL
+    arrays as borrowed device pointers gives the host-array drop-in's results bit
+    for bit; host pointers and inconsistent arrays are refused."""
+    import torch
+
+    case = chain_case(16)
+    orb = oracle_orbits(case, 8)
+    ref = oracle_ell(case, orb, 0)
+    say("eigvals_ref_device drop-in chain16")
+    E_h, r_h, kr_h = q.eigvals_ref(None, 50, ref.n_rows, ref.W, 2212456, ref.ind, ref.val, ref.track, False,
+                                   check=True)
+    d_ind = torch.from_numpy(ref.ind.view(np.int32).copy()).cuda()
+    d_val = torch.from_numpy(ref.val.copy()).cuda()
+    d_tr = torch.from_numpy(ref.track.copy()).cuda()
+    torch.cuda.synchronize()
+    E_d, r_d, kr_d = q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, d_ind.data_ptr(), d_val.data_ptr(),
+                                          d_tr.data_ptr(), False, check=True)
+    assert E_d == E_h and r_d == r_h and np.array_equal(kr_d, kr_h)
+    # a host pointer is not device memory
+    with pytest.raises(q.QMBError, match="not device memory"):
+        q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, ref.ind.ctypes.data, d_val.data_ptr(),
+                             d_tr.data_ptr(), False)
+    # a track entry beyond max_terms, then a column beyond nobv
+    bad_tr = d_tr.clone()
+    bad_tr[3] = ref.W + 1
+    with pytest.raises(q.QMBError, match="track out of range"):
+        q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, d_ind.data_ptr(), d_val.data_ptr(),
+                             bad_tr.data_ptr(), False)
+    bad_ind = d_ind.clone()
+    bad_ind[ref.W * 5 + 1] = ref.n_rows + 7
+    with pytest.raises(q.QMBError, match="column out of range"):
+        q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, bad_ind.data_ptr(), d_val.data_ptr(),
+                             d_tr.data_ptr(), False)
+    torch.cuda.synchronize()

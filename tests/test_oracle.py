@@ -15,7 +15,7 @@ import pytest
 
 import oracle
 from oracle import model as OM
-from tests.helpers import (brute_orbits, chain_case, dense_spectrum, ell_dense_nonnull, oracle_ell, oracle_orbits,
+from tests.helpers import (brute_orbits, burnside_count, chain_case, dense_spectrum, ell_dense_nonnull, oracle_ell, oracle_orbits,
                            square_case)
 
 
@@ -288,3 +288,73 @@ def test_planted_mistake_is_caught():
         if E.shape[0]:
             parts.append(np.linalg.eigvalsh(E))
     assert not np.allclose(np.sort(np.concatenate(parts)), dense, atol=1e-6)
+
+
+@pytest.mark.parametrize("label,case,n_up,alphas", [
+    ("chain12_flip", chain_case(12, 1.0, 1.0, 0.3, flip=True), 6, [0, 5, 12, 17]),
+    ("square4x4_flip", square_case(4, 4, 1.0, 0.8, flip=True), 8, [0, 5, 26]),
+    ("chain13_odd", chain_case(13, 1.0, 0.7, 0.2), 6, [0, 2]),
+])
+def test_rows_from_reps_equal_full_table_build(label, case, n_up, alphas):
+    """The sampled-row builder (brute-force orbit data, binary search in the
+    representative list; used where the sector is too large for the full
+    tables) equals the full-table build row for row, in both layouts."""
+    orb = oracle_orbits(case, n_up)
+    rng = np.random.default_rng(3)
+    for a in alphas:
+        ref = oracle_ell(case, orb, a)
+        cplx = ref.is_complex
+        rows = np.sort(rng.choice(orb.n_orb, min(orb.n_orb, 300), replace=False))
+        ind, val, track = oracle.ell_rows_from_reps(case["n_sites"], case["group_el"], case["flip"], case["chars"],
+                                                    case["table"], a, orb.reps, rows, ref.W, cplx)
+        W = ref.W
+        for i, r in enumerate(rows):
+            n = int(ref.track[r])
+            assert track[i] == n
+            assert np.array_equal(ind[i * W:i * W + max(n, 1)], ref.ind[r * W:r * W + max(n, 1)])
+            assert np.array_equal(np.asarray(val)[i * W:i * W + n], ref.values[r * W:r * W + n])
+        # compact list: only the irrep's compatible orbits; columns are compact row indices
+        keep = ~orb.null_mask(a)
+        creps = orb.reps[keep]
+        cidx = np.cumsum(keep) - 1
+        crow = rng.choice(len(creps), min(len(creps), 200), replace=False)
+        ind, val, track = oracle.ell_rows_from_reps(case["n_sites"], case["group_el"], case["flip"], case["chars"],
+                                                    case["table"], a, creps, crow, W, cplx)
+        full = np.flatnonzero(keep)[crow]
+        for i, r in enumerate(full):
+            n = int(ref.track[r])
+            assert track[i] == n
+            assert np.array_equal(ind[i * W + 1:i * W + n], cidx[ref.ind[r * W + 1:r * W + n]])
+            assert np.array_equal(np.asarray(val)[i * W:i * W + n], ref.values[r * W:r * W + n])
+
+
+def test_rows_from_reps_detects_missing_and_false_representatives():
+    case = chain_case(12, 1.0, 1.0, 0.3, flip=True)
+    orb = oracle_orbits(case, 6)
+    W = OM.max_terms_rule(len(case["table"][3]), 6, 12)
+    reps = orb.reps
+    # a representative dropped from the list: some row's target orbit is then missing
+    drop = np.delete(reps, len(reps) // 2)
+    with pytest.raises(ValueError, match="missing"):
+        oracle.ell_rows_from_reps(12, case["group_el"], case["flip"], case["chars"], case["table"], 0, drop,
+                                  np.arange(len(drop)), W, False)
+    # a listed state that is not its orbit's minimum
+    bad = reps.copy()
+    s = int(bad[5])
+    img = max(oracle.apply_elem(s, case["group_el"][g], int(case["flip"][g]), 12) for g in range(len(case["flip"])))
+    bad[5] = img
+    with pytest.raises(ValueError, match="minimum"):
+        oracle.ell_rows_from_reps(12, case["group_el"], case["flip"], case["chars"], case["table"], 0, bad,
+                                  np.array([5]), W, False)
+
+
+def test_burnside_count_matches_orbit_build():
+    """The Burnside counting rule used for the full-size sector tests, against the
+    oracle's orbit build (with and without spin flip, odd N, 2-D)."""
+    for case, n_up in ((chain_case(12, flip=True), 6), (square_case(4, 4, flip=True), 8), (chain_case(13), 6),
+                       (square_case(4, 3), 6), (chain_case(16, flip=True), 8)):
+        orb = oracle.Orbits(case["n_sites"], n_up, case["group_el"], case["chars"][:1], flip=case["flip"])
+        assert burnside_count(case, n_up) == orb.n_orb
+    # the configured sector sizes
+    assert burnside_count(chain_case(28, flip=True), 14) == 716728
+    assert burnside_count(chain_case(30), 15) == 5170604
