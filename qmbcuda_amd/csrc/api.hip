@@ -8,6 +8,7 @@
 #include <unordered_map>
 
 #include "internal.hpp"
+#include "terms.hpp"
 
 namespace qmb {
 
@@ -462,6 +463,11 @@ qmb_status qmb_ctx_create(int device, qmb_ctx **out) {
             throw Error{QMB_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
         }
         c->pool.stream = c->stream;
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
+            c->pool.max_cached = std::max<size_t>(c->pool.max_cached, total_b / 4 * 3);
+        else
+            (void)hipGetLastError();
         try {
             ensure_binom(c);
         } catch (...) {
@@ -575,6 +581,7 @@ qmb_status qmb_group_create(int n_sites, int n_elems, const uint32_t *group_el, 
         mix(g->flip.data(), g->flip.size());
         mix(g->chars.data(), g->chars.size() * sizeof(double));
         g->hash = hsh;
+        g->orbit_norms = orbit_norm_table(n_elems);
         detect_torus(*g);
         build_lut(*g);
         if (!g->lut) build_coset_ring(*g);
@@ -631,6 +638,11 @@ qmb_status qmb_terms_create(int n_sites, int n_terms, int max_ops, const int32_t
                 dn += ty == QMB_OP_SM;
             }
             QMB_REQUIRE(up == dn, "qmb_terms_create: term " + std::to_string(h) + " does not conserve Sz");
+        }
+        t->compiled.resize((size_t)n_terms * sizeof(TermC));
+        for (int h = 0; h < n_terms; ++h) {
+            const TermC c = compile_term(*t, h);
+            std::memcpy(t->compiled.data() + (size_t)h * sizeof(TermC), &c, sizeof(TermC));
         }
         *out = t.release();
     });

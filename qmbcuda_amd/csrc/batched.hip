@@ -61,7 +61,9 @@ struct BTask {
 
 constexpr int kVecPerBlock = kThreads * 8;
 
-template <bool CPLX>
+// NT: matrix stream past the caches (nontemporal) or plain loads; U: slots per
+// group of the packed row products. Both are picked by the autotuner.
+template <bool CPLX, bool NT, int U>
 __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ insts, const BTask *__restrict__ tasks,
                                                     const int *__restrict__ active, const int *__restrict__ it_p,
                                                     double *__restrict__ part) {
@@ -83,9 +85,10 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
         const int64_t p0 = base + lane;
         if (!CPLX) {
             const double xr = r < I.n_rows ? I.P[r] : 0.0;
-            const double acc = (I.packed ? sell_row_packed<true>(reinterpret_cast<const uint32_t *>(I.col), I.vtab, I.P, p0,
-                                                           L, len, I.col_bits, r < I.n_rows ? I.diag[r] : 0.0, xr)
-                                         : sell_row_real<true>(I.col, I.val, I.P, p0, L, len)) *
+            const double acc = (I.packed ? sell_row_packed<NT, U>(reinterpret_cast<const uint32_t *>(I.col), I.vtab,
+                                                                I.P, p0, L, len, I.col_bits,
+                                                                r < I.n_rows ? I.diag[r] : 0.0, xr)
+                                         : sell_row_real<NT>(I.col, I.val, I.P, p0, L, len)) *
                                s;
             if (r < I.n_rows) {
                 I.Y[r] = acc;
@@ -96,12 +99,12 @@ __global__ __launch_bounds__(kThreads) void bk_spmv(const BInst *__restrict__ in
         } else {
             const double2 xr = r < I.n_rows ? reinterpret_cast<const double2 *>(I.P)[r] : make_double2(0.0, 0.0);
             double2 a = I.packed
-                            ? sell_row_cplx_packed<true>(reinterpret_cast<const uint32_t *>(I.col), I.vtab,
+                            ? sell_row_cplx_packed<NT, U>(reinterpret_cast<const uint32_t *>(I.col), I.vtab,
                                                    I.vtab + kMaxValues, I.P, p0, L, len, I.col_bits, I.re_bits,
                                                    r < I.n_rows ? reinterpret_cast<const double2 *>(I.diag)[r]
                                                                 : make_double2(0.0, 0.0),
                                                    xr)
-                            : sell_row_cplx<true>(I.col, I.val, I.P, p0, L, len);
+                            : sell_row_cplx<NT>(I.col, I.val, I.P, p0, L, len);
             a.x *= s;
             a.y *= s;
             if (r < I.n_rows) {
@@ -410,25 +413,49 @@ struct Family {
     std::vector<BTask> spmv_tasks, vec_tasks;
     DevBuf d_spmv_tasks, d_vec_tasks;
     int spmv_first_block = 0, vec_first_block = 0;
-    int cpb = 4;
+    int cpb = 4;      // chunks per block
+    int variant = 0;  // index into spmv_variants()
 };
 
-// (complex, log2 rows, instances) -> chunks per block; shared by every context
-// of the process, hence the lock
-static std::map<std::tuple<int, int, int>, int> g_tile_cache;
+using SpmvKernel = void (*)(const BInst *, const BTask *, const int *, const int *, double *);
+
+// The SpMV shapes the autotuner chooses among, per family: (stream policy,
+// slots per group). Entry 0 is the streamed 8-slot form.
+struct SpmvVariant {
+    bool nt;
+    int slots;
+    SpmvKernel real, cplx;
+};
+static const std::vector<SpmvVariant> &spmv_variants() {
+    static const std::vector<SpmvVariant> v = {
+        {true, 8, bk_spmv<false, true, 8>, bk_spmv<true, true, 8>},
+        {true, 4, bk_spmv<false, true, 4>, bk_spmv<true, true, 4>},
+        {false, 4, bk_spmv<false, false, 4>, bk_spmv<true, false, 4>},
+        {false, 8, bk_spmv<false, false, 8>, bk_spmv<true, false, 8>},
+    };
+    return v;
+}
+
+struct TileChoice {
+    int cpb, variant;
+};
+
+// (complex, log2 rows, instances) -> tile choice; shared by every context of the
+// process, hence the lock
+static std::map<std::tuple<int, int, int>, TileChoice> g_tile_cache;
 static std::mutex g_tile_mutex;
 
-static bool tile_cache_get(const std::tuple<int, int, int> &key, int &cpb) {
+static bool tile_cache_get(const std::tuple<int, int, int> &key, TileChoice &c) {
     std::lock_guard<std::mutex> lock(g_tile_mutex);
     auto it = g_tile_cache.find(key);
     if (it == g_tile_cache.end()) return false;
-    cpb = it->second;
+    c = it->second;
     return true;
 }
 
-static void tile_cache_put(const std::tuple<int, int, int> &key, int cpb) {
+static void tile_cache_put(const std::tuple<int, int, int> &key, TileChoice c) {
     std::lock_guard<std::mutex> lock(g_tile_mutex);
-    g_tile_cache[key] = cpb;
+    g_tile_cache[key] = c;
 }
 
 }  // namespace qmb
@@ -602,18 +629,25 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         for (auto &f : fam)
             for (size_t q = 0; q < f.members.size(); ++q) hin[f.first_inst + q].vec_first -= f.vec_first_block;
 
-        // autotune the SpMV tile (chunks per block) per family
+        // autotune the SpMV shape per family: chunks per block x (stream policy,
+        // slots per group), each candidate timed over 3 launches after a warm one
         for (auto &f : fam) {
             if (f.members.empty()) continue;
             int64_t rows = 0;
             for (int i : f.members) rows += ells[i]->n_rows;
             const auto key = std::make_tuple((int)f.cplx, (int)std::log2((double)rows + 1.0), (int)f.members.size());
-            if (tile_cache_get(key, f.cpb)) continue;
+            TileChoice tc{};
+            if (tile_cache_get(key, tc)) {
+                f.cpb = tc.cpb;
+                f.variant = tc.variant;
+                continue;
+            }
             float best = 1e30f;
-            int best_cpb = 4;
+            TileChoice best_c{4, 0};
             hipEvent_t ea, eb;
             QMB_HIP(hipEventCreate(&ea));
             QMB_HIP(hipEventCreate(&eb));
+            const auto &vars = spmv_variants();
             for (int cpb : {1, 2, 4, 8, 16, 32}) {
                 int fb = 0;
                 build_spmv_tasks(f, cpb, fb);
@@ -621,28 +655,31 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                 DevBuf dt(f.spmv_tasks.size() * sizeof(BTask));
                 QMB_HIP(hipMemcpy(dt.p, f.spmv_tasks.data(), f.spmv_tasks.size() * sizeof(BTask),
                                   hipMemcpyHostToDevice));
-                auto k = f.cplx ? bk_spmv<true> : bk_spmv<false>;
-                hipLaunchKernelGGL(k, dim3((unsigned)f.spmv_tasks.size()), dim3(kThreads), 0, ctx->stream,
-                                   d_inst.as<BInst>(), dt.as<BTask>(), d_active.as<int>(), d_it.as<int>(),
-                                   part.as<double>());
-                QMB_HIP(hipEventRecord(ea, ctx->stream));
-                for (int rep = 0; rep < 3; ++rep)
+                for (int vi = 0; vi < (int)vars.size(); ++vi) {
+                    const SpmvKernel k = f.cplx ? vars[vi].cplx : vars[vi].real;
                     hipLaunchKernelGGL(k, dim3((unsigned)f.spmv_tasks.size()), dim3(kThreads), 0, ctx->stream,
                                        d_inst.as<BInst>(), dt.as<BTask>(), d_active.as<int>(), d_it.as<int>(),
                                        part.as<double>());
-                QMB_HIP(hipEventRecord(eb, ctx->stream));
-                QMB_HIP(hipEventSynchronize(eb));
-                float t = 0.f;
-                QMB_HIP(hipEventElapsedTime(&t, ea, eb));
-                if (t < best) {
-                    best = t;
-                    best_cpb = cpb;
+                    QMB_HIP(hipEventRecord(ea, ctx->stream));
+                    for (int rep = 0; rep < 3; ++rep)
+                        hipLaunchKernelGGL(k, dim3((unsigned)f.spmv_tasks.size()), dim3(kThreads), 0, ctx->stream,
+                                           d_inst.as<BInst>(), dt.as<BTask>(), d_active.as<int>(), d_it.as<int>(),
+                                           part.as<double>());
+                    QMB_HIP(hipEventRecord(eb, ctx->stream));
+                    QMB_HIP(hipEventSynchronize(eb));
+                    float t = 0.f;
+                    QMB_HIP(hipEventElapsedTime(&t, ea, eb));
+                    if (t < best) {
+                        best = t;
+                        best_c = TileChoice{cpb, vi};
+                    }
                 }
             }
             (void)hipEventDestroy(ea);
             (void)hipEventDestroy(eb);
-            f.cpb = best_cpb;
-            tile_cache_put(key, best_cpb);
+            f.cpb = best_c.cpb;
+            f.variant = best_c.variant;
+            tile_cache_put(key, best_c);
         }
         // final SpMV task tables; spmv block ids are family-local too
         for (auto &f : fam) {
@@ -664,9 +701,9 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                 const unsigned nbs = (unsigned)f.spmv_tasks.size(), nvb = (unsigned)f.vec_tasks.size();
                 const int vs = f.cplx ? 2 : 1;
                 const unsigned nm = (unsigned)f.members.size();
-                hipLaunchKernelGGL(f.cplx ? bk_spmv<true> : bk_spmv<false>, dim3(nbs), dim3(kThreads), 0, s,
-                                   d_inst.as<BInst>(), f.d_spmv_tasks.as<BTask>(), d_active.as<int>(), d_it.as<int>(),
-                                   part.as<double>());
+                const SpmvKernel ks = f.cplx ? spmv_variants()[f.variant].cplx : spmv_variants()[f.variant].real;
+                hipLaunchKernelGGL(ks, dim3(nbs), dim3(kThreads), 0, s, d_inst.as<BInst>(), f.d_spmv_tasks.as<BTask>(),
+                                   d_active.as<int>(), d_it.as<int>(), part.as<double>());
                 hipLaunchKernelGGL(bk_finish, dim3(nm), dim3(kThreads), 0, s, d_inst.as<BInst>(), d_active.as<int>(),
                                    d_it.as<int>(), part.as<double>(), f.first_inst, 0);
                 hipLaunchKernelGGL(bk_update, dim3(nvb), dim3(kThreads), 0, s, d_inst.as<BInst>(),

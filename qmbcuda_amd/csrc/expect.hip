@@ -24,8 +24,8 @@
 // operators and real vectors.
 #include <cmath>
 #include <complex>
-#include <string>
 #include <memory>
+#include <string>
 
 #include "device_common.hpp"
 #include "reduce.hpp"
@@ -243,6 +243,8 @@ qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a,
             d_x.alloc(xbytes);
             QMB_HIP(hipMemcpyAsync(d_x.p, x, xbytes, hipMemcpyHostToDevice, ctx->stream));
             xd = d_x.as<double>();
+        } else {
+            require_device_range(ctx, x, xbytes, "qmb_expectation: x");
         }
         const DeviceTerms dt(*a, ch, g->G);
         const int blocks = (int)std::min<int64_t>(kExpMaxBlocks, (n + kExpThreads - 1) / kExpThreads);

@@ -57,18 +57,23 @@ qmb_status guarded(F &&f) {
 
 // ---- device memory ------------------------------------------------------------
 // A context caches the device blocks its buffers release and hands them out
-// again (hipMalloc/hipFree cost 10-20 us each, about 40 per solve). A released
-// block is reused only once the context's stream has drained, so no queued
-// kernel and no synchronous copy can touch a block that changed hands; blocks
-// larger than 4 GiB are freed at once and the cache holds at most 16 GiB.
+// again: hipMalloc/hipFree cost 10-20 us each (about 40 per small solve), and
+// memory freed to the driver is scrubbed before it can be mapped again, which
+// made a repeated 6x6 solve (~170 GB of matrix, temporaries and kept Lanczos
+// vectors) several times slower than the first. A released block is reused only
+// once the context's stream has drained, so no queued kernel and no synchronous
+// copy can touch a block that changed hands. Blocks of 1 GiB and more are reused
+// for requests within 1/8 of their size, smaller ones within a factor 2; the
+// cache holds at most max_cached bytes (3/4 of the device memory) and is
+// emptied when an allocation fails, before it is retried.
 struct DevPool {
     hipStream_t stream = nullptr;
     std::mutex m;
     std::multimap<size_t, void *> ready;    // free blocks by size
     std::vector<std::pair<size_t, void *>> pending;  // released while the stream may still use them
     size_t cached = 0;
-    static constexpr size_t kMaxBlock = (size_t)4 << 30;
-    static constexpr size_t kMaxCached = (size_t)16 << 30;
+    size_t max_cached = (size_t)16 << 30;  // set from the device memory when the context is created
+    static constexpr size_t kBig = (size_t)1 << 30;
 
     void *get(size_t n, size_t &cap) {
         {
@@ -78,7 +83,7 @@ struct DevPool {
                 pending.clear();
             }
             auto it = ready.lower_bound(n);
-            if (it != ready.end() && it->first <= 2 * n) {
+            if (it != ready.end() && it->first <= (n >= kBig ? n + n / 8 : 2 * n)) {
                 void *p = it->second;
                 cap = it->first;
                 cached -= cap;
@@ -108,7 +113,7 @@ struct DevPool {
     void put(void *p, size_t cap) {
         {
             std::lock_guard<std::mutex> lock(m);
-            if (cap <= kMaxBlock && cached + cap <= kMaxCached) {
+            if (cached + cap <= max_cached) {
                 pending.emplace_back(cap, p);
                 cached += cap;
                 return;
@@ -340,12 +345,14 @@ struct qmb_group : qmb::Counted {
     std::vector<uint8_t> comp;     // [G*G]: index of a o b (b applied first)
     int n_cos = 0, cos_rev = 0;    // lut 3: cosets of the ring subgroup and their element table
     std::vector<uint8_t> cos_tab;  // [kCosBytes]
+    std::vector<double> orbit_norms;  // [G + 1]: sqrt(G m)/G for stabiliser size m
 };
 
 struct qmb_terms {
     int n_sites = 0, n_terms = 0, max_ops = 0;
     std::vector<int32_t> op_type, site, n_ops;
     std::vector<double> scalar;  // re, im pairs
+    std::vector<uint8_t> compiled;  // the terms compiled to masks (TermC[n_terms], host), made at creation
 };
 
 struct qmb_basis : qmb::Counted {
@@ -399,6 +406,24 @@ struct qmb_ell : qmb::Counted {
 };
 
 namespace qmb {
+
+// A caller's buffer must be device memory of the context's GPU with `bytes`
+// bytes inside one allocation, or QMB_ERR_INVALID (a host pointer handed to a
+// kernel would fault the GPU).
+inline void require_device_range(const qmb_ctx *ctx, const void *p, size_t bytes, const std::string &what) {
+    hipPointerAttribute_t a{};
+    hipError_t r = hipPointerGetAttributes(&a, p);
+    if (r != hipSuccess) (void)hipGetLastError();
+    QMB_REQUIRE(r == hipSuccess && (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) &&
+                    a.device == ctx->device,
+                what + " is not device memory of the context's GPU");
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    r = hipMemGetAddressRange(&base, &size, const_cast<void *>(p));
+    if (r != hipSuccess) (void)hipGetLastError();
+    QMB_REQUIRE(r == hipSuccess && static_cast<const char *>(p) + bytes <= static_cast<const char *>(base) + size,
+                what + " is smaller than the data it must hold");
+}
 
 // Exclusive scans on device (3-phase). Values come from a functor.
 int64_t scan_popcount_dict(qmb_ctx *ctx, uint64_t *dict, int64_t n_words);

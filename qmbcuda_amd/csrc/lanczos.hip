@@ -1138,16 +1138,6 @@ qmb_ell *ell_from_reference_device(qmb_ctx *ctx, int64_t n, int W, const uint32_
     return e.release();
 }
 
-// The caller's pointer must be device memory of the context's device.
-void require_device_ptr(const qmb_ctx *ctx, const void *p, const char *what) {
-    hipPointerAttribute_t a{};
-    const hipError_t r = hipPointerGetAttributes(&a, p);
-    if (r != hipSuccess) (void)hipGetLastError();
-    QMB_REQUIRE(r == hipSuccess && (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) &&
-                    a.device == ctx->device,
-                std::string("qmb_lanczos_eigvals_ref_device: ") + what + " is not device memory of the context's GPU");
-}
-
 // Fixed NIter steps from the seeded start on a reference-layout ELL: the
 // reference free function's semantics (lanczos.cu:128-317).
 double eigvals_ref_run(qmb_ctx *ctx, qmb_ell *e, double *E_tmp, int NIter, int nobv, long long seed,
@@ -1330,9 +1320,11 @@ double qmb_lanczos_eigvals_ref_device(double *E_tmp, int NStates, int NIter, int
                     "qmb_lanczos_eigvals_ref_device: bad sizes");
         qmb_ctx *ctx = default_ctx();
         QMB_HIP(hipSetDevice(ctx->device));
-        require_device_ptr(ctx, index_mat, "index_mat");
-        require_device_ptr(ctx, vals_mat, "vals_mat");
-        require_device_ptr(ctx, track_vec, "track_vec");
+        const size_t nw = (size_t)nobv * max_terms;
+        require_device_range(ctx, index_mat, nw * sizeof(uint32_t), "qmb_lanczos_eigvals_ref_device: index_mat");
+        require_device_range(ctx, vals_mat, nw * (is_complex ? 2 : 1) * sizeof(double),
+                             "qmb_lanczos_eigvals_ref_device: vals_mat");
+        require_device_range(ctx, track_vec, (size_t)nobv * sizeof(int16_t), "qmb_lanczos_eigvals_ref_device: track_vec");
         std::unique_ptr<qmb_ell> e(
             ell_from_reference_device(ctx, nobv, max_terms, index_mat, vals_mat, track_vec, is_complex != 0));
         ret = eigvals_ref_run(ctx, e.get(), E_tmp, NIter, nobv, seed, eig_vec_Krylov_out, exportKrylovVec,

@@ -204,15 +204,19 @@ __global__ __launch_bounds__(64) void mlp_wgrad2(int n_rows, const uint16_t *__r
 }
 
 // ---- backward, small parameters (VALU) over a group of rows ----------------
-// per group partial of W1, b1, b2, W3, b3 (W2 comes from mlp_wgrad2).
+// per group partial of W1, b1, b2, W3, b3 (W2 comes from mlp_wgrad2). Groups of
+// 256 rows (one coupling's patterns): four times as many blocks as the MFMA
+// weight gradient's groups, so the row loops are short.
+constexpr int kRowsPerSmallGroup = 256;
+
 __global__ __launch_bounds__(kH) void mlp_wgrad_small(const double *__restrict__ coup, int n_rows,
                                                       const float *__restrict__ dout, const float *__restrict__ H2,
                                                       const float *__restrict__ dA2, const float *__restrict__ dA1,
                                                       float *__restrict__ part) {
     const int k = threadIdx.x;  // hidden unit
-    const int g0 = blockIdx.x * kRowsPerGroup;
+    const int g0 = blockIdx.x * kRowsPerSmallGroup;
     float gw1[kIn] = {}, gb1 = 0.f, gb2 = 0.f, gw3a = 0.f, gw3b = 0.f, gb3a = 0.f, gb3b = 0.f;
-    for (int r = g0; r < g0 + kRowsPerGroup && r < n_rows; ++r) {
+    for (int r = g0; r < g0 + kRowsPerSmallGroup && r < n_rows; ++r) {
         const int p = r & 255, c = r >> 8;
         const float d1 = dA1[(size_t)r * kH + k];
         for (int i = 0; i < kWin; ++i) gw1[i] += d1 * (((p >> i) & 1) ? 1.f : -1.f);
@@ -240,15 +244,16 @@ __global__ __launch_bounds__(kH) void mlp_wgrad_small(const double *__restrict__
     }
 }
 
-// grad = scale * sum over groups of the partials (W2 from part2, the rest from part).
-__global__ __launch_bounds__(256) void mlp_reduce_grad(int n_groups, const float *__restrict__ part,
+// grad = scale * sum over groups of the partials (W2 from part2 over n_groups2
+// MFMA groups, the rest from part over n_groups small groups).
+__global__ __launch_bounds__(256) void mlp_reduce_grad(int n_groups, int n_groups2, const float *__restrict__ part,
                                                        const float *__restrict__ part2, float scale,
                                                        float *__restrict__ grad) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kParams) return;
     float s = 0.f;
     if (i >= oW2 && i < ob2) {
-        for (int g = 0; g < n_groups; ++g) s += part2[(size_t)g * kH * kH + (i - oW2)];
+        for (int g = 0; g < n_groups2; ++g) s += part2[(size_t)g * kH * kH + (i - oW2)];
     } else {
         for (int g = 0; g < n_groups; ++g) s += part[(size_t)g * kParams + i];
     }
@@ -535,6 +540,190 @@ __global__ __launch_bounds__(256) void mlp_finish_sum2(const double *__restrict_
     }
 }
 
+// ---- the per-sample kernels batched over every sample of a training step ----
+// One launch per phase covers all samples: block b works on sample task[b],
+// with the same row-to-thread assignment and the same partial-sum layout per
+// sample as the single-sample kernels above, so the loss and gradient are the
+// ones those kernels give, bit for bit.
+struct BSample {
+    int64_t n, off;  // rows; offset of the sample's rows in the concatenated alpha/beta
+    const uint64_t *reps;
+    const uint8_t *stab;
+    const double *u;
+    const uint8_t *win;
+    uint64_t sub_a;
+    int n_win, G, blk0, nblk;  // blocks [blk0, blk0 + nblk) of every batched launch
+};
+
+__global__ __launch_bounds__(256) void mlp_batch_logpsi(const BSample *__restrict__ S, const int *__restrict__ task,
+                                                        const float *__restrict__ tabs, double *__restrict__ alpha,
+                                                        double *__restrict__ beta, double *__restrict__ part) {
+    __shared__ float t[512];
+    __shared__ double lds[kWaves];
+    __shared__ uint8_t wl[kMaxWin * kWin];
+    const int si = task[blockIdx.x];
+    const BSample s = S[si];
+    const float *tab = tabs + (size_t)si * 512;
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) t[i] = tab[i];
+    stage_windows(wl, s.win, s.n_win);
+    __syncthreads();
+    const int lb = blockIdx.x - s.blk0;
+    double mx = -1e300;
+    for (int64_t r = (int64_t)lb * blockDim.x + threadIdx.x; r < s.n; r += (int64_t)s.nblk * blockDim.x) {
+        const uint64_t st = s.reps[r];
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < s.n_win; ++q) {
+            const int p = window_pattern(st, wl + q * kWin);
+            a += (double)t[2 * p];
+            b += (double)t[2 * p + 1];
+        }
+        alpha[s.off + r] = a;
+        beta[s.off + r] = b + kMarshallPi * (double)__popcll(st & s.sub_a);
+        mx = fmax(mx, a);
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lds[0];
+        for (int i = 1; i < kWaves; ++i) m = fmax(m, lds[i]);
+        part[blockIdx.x] = m;
+    }
+}
+
+// One block per sample: stats[8 si + 5] = max of the sample's block maxima.
+__global__ __launch_bounds__(256) void mlp_batch_finish_max(const BSample *__restrict__ S,
+                                                            const double *__restrict__ part,
+                                                            double *__restrict__ stats) {
+    __shared__ double lds[kWaves];
+    const BSample s = S[blockIdx.x];
+    double mx = -1e300;
+    for (int i = threadIdx.x; i < s.nblk; i += blockDim.x) mx = fmax(mx, part[s.blk0 + i]);
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lds[0];
+        for (int i = 1; i < kWaves; ++i) m = fmax(m, lds[i]);
+        stats[(size_t)blockIdx.x * 8 + 5] = m;
+    }
+}
+
+// Partials of Re/Im <u, c>, <c, c>, <u, u>: part[q * total + block].
+__global__ __launch_bounds__(256) void mlp_batch_overlap(const BSample *__restrict__ S, const int *__restrict__ task,
+                                                         const double *__restrict__ alpha,
+                                                         const double *__restrict__ beta,
+                                                         const double *__restrict__ stats, double *__restrict__ part,
+                                                         int total) {
+    __shared__ double lds[kWaves];
+    const int si = task[blockIdx.x];
+    const BSample s = S[si];
+    const double am = stats[(size_t)si * 8 + 5];
+    const int lb = blockIdx.x - s.blk0;
+    double o_re = 0, o_im = 0, nc = 0, nu = 0;
+    for (int64_t r = (int64_t)lb * blockDim.x + threadIdx.x; r < s.n; r += (int64_t)s.nblk * blockDim.x) {
+        const double mag = exp(alpha[s.off + r] - am) * sqrt((double)s.G / (double)s.stab[r]);
+        const double br = beta[s.off + r];
+        const double cr = mag * cos(br), ci = mag * sin(br);
+        const double ur = s.u[r];
+        o_re += ur * cr;
+        o_im += ur * ci;
+        nc += mag * mag;
+        nu += ur * ur;
+    }
+    const double v[4] = {o_re, o_im, nc, nu};
+    for (int q = 0; q < 4; ++q) {
+        const double t = block_sum(v[q], lds);
+        if (threadIdx.x == 0) part[(size_t)q * total + blockIdx.x] = t;
+    }
+}
+
+// One block per sample: stats[8 si + 0..3] = Re O, Im O, Nc, Nu; stats[8 si + 4] = loss.
+__global__ __launch_bounds__(256) void mlp_batch_finish_overlap(const BSample *__restrict__ S,
+                                                                const double *__restrict__ part, int total,
+                                                                double *__restrict__ stats) {
+    __shared__ double lds[kWaves];
+    const BSample s = S[blockIdx.x];
+    double v[4];
+    for (int q = 0; q < 4; ++q) v[q] = sum_partials(part + (size_t)q * total + s.blk0, s.nblk, lds);
+    if (threadIdx.x == 0) {
+        double *st = stats + (size_t)blockIdx.x * 8;
+        for (int q = 0; q < 4; ++q) st[q] = v[q];
+        const double o2 = v[0] * v[0] + v[1] * v[1];
+        st[4] = 1.0 - o2 / (v[2] * v[3]);
+    }
+}
+
+__global__ __launch_bounds__(256) void mlp_zero_u64(unsigned long long *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+// Per-pattern gradient histograms (fixed point) of every sample: each block adds
+// its LDS histogram into its sample's hist[512] (integer sums, any order).
+__global__ __launch_bounds__(256) void mlp_batch_grad(const BSample *__restrict__ S, const int *__restrict__ task,
+                                                      const double *__restrict__ alpha,
+                                                      const double *__restrict__ beta,
+                                                      const double *__restrict__ stats,
+                                                      unsigned long long *__restrict__ hist) {
+    __shared__ unsigned long long h[512];
+    __shared__ uint8_t wl[kMaxWin * kWin];
+    const int si = task[blockIdx.x];
+    const BSample s = S[si];
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) h[i] = 0ull;
+    stage_windows(wl, s.win, s.n_win);
+    __syncthreads();
+    const double *st = stats + (size_t)si * 8;
+    const double am = st[5];
+    const double ore = st[0], oim = st[1], nc = st[2], nu = st[3];
+    const double o2 = ore * ore + oim * oim;
+    const int lb = blockIdx.x - s.blk0;
+    for (int64_t r = (int64_t)lb * blockDim.x + threadIdx.x; r < s.n; r += (int64_t)s.nblk * blockDim.x) {
+        const double mag = exp(alpha[s.off + r] - am) * sqrt((double)s.G / (double)s.stab[r]);
+        const double br = beta[s.off + r];
+        const double cr = mag * cos(br), ci = mag * sin(br);
+        const double wr = s.u[r] * (ore * cr + oim * ci);
+        const double wi = s.u[r] * (ore * ci - oim * cr);
+        const double ga = -(2.0 * wr * nc - o2 * 2.0 * mag * mag) / (nc * nc * nu);
+        const double gb = -(-2.0 * wi) / (nc * nu);
+        const long long qa = (long long)llrint(ga * kFix), qb = (long long)llrint(gb * kFix);
+        const uint64_t stt = s.reps[r];
+        for (int q = 0; q < s.n_win; ++q) {
+            const int p = window_pattern(stt, wl + q * kWin);
+            atomicAdd(&h[2 * p], (unsigned long long)qa);
+            atomicAdd(&h[2 * p + 1], (unsigned long long)qb);
+        }
+    }
+    __syncthreads();
+    unsigned long long *hs = hist + (size_t)si * 512;
+    for (int i = threadIdx.x; i < 512; i += blockDim.x)
+        if (h[i]) atomicAdd(&hs[i], h[i]);
+}
+
+// dout rows of every sample (512 per sample) from its histogram; loss = mean of
+// the samples' losses summed in sample order.
+__global__ __launch_bounds__(256) void mlp_batch_finish(const unsigned long long *__restrict__ hist, int ns,
+                                                        float *__restrict__ dout, float scale,
+                                                        const double *__restrict__ stats, double *__restrict__ loss) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)ns * 512;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dout[i] = (float)((double)(long long)hist[i] / kFix * scale);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double mean = 0.0;
+        for (int k = 0; k < ns; ++k) mean += stats[(size_t)k * 8 + 4];
+        *loss = mean / ns;
+    }
+}
+
+// Loss only: the mean of the samples' losses.
+__global__ __launch_bounds__(64) void mlp_batch_loss(const double *__restrict__ stats, int ns,
+                                                     double *__restrict__ loss) {
+    if (threadIdx.x != 0) return;
+    double mean = 0.0;
+    for (int k = 0; k < ns; ++k) mean += stats[(size_t)k * 8 + 4];
+    *loss = mean / ns;
+}
+
 }  // namespace qmb
 
 using namespace qmb;
@@ -555,12 +744,22 @@ struct MlpSample {
     double delta, j2;
 };
 
+// Device tables and work arrays of the batched per-sample kernels, rebuilt when
+// the sample list changes.
+struct MlpBatch {
+    bool valid = false;
+    int ns = 0, total_blocks = 0;
+    int64_t total_rows = 0;
+    DevBuf samples, task, alpha, beta, part, stats, hist, loss;
+};
+
 struct qmb_mlp : Counted {
     Ref<qmb_ctx> ctx;
     DevBuf theta, m1, m2, grad, w2bf, w2t;
     int64_t step = 0;
     std::vector<MlpGeom> geoms;
     std::vector<MlpSample> samples;
+    MlpBatch batch;
     // row buffers (sized for the current row count)
     int rows_cap = 0;
     DevBuf coup, out, H1, H1q, H2, dout, dA2, dA2q, dA1, part, part2;
@@ -580,7 +779,8 @@ void qmb_mlp::ensure_rows(int rows) {
     dA2q.alloc((size_t)cap * kH * sizeof(uint16_t));
     dA1.alloc((size_t)cap * kH * sizeof(float));
     const int groups = (cap + kRowsPerGroup - 1) / kRowsPerGroup;
-    part.alloc((size_t)groups * kParams * sizeof(float));
+    const int groups_s = (cap + kRowsPerSmallGroup - 1) / kRowsPerSmallGroup;
+    part.alloc((size_t)groups_s * kParams * sizeof(float));
     part2.alloc((size_t)groups * kH * kH * sizeof(float));
     rows_cap = cap;
 }
@@ -660,28 +860,88 @@ void state_logpsi(qmb_ctx *ctx, const qmb_basis *b, const MlpGeom &gm, const flo
     QMB_HIP(hipGetLastError());
 }
 
-// Loss (and dout rows of this sample when dout_rows != NULL) of one sample.
-void sample_loss(qmb_ctx *ctx, const MlpSample &s, const MlpGeom &gm, const float *tab, StateWork &w,
-                 float *dout_rows, float scale) {
-    const qmb_basis *b = s.basis;
-    state_logpsi(ctx, b, gm, tab, w);
-    hipLaunchKernelGGL(mlp_state_overlap, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, b->group->G,
-                       b->stab.as<uint8_t>(), w.alpha.as<double>(), w.beta.as<double>(), s.u.as<double>(),
-                       w.amax.as<double>(), w.part.as<double>(), w.nb);
-    QMB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(mlp_finish_overlap, dim3(1), dim3(256), 0, ctx->stream, w.part.as<double>(), w.nb,
-                       w.stats.as<double>());
-    QMB_HIP(hipGetLastError());
-    if (dout_rows) {
-        hipLaunchKernelGGL(mlp_state_grad, dim3(w.nb), dim3(256), 0, ctx->stream, b->n_rows, gm.n_win,
-                           gm.win.as<uint8_t>(), b->group->G, b->reps.as<uint64_t>(), b->stab.as<uint8_t>(), w.alpha.as<double>(),
-                           w.beta.as<double>(), s.u.as<double>(), w.amax.as<double>(), w.stats.as<double>(),
-                           w.hist.as<unsigned long long>());
-        QMB_HIP(hipGetLastError());
-        hipLaunchKernelGGL(mlp_finish_hist, dim3(512 / kWaves), dim3(256), 0, ctx->stream,
-                           w.hist.as<unsigned long long>(), w.nb, dout_rows, scale);
-        QMB_HIP(hipGetLastError());
+// Sample table and work arrays of the batched kernels: each sample keeps the
+// block count of the single-sample kernels (min(1024, ceil(n/256))).
+void ensure_batch(qmb_mlp *m) {
+    MlpBatch &B = m->batch;
+    if (B.valid) return;
+    const int ns = (int)m->samples.size();
+    std::vector<BSample> hs(ns);
+    std::vector<int> task;
+    int64_t off = 0;
+    for (int i = 0; i < ns; ++i) {
+        const MlpSample &sm = m->samples[i];
+        const MlpGeom &gm = m->geoms[sm.geom];
+        const qmb_basis *b = sm.basis;
+        BSample &d = hs[i];
+        d.n = b->n_rows;
+        d.off = off;
+        d.reps = b->reps.as<uint64_t>();
+        d.stab = b->stab.as<uint8_t>();
+        d.u = sm.u.as<double>();
+        d.win = gm.win.as<uint8_t>();
+        d.sub_a = gm.sub_a;
+        d.n_win = gm.n_win;
+        d.G = b->group->G;
+        d.blk0 = (int)task.size();
+        d.nblk = (int)std::min<int64_t>(1024, (b->n_rows + 255) / 256);
+        for (int k = 0; k < d.nblk; ++k) task.push_back(i);
+        off += b->n_rows;
     }
+    B.ns = ns;
+    B.total_blocks = (int)task.size();
+    B.total_rows = off;
+    B.samples.alloc((size_t)ns * sizeof(BSample));
+    B.task.alloc(task.size() * sizeof(int));
+    B.alpha.alloc((size_t)off * sizeof(double));
+    B.beta.alloc((size_t)off * sizeof(double));
+    B.part.alloc((size_t)B.total_blocks * 4 * sizeof(double));
+    B.stats.alloc((size_t)ns * 8 * sizeof(double));
+    B.hist.alloc((size_t)ns * 512 * sizeof(unsigned long long));
+    B.loss.alloc(sizeof(double));
+    QMB_HIP(hipMemcpy(B.samples.p, hs.data(), hs.size() * sizeof(BSample), hipMemcpyHostToDevice));
+    QMB_HIP(hipMemcpy(B.task.p, task.data(), task.size() * sizeof(int), hipMemcpyHostToDevice));
+    B.valid = true;
+}
+
+// Loss (and the dout rows of every sample when want_grad) of all samples, one
+// launch per phase; the mean loss is left in batch.loss.
+void batch_loss(qmb_mlp *m, bool want_grad) {
+    qmb_ctx *ctx = m->ctx;
+    ensure_batch(m);
+    MlpBatch &B = m->batch;
+    const BSample *S = B.samples.as<BSample>();
+    const int *task = B.task.as<int>();
+    const unsigned nb = (unsigned)B.total_blocks, ns = (unsigned)B.ns;
+    hipLaunchKernelGGL(mlp_batch_logpsi, dim3(nb), dim3(256), 0, ctx->stream, S, task, m->out.as<float>(),
+                       B.alpha.as<double>(), B.beta.as<double>(), B.part.as<double>());
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_batch_finish_max, dim3(ns), dim3(256), 0, ctx->stream, S, B.part.as<double>(),
+                       B.stats.as<double>());
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_batch_overlap, dim3(nb), dim3(256), 0, ctx->stream, S, task, B.alpha.as<double>(),
+                       B.beta.as<double>(), B.stats.as<double>(), B.part.as<double>(), B.total_blocks);
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_batch_finish_overlap, dim3(ns), dim3(256), 0, ctx->stream, S, B.part.as<double>(),
+                       B.total_blocks, B.stats.as<double>());
+    QMB_HIP(hipGetLastError());
+    if (!want_grad) {
+        hipLaunchKernelGGL(mlp_batch_loss, dim3(1), dim3(64), 0, ctx->stream, B.stats.as<double>(), B.ns,
+                           B.loss.as<double>());
+        QMB_HIP(hipGetLastError());
+        return;
+    }
+    const int64_t nh = (int64_t)B.ns * 512;
+    hipLaunchKernelGGL(mlp_zero_u64, dim3((unsigned)std::min<int64_t>(1024, (nh + 255) / 256)), dim3(256), 0,
+                       ctx->stream, B.hist.as<unsigned long long>(), nh);
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_batch_grad, dim3(nb), dim3(256), 0, ctx->stream, S, task, B.alpha.as<double>(),
+                       B.beta.as<double>(), B.stats.as<double>(), B.hist.as<unsigned long long>());
+    QMB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mlp_batch_finish, dim3((unsigned)std::min<int64_t>(1024, (nh + 255) / 256)), dim3(256), 0,
+                       ctx->stream, B.hist.as<unsigned long long>(), B.ns, m->dout.as<float>(), 1.0f / B.ns,
+                       B.stats.as<double>(), B.loss.as<double>());
+    QMB_HIP(hipGetLastError());
 }
 
 // Learned start vector of a k=0 ring basis into dx (device, n_rows values),
@@ -729,14 +989,7 @@ double loss_and_grad(qmb_mlp *m, bool want_grad) {
     }
     forward_rows(m, coup, want_grad);
     const int rows = ns * 256;
-    StateWork w;
-    DevBuf losses((size_t)ns * sizeof(double));
-    for (int i = 0; i < ns; ++i) {
-        sample_loss(ctx, m->samples[i], m->geoms[m->samples[i].geom], m->out.as<float>() + (size_t)i * 512, w,
-                    want_grad ? m->dout.as<float>() + (size_t)i * 512 : nullptr, 1.0f / ns);
-        QMB_HIP(hipMemcpyAsync(losses.as<double>() + i, w.stats.as<double>() + 4, sizeof(double),
-                               hipMemcpyDeviceToDevice, ctx->stream));
-    }
+    batch_loss(m, want_grad);
     if (want_grad) {
         hipLaunchKernelGGL(mlp_backward_act, dim3((rows + 31) / 32), dim3(64), 0, ctx->stream, m->theta.as<float>(),
                            m->w2t.as<uint16_t>(), rows, m->dout.as<float>(), m->H1.as<float>(), m->H2.as<float>(),
@@ -746,20 +999,19 @@ double loss_and_grad(qmb_mlp *m, bool want_grad) {
         hipLaunchKernelGGL(mlp_wgrad2, dim3(groups, 16), dim3(64), 0, ctx->stream, rows, m->dA2q.as<uint16_t>(),
                            m->H1q.as<uint16_t>(), m->part2.as<float>());
         QMB_HIP(hipGetLastError());
-        hipLaunchKernelGGL(mlp_wgrad_small, dim3(groups), dim3(kH), 0, ctx->stream, m->coup.as<double>(), rows,
+        const int groups_s = (rows + kRowsPerSmallGroup - 1) / kRowsPerSmallGroup;
+        hipLaunchKernelGGL(mlp_wgrad_small, dim3(groups_s), dim3(kH), 0, ctx->stream, m->coup.as<double>(), rows,
                            m->dout.as<float>(), m->H2.as<float>(), m->dA2.as<float>(), m->dA1.as<float>(),
                            m->part.as<float>());
         QMB_HIP(hipGetLastError());
-        hipLaunchKernelGGL(mlp_reduce_grad, dim3((kParams + 255) / 256), dim3(256), 0, ctx->stream, groups,
+        hipLaunchKernelGGL(mlp_reduce_grad, dim3((kParams + 255) / 256), dim3(256), 0, ctx->stream, groups_s, groups,
                            m->part.as<float>(), m->part2.as<float>(), 1.0f, m->grad.as<float>());
         QMB_HIP(hipGetLastError());
     }
-    std::vector<double> hl(ns);
-    QMB_HIP(hipMemcpyAsync(hl.data(), losses.p, ns * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    double *hl = ctx->host_stage<double>(sizeof(double));
+    QMB_HIP(hipMemcpyAsync(hl, m->batch.loss.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     QMB_HIP(hipStreamSynchronize(ctx->stream));
-    double mean = 0.0;
-    for (double v : hl) mean += v;
-    return mean / ns;
+    return *hl;
 }
 
 std::vector<float> init_params(uint64_t seed) {
@@ -864,6 +1116,7 @@ qmb_status qmb_mlp_add_sample(qmb_mlp *m, const qmb_basis *b, const double *grou
         s.u.alloc((size_t)b->n_rows * sizeof(double));
         QMB_HIP(hipMemcpy(s.u.p, ground_state, (size_t)b->n_rows * sizeof(double), hipMemcpyHostToDevice));
         m->samples.push_back(std::move(s));
+        m->batch.valid = false;
     });
 }
 
@@ -900,6 +1153,7 @@ qmb_status qmb_mlp_clear_samples(qmb_mlp *m) {
         QMB_REQUIRE(m, "qmb_mlp_clear_samples: NULL argument");
         QMB_HIP(hipStreamSynchronize(m->ctx->stream));
         m->samples.clear();
+        m->batch = MlpBatch{};
     });
 }
 

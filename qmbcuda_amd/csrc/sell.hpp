@@ -76,17 +76,16 @@ __device__ inline double sell_row_real(const int32_t *__restrict__ col, const do
 // Packed real rows: slot 0 is the diagonal d (column = the row r itself); slot
 // k >= 1 holds w = col | (index << cb) with the value vt[index]. Same products in
 // the same order as sell_row_real on the unpacked matrix.
-template <bool NT>
+// Slots are taken in groups of U from slot 1 (8 when the matrix is streamed
+// from HBM: more loads in flight; 4 when it is cached); the next group's words
+// are loaded before this group's gathers, so the two latencies overlap.
+template <bool NT, int U = (NT ? 8 : 4)>
 __device__ inline double sell_row_packed(const uint32_t *__restrict__ w, const double *vt,
                                          const double *__restrict__ x, int64_t p0, int L, int len, int cb,
                                          double d, double xr) {
     const uint32_t mask = (1u << cb) - 1u;
     double acc = 0.0;
     if (len > 0) acc += d * xr;
-    // groups of U slots from slot 1 (8 when the matrix is streamed from HBM: more
-    // loads in flight; 4 when it is cached); the next group's words are loaded
-    // before this group's gathers, so the two latencies overlap
-    constexpr int U = NT ? 8 : 4;
     const int n4 = (L - 1) / U;
     uint32_t jn[U];
 #pragma unroll
@@ -174,8 +173,8 @@ __device__ inline double2 sell_row_cplx(const int32_t *__restrict__ col, const d
 // Packed complex rows: slot 0 is the diagonal d; slot k >= 1 holds
 // w = col | (re index << cb) | (im index << (cb + rb)) with the value
 // (vre[re index], vim[im index]). Same products in the same order as
-// sell_row_cplx on the unpacked matrix.
-template <bool NT>
+// sell_row_cplx on the unpacked matrix; groups of U slots as in sell_row_packed.
+template <bool NT, int U = 4>
 __device__ inline double2 sell_row_cplx_packed(const uint32_t *__restrict__ w, const double *vre, const double *vim,
                                                const double *__restrict__ x, int64_t p0, int L, int len, int cb,
                                                int rb, double2 d, double2 xr) {
@@ -188,26 +187,28 @@ __device__ inline double2 sell_row_cplx_packed(const uint32_t *__restrict__ w, c
         ai += d.x * xr.y + d.y * xr.x;
     }
     // next group's words loaded before this group's gathers (as in sell_row_packed)
-    const int n4 = (L - 1) / 4;
-    uint32_t jn[4] = {0u, 0u, 0u, 0u};
+    const int n4 = (L - 1) / U;
+    uint32_t jn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = 0u;
     if (n4 > 0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < U; ++u)
             if (1 + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(1 + u) * 64);
     }
     int k = 1;
-    for (int q = 0; q < n4; ++q, k += 4) {
-        uint32_t j[4];
+    for (int q = 0; q < n4; ++q, k += U) {
+        uint32_t j[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) j[u] = jn[u];
+        for (int u = 0; u < U; ++u) j[u] = jn[u];
         if (q + 1 < n4) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (k + 4 + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(k + 4 + u) * 64);
+            for (int u = 0; u < U; ++u)
+                if (k + U + u < len) jn[u] = stream_load<NT>(w + p0 + (int64_t)(k + U + u) * 64);
         }
-        double2 g[4], v[4];
+        double2 g[U], v[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             if (k + u < len) {
                 g[u] = x2[j[u] & mask];
                 v[u].x = vre[(j[u] >> cb) & rmask];
@@ -215,7 +216,7 @@ __device__ inline double2 sell_row_cplx_packed(const uint32_t *__restrict__ w, c
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             if (k + u < len) {
                 ar += v[u].x * g[u].x - v[u].y * g[u].y;
                 ai += v[u].x * g[u].y + v[u].y * g[u].x;

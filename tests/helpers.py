@@ -157,3 +157,74 @@ def burnside_count(case, n_up):
         total += poly.get(n_up, 0)
     assert total % ge.shape[0] == 0
     return total // ge.shape[0]
+
+
+class DeviceBuffer:
+    """Device memory for tests that hand device pointers to the C ABI, taken from
+    the HIP runtime libqmb itself links (libamdhip64.so.7). torch ships a HIP
+    runtime of its own, so its tensors cannot be shared with libqmb in one process."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        import ctypes as C
+
+        if cls._hip is None:
+            import qmbcuda_amd as q
+
+            q.lib()  # loads the runtime as libqmb's dependency
+            try:
+                h = C.CDLL("libamdhip64.so.7")
+            except OSError:
+                h = C.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMalloc.restype = C.c_int
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipMemcpy.restype = C.c_int
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipFree.restype = C.c_int
+            h.hipMemGetAddressRange.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_size_t), C.c_void_p]
+            h.hipMemGetAddressRange.restype = C.c_int
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes):
+        import ctypes as C
+
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        assert self.hip().hipMalloc(C.byref(p), max(self.nbytes, 8)) == 0
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes)
+        assert cls.hip().hipMemcpy(buf.ptr, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        return buf
+
+    def reported_size(self):
+        """Size of the allocation the runtime reports for this buffer (what the
+        library's range checks see)."""
+        import ctypes as C
+
+        base, size = C.c_void_p(), C.c_size_t()
+        assert self.hip().hipMemGetAddressRange(C.byref(base), C.byref(size), self.ptr) == 0
+        return base.value, size.value
+
+    def to_array(self, dtype, count):
+        out = np.zeros(count, dtype)
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

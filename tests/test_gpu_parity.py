@@ -15,7 +15,8 @@ import pytest
 import oracle
 import qmbcuda_amd as q
 from oracle import model as OM
-from tests.helpers import chain_case, dense_spectrum, ell_dense_nonnull, oracle_ell, oracle_orbits, square_case
+from tests.helpers import (DeviceBuffer, chain_case, dense_spectrum, ell_dense_nonnull, oracle_ell, oracle_orbits,
+                           square_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -330,8 +331,12 @@ def test_eigenvector_second_pass(ctx):
     _, _, b, e = gpu_objects(ctx, case, 7, 0, q.LAYOUT_COMPACT)
     ev, rep, kr, x = q.lanczos(e, max_iter=300, tol=1e-13, want_eigvec=True)
     assert abs(np.linalg.norm(x) - 1) < 1e-8
-    assert rep.eig_residual < 1e-5
-    np.testing.assert_allclose(E @ x, ev[0] * x, atol=1e-5)
+    # the convergence test guarantees a Ritz residual <= sqrt(tol) max(1, |E|); the
+    # rebuilt vector's explicit residual must meet it (factor 2 for rounding)
+    bound = 2.0 * np.sqrt(1e-13) * max(1.0, abs(ev[0]))
+    assert rep.residual[0] <= bound / 2.0
+    assert rep.eig_residual <= bound
+    assert np.linalg.norm(E @ x - ev[0] * x) <= bound
 
 
 def test_reference_signature_drop_in(ctx):
@@ -352,34 +357,36 @@ def test_reference_signature_device_pointers(ctx):
     """qmb_lanczos_eigvals_ref_device: the reference's free function with its ELL
     arrays as borrowed device pointers gives the host-array drop-in's results bit
     for bit; host pointers and inconsistent arrays are refused."""
-    import torch
-
     case = chain_case(16)
     orb = oracle_orbits(case, 8)
     ref = oracle_ell(case, orb, 0)
     say("eigvals_ref_device drop-in chain16")
     E_h, r_h, kr_h = q.eigvals_ref(None, 50, ref.n_rows, ref.W, 2212456, ref.ind, ref.val, ref.track, False,
                                    check=True)
-    d_ind = torch.from_numpy(ref.ind.view(np.int32).copy()).cuda()
-    d_val = torch.from_numpy(ref.val.copy()).cuda()
-    d_tr = torch.from_numpy(ref.track.copy()).cuda()
-    torch.cuda.synchronize()
-    E_d, r_d, kr_d = q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, d_ind.data_ptr(), d_val.data_ptr(),
-                                          d_tr.data_ptr(), False, check=True)
+    d_ind = DeviceBuffer.from_array(ref.ind)
+    d_val = DeviceBuffer.from_array(ref.val)
+    d_tr = DeviceBuffer.from_array(ref.track)
+    E_d, r_d, kr_d = q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, d_ind.ptr, d_val.ptr, d_tr.ptr, False,
+                                          check=True)
     assert E_d == E_h and r_d == r_h and np.array_equal(kr_d, kr_h)
     # a host pointer is not device memory
     with pytest.raises(q.QMBError, match="not device memory"):
-        q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, ref.ind.ctypes.data, d_val.data_ptr(),
-                             d_tr.data_ptr(), False)
+        q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, ref.ind.ctypes.data, d_val.ptr, d_tr.ptr, False)
+    # a device array shorter than nobv * max_terms
+    short = DeviceBuffer.from_array(ref.ind[: len(ref.ind) // 2])
+    if short.reported_size() == (short.ptr, short.nbytes):  # the runtime reports the exact extent
+        with pytest.raises(q.QMBError, match="smaller than"):
+            q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, short.ptr, d_val.ptr, d_tr.ptr, False)
     # a track entry beyond max_terms, then a column beyond nobv
-    bad_tr = d_tr.clone()
+    bad_tr = ref.track.copy()
     bad_tr[3] = ref.W + 1
+    d_bad_tr = DeviceBuffer.from_array(bad_tr)
     with pytest.raises(q.QMBError, match="track out of range"):
-        q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, d_ind.data_ptr(), d_val.data_ptr(),
-                             bad_tr.data_ptr(), False)
-    bad_ind = d_ind.clone()
+        q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, d_ind.ptr, d_val.ptr, d_bad_tr.ptr, False)
+    bad_ind = ref.ind.copy()
     bad_ind[ref.W * 5 + 1] = ref.n_rows + 7
+    d_bad_ind = DeviceBuffer.from_array(bad_ind)
     with pytest.raises(q.QMBError, match="column out of range"):
-        q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, bad_ind.data_ptr(), d_val.data_ptr(),
-                             d_tr.data_ptr(), False)
-    torch.cuda.synchronize()
+        q.eigvals_ref_device(50, ref.n_rows, ref.W, 2212456, d_bad_ind.ptr, d_val.ptr, d_tr.ptr, False)
+    for b_ in (d_ind, d_val, d_tr, short, d_bad_tr, d_bad_ind):
+        b_.free()

@@ -220,7 +220,8 @@ def test_gpu_expectation_c2_full_size(ctx):
     b = q.Basis(ctx, g, 14, 0, q.LAYOUT_COMPACT)
     e = q.Ell(ctx, b, th)
     ev, rep, _, vec = q.lanczos(e, max_iter=500, tol=1e-13, want_eigvec=True)
-    assert rep.converged and rep.eig_residual < 1e-5
+    # the explicit residual of the rebuilt vector meets the convergence test's bound
+    assert rep.converged and rep.eig_residual <= 2.0 * np.sqrt(1e-13) * max(1.0, abs(ev[0]))
     nv = float(vec @ vec)
     eh, avg = b.expectation(th, vec)
     assert not avg
