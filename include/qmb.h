@@ -308,6 +308,19 @@ enum { QMB_EXPECT_GROUP_AVERAGE = 1 };
 qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a, int is_complex,
                            const double *x, int x_on_device, int flags, double *out, int *averaged);
 
+/* qmb_expectation with nothing allocated: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). x_dev must be device
+ * memory of the context's GPU holding the basis's n_rows values (interleaved
+ * re/im when is_complex); the result is the same as qmb_expectation's, bit for
+ * bit. The per-block partial sums, the compiled terms and the group tables live
+ * in a 1 MiB scratch array that the library keeps on each GPU until the process
+ * ends: a table of more than 15292 terms is refused (QMB_ERR_INVALID; use
+ * qmb_expectation). Every call on one GPU shares that scratch: calls that
+ * overlap on one GPU (from several threads or contexts) race. */
+qmb_status qmb_expectation_noalloc(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a, int is_complex,
+                                   const double *x_dev, int flags, double *out, int *averaged);
+
 /* ---- batched Lanczos over many sectors (new; no reference counterpart) -------
  * The reference solves irreps one after another (HeisenbergGenericXXZ_AbelianSymms_CUDA.cu:127-229).
  * This solves n independent ELLs together: one launch per phase covers every

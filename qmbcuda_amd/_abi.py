@@ -118,6 +118,7 @@ SIGNATURES = [
     ("qmb_lanczos_checkpointed", C.c_int, [_vp, _vp, _vp, C.c_char_p, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                            _vp]),
     ("qmb_expectation", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp, C.POINTER(C.c_int)]),
+    ("qmb_expectation_noalloc", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, C.POINTER(C.c_int)]),
     ("qmb_spmv_host", C.c_int, [_vp, _vp, _vp, _vp]),
     ("qmb_spmv_device", C.c_int, [_vp, _vp, _vp, _vp]),
     ("qmb_spmv_benchmark", C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_double)]),

@@ -24,7 +24,9 @@
 // operators and real vectors.
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "device_common.hpp"
@@ -209,11 +211,121 @@ static bool commutes_with_group(const qmb_terms &t, const qmb_group &g) {
     return true;
 }
 
+// ---- the entry point that allocates nothing ----------------------------------
+// Scratch of qmb_expectation_noalloc: one file-scope device array per GPU, held
+// until the process ends. Layout: [0, 256) the result (re, im); [256, 256 +
+// 64 KiB) the per-block partial sums; then the compiled terms, the character row
+// (G double2) and the orbit-norm table (G + 1 doubles), each 256-byte aligned.
+// Host code never uses the array's name as an address: the device address comes
+// from hipGetSymbolAddress, once per device.
+constexpr size_t kScratchBytes = (size_t)1 << 20;
+constexpr size_t kScratchPart = 256;
+constexpr size_t kScratchTerms = kScratchPart + (size_t)kExpMaxBlocks * 2 * sizeof(double);
+constexpr size_t kScratchTabBytes = 4096;  // chi (<= 128 x 16 B) + norms (<= 129 x 8 B), aligned
+constexpr int kNoAllocMaxTerms = (int)((kScratchBytes - kScratchTerms - kScratchTabBytes) / sizeof(TermC));
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on `device` (the current device), looked up once per device.
+static unsigned char *scratch_ptr(int device) {
+    static std::mutex mu;
+    static unsigned char *ptrs[kMaxDevices] = {};
+    QMB_REQUIRE(device >= 0 && device < kMaxDevices, "qmb_expectation_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[device]) {
+        void *p = nullptr;
+        QMB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        QMB_REQUIRE(p != nullptr, "qmb_expectation_noalloc: no device address for the scratch array");
+        ptrs[device] = static_cast<unsigned char *>(p);
+    }
+    return ptrs[device];
+}
+
+// The partial sums in block order, as the host loop of qmb_expectation adds them.
+__global__ __launch_bounds__(64) void expect_finish(const double *__restrict__ partials, int blocks,
+                                                    double *__restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double sr = 0.0, si = 0.0;
+    for (int i = 0; i < blocks; ++i) {
+        sr += partials[2 * i];
+        si += partials[2 * i + 1];
+    }
+    out[0] = sr;
+    out[1] = si;
+}
+
 }  // namespace qmb
 
 using namespace qmb;
 
 extern "C" {
+
+qmb_status qmb_expectation_noalloc(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a, int is_complex,
+                                   const double *x_dev, int flags, double *out, int *averaged) {
+    return guarded([&] {
+        QMB_REQUIRE(ctx && b && a && x_dev && out, "qmb_expectation_noalloc: NULL argument");
+        QMB_REQUIRE(b->ctx == ctx, "qmb_expectation_noalloc: basis belongs to another context");
+        QMB_REQUIRE(a->n_sites == b->n_sites, "qmb_expectation_noalloc: term table and basis disagree on n_sites");
+        QMB_REQUIRE(is_complex == 0 || is_complex == 1, "qmb_expectation_noalloc: is_complex must be 0 or 1");
+        QMB_REQUIRE(flags == 0 || flags == QMB_EXPECT_GROUP_AVERAGE, "qmb_expectation_noalloc: unknown flags");
+        QMB_REQUIRE(a->n_terms <= kNoAllocMaxTerms,
+                    "qmb_expectation_noalloc: " + std::to_string(a->n_terms) + " terms exceed the scratch limit of " +
+                        std::to_string(kNoAllocMaxTerms) + " (use qmb_expectation)");
+        const qmb_group *g = b->group;
+        QMB_REQUIRE((size_t)g->G * 16 + 256 + (size_t)(g->G + 1) * 8 <= kScratchTabBytes,
+                    "qmb_expectation_noalloc: group too large for the scratch tables");
+        const double *ch = &g->chars[(size_t)b->irrep * g->G * 2];
+        if (!is_complex)
+            for (int e = 0; e < g->G; ++e)
+                QMB_REQUIRE(ch[2 * e + 1] == 0.0, "qmb_expectation_noalloc: a real vector needs a real character row");
+        const bool avg = (flags & QMB_EXPECT_GROUP_AVERAGE) || !commutes_with_group(*a, *g);
+        if (averaged) *averaged = avg ? 1 : 0;
+        out[0] = 0.0;
+        out[1] = 0.0;
+        const int64_t n = b->n_rows;
+        if (n == 0 || a->n_terms == 0) return;
+        QMB_HIP(hipSetDevice(ctx->device));
+        QMB_REQUIRE(ctx->binom.p != nullptr, "qmb_expectation_noalloc: context has no binomial table");
+        require_device_range(ctx, x_dev, (size_t)n * (is_complex ? 16 : 8), "qmb_expectation_noalloc: x_dev");
+        unsigned char *sp = scratch_ptr(ctx->device);
+        // tables into the scratch (host copies made when the handles were created)
+        const int nt = a->n_terms;
+        auto up256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
+        const size_t o_chi = kScratchTerms + up256((size_t)nt * sizeof(TermC));
+        const size_t o_nrm = o_chi + up256((size_t)g->G * 16);
+        QMB_HIP(hipMemcpyAsync(sp + kScratchTerms, a->compiled.data(), (size_t)nt * sizeof(TermC),
+                               hipMemcpyHostToDevice, ctx->stream));
+        QMB_HIP(hipMemcpyAsync(sp + o_chi, ch, (size_t)g->G * 16, hipMemcpyHostToDevice, ctx->stream));
+        QMB_HIP(hipMemcpyAsync(sp + o_nrm, g->orbit_norms.data(), (size_t)(g->G + 1) * sizeof(double),
+                               hipMemcpyHostToDevice, ctx->stream));
+        const TermsView tv{nt, reinterpret_cast<const TermC *>(sp + kScratchTerms)};
+        const double2 *chi = reinterpret_cast<const double2 *>(sp + o_chi);
+        const double *nrm = reinterpret_cast<const double *>(sp + o_nrm);
+        double *part = reinterpret_cast<double *>(sp + kScratchPart);
+        double *res = reinterpret_cast<double *>(sp);
+        const int blocks = (int)std::min<int64_t>(kExpMaxBlocks, (n + kExpThreads - 1) / kExpThreads);
+        const bool narrow = b->n_sites <= 32;
+        using K = void (*)(GroupView, const uint64_t *, int, int64_t, const uint64_t *, const uint8_t *,
+                           const uint8_t *, const uint64_t *, TermsView, const double2 *,
+                           const double *, const double *, double *);
+        K k;
+        if (narrow)
+            k = is_complex ? (avg ? expect_rows<uint32_t, true, true> : expect_rows<uint32_t, true, false>)
+                           : (avg ? expect_rows<uint32_t, false, true> : expect_rows<uint32_t, false, false>);
+        else
+            k = is_complex ? (avg ? expect_rows<uint64_t, true, true> : expect_rows<uint64_t, true, false>)
+                           : (avg ? expect_rows<uint64_t, false, true> : expect_rows<uint64_t, false, false>);
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(kExpThreads), 0, ctx->stream, b->view(), ctx->binom.as<uint64_t>(),
+                           b->layout, n, b->reps.as<uint64_t>(), b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(),
+                           b->dict.as<uint64_t>(), tv, chi, nrm, x_dev, part);
+        QMB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(expect_finish, dim3(1), dim3(64), 0, ctx->stream, part, blocks, res);
+        QMB_HIP(hipGetLastError());
+        QMB_HIP(hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
 
 qmb_status qmb_expectation(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *a, int is_complex, const double *x,
                            int x_on_device, int flags, double *out, int *averaged) {
