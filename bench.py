@@ -283,6 +283,8 @@ def run_c3(args, ctx=None):
         walls["batched_" + mode] = time.perf_counter() - t0
         if mode == "partial":
             reps_partial = reps
+    # the same batched solve with HIP events around its graph replays (GPU busy time)
+    _, reps_prof = q.lanczos_batched(ells, n_eigs=4, max_iter=400, tol=1e-13, reorth=2, profile=True)
     for mode, rmode in (("full", 1), ("partial", 2)):
         t0 = time.perf_counter()
         evals["sequential_" + mode] = np.array([q.lanczos(e, n_eigs=4, max_iter=400, tol=1e-13, reorth=rmode)[0]
@@ -299,6 +301,12 @@ def run_c3(args, ctx=None):
            "reorth": "partial (Simon 1984; batched: decided per instance on the device)",
            "ell_build_s": round(build_s, 3), "max_abs_dE_vs_sequential_full": dE,
            "autotune_warmup_ms": round(tune_s * 1e3, 2),
+           "batched_partial_profile": {"wall_ms": round(reps_prof[0].ms_total, 2),
+                                       "graph_replays": int(reps_prof[0].n_spmv_timed),
+                                       "replays_gpu_ms": round(reps_prof[0].ms_spmv, 2),
+                                       "host_check_ms": round(reps_prof[0].ms_host, 2),
+                                       "note": "wall - replays = set-up (tables, graph capture) and GPU idle "
+                                               "between replays"},
            "iters_batched_max": int(max(r.iters for r in reps_partial)),
            "reorth_steps_batched_partial": int(sum(r.n_reorth for r in reps_partial)),
            "sector_dims": [min(rows), max(rows)],

@@ -327,7 +327,11 @@ qmb_status qmb_expectation_noalloc(qmb_ctx *ctx, const qmb_basis *b, const qmb_t
  * instance (a work table of row chunks), the iteration chunk is captured once as
  * a HIP graph and replayed, converged instances drop out. evals[n*n_eigs],
  * reports[n] (may be NULL). reorth = 2 (partial) decides on the device, per
- * instance and step, inside the captured iteration. */
+ * instance and step, inside the captured iteration. The host checks the
+ * instances' tridiagonal problems on several threads while the next chunk runs.
+ * With opts->profile, every report's ms_spmv is the summed GPU time of the graph
+ * replays (HIP events) and n_spmv_timed their count; ms_total is the call's wall
+ * time and ms_host the host's check time. */
 qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
                                const qmb_lanczos_opts *opts, double *evals,
                                qmb_lanczos_report *reports);

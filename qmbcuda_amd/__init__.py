@@ -310,12 +310,14 @@ def lanczos_checkpointed(ell: Ell, path, every=0, resume=False, stop_after=0, n_
     return ev, rep, kr[: rep.iters], vec, st == 0
 
 
-def lanczos_batched(ells, n_eigs=1, max_iter=300, tol=1e-13, seed=2212456, reorth=None, check_every=10):
-    """Solve several ELLs together (one launch per phase for all of them)."""
+def lanczos_batched(ells, n_eigs=1, max_iter=300, tol=1e-13, seed=2212456, reorth=None, check_every=10,
+                    profile=False):
+    """Solve several ELLs together (one launch per phase for all of them).
+    profile: time the graph replays (report.ms_spmv, report.n_spmv_timed)."""
     if reorth is None:
         reorth = n_eigs > 1
     o = default_opts(max_iter=max_iter, n_eigs=n_eigs, tol=tol, seed=seed, reorth=int(reorth),
-                     check_every=check_every)
+                     check_every=check_every, profile=int(profile))
     arr = (C.c_void_p * len(ells))(*[e.h for e in ells])
     ev = np.zeros(len(ells) * n_eigs, np.float64)
     reps = (LanczosReport * len(ells))()

@@ -23,6 +23,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <tuple>
 
 #include "internal.hpp"
@@ -763,15 +764,36 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         std::vector<std::vector<double>> ev(n, std::vector<double>(k_eigs, std::nan(""))),
             res(n, std::vector<double>(k_eigs, std::nan("")));
         std::vector<int> iters(n, 0), conv(n, 0), brk(n, 0);
+        std::vector<char> done_now(n, 0);
         int launched = 0, n_active = n;
         double ms_host = 0.0;
-        std::vector<double> w(k_eigs + 1), zl(k_eigs + 1);
+        // with opts.profile: HIP events around every graph replay (the GPU's busy
+        // time; the rest of the wall time is set-up and host gaps)
+        std::vector<hipEvent_t> rev;
         auto launch_chunk = [&](int slot) {
+            if (o.profile) {
+                hipEvent_t e0, e1;
+                QMB_HIP(hipEventCreate(&e0));
+                QMB_HIP(hipEventCreate(&e1));
+                rev.push_back(e0);
+                rev.push_back(e1);
+                QMB_HIP(hipEventRecord(e0, ctx->stream));
+            }
             QMB_HIP(hipGraphLaunch(exec, ctx->stream));
+            if (o.profile) QMB_HIP(hipEventRecord(rev.back(), ctx->stream));
             launched += chunk;
             QMB_HIP(hipMemcpyAsync(hsnap[slot], scal.p, snap * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
             QMB_HIP(hipEventRecord(evs[slot], ctx->stream));
         };
+        struct EventList {
+            std::vector<hipEvent_t> *v;
+            ~EventList() {
+                for (auto e : *v) (void)hipEventDestroy(e);
+            }
+        } rev_guard{&rev};
+        // host threads for the per-instance tridiagonal checks (independent problems)
+        const int n_threads =
+            (int)std::max(1u, std::min({std::thread::hardware_concurrency(), 16u, (unsigned)((n + 3) / 4)}));
         launch_chunk(0);
         int pend = 0, pend_it = launched;
         while (true) {
@@ -782,8 +804,8 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             const double *hs = hsnap[pend];
             auto th0 = std::chrono::steady_clock::now();
             bool changed = false;
-            for (int pos = 0; pos < n; ++pos) {
-                if (!active[pos]) continue;
+            // instance pos: tridiagonal solve of its snapshot, convergence test; done_now[pos] = 1 when it stops
+            auto check = [&](int pos, std::vector<double> &w, std::vector<double> &zl) {
                 const double *a = hs + (size_t)pos * (3 * M + 2);
                 const double *b = a + M;
                 double tsc = 0.0;
@@ -817,14 +839,35 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                     res[pos][j] = j < kq ? std::fabs(b[m] * zl[j]) : std::nan("");
                 }
                 iters[pos] = m;
+                done_now[pos] = 0;
                 if (ok || bd || it >= M) {
                     conv[pos] = ok ? 1 : 0;
                     brk[pos] = bd ? 1 : 0;
+                    done_now[pos] = 1;
+                }
+            };
+            std::vector<int> act;
+            for (int pos = 0; pos < n; ++pos)
+                if (active[pos]) act.push_back(pos);
+            const int T = std::min<int>(n_threads, (int)act.size());
+            auto run_part = [&](int t) {
+                std::vector<double> w(k_eigs + 1), zl(k_eigs + 1);
+                for (size_t i = t; i < act.size(); i += T) check(act[i], w, zl);
+            };
+            if (T <= 1) {
+                run_part(0);
+            } else {
+                std::vector<std::thread> pool;
+                for (int t = 1; t < T; ++t) pool.emplace_back(run_part, t);
+                run_part(0);
+                for (auto &th : pool) th.join();
+            }
+            for (int pos : act)
+                if (done_now[pos]) {
                     active[pos] = 0;
                     --n_active;
                     changed = true;
                 }
-            }
             ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
             if (!next) break;  // no chunk in flight: that snapshot was the last one
             if (changed && n_active > 0) {
@@ -839,6 +882,12 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         (void)hipGraphDestroy(graph);
         const double wall =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+        double ms_replays = 0.0;
+        for (size_t i = 0; i + 1 < rev.size(); i += 2) {
+            float t = 0.f;
+            QMB_HIP(hipEventElapsedTime(&t, rev[i], rev[i + 1]));
+            ms_replays += t;
+        }
         bool all_conv = true;
         std::vector<double> hpro((size_t)n * 3, 0.0);
         QMB_HIP(hipMemcpyAsync(hpro.data(), pros.p, hpro.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -859,6 +908,8 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                 for (int j = 0; j < std::min(k_eigs, 8); ++j) r.residual[j] = res[pos][j];
                 r.ms_total = wall;
                 r.ms_host = ms_host;
+                r.ms_spmv = ms_replays;
+                r.n_spmv_timed = (int)(rev.size() / 2);
             }
         }
         if (!all_conv) throw Error{QMB_ERR_NOT_CONVERGED, "qmb_lanczos_batched: some instances did not converge"};

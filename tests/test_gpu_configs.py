@@ -183,3 +183,34 @@ def test_c4_learned_start_configured_sizes(ctx):
         assert rep_r.converged and rep_l.converged and not fb
         assert abs(ev_l[0] - ev_r[0]) < 1e-10 * abs(ev_r[0])
         assert rep_l.iters < rep_r.iters
+
+
+def test_c3_full_size_sectors_vs_oracle(ctx):
+    """Batched uneven sectors at full size (N=24 chain): the batched solve of all
+    72 (k, n_up) sectors against the oracle's own converged Lanczos on three of
+    them (real k=0 and k=12, complex k=5), lowest eigenvalue to 1e-10."""
+    lat, info = LM.chain(24, 1.0, 1.0, 0.0)
+    ge, fl, ch = LM.translation_group(lat)
+    g = q.Group(ge, ch)
+    t = q.Terms(24, LM.term_table(lat, info))
+    say("C3 72 sectors batched")
+    keep, ells = [], []
+    for n_up in (12, 14, 16):
+        for k in range(24):
+            b = q.Basis(ctx, g, n_up, k, q.LAYOUT_COMPACT)
+            keep.append(b)
+            ells.append(q.Ell(ctx, b, t))
+    ev, reps = q.lanczos_batched(ells, n_eigs=4, max_iter=400, tol=1e-13, reorth=2)
+    assert all(r.converged for r in reps)
+    case = chain_case(24)
+    for n_up, k in ((12, 0), (14, 5), (16, 12)):
+        say(f"C3 oracle sector n_up={n_up} k={k}")
+        orb = oracle.Orbits(24, n_up, case["group_el"], case["chars"][[k]], flip=None)
+        cplx = not np.all(case["chars"][k].imag == 0)
+        W = OM.max_terms_rule(len(case["table"][3]), n_up, 24)
+        ref = oracle.Ell(orb, case["table"], 0, W, cplx)
+        st, e0, it, _, _ = ref.lanczos_converge(600, 2212456, tol=1e-13, check_every=5)
+        assert st == 0
+        got = ev[[12, 14, 16].index(n_up) * 24 + k][0]
+        print(f"n_up={n_up} k={k}: batched {got:.12f}, oracle {e0:.12f}", flush=True)
+        assert abs(got - e0) < 1e-10 * abs(e0)
