@@ -146,7 +146,7 @@ def cpu_baseline(cfg, table, threads, spmv_bytes):
 def c5_learned(ctx, lat6, b6, e6, e0_plain, rep_plain, lanczos_s_plain, total_s_plain, args):
     """The same 6x6 solve from the learned start: the window network (2x4 and
     4x2 windows, Marshall sublattice prior) trained on the ground states of 4x4,
-    4x6 and 6x4 lattices at Delta in {0.8, ..., 1.2}, then Lanczos on the 6x6
+    4x6 and 6x4 lattices at Delta in {0.9, 1.0, 1.1}, then Lanczos on the 6x6
     from its start vector (inference on the device, inside the timed span).
     Reported both for the Lanczos part alone and end to end (training data,
     training, inference and Lanczos against the plain solve's Lanczos)."""
@@ -156,8 +156,10 @@ def c5_learned(ctx, lat6, b6, e6, e0_plain, rep_plain, lanczos_s_plain, total_s_
     net = q.Mlp(ctx, seed=20261202)
     keep = []
     t0 = time.perf_counter()
+    # (more couplings, more steps, or 4x8 and 8x4 lattices in the training set leave
+    # the 6x6 solve at the same step count)
     for (n1, n2) in ((4, 4), (4, 6), (6, 4)):
-        for d in (0.8, 0.9, 1.0, 1.1, 1.2):
+        for d in (0.9, 1.0, 1.1):
             lat, info = LM.square(n1, n2, 1.0, d)
             ge, fl, ch = LM.translation_group(lat, spin_flip=True)
             g = q.Group(ge, ch, fl)
@@ -190,7 +192,7 @@ def c5_learned(ctx, lat6, b6, e6, e0_plain, rep_plain, lanczos_s_plain, total_s_
             "abs_dE_vs_plain": abs(ev[0] - e0_plain), "fallback": fb, "converged": bool(rep.converged),
             "train_loss_first_last": [float(hist[0]), float(hist[-1])], "train_steps": args.c5_steps,
             "train_s": round(train_s, 3), "data_gen_s": round(data_s, 3),
-            "training_lattices": "4x4, 4x6, 6x4 at Delta 0.8..1.2 (J2 = 0), k=(0,0) flip-even"}
+            "training_lattices": "4x4, 4x6, 6x4 at Delta 0.9, 1.0, 1.1 (J2 = 0), k=(0,0) flip-even"}
 
 
 def cpu_baseline_shard(e, iters, threads, spmv_bytes, shard_rows=1 << 21):
@@ -692,7 +694,7 @@ def main():
     ap.add_argument("--c4-steps", type=int, default=400)
     ap.add_argument("--c4-eval", type=int, default=16)
     ap.add_argument("--c4-n", type=int, default=30)
-    ap.add_argument("--c5-steps", type=int, default=300)
+    ap.add_argument("--c5-steps", type=int, default=150)
     ap.add_argument("--no-learned", action="store_true", help="c5: skip the learned-start solve")
     ap.add_argument("--tol", type=float, default=1e-13)
     ap.add_argument("--no-cpu-baseline", action="store_true")
