@@ -360,96 +360,6 @@ __global__ __launch_bounds__(256) void mlp_finish_max(const double *__restrict__
     }
 }
 
-// c_r = exp(alpha - amax + i beta) sqrt(G/m_r); partials of Re/Im <u, c>, <c, c>, <u, u>.
-__global__ __launch_bounds__(256) void mlp_state_overlap(int64_t n, int G, const uint8_t *__restrict__ stab,
-                                                         const double *__restrict__ alpha,
-                                                         const double *__restrict__ beta,
-                                                         const double *__restrict__ u, const double *__restrict__ amax,
-                                                         double *__restrict__ part, int nb) {
-    __shared__ double lds[kWaves];
-    const double am = *amax;
-    double o_re = 0, o_im = 0, nc = 0, nu = 0;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-        const double mag = exp(alpha[r] - am) * sqrt((double)G / (double)stab[r]);
-        const double cr = mag * cos(beta[r]), ci = mag * sin(beta[r]);
-        const double ur = u ? u[r] : 0.0;
-        o_re += ur * cr;
-        o_im += ur * ci;
-        nc += mag * mag;
-        nu += ur * ur;
-    }
-    const double v[4] = {o_re, o_im, nc, nu};
-    for (int q = 0; q < 4; ++q) {
-        const double t = block_sum(v[q], lds);
-        if (threadIdx.x == 0) part[q * nb + blockIdx.x] = t;
-    }
-}
-
-// One block: sums of the 4 partial rows -> stats[0..3] = Re O, Im O, Nc, Nu; stats[4] = loss.
-__global__ __launch_bounds__(256) void mlp_finish_overlap(const double *__restrict__ part, int nb,
-                                                          double *__restrict__ stats) {
-    __shared__ double lds[kWaves];
-    double s[4];
-    for (int q = 0; q < 4; ++q) s[q] = sum_partials(part + q * nb, nb, lds);
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < 4; ++q) stats[q] = s[q];
-        const double o2 = s[0] * s[0] + s[1] * s[1];
-        stats[4] = 1.0 - o2 / (s[2] * s[3]);
-    }
-}
-
-// dL/dalpha_r, dL/dbeta_r (L = 1 - F) accumulated per window pattern in
-// fixed point (order-independent integer sums): hist[block][256][2].
-__global__ __launch_bounds__(256) void mlp_state_grad(int64_t n, int n_win, const uint8_t *__restrict__ win, int G,
-                                                      const uint64_t *__restrict__ reps,
-                                                      const uint8_t *__restrict__ stab,
-                                                      const double *__restrict__ alpha,
-                                                      const double *__restrict__ beta, const double *__restrict__ u,
-                                                      const double *__restrict__ amax,
-                                                      const double *__restrict__ stats,
-                                                      unsigned long long *__restrict__ hist) {
-    __shared__ unsigned long long h[512];
-    __shared__ uint8_t wl[kMaxWin * kWin];
-    for (int i = threadIdx.x; i < 512; i += blockDim.x) h[i] = 0ull;
-    stage_windows(wl, win, n_win);
-    __syncthreads();
-    const double am = *amax;
-    const double ore = stats[0], oim = stats[1], nc = stats[2], nu = stats[3];
-    const double o2 = ore * ore + oim * oim;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-        const double mag = exp(alpha[r] - am) * sqrt((double)G / (double)stab[r]);
-        const double cr = mag * cos(beta[r]), ci = mag * sin(beta[r]);
-        // w = conj(O) u c
-        const double wr = u[r] * (ore * cr + oim * ci);
-        const double wi = u[r] * (ore * ci - oim * cr);
-        const double ga = -(2.0 * wr * nc - o2 * 2.0 * mag * mag) / (nc * nc * nu);
-        const double gb = -(-2.0 * wi) / (nc * nu);
-        const long long qa = (long long)llrint(ga * kFix), qb = (long long)llrint(gb * kFix);
-        const uint64_t s = reps[r];
-        for (int q = 0; q < n_win; ++q) {
-            const int p = window_pattern(s, wl + q * kWin);
-            atomicAdd(&h[2 * p], (unsigned long long)qa);
-            atomicAdd(&h[2 * p + 1], (unsigned long long)qb);
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < 512; i += blockDim.x) hist[(size_t)blockIdx.x * 512 + i] = h[i];
-}
-
-// dout[row][o] for the sample's 256 rows = sum over blocks of the histograms / 2^36.
-// Grid (512 / 64): a wavefront per bin, lanes striding over the block
-// histograms; integer sums, so the result does not depend on the order.
-__global__ __launch_bounds__(256) void mlp_finish_hist(const unsigned long long *__restrict__ hist, int nb,
-                                                       float *__restrict__ dout_rows, float scale) {
-    const int lane = threadIdx.x & 63;
-    for (int i = blockIdx.x * kWaves + (threadIdx.x >> 6); i < 512; i += gridDim.x * kWaves) {
-        long long s = 0;
-        for (int b = lane; b < nb; b += 64) s += (long long)hist[(size_t)b * 512 + i];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-        if (lane == 0) dout_rows[i] = (float)((double)s / kFix * scale);
-    }
-}
-
 // Real start vector x_r = |c_r| cos(beta_r - beta_ref) with beta_ref the phase
 // of the first representative of largest amplitude (alpha == amax). Block
 // partials: part[block] = sum of x_r^2, part[gridDim.x + block] = sum of xi_r^2
@@ -542,9 +452,8 @@ __global__ __launch_bounds__(256) void mlp_finish_sum2(const double *__restrict_
 
 // ---- the per-sample kernels batched over every sample of a training step ----
 // One launch per phase covers all samples: block b works on sample task[b],
-// with the same row-to-thread assignment and the same partial-sum layout per
-// sample as the single-sample kernels above, so the loss and gradient are the
-// ones those kernels give, bit for bit.
+// with the row-to-thread assignment and the per-sample block count of the
+// single-sample kernel above, so every sample's sums are formed in a fixed order.
 struct BSample {
     int64_t n, off;  // rows; offset of the sample's rows in the concatenated alpha/beta
     const uint64_t *reps;
@@ -678,20 +587,47 @@ __global__ __launch_bounds__(256) void mlp_batch_grad(const BSample *__restrict_
     const double ore = st[0], oim = st[1], nc = st[2], nu = st[3];
     const double o2 = ore * ore + oim * oim;
     const int lb = blockIdx.x - s.blk0;
-    for (int64_t r = (int64_t)lb * blockDim.x + threadIdx.x; r < s.n; r += (int64_t)s.nblk * blockDim.x) {
-        const double mag = exp(alpha[s.off + r] - am) * sqrt((double)s.G / (double)s.stab[r]);
-        const double br = beta[s.off + r];
-        const double cr = mag * cos(br), ci = mag * sin(br);
-        const double wr = s.u[r] * (ore * cr + oim * ci);
-        const double wi = s.u[r] * (ore * ci - oim * cr);
-        const double ga = -(2.0 * wr * nc - o2 * 2.0 * mag * mag) / (nc * nc * nu);
-        const double gb = -(-2.0 * wi) / (nc * nu);
-        const long long qa = (long long)llrint(ga * kFix), qb = (long long)llrint(gb * kFix);
-        const uint64_t stt = s.reps[r];
+    const int lane = threadIdx.x & 63;
+    // wave-uniform loop (lane l of a wave handles row r0 + l, the same rows as a
+    // thread-strided loop): consecutive representatives share their high sites, so
+    // a window there often has one pattern across the wave; its 64 additions are
+    // then summed in registers and added once instead of contending for one bin
+    for (int64_t r0 = (int64_t)lb * blockDim.x + (threadIdx.x & ~63); r0 < s.n; r0 += (int64_t)s.nblk * blockDim.x) {
+        const int64_t r = r0 + lane;
+        const bool act = r < s.n;
+        long long qa = 0, qb = 0;
+        uint64_t stt = 0;
+        if (act) {
+            const double mag = exp(alpha[s.off + r] - am) * sqrt((double)s.G / (double)s.stab[r]);
+            const double br = beta[s.off + r];
+            const double cr = mag * cos(br), ci = mag * sin(br);
+            const double wr = s.u[r] * (ore * cr + oim * ci);
+            const double wi = s.u[r] * (ore * ci - oim * cr);
+            const double ga = -(2.0 * wr * nc - o2 * 2.0 * mag * mag) / (nc * nc * nu);
+            const double gb = -(-2.0 * wi) / (nc * nu);
+            qa = (long long)llrint(ga * kFix);
+            qb = (long long)llrint(gb * kFix);
+            stt = s.reps[r];
+        }
+        const uint64_t st0 = __shfl(stt, 0, 64);  // lane 0's row exists (r0 < n)
+        if (!act) stt = st0;                     // rows past the end add 0 to lane 0's bins
         for (int q = 0; q < s.n_win; ++q) {
             const int p = window_pattern(stt, wl + q * kWin);
-            atomicAdd(&h[2 * p], (unsigned long long)qa);
-            atomicAdd(&h[2 * p + 1], (unsigned long long)qb);
+            const int p0 = __shfl(p, 0, 64);
+            if (__all(p == p0)) {
+                long long sa = qa, sb = qb;
+                for (int off = 32; off > 0; off >>= 1) {
+                    sa += __shfl_xor(sa, off, 64);
+                    sb += __shfl_xor(sb, off, 64);
+                }
+                if (lane == 0) {
+                    atomicAdd(&h[2 * p0], (unsigned long long)sa);
+                    atomicAdd(&h[2 * p0 + 1], (unsigned long long)sb);
+                }
+            } else {
+                atomicAdd(&h[2 * p], (unsigned long long)qa);
+                atomicAdd(&h[2 * p + 1], (unsigned long long)qb);
+            }
         }
     }
     __syncthreads();
@@ -833,7 +769,7 @@ void forward_rows(qmb_mlp *m, const std::vector<double> &coup, bool store) {
 
 // alpha/beta/amax of one state set for the coupling whose table starts at tab.
 struct StateWork {
-    DevBuf alpha, beta, part, amax, stats, hist;
+    DevBuf alpha, beta, part, amax, stats;
     int64_t cap = 0;
     int nb = 0;
     void ensure(int64_t n) {
@@ -844,7 +780,6 @@ struct StateWork {
         part.alloc((size_t)nb * 4 * sizeof(double));
         amax.alloc(sizeof(double));
         stats.alloc(8 * sizeof(double));
-        hist.alloc((size_t)nb * 512 * sizeof(unsigned long long));
         cap = n;
     }
 };
