@@ -36,6 +36,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, AMD specification
 
 METRIC = "Lanczos time-to-E0 at |ΔE|<1e-10, learned/batched vs plain; SpMV HBM GB/s"
+# every benchmark matrix is built with QMB_LOCALITY_SLOTS: the same matrix, each
+# row's slots ordered by the terms' sites (high sites first) for gather locality
+ELL_SLOTS = "locality order (QMB_LOCALITY_SLOTS: terms sorted on lowest, then highest site, descending)"
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -75,7 +78,7 @@ def gpu_step(ctx, group, table, cfg, tol):
     b = q.Basis(ctx, group, cfg["n"] // 2, 0, q.LAYOUT_COMPACT)
     t1 = time.perf_counter()
     terms = q.Terms(cfg["n"], table)
-    e = q.Ell(ctx, b, terms)
+    e = q.Ell(ctx, b, terms, locality_slots=True)
     t2 = time.perf_counter()
     # HIP events around every 8th fused SpMV (the kernel's duration for the roofline;
     # events around every launch would add ~0.4 ms to the solve)
@@ -165,7 +168,7 @@ def c5_learned(ctx, lat6, b6, e6, e0_plain, rep_plain, lanczos_s_plain, total_s_
             g = q.Group(ge, ch, fl)
             t = q.Terms(n1 * n2, LM.term_table(lat, info))
             b = q.Basis(ctx, g, n1 * n2 // 2, 0, q.LAYOUT_COMPACT)
-            e = q.Ell(ctx, b, t)
+            e = q.Ell(ctx, b, t, locality_slots=True)
             net.set_windows(b, LM.square_windows(lat), LM.sublattice_mask(lat))
             _, _, _, u = q.lanczos(e, max_iter=300, want_eigvec=True)
             net.add_sample(b, u, d, 0.0)
@@ -267,7 +270,7 @@ def run_c3(args, ctx=None):
     for n_up in (12, 14, 16):
         for k in range(24):
             g, t, b = _sector(ctx, lat_info, n_up, k, group=group)
-            e = q.Ell(ctx, b, t)
+            e = q.Ell(ctx, b, t, locality_slots=True)
             keep += [t, b]
             ells.append(e)
     build_s = time.perf_counter() - t0
@@ -296,7 +299,7 @@ def run_c3(args, ctx=None):
     ts = min(walls["sequential_partial"], walls["sequential_full"])
     dE = max(float(np.max(np.abs(evals[a] - evals["sequential_full"]))) for a in evals)
     out = {"metric": METRIC, "config": {"workload": "C3: XXZ chain N=24, 24 k x n_up {12,14,16}, 72 sectors, "
-                                                    "lowest 4 each, FP64"},
+                                                    "lowest 4 each, FP64", "ell_slots": ELL_SLOTS},
            "value": round(ts / tb, 3), "unit": "x (best sequential wall / batched wall)", "higher_is_better": True,
            "batched_ms": round(tb * 1e3, 2), "sequential_ms": round(ts * 1e3, 2),
            "wall_ms": {k: round(v * 1e3, 2) for k, v in walls.items()},
@@ -335,7 +338,7 @@ def run_c2p(args):
         t0 = time.perf_counter()
         b = q.Basis(ctx, group, 14, irrep, q.LAYOUT_COMPACT)
         t1 = time.perf_counter()
-        e = q.Ell(ctx, b, q.Terms(28, table))
+        e = q.Ell(ctx, b, q.Terms(28, table), locality_slots=True)
         t2 = time.perf_counter()
         ev, rep, _, _ = q.lanczos(e, max_iter=1000, tol=args.tol, seed=2212456)
         t3 = time.perf_counter()
@@ -356,7 +359,8 @@ def run_c2p(args):
 
     gs = min(range(2), key=lambda i: parts[i][0]["e0"])
     out = {"metric": METRIC, "config": {"workload": "C2 chain N=28 with the site reflection: translations x spin "
-                                                    "flip x reflection (112 elements), k=0 flip-even, both parities"},
+                                                    "flip x reflection (112 elements), k=0 flip-even, both parities",
+                      "ell_slots": ELL_SLOTS},
            "value": mean(parts[gs], "total_ms"), "unit": "ms time-to-E0 (ground-state parity sector)",
            "higher_is_better": False,
            "plain_translation_flip_ms": mean(plain, "total_ms"),
@@ -396,7 +400,7 @@ def run_c4(args, ctx=None):
             for j2 in j2s:
                 lat_, info = LM.chain(n, 1.0, float(d), float(j2))
                 t = q.Terms(n, LM.term_table(lat_, info))
-                e = q.Ell(ctx, b, t)
+                e = q.Ell(ctx, b, t, locality_slots=True)
                 ev, rep, _, u = q.lanczos(e, max_iter=600, want_eigvec=True)
                 net.add_sample(b, u, float(d), float(j2))
                 keep.append(t)
@@ -421,7 +425,7 @@ def run_c4(args, ctx=None):
     for d, j2 in pairs:
         lat_, info = LM.chain(n, 1.0, d, j2)
         t = q.Terms(n, LM.term_table(lat_, info))
-        e = q.Ell(ctx, b, t)
+        e = q.Ell(ctx, b, t, locality_slots=True)
         q.lanczos(e, max_iter=5, allow_unconverged=True)  # warm the kernels for this matrix
         t1 = time.perf_counter()
         ev_r, rep_r, _, _ = q.lanczos(e, max_iter=1500, tol=1e-13)
@@ -440,7 +444,8 @@ def run_c4(args, ctx=None):
     e2e_l = overhead_ms + ms_l * n_pairs
     out = {"metric": METRIC,
            "config": {"workload": f"C4: MLP trained on N=20,22,24 k=0 x {args.c4_grid}x{args.c4_grid} grid; "
-                                  f"evaluated on N={n} Sz=0 k=0 (dim {b.n_rows}) at {len(pairs)} held-out couplings"},
+                                  f"evaluated on N={n} Sz=0 k=0 (dim {b.n_rows}) at {len(pairs)} held-out couplings",
+                      "ell_slots": ELL_SLOTS},
            "value": round(ms_r / ms_l, 3), "unit": "x (random-start wall / learned-start wall, incl. inference)",
            "higher_is_better": True, "iters_random_mean": it_r, "iters_learned_mean": it_l,
            "ms_random_mean": round(ms_r, 2), "ms_learned_mean": round(ms_l, 2),
@@ -457,22 +462,23 @@ def run_c4(args, ctx=None):
 
 
 C5_WORKLOAD = ("6x6 periodic square-lattice Heisenberg J=1 Delta=1, Sz=0, k=(0,0), spin-flip even "
-               "(dim 126,056,625, <= 73 ELL entries/row), FP64 Lanczos with partial reorthogonalisation "
-               "to |dE| < 1e-10")
+               "(dim 126,056,625, <= 73 ELL entries/row, slots in locality order), FP64 Lanczos with partial "
+               "reorthogonalisation to |dE| < 1e-10")
 
 
 def c5_step(ctx, group, table):
-    """One full 6x6 solve: device basis, device ELL, Lanczos with partial
-    reorthogonalisation (every Lanczos vector kept, 1 GB each: 100 steps bound
-    the memory at 100 GB beside the 23 GB packed matrix; a step costs ~30 ms,
-    the convergence test well under 1 ms, so it runs after every step) with HIP
-    events around every fused SpMV launch."""
+    """One full 6x6 solve: device basis, device ELL (slots in locality order:
+    the same matrix, 0.34 instead of 0.54 distinct lines per gathered value),
+    Lanczos with partial reorthogonalisation (every Lanczos vector kept, 1 GB
+    each: 100 steps bound the memory at 100 GB beside the 23 GB packed matrix;
+    a step costs ~25 ms, the convergence test well under 1 ms, so it runs after
+    every step) with HIP events around every fused SpMV launch."""
     import qmbcuda_amd as q
 
     t0 = time.perf_counter()
     b = q.Basis(ctx, group, 18, 0, q.LAYOUT_COMPACT)
     t1 = time.perf_counter()
-    e = q.Ell(ctx, b, q.Terms(36, table))
+    e = q.Ell(ctx, b, q.Terms(36, table), locality_slots=True)
     t2 = time.perf_counter()
     ev, rep, _, _ = q.lanczos(e, max_iter=100, tol=1e-13, seed=2212456, profile=1, reorth=2, check_every=1)
     t3 = time.perf_counter()
@@ -535,7 +541,7 @@ def run_headline(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": C5_WORKLOAD, "lanczos": "seeded random start, tol 1e-13 on the Ritz values",
-                   "parallelism": "single GPU"},
+                   "ell_slots": ELL_SLOTS, "parallelism": "single GPU"},
         "time_to_e0_ms": round(ms_per_step, 3),
         "step_ms": [round(r["total_ms"], 1) for r in runs],
         "breakdown_ms": {k: round(sum(r[k] for r in runs) / len(runs), 2)
@@ -648,7 +654,7 @@ def run_single(args, cfg, ctx=None, steps=None, warmup=None, observables=True):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "lanczos": "plain, seeded random start, tol 1e-13 rel",
-                   "parallelism": "single GPU"},
+                   "ell_slots": ELL_SLOTS, "parallelism": "single GPU"},
         "time_to_e0_ms": round(ms_per_step, 4),
         "breakdown_ms": {k: round(sum(r[k] for r in runs) / len(runs), 3)
                          for k in ("basis_ms", "ell_ms", "lanczos_ms")},

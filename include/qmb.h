@@ -166,6 +166,15 @@ typedef struct {
  * to the unpacked form); the matrix stream is 4 instead of 12 (20 complex)
  * bytes per entry. value_type | QMB_KEEP_FP64_VALUES keeps the unpacked form. */
 enum { QMB_KEEP_FP64_VALUES = 16 };
+/* value_type | QMB_LOCALITY_SLOTS: the slots of each row follow the terms
+ * sorted on (lowest site, then highest site) descending, stable (a term with no
+ * operator first), instead of table order. Same matrix, same entries; each
+ * row's sum runs in the new slot order. Rows are in colex order, so the 64 rows
+ * of a chunk share their high sites and, with those terms first, slot k of the
+ * chunk's rows gathers nearby columns (6x6 Sz=0 k=0 sector: 0.54 -> 0.34
+ * distinct 128 B lines per gathered value, fused Lanczos SpMV 26.3 -> 24.8 ms).
+ * No counterpart in the reference, whose slot order is the table order. */
+enum { QMB_LOCALITY_SLOTS = 32 };
 qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t,
                          int value_type, int max_terms, qmb_ell **out);
 qmb_status qmb_ell_destroy(qmb_ell *e);

@@ -43,6 +43,8 @@ struct EDOptions {
     int device = 0;
     bool batched = false;          // not fixed_iter: solve all irreps in one qmb_lanczos_batched call
                                    // (the ground-state irrep is then re-solved alone for its Krylov vector)
+    bool locality_slots = false;   // QMB_LOCALITY_SLOTS: row slots in locality order (same matrix; no
+                                   // counterpart in the reference, whose slots follow the term table)
 };
 
 template <class T>
@@ -184,7 +186,7 @@ class HeisenbergXXZAbelianSymms {
         qmb_basis_info bi;
         qmb_basis_get_info(h.b, &bi);
         if (bi.n_rows == 0) throw Status(QMB_ERR_INVALID, "ComputeStaticExpValZeroT: no ground state");
-        check(qmb_ell_build(h.ctx, h.b, h.th, real ? QMB_REAL : QMB_COMPLEX,
+        check(qmb_ell_build(h.ctx, h.b, h.th, (real ? QMB_REAL : QMB_COMPLEX) | slot_flags(),
                             max_terms_in > 0 ? max_terms_in : max_terms, &h.e),
               "qmb_ell_build");
         qmb_lanczos_opts o;
@@ -250,8 +252,9 @@ class HeisenbergXXZAbelianSymms {
                 qmb_basis_get_info(bases[a], &bi);
                 sector_dims.push_back(bi.n_rows - bi.n_null);
                 if (bi.n_rows == 0) continue;
-                check(qmb_ell_build(ctx, bases[a], terms, irrep_is_real(a, ch, G) ? QMB_REAL : QMB_COMPLEX,
-                                    max_terms, &ells[a]),
+                check(qmb_ell_build(ctx, bases[a], terms,
+                                    (irrep_is_real(a, ch, G) ? QMB_REAL : QMB_COMPLEX) | slot_flags(), max_terms,
+                                    &ells[a]),
                       "qmb_ell_build");
                 batch.push_back(ells[a]);
                 which.push_back(a);
@@ -306,7 +309,7 @@ class HeisenbergXXZAbelianSymms {
             iterations.push_back(0);
             return;
         }
-        qmb_status st = qmb_ell_build(ctx, b, terms, real ? QMB_REAL : QMB_COMPLEX, max_terms, &e);
+        qmb_status st = qmb_ell_build(ctx, b, terms, (real ? QMB_REAL : QMB_COMPLEX) | slot_flags(), max_terms, &e);
         if (st != QMB_OK) {
             qmb_basis_destroy(b);
             check(st, "qmb_ell_build");
@@ -368,6 +371,7 @@ class HeisenbergXXZAbelianSymms {
     int hop_f_, NStates_, NIter_;
     double tol_norm_;
     EDOptions opts_;
+    int slot_flags() const { return opts_.locality_slots ? QMB_LOCALITY_SLOTS : 0; }
     std::vector<double> ch_;      // characters [NIr][G] (re, im)
     std::vector<uint8_t> flips_;  // spin flip of each element
     int max_terms = 0;

@@ -275,6 +275,24 @@ class Ell:
         return r, x
 
 
+def locality_order(term_table):
+    """The term table with its terms reordered as an ELL built with
+    QMB_LOCALITY_SLOTS applies them: terms sorted on (lowest site, then highest
+    site) descending, stable, a term without operators first. Pure-Python
+    restatement of the rule stated in include/qmb.h; the reference has no such
+    order (its slots follow the table, utils.cu:133-161)."""
+    op, site, scalar, n_ops, h_col = term_table
+    op = np.asarray(op, np.int32).reshape(len(n_ops), h_col)
+    site = np.asarray(site, np.int32).reshape(len(n_ops), h_col)
+    keys = []
+    for h in range(len(n_ops)):
+        s = [int(site[h, k]) for k in range(int(n_ops[h]))]
+        keys.append((min(s), max(s)) if s else (float("inf"), float("inf")))
+    order = sorted(range(len(n_ops)), key=lambda h: (-keys[h][0], -keys[h][1]))
+    return (np.ascontiguousarray(op[order].reshape(-1)), np.ascontiguousarray(site[order].reshape(-1)),
+            np.ascontiguousarray(np.asarray(scalar)[order]), np.ascontiguousarray(np.asarray(n_ops)[order]), h_col)
+
+
 def ell_rows_from_reps(n_sites, group_el, flip, chars, term_table, alpha, reps, rows, max_terms, is_complex,
                        tol_norm=1e-5):
     """Rows `rows` of the sector matrix whose rows are the ascending representatives

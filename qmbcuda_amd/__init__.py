@@ -13,8 +13,8 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import (COMPLEX, EXPECT_GROUP_AVERAGE, KEEP_FP64_VALUES, LAYOUT_COMPACT, LAYOUT_REFERENCE, REAL,
-                   START_RANDOM, START_USER, BasisInfo, EllInfo, LanczosOpts, LanczosReport)
+from ._abi import (COMPLEX, EXPECT_GROUP_AVERAGE, KEEP_FP64_VALUES, LAYOUT_COMPACT, LAYOUT_REFERENCE,
+                   LOCALITY_SLOTS, REAL, START_RANDOM, START_USER, BasisInfo, EllInfo, LanczosOpts, LanczosReport)
 from . import models
 
 __all__ = ["QMBError", "Context", "Group", "Terms", "Basis", "Ell", "lanczos", "lanczos_batched", "models",
@@ -178,15 +178,18 @@ class Basis:
 
 class Ell:
     def __init__(self, ctx: Context, basis: Basis, terms: Terms, value_type=None, max_terms: int = 0,
-                 keep_fp64_values: bool = False):
-        """keep_fp64_values: store every value in FP64 (no packed value table)."""
+                 keep_fp64_values: bool = False, locality_slots: bool = False):
+        """keep_fp64_values: store every value in FP64 (no packed value table).
+        locality_slots: slots follow the terms sorted on (lowest site, highest
+        site) descending instead of table order (QMB_LOCALITY_SLOTS): the same
+        matrix, with better gather locality on large sectors."""
         if value_type is None:
             real = models.is_real_irrep(basis.group.chars, basis.irrep) and np.all(terms.scalars.imag == 0)
             value_type = REAL if real else COMPLEX
         self.ctx, self.basis, self.terms = ctx, basis, terms
         self.value_type = value_type
         h = C.c_void_p()
-        flags = KEEP_FP64_VALUES if keep_fp64_values else 0
+        flags = (KEEP_FP64_VALUES if keep_fp64_values else 0) | (LOCALITY_SLOTS if locality_slots else 0)
         _check(lib().qmb_ell_build(ctx.h, basis.h, terms.h, value_type | flags, max_terms, C.byref(h)))
         self.h = h
 

@@ -23,6 +23,7 @@ REAL, COMPLEX = 0, 1
 START_RANDOM, START_USER = 0, 1
 EXPECT_GROUP_AVERAGE = 1
 KEEP_FP64_VALUES = 16
+LOCALITY_SLOTS = 32
 
 
 class BasisInfo(C.Structure):

@@ -503,7 +503,8 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         QMB_REQUIRE(b->ctx == ctx, "qmb_ell_build: basis belongs to another context");
         QMB_REQUIRE(t->n_sites == b->n_sites, "qmb_ell_build: term table and basis disagree on n_sites");
         const bool keep_fp64 = (value_type & QMB_KEEP_FP64_VALUES) != 0;
-        value_type &= ~QMB_KEEP_FP64_VALUES;
+        const bool locality = (value_type & QMB_LOCALITY_SLOTS) != 0;
+        value_type &= ~(QMB_KEEP_FP64_VALUES | QMB_LOCALITY_SLOTS);
         QMB_REQUIRE(value_type == QMB_REAL || value_type == QMB_COMPLEX, "qmb_ell_build: bad value type");
         const qmb_group *g = b->group;
         const double *ch = &g->chars[(size_t)b->irrep * g->G * 2];
@@ -525,7 +526,8 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         e->n_rows = b->n_rows;
         e->n_chunks = (b->n_rows + kChunk - 1) / kChunk;
         // term table and irrep data on device
-        const DeviceTerms dt(*t, ch, g->G);
+        const std::vector<int> order = locality ? locality_order(*t) : std::vector<int>();
+        const DeviceTerms dt(*t, ch, g->G, locality ? &order : nullptr);
         const TermsView tv = dt.view;
         DevBuf d_err(sizeof(int));
         QMB_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), ctx->stream));

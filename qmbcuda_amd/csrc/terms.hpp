@@ -94,22 +94,53 @@ inline std::vector<double> orbit_norm_table(int G) {
     return nrm;
 }
 
-// A term table uploaded to the device (compiled terms) together with the
-// irrep's character row and the orbit norm table of its group: one host blob,
-// one copy.
+// Locality order of a term table: the terms sorted on (lowest site, then
+// highest site) descending, stable. A term without operators counts as lying
+// above every site. Basis rows are sorted by colex rank, which the highest
+// occupied sites dominate, so the 64 rows of one chunk share their high sites:
+// terms on high sites act on the same rows of the chunk, and when they come
+// first, slot k of those rows comes from the same term and gathers nearby
+// columns.
+inline std::vector<int> locality_order(const qmb_terms &t) {
+    std::vector<int> lo(t.n_terms), hi(t.n_terms), ord(t.n_terms);
+    for (int h = 0; h < t.n_terms; ++h) {
+        int a = t.n_sites, b = t.n_sites;
+        if (t.n_ops[h] > 0) {
+            a = t.n_sites;
+            b = -1;
+            for (int k = 0; k < t.n_ops[h]; ++k) {
+                const int s = t.site[h * t.max_ops + k];
+                a = std::min(a, s);
+                b = std::max(b, s);
+            }
+        }
+        lo[h] = a;
+        hi[h] = b;
+        ord[h] = h;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        if (lo[x] != lo[y]) return lo[x] > lo[y];
+        return hi[x] > hi[y];
+    });
+    return ord;
+}
+
+// A term table uploaded to the device (compiled terms, in table order or in
+// the given order) together with the irrep's character row and the orbit norm
+// table of its group: one host blob, one copy.
 struct DeviceTerms {
     DevBuf blob;
     TermsView view{};
     const double2 *chi = nullptr;  // [G]
     const double *nrm = nullptr;   // [G + 1]
-    DeviceTerms(const qmb_terms &t, const double *chi_row, int G) {
+    DeviceTerms(const qmb_terms &t, const double *chi_row, int G, const std::vector<int> *order = nullptr) {
         const int nt = t.n_terms;
         auto up16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
         const size_t o_terms = 0, o_chi = up16((size_t)std::max(1, nt) * sizeof(TermC)),
                      o_nrm = up16(o_chi + (size_t)G * 16), total = up16(o_nrm + (size_t)(G + 1) * 8);
         std::vector<unsigned char> h(total, 0);
         TermC *tc = reinterpret_cast<TermC *>(h.data() + o_terms);
-        for (int k = 0; k < nt; ++k) tc[k] = compile_term(t, k);
+        for (int k = 0; k < nt; ++k) tc[k] = compile_term(t, order ? (*order)[k] : k);
         if (chi_row) std::memcpy(h.data() + o_chi, chi_row, (size_t)G * 16);
         const std::vector<double> nt_tab = orbit_norm_table(G);
         std::memcpy(h.data() + o_nrm, nt_tab.data(), nt_tab.size() * 8);

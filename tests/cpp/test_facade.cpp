@@ -162,10 +162,11 @@ static void print_tables(const Case &c) {
     std::printf("]}\n");
 }
 
-static void solve(const Case &c, bool fixed, bool batched = false) {
+static void solve(const Case &c, bool fixed, bool batched = false, bool locality = false) {
     EDOptions o;
     o.fixed_iter = fixed;
     o.batched = batched;
+    o.locality_slots = locality;
     HeisenbergXXZAbelianSymms<cd> ham(c.model, c.group, -1, -1, 1, 2212456, fixed ? 60 : 50, 1e-5, o);
     const auto E = ham.get_E_vecs();
     std::printf("{\"E\": [");
@@ -182,8 +183,9 @@ static void solve(const Case &c, bool fixed, bool batched = false) {
 
 // Ground-state expectation values (ComputeStaticExpValZeroT), including the
 // reference example's Sz_0 Sz_3 (Q/examples/examples.cu:897-909).
-static void expect(const Case &c) {
+static void expect(const Case &c, bool locality = false) {
     EDOptions o;
+    o.locality_slots = locality;
     HeisenbergXXZAbelianSymms<cd> ham(c.model, c.group, -1, -1, 1, 2212456, 50, 1e-5, o);
     Operator<cd> Sz0(0, OperatorType::Sz, 1.0), Sz1(3, OperatorType::Sz, 1.0);
     ManyBodyOperator<cd> szsz;
@@ -283,7 +285,10 @@ static void ref_cuda(const Case &c) {
 
 int main(int argc, char **argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s tables|solve|solve_fixed|solve_batched|expect|ref_cuda <case>\n", argv[0]);
+        std::fprintf(stderr,
+                     "usage: %s tables|solve|solve_fixed|solve_batched|solve_locality|expect|expect_locality|"
+                     "ref_cuda <case>\n",
+                     argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -297,6 +302,10 @@ int main(int argc, char **argv) {
             solve(c, true);
         else if (mode == "solve_batched")
             solve(c, false, true);
+        else if (mode == "solve_locality")
+            solve(c, false, false, true);
+        else if (mode == "expect_locality")
+            expect(c, true);
         else if (mode == "expect")
             expect(c);
         else if (mode == "ref_cuda")

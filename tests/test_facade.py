@@ -123,6 +123,28 @@ def test_facade_driver_batched_equals_sequential(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", ["chain16", "square2x4_example"])
+def test_facade_locality_slots(name):
+    """EDOptions::locality_slots (QMB_LOCALITY_SLOTS on every ELL the driver
+    builds): the same energies per irrep, the same ground-state irrep, and the
+    same ground-state expectation values (the regenerated state uses the same
+    slot order as the first solve)."""
+    print(f"[gpu-case] facade solve_locality {name}", flush=True)
+    seq = run("solve", name)
+    loc = run("solve_locality", name)
+    assert loc["GS_irrep"] == seq["GS_irrep"]
+    for a, b in zip(seq["E"], loc["E"]):
+        assert (a is None) == (b is None)
+        if a is not None:
+            assert abs(a - b) < 1e-10 * max(1, abs(a))
+    ex, exl = run("expect", name), run("expect_locality", name)
+    assert abs(ex["E_gs"] - exl["E_gs"]) < 1e-10 * max(1, abs(ex["E_gs"]))
+    for key, tol in (("szsz", 1e-5), ("H", 1e-10)):  # the state carries the Lanczos tolerance
+        v, w = complex(*ex[key]), complex(*exl[key])
+        assert abs(v - w) < tol * max(1, abs(v)), (key, v, w)
+
+
+@pytest.mark.gpu
 def test_facade_fixed_iteration_mode():
     print("[gpu-case] facade solve_fixed chain16", flush=True)
     got = run("solve_fixed", "chain16")

@@ -49,7 +49,8 @@ def test_c5_square6x6_full_size(ctx):
     b = q.Basis(ctx, g, 18, 0, q.LAYOUT_COMPACT)
     n = b.n_rows
     assert n == burnside_count(case, 18) == 126056625
-    e = q.Ell(ctx, b, t)
+    # slots in locality order, as the benchmark builds the sector
+    e = q.Ell(ctx, b, t, locality_slots=True)
     inf = e.info()
     assert inf.n_rows == n and inf.max_row_len <= 73 and inf.packed == 1
 
@@ -62,7 +63,8 @@ def test_c5_square6x6_full_size(ctx):
     starts = np.concatenate([[0, n - 100], starts])
     rows = np.concatenate([np.arange(s, s + 100) for s in starts])
     ind_or, val_or, tr_or = oracle.ell_rows_from_reps(36, case["group_el"], case["flip"], case["chars"],
-                                                      case["table"], 0, reps, rows, W, False)
+                                                      oracle.locality_order(case["table"]), 0, reps, rows, W,
+                                                      False)
     for i, s in enumerate(starts):
         ind, val, tr = e.export_rows(int(s), 100)
         sl = slice(i * 100 * W, (i + 1) * 100 * W)

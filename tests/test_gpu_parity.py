@@ -132,6 +132,39 @@ def test_packed_values_equal_fp64_values(ctx, label, mk, n_up, alphas):
             assert e1[0] == e2[0] and r1.iters == r2.iters
 
 
+@pytest.mark.parametrize("label,mk,n_up,alphas", [CASES[0], CASES[3], CASES[4]])
+def test_locality_slots_bit_exact(ctx, label, mk, n_up, alphas):
+    """QMB_LOCALITY_SLOTS: the library's reordered slots equal the oracle's ELL
+    built from the table in locality order (columns, values, track and SpMV bit
+    for bit, FP64 and packed), and the eigenvalue is the table-order one."""
+    case = mk()
+    loc = dict(case, table=oracle.locality_order(case["table"]))
+    orb = oracle_orbits(case, n_up)
+    for a in alphas:
+        say(f"locality slots {label} irrep={a}")
+        ref = oracle_ell(loc, orb, a)
+        vt = q.COMPLEX if ref.is_complex else q.REAL
+        g = q.Group(case["group_el"], case["chars"], case["flip"])
+        t = q.Terms(case["n_sites"], case["table"])
+        b = q.Basis(ctx, g, n_up, a, q.LAYOUT_REFERENCE)
+        rng = np.random.default_rng(a + 3)
+        x = rng.normal(size=orb.n_orb)
+        if ref.is_complex:
+            x = x + 1j * rng.normal(size=orb.n_orb)
+        for fp64 in (False, True):
+            e = q.Ell(ctx, b, t, value_type=vt, max_terms=ref.W, keep_fp64_values=fp64, locality_slots=True)
+            ind, val, track = e.export_reference()
+            assert np.array_equal(track, ref.track)
+            assert np.array_equal(ind, ref.ind)
+            assert np.array_equal(val.view(np.float64), ref.val)
+            assert np.array_equal(e.spmv(x), ref.spmv(x))
+        e_tab = q.Ell(ctx, b, t, value_type=vt, max_terms=ref.W)
+        assert not np.array_equal(e_tab.export_reference()[0], ind)
+        e1, _, _, _ = q.lanczos(e, max_iter=300, tol=1e-13)
+        e2, _, _, _ = q.lanczos(e_tab, max_iter=300, tol=1e-13)
+        assert abs(e1[0] - e2[0]) <= 1e-10 * max(1.0, abs(e2[0]))
+
+
 def test_compact_layout_is_reference_without_null_rows(ctx):
     case = chain_case(12, 1.0, 1.0, 0.3, flip=True)
     orb = oracle_orbits(case, 6)

@@ -358,3 +358,41 @@ def test_burnside_count_matches_orbit_build():
     # the configured sector sizes
     assert burnside_count(chain_case(28, flip=True), 14) == 716728
     assert burnside_count(chain_case(30), 15) == 5170604
+
+
+def test_locality_order_by_hand():
+    """The locality slot order (terms sorted on lowest site, then highest site,
+    descending, stable; operator-free terms first) on a table whose order is
+    worked out by hand, and the same matrix from the reordered table: every row
+    holds the same off-diagonal (column, value) pairs, only in another slot
+    order, and the same diagonal up to the rounding of its sum (the diagonal
+    terms are added in the new order)."""
+    Z, P, M = OM.OP_SZ, OM.OP_SP, OM.OP_SM
+    op = np.array([Z, Z, M, P, Z, Z, M, P, 0, 0, P, M, Z, 0], np.int32)
+    site = np.array([0, 1, 1, 2, 3, 0, 2, 3, 0, 0, 3, 0, 2, 0], np.int32)
+    scal = np.arange(1, 8).astype(np.complex128)
+    n_ops = np.array([2, 2, 2, 2, 0, 2, 1], np.int32)
+    # keys (lo, hi): (0,1) (1,2) (0,3) (2,3) none (0,3) (2,2)
+    op2, site2, scal2, n_ops2, mo = oracle.locality_order((op, site, scal, n_ops, 2))
+    assert np.array_equal(scal2.real.astype(int) - 1, [4, 3, 6, 1, 2, 5, 0])
+    assert np.array_equal(n_ops2, [0, 2, 1, 2, 2, 2, 2]) and mo == 2
+    assert np.array_equal(site2.reshape(-1, 2)[1], [2, 3]) and np.array_equal(op2.reshape(-1, 2)[1], [M, P])
+    for case, n_up, alphas in ((chain_case(12, 1.0, 0.7, 0.3, flip=True), 6, (0, 5)),
+                               (square_case(4, 3, 1.0, 0.8), 6, (1, 7))):
+        orb = oracle_orbits(case, n_up)
+        loc = dict(case, table=oracle.locality_order(case["table"]))
+        for a in alphas:
+            e1, e2 = oracle_ell(case, orb, a), oracle_ell(loc, orb, a)
+            assert np.array_equal(e1.track, e2.track)
+            W = e1.W
+            key = lambda t: (t[0], complex(t[1]).real, complex(t[1]).imag)  # noqa: E731
+            for r in range(e1.n_rows):
+                n = int(e1.track[r])
+                if n == 0:
+                    continue
+                assert e1.ind[r * W] == e2.ind[r * W] == r
+                assert abs(e1.values[r * W] - e2.values[r * W]) <= 1e-14
+                p1 = sorted(zip(e1.ind[r * W + 1:r * W + n].tolist(), e1.values[r * W + 1:r * W + n].tolist()), key=key)
+                p2 = sorted(zip(e2.ind[r * W + 1:r * W + n].tolist(), e2.values[r * W + 1:r * W + n].tolist()), key=key)
+                assert p1 == p2
+            assert not np.array_equal(e1.ind, e2.ind)  # the slot order did change
