@@ -51,7 +51,10 @@ qmb::GroupView qmb_basis::view() const {
 
 namespace qmb {
 
-constexpr int kBasisThreads = 256;
+// 8 waves per block share the staged binomial and group tables (about 28 KB of
+// LDS), so more waves fit a CU than with 4-wave blocks (6x6 sector basis: 43.5
+// -> 35 ms)
+constexpr int kBasisThreads = 512;
 
 __device__ inline bool compatible(uint64_t slo, uint64_t shi, uint64_t tlo, uint64_t thi) {
     return ((slo & ~tlo) | (shi & ~thi)) == 0;

@@ -748,9 +748,14 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
         // every instance overlap the GPU's next chunk. An instance that converged
         // at chunk c is dropped from chunk c+2 on (its extra steps are discarded).
         const size_t snap = (size_t)n * (3 * M + 2);
-        PinnedBuf hbuf(2 * snap * sizeof(double) + (size_t)n * sizeof(int));
+        PinnedBuf hbuf(2 * snap * sizeof(double) + 2 * (size_t)n * sizeof(int));
         double *hsnap[2] = {hbuf.as<double>(), hbuf.as<double>() + snap};
-        int *hact = reinterpret_cast<int *>(hbuf.as<double>() + 2 * snap);
+        // two staging copies of the active mask, used in turn: the copy queued after
+        // chunk c may still be pending when the host next updates the mask (it waits
+        // only for chunk c's snapshot event), but not two updates later
+        int *hact[2] = {reinterpret_cast<int *>(hbuf.as<double>() + 2 * snap),
+                        reinterpret_cast<int *>(hbuf.as<double>() + 2 * snap) + n};
+        int hsel = 0;
         hipEvent_t evs[2];
         QMB_HIP(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
         QMB_HIP(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
@@ -871,8 +876,9 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
             if (!next) break;  // no chunk in flight: that snapshot was the last one
             if (changed && n_active > 0) {
-                std::memcpy(hact, active.data(), n * sizeof(int));
-                QMB_HIP(hipMemcpyAsync(d_active.p, hact, n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+                std::memcpy(hact[hsel], active.data(), n * sizeof(int));
+                QMB_HIP(hipMemcpyAsync(d_active.p, hact[hsel], n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+                hsel = 1 - hsel;
             }
             pend = 1 - pend;
             pend_it = launched;

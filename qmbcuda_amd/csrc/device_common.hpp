@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -497,18 +498,67 @@ __device__ inline S orbit_min_coset(S x, const GroupView &gv, const GroupLds &t,
     return best;
 }
 
+// Colex rank by nibbles: T[(k * n_c + c) * 16 + v] is the rank contribution of
+// nibble k (sites 4k..4k+3) holding v when c set bits lie below it, i.e. the sum
+// over v's set bits i (ascending, j of v's bits below i) of C(4k + i, c + j + 1).
+// ceil(n_sites / 4) table reads instead of one per set bit; same value as
+// colex_rank for every state with at most n_c - 1 set bits.
+template <class S>
+__host__ __device__ inline int64_t colex_rank_nib(S x, const uint64_t *tab, int n_nib, int n_c) {
+    int64_t r = 0;
+    int c = 0;
+    for (int k = 0; k < n_nib; ++k) {
+        const unsigned v = (unsigned)(x >> (4 * k)) & 15u;
+        r += (int64_t)tab[(k * n_c + c) * 16 + (int)v];
+        c += __builtin_popcount(v);
+    }
+    return r;
+}
+
+// Host: the nibble rank table for n_sites sites and states of n_up set bits
+// (n_nib = ceil(n_sites / 4), n_c = n_up + 1); entries that no such state reads
+// (more than n_up bits up to that nibble) hold the same formula.
+inline std::vector<uint64_t> rank_nibble_table(int n_sites, int n_up, int &n_nib, int &n_c) {
+    n_nib = (n_sites + 3) / 4;
+    n_c = n_up + 1;
+    const int P = 4 * n_nib + 1;
+    std::vector<std::vector<uint64_t>> C(P, std::vector<uint64_t>(P + 8, 0));
+    for (int n = 0; n < P; ++n) {
+        C[n][0] = 1;
+        for (int k = 1; k <= n; ++k) C[n][k] = C[n - 1][k - 1] + (k <= n - 1 ? C[n - 1][k] : 0);
+    }
+    auto binom = [&](int n, int k) -> uint64_t { return (k < 0 || k > n) ? 0 : C[n][k]; };
+    std::vector<uint64_t> t((size_t)n_nib * n_c * 16, 0);
+    for (int k = 0; k < n_nib; ++k)
+        for (int c = 0; c < n_c; ++c)
+            for (int v = 0; v < 16; ++v) {
+                uint64_t r = 0;
+                int j = 0;
+                for (int i = 0; i < 4; ++i)
+                    if ((v >> i) & 1) {
+                        r += binom(4 * k + i, c + j + 1);
+                        ++j;
+                    }
+                t[((size_t)k * n_c + c) * 16 + v] = r;
+            }
+    return t;
+}
+
 // Row of a representative through the rank dictionary (rank -> record -> rows
 // before the word + popcount of the lower bits). Returns -1 when the
 // representative is not a row.
-template <class S>
-__device__ inline int64_t row_of_rep(S rep, const uint64_t *bin_lds, const uint64_t *dict) {
-    const int64_t rk = colex_rank<S>(rep, bin_lds);
+__device__ inline int64_t row_of_rank(int64_t rk, const uint64_t *dict) {
     const ulonglong2 rec = reinterpret_cast<const ulonglong2 *>(dict)[rk >> 6];
     const uint64_t w = rec.x;
     const int b = (int)(rk & 63);
     if (!((w >> b) & 1ULL)) return -1;
     const uint64_t below = b ? (w & ((1ULL << b) - 1ULL)) : 0ULL;
     return (int64_t)rec.y + __popcll(below);
+}
+
+template <class S>
+__device__ inline int64_t row_of_rep(S rep, const uint64_t *bin_lds, const uint64_t *dict) {
+    return row_of_rank(colex_rank<S>(rep, bin_lds), dict);
 }
 
 }  // namespace qmb

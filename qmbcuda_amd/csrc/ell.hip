@@ -19,7 +19,11 @@
 
 namespace qmb {
 
-constexpr int kEllThreads = 256;
+// 8 waves per block share the staged tables (rank, group, characters), so more
+// waves fit a CU's LDS than with 4-wave blocks (6x6 ELL build: 145 ms with
+// 4-wave blocks and the binomial rank loop, 131 ms with 8-wave blocks and the
+// nibble rank table)
+constexpr int kEllThreads = 512;
 
 constexpr int kSetSlots = 4096;  // global set of distinct values
 constexpr int kLdsSet = 256;     // per-block set
@@ -45,16 +49,19 @@ __device__ inline bool set_insert(unsigned long long *set, int slots, unsigned l
     return false;
 }
 
+// Rank tables in dynamic LDS: the nibble table (rank_tab, n_nib > 0) when it
+// fits, else the binomial table (binom).
 template <class S, bool FILL, bool CPLX>
-__global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom, int layout,
+__global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom,
+                                                        const uint64_t *rank_tab, int n_nib, int n_c, int layout,
                                                         int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
                                                         const uint8_t *nullf, const uint64_t *dict,
                                                         TermsView tv, const double2 *chi,
                                                         const double *nrm, int W, uint16_t *row_len, int *err,
                                                         int all_rows, const int64_t *chunk_ptr, int32_t *col,
                                                         double *val) {
+    extern __shared__ uint64_t rank_lds[];
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
-    __shared__ uint64_t bin[64 * kBinJ];
     __shared__ double2 s_chi[kMaxGroup];
     __shared__ double s_nrm[kMaxGroup + 1];
     __shared__ GroupLds gt;
@@ -64,7 +71,11 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
     if (lookups) {
         stage_perm(perm_lds, gv);
         stage_group(gt, gv);
-        stage_binom(bin, binom);
+        if (n_nib) {
+            for (int i = threadIdx.x; i < n_nib * n_c * 16; i += blockDim.x) rank_lds[i] = rank_tab[i];
+        } else {
+            stage_binom(rank_lds, binom);
+        }
         for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
         for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
         __syncthreads();
@@ -122,7 +133,8 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                               : gv.lut == 2 ? orbit_min_ring<S>(tgt, gv, gt, gm, mult)
                               : gv.lut == 3 ? orbit_min_coset<S>(tgt, gv, gt, perm_lds, gm, mult)
                                             : orbit_min<S>(tgt, gv, perm_lds, gm, mult);
-                const int64_t r2 = row_of_rep<S>(rep, bin, dict);
+                const int64_t r2 = row_of_rank(
+                    n_nib ? colex_rank_nib<S>(rep, rank_lds, n_nib, n_c) : colex_rank<S>(rep, rank_lds), dict);
                 if (r2 < 0) continue;
                 if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
                 if (FILL) {
@@ -493,6 +505,30 @@ double qmb_ell::format_bytes() const {
 
 extern "C" {
 
+namespace qmb {
+// The rank lookup table of a basis for the ELL kernels: the nibble table on the
+// device when it fits the LDS budget, else none (the kernels stage the binomial
+// table). lds_bytes = the dynamic LDS of a launch that makes lookups.
+struct RankTables {
+    static constexpr size_t kMaxBytes = 48 * 1024;
+    DevBuf buf;
+    int n_nib = 0, n_c = 0;
+    size_t lds_bytes = 64 * kBinJ * sizeof(uint64_t);
+    explicit RankTables(const qmb_basis &b) {
+        int nn = 0, nc = 0;
+        const std::vector<uint64_t> t = rank_nibble_table(b.n_sites, b.n_up, nn, nc);
+        const size_t bytes = t.size() * sizeof(uint64_t);
+        if (bytes > kMaxBytes) return;
+        buf.alloc(bytes);
+        QMB_HIP(hipMemcpy(buf.p, t.data(), bytes, hipMemcpyHostToDevice));
+        n_nib = nn;
+        n_c = nc;
+        lds_bytes = bytes;
+    }
+    const uint64_t *tab() const { return n_nib ? buf.as<uint64_t>() : nullptr; }
+};
+}  // namespace qmb
+
 qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, int value_type, int max_terms,
                          qmb_ell **out) {
     return guarded([&] {
@@ -535,14 +571,17 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         const int64_t n = b->n_rows;
         e->row_len.alloc((size_t)std::max<int64_t>(n, 1) * sizeof(uint16_t));
         e->chunk_ptr.alloc((size_t)(e->n_chunks + 1) * sizeof(int64_t));
+        const RankTables rank(*b);
         const unsigned blocks = (unsigned)std::max<int64_t>(1, (n + kEllThreads - 1) / kEllThreads);
         const bool cplx = value_type == QMB_COMPLEX;
         if (n > 0) {
             const bool narrow = b->n_sites <= 32;
             auto kcount = narrow ? (cplx ? ell_rows<uint32_t, false, true> : ell_rows<uint32_t, false, false>)
                                  : (cplx ? ell_rows<uint64_t, false, true> : ell_rows<uint64_t, false, false>);
-            hipLaunchKernelGGL(kcount, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
-                               ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
+            // the count pass of a sector without null orbits makes no orbit lookups
+            hipLaunchKernelGGL(kcount, dim3(blocks), dim3(kEllThreads), b->all_rows ? 0 : rank.lds_bytes,
+                               ctx->stream, gv, ctx->binom.as<uint64_t>(), rank.tab(), rank.n_nib, rank.n_c,
+                               b->layout, n, b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->dict.as<uint64_t>(),
                                tv, dt.chi, dt.nrm, W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), b->all_rows ? 1 : 0, nullptr, nullptr,
@@ -564,8 +603,9 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
             const bool narrow = b->n_sites <= 32;
             auto kfill = narrow ? (cplx ? ell_rows<uint32_t, true, true> : ell_rows<uint32_t, true, false>)
                                 : (cplx ? ell_rows<uint64_t, true, true> : ell_rows<uint64_t, true, false>);
-            hipLaunchKernelGGL(kfill, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, gv,
-                               ctx->binom.as<uint64_t>(), b->layout, n, b->reps.as<uint64_t>(),
+            hipLaunchKernelGGL(kfill, dim3(blocks), dim3(kEllThreads), rank.lds_bytes, ctx->stream, gv,
+                               ctx->binom.as<uint64_t>(), rank.tab(), rank.n_nib, rank.n_c, b->layout, n,
+                               b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->dict.as<uint64_t>(),
                                tv, dt.chi, dt.nrm, W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), 0, e->chunk_ptr.as<int64_t>(),

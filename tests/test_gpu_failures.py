@@ -82,15 +82,17 @@ def _learned_case(ctx, n=16, delta=0.9, j2=0.2):
 
 
 def test_learned_start_unusable_vector_falls_back(ctx):
-    """NaN or infinite network parameters make the learned vector unusable: the
-    seeded random start is used instead (fallback 1) and the result equals the
-    plain run from that seed bit for bit."""
+    """NaN or infinite network parameters make the learned vector unusable (NaN,
+    infinite, or zero amplitudes everywhere): the seeded random start is used
+    instead (fallback 1) and the result equals the plain run from that seed bit
+    for bit."""
     keep, net = _learned_case(ctx)
     b, e = keep[2], keep[3]
     ev_p, rep_p, _, _ = q.lanczos(e, max_iter=400, tol=1e-13, check_every=10)
     theta0 = net.params()
     for label, bad in (("NaN weights", lambda th: np.full_like(th, np.nan)),
-                       ("infinite log-amplitude bias", lambda th: _set_bias(th, np.inf))):
+                       ("infinite log-amplitude bias", lambda th: _set_bias(th, np.inf)),
+                       ("log-amplitude bias -inf: every amplitude zero", lambda th: _set_bias(th, -np.inf))):
         say(f"learned fallback 1: {label}")
         net.set_params(bad(theta0.copy()))
         x, fb_v = net.start_vector(b, 0.9, 0.2)

@@ -1,8 +1,9 @@
 """Host checks of device-code building blocks, built by __graft_entry__.build():
 the term compiler (compiled masks and scalars equal the operator-by-operator
-application bit for bit, tests/cpp/test_terms) and the bit-sliced ring window
+application bit for bit, tests/cpp/test_terms), the bit-sliced ring window
 scan of the ring orbit minimum (equal to a plain scan of every rotation's
-window, tests/cpp/test_ring_windows)."""
+window, tests/cpp/test_ring_windows) and the nibble-table colex rank of the ELL
+kernels (equal to the rank's definition, tests/cpp/test_rank)."""
 import os
 import subprocess
 
@@ -26,3 +27,7 @@ def test_compiled_terms_equal_sequential_application():
 
 def test_bit_sliced_ring_windows_equal_plain_scan():
     run("test_ring_windows")
+
+
+def test_nibble_table_rank_equals_colex_rank():
+    run("test_rank")
