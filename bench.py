@@ -5,7 +5,8 @@ Headline workload (default): the 6x6 periodic square-lattice Heisenberg model
 (J=1, Delta=1), Sz=0, k=(0,0), spin-flip even - 126,056,625 rows, <= 73 ELL
 entries per row (packed column | value-index words, FP64 diagonal, slots in
 locality order) built on the device - solved to |dE| < 1e-10 (tolerance 1e-13
-on the Ritz values) by FP64 Lanczos from the seeded random start.
+on the Ritz values) by FP64 Lanczos with partial reorthogonalisation from the
+seeded random start.
 
 One step = one full ground-state solve from the model tables: device basis
 build -> device ELL assembly -> Lanczos loop (fused SpMV + update kernels, host
@@ -181,7 +182,7 @@ def c5_learned(ctx, lat6, b6, e6, e0_plain, rep_plain, lanczos_s_plain, total_s_
     train_s = time.perf_counter() - t0
     net.set_windows(b6, LM.square_windows(lat6), LM.sublattice_mask(lat6))
     t0 = time.perf_counter()
-    ev, rep, fb, _ = net.lanczos(b6, e6, 1.0, 0.0, max_iter=200, tol=1e-13, reorth=0, check_every=1)
+    ev, rep, fb, _ = net.lanczos(b6, e6, 1.0, 0.0, max_iter=100, tol=1e-13, reorth=2, check_every=1)
     wall = time.perf_counter() - t0
     e2e = data_s + train_s + wall
     gain = lanczos_s_plain - wall
@@ -462,16 +463,18 @@ def run_c4(args, ctx=None):
 
 
 C5_WORKLOAD = ("6x6 periodic square-lattice Heisenberg J=1 Delta=1, Sz=0, k=(0,0), spin-flip even "
-               "(dim 126,056,625, <= 73 ELL entries/row, slots in locality order), FP64 Lanczos to |dE| < 1e-10")
+               "(dim 126,056,625, <= 73 ELL entries/row, slots in locality order), FP64 Lanczos with partial "
+               "reorthogonalisation to |dE| < 1e-10")
 
 
 def c5_step(ctx, group, table):
     """One full 6x6 solve: device basis, device ELL (slots in locality order:
     the same matrix, 0.34 instead of 0.54 distinct lines per gathered value),
-    plain Lanczos (the ground state converges in 44 steps with or without
-    partial reorthogonalisation, to the same E0, and the plain recurrence skips
-    the orthogonality work and the 1 GB kept per step; a step costs ~25 ms, the
-    convergence test well under 1 ms, so it runs after every step) with HIP
+    Lanczos with partial reorthogonalisation, the workload's selective scheme
+    (every Lanczos vector kept, 1 GB each: 100 steps bound the memory at 100 GB
+    beside the 23 GB packed matrix; the ground state converges in 44 steps
+    either way and the plain recurrence would save ~24 ms; a step costs ~25 ms,
+    the convergence test well under 1 ms, so it runs after every step) with HIP
     events around every fused SpMV launch."""
     import qmbcuda_amd as q
 
@@ -480,7 +483,7 @@ def c5_step(ctx, group, table):
     t1 = time.perf_counter()
     e = q.Ell(ctx, b, q.Terms(36, table), locality_slots=True)
     t2 = time.perf_counter()
-    ev, rep, _, _ = q.lanczos(e, max_iter=200, tol=1e-13, seed=2212456, profile=1, reorth=0, check_every=1)
+    ev, rep, _, _ = q.lanczos(e, max_iter=100, tol=1e-13, seed=2212456, profile=1, reorth=2, check_every=1)
     t3 = time.perf_counter()
     inf = e.info()
     return dict(e0=float(ev[0]), iters=int(rep.iters), n_spmv=int(rep.n_spmv), ms_spmv=float(rep.ms_spmv),
