@@ -397,16 +397,21 @@ def run_c4(args, ctx=None):
         group = q.Group(ge, ch)
         b = q.Basis(ctx, group, n // 2, 0, q.LAYOUT_COMPACT)
         keep += [group, b]
+        ells, coup = [], []
         for d in deltas:
             for j2 in j2s:
                 lat_, info = LM.chain(n, 1.0, float(d), float(j2))
                 t = q.Terms(n, LM.term_table(lat_, info))
-                e = q.Ell(ctx, b, t, locality_slots=True)
-                # training targets to tol 1e-9 (eigenvector error ~3e-5, below the
-                # bf16 network's resolution); the evaluation solves use 1e-13
-                ev, rep, _, u = q.lanczos(e, max_iter=600, tol=1e-9, want_eigvec=True)
-                net.add_sample(b, u, float(d), float(j2))
+                ells.append(q.Ell(ctx, b, t, locality_slots=True))
+                coup.append((float(d), float(j2)))
                 keep.append(t)
+        # the grid's ground states of this size in one batched solve with vectors;
+        # training targets to tol 1e-9 (eigenvector error ~3e-5, below the bf16
+        # network's resolution); the evaluation solves use 1e-13
+        _, _, vecs = q.lanczos_batched(ells, max_iter=200, tol=1e-9, reorth=2, want_eigvec=True)
+        for (d, j2), u in zip(coup, vecs):
+            net.add_sample(b, u, d, j2)
+        del ells, vecs
     ctx.synchronize()
     data_s = time.perf_counter() - t0
     t0 = time.perf_counter()

@@ -344,6 +344,16 @@ qmb_status qmb_expectation_noalloc(qmb_ctx *ctx, const qmb_basis *b, const qmb_t
 qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
                                const qmb_lanczos_opts *opts, double *evals,
                                qmb_lanczos_report *reports);
+/* qmb_lanczos_batched with every instance's ground-state vector: vecs[i] (host,
+ * n_rows doubles, 2 n_rows interleaved for a complex ELL) gets x = sum_j z_j
+ * phi_j, z the unit eigenvector of the instance's tridiagonal matrix for its
+ * lowest eigenvalue (z_0 >= 0), phi_j its kept Lanczos vectors - the vector
+ * qmb_lanczos returns with want_eigvec and reorthogonalisation. Needs
+ * opts->reorth = 1 or 2 (the vectors are kept); opts->want_eigvec is ignored.
+ * For many small sectors at once (training ground states of a learned start). */
+qmb_status qmb_lanczos_batched_vectors(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
+                                       const qmb_lanczos_opts *opts, double *evals, double *const *vecs,
+                                       qmb_lanczos_report *reports);
 
 /* ---- learned start vector (new; no reference counterpart) -------------------
  * A translation-invariant network over 8-spin windows (+-1 spins, Delta, J2/J1

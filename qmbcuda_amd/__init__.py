@@ -314,18 +314,26 @@ def lanczos_checkpointed(ell: Ell, path, every=0, resume=False, stop_after=0, n_
 
 
 def lanczos_batched(ells, n_eigs=1, max_iter=300, tol=1e-13, seed=2212456, reorth=None, check_every=10,
-                    profile=False):
+                    profile=False, want_eigvec=False):
     """Solve several ELLs together (one launch per phase for all of them).
-    profile: time the graph replays (report.ms_spmv, report.n_spmv_timed)."""
+    profile: time the graph replays (report.ms_spmv, report.n_spmv_timed).
+    want_eigvec: also return every instance's ground-state vector
+    (qmb_lanczos_batched_vectors; needs reorth 1 or 2, partial by default).
+    Returns (evals[n, n_eigs], reports) or (evals, reports, vectors)."""
     if reorth is None:
-        reorth = n_eigs > 1
+        reorth = 2 if want_eigvec else int(n_eigs > 1)
     o = default_opts(max_iter=max_iter, n_eigs=n_eigs, tol=tol, seed=seed, reorth=int(reorth),
                      check_every=check_every, profile=int(profile))
     arr = (C.c_void_p * len(ells))(*[e.h for e in ells])
     ev = np.zeros(len(ells) * n_eigs, np.float64)
     reps = (LanczosReport * len(ells))()
-    _check(lib().qmb_lanczos_batched(ells[0].ctx.h, len(ells), arr, C.byref(o), _ptr(ev), reps))
-    return ev.reshape(len(ells), n_eigs), list(reps)
+    if not want_eigvec:
+        _check(lib().qmb_lanczos_batched(ells[0].ctx.h, len(ells), arr, C.byref(o), _ptr(ev), reps))
+        return ev.reshape(len(ells), n_eigs), list(reps)
+    vecs = [np.zeros(e.basis.n_rows, np.complex128 if e.is_complex else np.float64) for e in ells]
+    vp = (C.c_void_p * len(ells))(*[v.ctypes.data for v in vecs])
+    _check(lib().qmb_lanczos_batched_vectors(ells[0].ctx.h, len(ells), arr, C.byref(o), _ptr(ev), vp, reps))
+    return ev.reshape(len(ells), n_eigs), list(reps), vecs
 
 
 class Mlp:

@@ -132,6 +132,7 @@ SIGNATURES = [
      [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_longlong, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
       C.POINTER(C.c_int)]),
     ("qmb_lanczos_batched", C.c_int, [_vp, C.c_int, _vp, C.POINTER(LanczosOpts), _vp, _vp]),
+    ("qmb_lanczos_batched_vectors", C.c_int, [_vp, C.c_int, _vp, C.POINTER(LanczosOpts), _vp, _vp, _vp]),
     ("qmb_mlp_num_params", C.c_int, []),
     ("qmb_mlp_create", C.c_int, [_vp, C.c_uint64, _pp]),
     ("qmb_mlp_destroy", C.c_int, [_vp]),

@@ -463,10 +463,23 @@ static void tile_cache_put(const std::tuple<int, int, int> &key, TileChoice c) {
 
 using namespace qmb;
 
-extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const qmb_lanczos_opts *opts,
-                                          double *evals, qmb_lanczos_report *reports) {
-    return guarded([&] {
-        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
+namespace qmb {
+
+// x = sum_j z[j] V[j * cnt + i] over the instance's kept Lanczos vectors (the
+// single solver's basis combination, same order of the sum)
+__global__ __launch_bounds__(kThreads) void bk_combine(int64_t cnt, int m, const double *__restrict__ V,
+                                                       const double *__restrict__ z, double *__restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < m; ++j) s = s + z[j] * V[(size_t)j * cnt + i];
+        x[i] = s;
+    }
+}
+
+// vecs: NULL (eigenvalues only) or one host array per instance
+static void run_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const qmb_lanczos_opts *opts,
+                        double *evals, double *const *vecs, qmb_lanczos_report *reports) {
+    {
         QMB_REQUIRE(ctx && ells && opts && evals && n >= 1, "qmb_lanczos_batched: bad argument");
         const qmb_lanczos_opts o = *opts;
         QMB_REQUIRE(o.max_iter >= 1 && o.max_iter <= 5000, "qmb_lanczos_batched: max_iter must be in [1, 5000]");
@@ -770,6 +783,7 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
             res(n, std::vector<double>(k_eigs, std::nan("")));
         std::vector<int> iters(n, 0), conv(n, 0), brk(n, 0);
         std::vector<char> done_now(n, 0);
+        std::vector<std::vector<double>> zvec(vecs ? n : 0);  // lowest Ritz vector of T, per stopped instance
         int launched = 0, n_active = n;
         double ms_host = 0.0;
         // with opts.profile: HIP events around every graph replay (the GPU's busy
@@ -849,6 +863,10 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                     conv[pos] = ok ? 1 : 0;
                     brk[pos] = bd ? 1 : 0;
                     done_now[pos] = 1;
+                    if (vecs) {
+                        zvec[pos].assign(m, 0.0);
+                        tridiag_lowest(m, a, b, 1, w.data(), zl.data(), zvec[pos].data());
+                    }
                 }
             };
             std::vector<int> act;
@@ -918,6 +936,54 @@ extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *co
                 r.n_spmv_timed = (int)(rev.size() / 2);
             }
         }
+        if (vecs) {
+            // ground-state vectors from the kept Krylov bases (the instances' extra
+            // steps after convergence wrote only slots past their own m)
+            DevBuf dz((size_t)M * sizeof(double));
+            for (int pos = 0; pos < n; ++pos) {
+                const int gi = order[pos];
+                const qmb_ell *e = ells[gi];
+                const int64_t cnt = e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1);
+                const int m = (int)zvec[pos].size();
+                DevBuf X((size_t)cnt * sizeof(double));
+                if (m > 0) {
+                    QMB_HIP(hipMemcpyAsync(dz.p, zvec[pos].data(), (size_t)m * sizeof(double),
+                                           hipMemcpyHostToDevice, ctx->stream));
+                    const unsigned g = (unsigned)std::min<int64_t>(4096, (cnt + kThreads * 4 - 1) / (kThreads * 4));
+                    hipLaunchKernelGGL(bk_combine, dim3(std::max(1u, g)), dim3(kThreads), 0, ctx->stream, cnt, m,
+                                       hin[pos].VB, dz.as<double>(), X.as<double>());
+                    QMB_HIP(hipGetLastError());
+                } else {
+                    QMB_HIP(hipMemsetAsync(X.p, 0, (size_t)cnt * sizeof(double), ctx->stream));
+                }
+                QMB_HIP(hipMemcpyAsync(vecs[gi], X.p, (size_t)cnt * sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+                QMB_HIP(hipStreamSynchronize(ctx->stream));  // dz and X are reused or freed next
+            }
+        }
         if (!all_conv) throw Error{QMB_ERR_NOT_CONVERGED, "qmb_lanczos_batched: some instances did not converge"};
+    }
+}
+
+}  // namespace qmb
+
+extern "C" qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const qmb_lanczos_opts *opts,
+                                          double *evals, qmb_lanczos_report *reports) {
+    return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
+        run_batched(ctx, n, ells, opts, evals, nullptr, reports);
+    });
+}
+
+extern "C" qmb_status qmb_lanczos_batched_vectors(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
+                                                  const qmb_lanczos_opts *opts, double *evals, double *const *vecs,
+                                                  qmb_lanczos_report *reports) {
+    return guarded([&] {
+        PoolScope pool_scope(ctx ? &ctx->pool : nullptr);
+        QMB_REQUIRE(vecs && opts, "qmb_lanczos_batched_vectors: bad argument");
+        QMB_REQUIRE(opts->reorth == 1 || opts->reorth == 2,
+                    "qmb_lanczos_batched_vectors: the vectors need kept Lanczos vectors (reorth = 1 or 2)");
+        for (int i = 0; i < n; ++i) QMB_REQUIRE(vecs[i], "qmb_lanczos_batched_vectors: NULL vector");
+        run_batched(ctx, n, ells, opts, evals, vecs, reports);
     });
 }

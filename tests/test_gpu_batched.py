@@ -85,6 +85,38 @@ def test_batched_equals_sequential_ground_states(ctx):
     assert abs(ev[labels.index((8, 0))][0] - (-7.142296360617)) < 1e-10
 
 
+@pytest.mark.parametrize("reorth", [1, 2])
+def test_batched_vectors_are_ground_states(ctx, reorth):
+    """qmb_lanczos_batched_vectors: every instance's vector (real and complex
+    sectors) is the dense ground state of its oracle sector matrix up to a phase
+    (|<x, g>| = 1 to 1e-9) with the convergence test's residual bound, and
+    matches the single solve's want_eigvec vector; reorth = 0 is refused."""
+    print(f"[gpu-case] batched vectors chain12 reorth={reorth}", flush=True)
+    keep, ells, labels = sector_ells(ctx, 12, [6, 4], J2=0.3, delta=0.9)
+    ev, reps, vecs = q.lanczos_batched(ells, max_iter=300, tol=1e-13, reorth=reorth, want_eigvec=True)
+    case = chain_case(12, 1.0, 0.9, 0.3)
+    orbs = {n_up: oracle_orbits(case, n_up) for n_up in (6, 4)}
+    for i, (n_up, a) in enumerate(labels):
+        x = vecs[i]
+        assert reps[i].converged
+        H = ell_dense_nonnull(oracle_ell(case, orbs[n_up], a), orbs[n_up], a)
+        E, V = np.linalg.eigh(H)
+        E0 = ev[i][0]
+        assert abs(E0 - E[0]) < 1e-10 * max(1.0, abs(E[0]))
+        nx = np.linalg.norm(x)
+        assert abs(nx - 1.0) < 1e-6
+        res = np.linalg.norm(H @ x - E0 * x)
+        assert res < 2 * np.sqrt(1e-13) * max(1.0, abs(E0)), (labels[i], res)
+        if E[1] - E[0] > 1e-6:  # non-degenerate ground state: the vector itself
+            assert abs(abs(np.vdot(V[:, 0], x)) - nx) < 1e-9, labels[i]
+        if reorth == 2 and i % 5 == 0:
+            _, _, _, xs = q.lanczos(ells[i], max_iter=300, tol=1e-13, reorth=2, want_eigvec=True)
+            assert abs(abs(np.vdot(xs, x)) - np.linalg.norm(xs) * nx) < 1e-9, labels[i]
+    if reorth == 1:
+        with pytest.raises(q.QMBError):
+            q.lanczos_batched(ells[:2], max_iter=300, tol=1e-13, reorth=0, want_eigvec=True)
+
+
 def test_batched_c3_full_size(ctx):
     """C3: N=24 chain, 24 momenta x n_up in {12, 14, 16}: 72 uneven sectors, lowest 4 each."""
     print("[gpu-case] batched C3 N=24 72 sectors", flush=True)
