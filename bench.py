@@ -4,9 +4,10 @@
 Headline workload (default): the 6x6 periodic square-lattice Heisenberg model
 (J=1, Delta=1), Sz=0, k=(0,0), spin-flip even - 126,056,625 rows, <= 73 ELL
 entries per row (packed column | value-index words, FP64 diagonal, slots in
-locality order) built on the device - solved to |dE| < 1e-10 (tolerance 1e-13
-on the Ritz values) by FP64 Lanczos with partial reorthogonalisation from the
-seeded random start.
+locality order) built on the device - solved to |dE| < 1e-10 (tolerance 1e-12
+on the Ritz values: the convergence test's bound r^2/gap <= 1e-12 |E0| = 2.4e-11;
+checked against a tol-1e-15 solve of the same matrix) by FP64 Lanczos with
+partial reorthogonalisation from the seeded random start.
 
 One step = one full ground-state solve from the model tables: device basis
 build -> device ELL assembly -> Lanczos loop (fused SpMV + update kernels, host
@@ -182,7 +183,7 @@ def c5_learned(ctx, lat6, b6, e6, e0_plain, rep_plain, lanczos_s_plain, total_s_
     train_s = time.perf_counter() - t0
     net.set_windows(b6, LM.square_windows(lat6), LM.sublattice_mask(lat6))
     t0 = time.perf_counter()
-    ev, rep, fb, _ = net.lanczos(b6, e6, 1.0, 0.0, max_iter=100, tol=1e-13, reorth=2, check_every=1)
+    ev, rep, fb, _ = net.lanczos(b6, e6, 1.0, 0.0, max_iter=100, tol=C5_TOL, reorth=2, check_every=1)
     wall = time.perf_counter() - t0
     e2e = data_s + train_s + wall
     gain = lanczos_s_plain - wall
@@ -469,6 +470,9 @@ def run_c4(args, ctx=None):
     return out
 
 
+# The 6x6 target is |dE| < 1e-10: tolerance 1e-12 on the Ritz values bounds the
+# error by r^2/gap <= 1e-12 |E0| = 2.4e-11 (44 -> 42 steps against 1e-13)
+C5_TOL = 1e-12
 C5_WORKLOAD = ("6x6 periodic square-lattice Heisenberg J=1 Delta=1, Sz=0, k=(0,0), spin-flip even "
                "(dim 126,056,625, <= 73 ELL entries/row, slots in locality order), FP64 Lanczos with partial "
                "reorthogonalisation to |dE| < 1e-10")
@@ -490,7 +494,7 @@ def c5_step(ctx, group, table):
     t1 = time.perf_counter()
     e = q.Ell(ctx, b, q.Terms(36, table), locality_slots=True)
     t2 = time.perf_counter()
-    ev, rep, _, _ = q.lanczos(e, max_iter=100, tol=1e-13, seed=2212456, profile=1, reorth=2, check_every=1)
+    ev, rep, _, _ = q.lanczos(e, max_iter=100, tol=C5_TOL, seed=2212456, profile=1, reorth=2, check_every=1)
     t3 = time.perf_counter()
     inf = e.info()
     return dict(e0=float(ev[0]), iters=int(rep.iters), n_spmv=int(rep.n_spmv), ms_spmv=float(rep.ms_spmv),
@@ -537,6 +541,9 @@ def run_headline(args):
     traffic = profile_traffic("spmv_sell_square_6x6")
     fmt_B = runs[0]["format_bytes"] + 8.0 * n_rows
     e0 = runs[-1]["e0"]
+    # accuracy check outside the timed region: the same matrix solved to tol 1e-15
+    ev_t, rep_t, _, _ = q.lanczos(last["ell"], max_iter=100, tol=1e-15, seed=2212456, reorth=2, check_every=1,
+                                  allow_unconverged=True)
     out = {
         "metric": METRIC,
         "value": round(n_spmv * B / wall / 1e9, 3),
@@ -550,7 +557,8 @@ def run_headline(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": C5_WORKLOAD, "lanczos": "seeded random start, tol 1e-13 on the Ritz values",
+        "config": {"workload": C5_WORKLOAD,
+                   "lanczos": "seeded random start, tol 1e-12 on the Ritz values (|dE| bound 2.4e-11)",
                    "ell_slots": ELL_SLOTS, "parallelism": "single GPU"},
         "time_to_e0_ms": round(ms_per_step, 3),
         "step_ms": [round(r["total_ms"], 1) for r in runs],
@@ -562,6 +570,8 @@ def run_headline(args):
         "spmv_per_s": round(n_spmv / wall, 3),
         "e0": e0,
         "e0_per_site": e0 / 36,
+        "abs_dE_vs_tight_solve": abs(e0 - float(ev_t[0])),
+        "tight_solve": {"tol": 1e-15, "iters": int(rep_t.iters), "e0": float(ev_t[0])},
         "abs_dE_per_site_vs_literature": abs(e0 / 36 - (-0.678872)),
         "literature": "E0/N = -0.678872 (Schulz, Ziman, Poilblanc 1996), to its published digits",
         "n_rows": n_rows,
