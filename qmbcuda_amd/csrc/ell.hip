@@ -503,8 +503,6 @@ double qmb_ell::format_bytes() const {
            (double)(n_values + n_values_im) * 8.0;
 }
 
-extern "C" {
-
 namespace qmb {
 // The rank lookup table of a basis for the ELL kernels: the nibble table on the
 // device when it fits the LDS budget, else none (the kernels stage the binomial
@@ -528,6 +526,8 @@ struct RankTables {
     const uint64_t *tab() const { return n_nib ? buf.as<uint64_t>() : nullptr; }
 };
 }  // namespace qmb
+
+extern "C" {
 
 qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, int value_type, int max_terms,
                          qmb_ell **out) {
