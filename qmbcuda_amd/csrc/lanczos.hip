@@ -312,30 +312,56 @@ __global__ __launch_bounds__(kThreads) void vec_axpy(int64_t cnt, double c, cons
 }
 
 // ---- reorthogonalisation (classical Gram-Schmidt, second pass on demand) ------
-// dots: coef[j] = <V_j, w> for j < nv; grid (blocks, nv).
+// dots: coef[j] = <V_j, w> for j < nv; grid (blocks, ceil(nv / kDotGroup)). A
+// block takes kDotGroup vectors at once, so w is read once per group instead of
+// once per vector; every dot keeps its own accumulator in the same index order,
+// so the partials are those of one vector per block.
+constexpr int kDotGroup = 4;
+
 template <bool CPLX>
-__global__ __launch_bounds__(kThreads) void reorth_dots(int64_t n, int64_t stride, const double *__restrict__ V,
+__global__ __launch_bounds__(kThreads) void reorth_dots(int64_t n, int64_t stride, int nv,
+                                                        const double *__restrict__ V,
                                                         const double *__restrict__ w, double *__restrict__ part,
                                                         int nb, const double *__restrict__ gate) {
     __shared__ double lds[kWaves];
     if (gated_off(gate)) return;
-    const int j = blockIdx.y;
-    const double *vj = V + (size_t)j * stride;
-    double sr = 0.0, si = 0.0;
+    const int j0 = blockIdx.y * kDotGroup;
+    const int nj = min(kDotGroup, nv - j0);
+    double sr[kDotGroup], si[kDotGroup];
+#pragma unroll
+    for (int q = 0; q < kDotGroup; ++q) {
+        sr[q] = 0.0;
+        si[q] = 0.0;
+    }
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (CPLX) {
-            const double vr = vj[2 * i], vi = vj[2 * i + 1], wr = w[2 * i], wi = w[2 * i + 1];
-            sr += vr * wr + vi * wi;
-            si += vr * wi - vi * wr;
+            const double wr = w[2 * i], wi = w[2 * i + 1];
+#pragma unroll
+            for (int q = 0; q < kDotGroup; ++q) {
+                if (q < nj) {
+                    const double *vj = V + (size_t)(j0 + q) * stride;
+                    const double vr = vj[2 * i], vi = vj[2 * i + 1];
+                    sr[q] += vr * wr + vi * wi;
+                    si[q] += vr * wi - vi * wr;
+                }
+            }
         } else {
-            sr += vj[i] * w[i];
+            const double wr = w[i];
+#pragma unroll
+            for (int q = 0; q < kDotGroup; ++q)
+                if (q < nj) sr[q] += V[(size_t)(j0 + q) * stride + i] * wr;
         }
     }
-    const double tr = block_sum(sr, lds);
-    const double ti = CPLX ? block_sum(si, lds) : 0.0;
-    if (threadIdx.x == 0) {
-        part[((size_t)j * nb + blockIdx.x) * 2] = tr;
-        part[((size_t)j * nb + blockIdx.x) * 2 + 1] = ti;
+#pragma unroll
+    for (int q = 0; q < kDotGroup; ++q) {
+        if (q < nj) {  // nj is uniform over the block: every thread takes part in each sum
+            const double tr = block_sum(sr[q], lds);
+            const double ti = CPLX ? block_sum(si[q], lds) : 0.0;
+            if (threadIdx.x == 0) {
+                part[((size_t)(j0 + q) * nb + blockIdx.x) * 2] = tr;
+                part[((size_t)(j0 + q) * nb + blockIdx.x) * 2 + 1] = ti;
+            }
+        }
     }
 }
 
@@ -702,8 +728,9 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             for (int pass = 0; pass < 2; ++pass) {
                 const double *g = pass ? gate : nullptr;
                 auto kd = cplx ? reorth_dots<true> : reorth_dots<false>;
-                hipLaunchKernelGGL(kd, dim3(rblocks, nv), dim3(kThreads), 0, ctx->stream, n, (int64_t)cnt,
-                                   VB.as<double>(), P.as<double>(), rpart.as<double>(), (int)rblocks, g);
+                hipLaunchKernelGGL(kd, dim3(rblocks, (nv + kDotGroup - 1) / kDotGroup), dim3(kThreads), 0,
+                                   ctx->stream, n, (int64_t)cnt, nv, VB.as<double>(), P.as<double>(),
+                                   rpart.as<double>(), (int)rblocks, g);
                 QMB_HIP(hipGetLastError());
                 hipLaunchKernelGGL(reorth_reduce, dim3(nv), dim3(kThreads), 0, ctx->stream, rpart.as<double>(),
                                    (int)rblocks, rcoef.as<double>(), g);
