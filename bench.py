@@ -722,7 +722,8 @@ def main():
     ap.add_argument("--c4-steps", type=int, default=100)
     ap.add_argument("--c4-eval", type=int, default=16)
     ap.add_argument("--c4-n", type=int, default=30)
-    ap.add_argument("--c5-steps", type=int, default=150)
+    # the 6x6 learned start takes 31 steps after 25, 50, 100 or 150 Adam steps
+    ap.add_argument("--c5-steps", type=int, default=50)
     ap.add_argument("--no-learned", action="store_true", help="c5: skip the learned-start solve")
     ap.add_argument("--tol", type=float, default=1e-13)
     ap.add_argument("--no-cpu-baseline", action="store_true")
