@@ -163,18 +163,28 @@ def c5_learned(ctx, lat6, b6, e6, e0_plain, rep_plain, lanczos_s_plain, total_s_
     t0 = time.perf_counter()
     # (more couplings, more steps, or 4x8 and 8x4 lattices in the training set leave
     # the 6x6 solve at the same step count)
+    samples = []  # (basis, lattice, Delta) of each training matrix
+    ells = []
     for (n1, n2) in ((4, 4), (4, 6), (6, 4)):
+        lat, _ = LM.square(n1, n2, 1.0, 1.0)
+        ge, fl, ch = LM.translation_group(lat, spin_flip=True)
+        g = q.Group(ge, ch, fl)
+        b = q.Basis(ctx, g, n1 * n2 // 2, 0, q.LAYOUT_COMPACT)
+        keep += [g, b]
         for d in (0.9, 1.0, 1.1):
-            lat, info = LM.square(n1, n2, 1.0, d)
-            ge, fl, ch = LM.translation_group(lat, spin_flip=True)
-            g = q.Group(ge, ch, fl)
-            t = q.Terms(n1 * n2, LM.term_table(lat, info))
-            b = q.Basis(ctx, g, n1 * n2 // 2, 0, q.LAYOUT_COMPACT)
-            e = q.Ell(ctx, b, t, locality_slots=True)
-            net.set_windows(b, LM.square_windows(lat), LM.sublattice_mask(lat))
-            _, _, _, u = q.lanczos(e, max_iter=300, want_eigvec=True)
-            net.add_sample(b, u, d, 0.0)
-            keep += [g, t, b, e]
+            lat_d, info = LM.square(n1, n2, 1.0, d)
+            t = q.Terms(n1 * n2, LM.term_table(lat_d, info))
+            ells.append(q.Ell(ctx, b, t, locality_slots=True))
+            samples.append((b, lat, d))
+            keep.append(t)
+    # the nine training ground states in one batched solve with vectors
+    _, _, vecs = q.lanczos_batched(ells, max_iter=300, tol=1e-9, reorth=2, want_eigvec=True)
+    for i in range(0, len(samples), 3):  # the windows of a basis are set before its samples
+        b, lat, _ = samples[i]
+        net.set_windows(b, LM.square_windows(lat), LM.sublattice_mask(lat))
+    for (b, lat, d), u in zip(samples, vecs):
+        net.add_sample(b, u, d, 0.0)
+    del ells, vecs
     ctx.synchronize()
     data_s = time.perf_counter() - t0
     t0 = time.perf_counter()
