@@ -350,7 +350,9 @@ qmb_status qmb_lanczos_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
  * lowest eigenvalue (z_0 >= 0), phi_j its kept Lanczos vectors - the vector
  * qmb_lanczos returns with want_eigvec and reorthogonalisation. Needs
  * opts->reorth = 1 or 2 (the vectors are kept); opts->want_eigvec is ignored.
- * For many small sectors at once (training ground states of a learned start). */
+ * For many small sectors at once (training ground states of a learned start).
+ * No counterpart in the reference (its eigenvector pass, lanczos.cu:31-104,
+ * serves one irrep at a time). */
 qmb_status qmb_lanczos_batched_vectors(qmb_ctx *ctx, int n, const qmb_ell *const *ells,
                                        const qmb_lanczos_opts *opts, double *evals, double *const *vecs,
                                        qmb_lanczos_report *reports);
