@@ -13,7 +13,7 @@ from tests.helpers import chain_case, ell_dense_nonnull, oracle_ell, oracle_orbi
 pytestmark = pytest.mark.gpu
 
 
-def sector_ells(ctx, n, n_ups, J2=0.0, delta=1.0):
+def sector_ells(ctx, n, n_ups, J2=0.0, delta=1.0, locality_slots=False):
     lat, info = LM.chain(n, 1.0, delta, J2)
     ge, fl, ch = LM.translation_group(lat)
     g = q.Group(ge, ch)
@@ -24,7 +24,7 @@ def sector_ells(ctx, n, n_ups, J2=0.0, delta=1.0):
             b = q.Basis(ctx, g, n_up, k, q.LAYOUT_COMPACT)
             if b.n_rows == 0:
                 continue
-            e = q.Ell(ctx, b, t)
+            e = q.Ell(ctx, b, t, locality_slots=locality_slots)
             keep.append(b)
             ells.append(e)
             labels.append((n_up, k))
@@ -115,6 +115,20 @@ def test_batched_vectors_are_ground_states(ctx, reorth):
     if reorth == 1:
         with pytest.raises(q.QMBError):
             q.lanczos_batched(ells[:2], max_iter=300, tol=1e-13, reorth=0, want_eigvec=True)
+
+
+def test_batched_locality_slots(ctx):
+    """The batch of locality-ordered matrices (the benchmark's matrices) gives the
+    table-order batch's lowest four eigenvalues of every sector (1e-10)."""
+    print("[gpu-case] batched chain12 locality slots", flush=True)
+    keep, ells, labels = sector_ells(ctx, 12, [6, 4], J2=0.3, delta=0.9)
+    keep2, ells2, labels2 = sector_ells(ctx, 12, [6, 4], J2=0.3, delta=0.9, locality_slots=True)
+    assert labels == labels2
+    ev, reps = q.lanczos_batched(ells, n_eigs=4, max_iter=300, tol=1e-13, reorth=2)
+    ev2, reps2 = q.lanczos_batched(ells2, n_eigs=4, max_iter=300, tol=1e-13, reorth=2)
+    for i in range(len(ells)):
+        assert reps[i].converged and reps2[i].converged
+        np.testing.assert_allclose(ev2[i], ev[i], atol=1e-10 * max(1.0, abs(ev[i][0])))
 
 
 def test_batched_c3_full_size(ctx):
