@@ -6,7 +6,6 @@ because both sides use the same FP64 operation sequence (no FMA contraction).
 Eigenvalues must agree within 1e-10 relative with the oracle's converged
 Lanczos and with dense diagonalisation where it fits.
 """
-import math
 import sys
 
 import numpy as np
@@ -15,7 +14,7 @@ import pytest
 import oracle
 import qmbcuda_amd as q
 from oracle import model as OM
-from tests.helpers import (DeviceBuffer, chain_case, dense_spectrum, ell_dense_nonnull, oracle_ell, oracle_orbits,
+from tests.helpers import (DeviceBuffer, chain_case, ell_dense_nonnull, oracle_ell, oracle_orbits,
                            square_case)
 
 pytestmark = pytest.mark.gpu
