@@ -131,7 +131,21 @@ def test_packed_values_equal_fp64_values(ctx, label, mk, n_up, alphas):
             assert e1[0] == e2[0] and r1.iters == r2.iters
 
 
-@pytest.mark.parametrize("label,mk,n_up,alphas", [CASES[0], CASES[3], CASES[4]])
+def chain12_with_constant():
+    """The chain12 flip table with an operator-free (constant) term inserted in
+    the middle: the locality order puts it first."""
+    case = chain_case(12, 1.0, 1.0, 0.3, flip=True)
+    op, site, sc, n_ops, mo = case["table"]
+    h = len(n_ops) // 2
+    op = np.insert(np.asarray(op).reshape(-1, mo), h, np.zeros(mo, np.int32), axis=0).reshape(-1)
+    site = np.insert(np.asarray(site).reshape(-1, mo), h, np.zeros(mo, np.int32), axis=0).reshape(-1)
+    sc = np.insert(np.asarray(sc), h, 0.3 + 0.0j)
+    n_ops = np.insert(np.asarray(n_ops), h, 0)
+    return dict(case, table=(op.astype(np.int32), site.astype(np.int32), sc, n_ops.astype(np.int32), mo))
+
+
+@pytest.mark.parametrize("label,mk,n_up,alphas", [CASES[0], CASES[3], CASES[4],
+                                                   ("chain12_constant", chain12_with_constant, 6, [0, 5])])
 def test_locality_slots_bit_exact(ctx, label, mk, n_up, alphas):
     """QMB_LOCALITY_SLOTS: the library's reordered slots equal the oracle's ELL
     built from the table in locality order (columns, values, track and SpMV bit
