@@ -322,6 +322,8 @@ def run_c3(args, ctx=None):
            "reorth": "partial (Simon 1984; batched: decided per instance on the device)",
            "ell_build_s": round(build_s, 3), "max_abs_dE_vs_sequential_full": dE,
            "autotune_warmup_ms": round(tune_s * 1e3, 2),
+           # a first batch of a new shape also pays the tuning call (8 steps + the tile sweep)
+           "speedup_incl_autotune": round(ts / (tb + tune_s), 3),
            "batched_partial_profile": {"wall_ms": round(reps_prof[0].ms_total, 2),
                                        "graph_replays": int(reps_prof[0].n_spmv_timed),
                                        "replays_gpu_ms": round(reps_prof[0].ms_spmv, 2),
