@@ -75,11 +75,14 @@ def test_c5_square6x6_full_size(ctx):
     print(f"sampled {len(rows)} rows, {nnz_sample} entries bit-exact", flush=True)
     del reps
 
-    say("C5 symmetry <x,Hy> = <Hx,y>")
+    say("C5 symmetry <x,Hy> = <Hx,y>; Hx on the sampled rows vs the oracle")
     x, y = rng.normal(size=n), rng.normal(size=n)
-    lhs, rhs = x @ e.spmv(y), e.spmv(x) @ y
+    hx = e.spmv(x)  # the streamed SpMV kernel (the matrix does not fit the MALL)
+    lhs, rhs = x @ e.spmv(y), hx @ y
     assert abs(lhs - rhs) < 1e-9 * abs(lhs)
-    del x, y
+    # the same products in the same slot order: bit for bit
+    assert np.array_equal(hx[rows], oracle.spmv_rows(W, ind_or, val_or, tr_or, x))
+    del x, y, hx
 
     say("C5 Lanczos, partial reorthogonalisation")
     ev, rep, _, _ = q.lanczos(e, max_iter=100, tol=1e-13, reorth=2, check_every=1)
