@@ -90,6 +90,21 @@ def test_c5_square6x6_full_size(ctx):
     print(f"E0 = {ev[0]:.12f}, E0/N = {ev[0] / 36:.10f}, {rep.iters} steps", flush=True)
     assert abs(ev[0] / 36 - E0_PER_SITE_6X6) < 5e-7
 
+    say("C5 ground-state observables")
+    ev_x, rep_x, _, xg = q.lanczos(e, max_iter=100, tol=1e-13, reorth=2, check_every=1, want_eigvec=True)
+    assert ev_x[0] == ev[0] and rep_x.converged
+    nx = float(xg @ xg)
+    eh, _ = b.expectation(t, xg)
+    assert abs(eh.real / nx - ev[0]) < 1e-9 * abs(ev[0])
+    # the singlet ground state is SU(2) and point-group invariant: every bond
+    # carries E0/72 of S_i.S_j and a third of it in Sz_i Sz_j (to the accuracy
+    # of the vector, ~sqrt(tol) for an operator that does not commute with H)
+    for j in (1, 6):  # the horizontal and the vertical neighbour of site 0
+        c, _ = b.expectation(q.Terms(36, LM.operator_table(LM.szsz(0, j))), xg)
+        print(f"<Sz_0 Sz_{j}> = {c.real / nx:.12f} (E0/216 = {ev[0] / 216:.12f})", flush=True)
+        assert abs(c.real / nx - ev[0] / 216) < 1e-6
+    del xg
+
     say("C5 learned start")
     net = q.Mlp(ctx, seed=20261202)
     keep = []
