@@ -192,11 +192,29 @@ def test_c4_learned_start_configured_sizes(ctx):
     b = q.Basis(ctx, g, 15, 0, q.LAYOUT_COMPACT)
     assert b.n_rows == 5170604
     rng = np.random.default_rng(20261202)
-    for _ in range(2):
+    for case_i in range(2):
         d, j2 = float(rng.uniform(0.5, 1.5)), float(rng.uniform(0.0, 0.6))
         say(f"C4 N=30 held-out Delta={d:.4f} J2={j2:.4f}")
         lat_, info = LM.chain(30, 1.0, d, j2)
-        e = q.Ell(ctx, b, q.Terms(30, LM.term_table(lat_, info)))
+        table = LM.term_table(lat_, info)
+        e = q.Ell(ctx, b, q.Terms(30, table))
+        if case_i == 0:
+            # sampled rows of the N=30 matrix (ring orbit minima) and the streamed
+            # SpMV's products on them, bit for bit against the oracle
+            say("C4 N=30 sampled rows and SpMV vs oracle")
+            reps, _, _ = b.export()
+            W = int(e.info().max_terms)
+            starts = np.concatenate([[0, b.n_rows - 64], np.sort(rng.integers(0, b.n_rows - 64, 60))])
+            rows = np.concatenate([np.arange(s0, s0 + 64) for s0 in starts])
+            ind_or, val_or, tr_or = oracle.ell_rows_from_reps(30, ge, None, ch, table, 0, reps, rows, W, False)
+            for i, s0 in enumerate(starts):
+                ind, val, tr = e.export_rows(int(s0), 64)
+                sl = slice(i * 64 * W, (i + 1) * 64 * W)
+                assert np.array_equal(tr, tr_or[i * 64:(i + 1) * 64]) and np.array_equal(ind, ind_or[sl])
+                assert np.array_equal(val, val_or[sl])
+            x = rng.normal(size=b.n_rows)
+            assert np.array_equal(e.spmv(x)[rows], oracle.spmv_rows(W, ind_or, val_or, tr_or, x))
+            del reps, x
         ev_r, rep_r, _, _ = q.lanczos(e, max_iter=1500, tol=1e-13)
         ev_l, rep_l, fb, _ = net.lanczos(b, e, d, j2, max_iter=1500, tol=1e-13)
         print(f"iterations random {rep_r.iters}, learned {rep_l.iters}", flush=True)
