@@ -226,7 +226,9 @@ def test_c4_learned_start_configured_sizes(ctx):
 def test_c3_full_size_sectors_vs_oracle(ctx):
     """Batched uneven sectors at full size (N=24 chain): the batched solve of all
     72 (k, n_up) sectors against the oracle's own converged Lanczos on three of
-    them (real k=0 and k=12, complex k=5), lowest eigenvalue to 1e-10."""
+    them (real k=0 and k=12, complex k=5), lowest eigenvalue to 1e-10; on the
+    same three the whole reference-layout ELL (columns, values, row lengths) and
+    the SpMV bit for bit."""
     lat, info = LM.chain(24, 1.0, 1.0, 0.0)
     ge, fl, ch = LM.translation_group(lat)
     g = q.Group(ge, ch)
@@ -247,6 +249,16 @@ def test_c3_full_size_sectors_vs_oracle(ctx):
         cplx = not np.all(case["chars"][k].imag == 0)
         W = OM.max_terms_rule(len(case["table"][3]), n_up, 24)
         ref = oracle.Ell(orb, case["table"], 0, W, cplx)
+        bref = q.Basis(ctx, g, n_up, k, q.LAYOUT_REFERENCE)
+        eref = q.Ell(ctx, bref, t, value_type=q.COMPLEX if cplx else q.REAL, max_terms=W)
+        ind, val, track = eref.export_reference()
+        assert np.array_equal(track, ref.track) and np.array_equal(ind, ref.ind)
+        assert np.array_equal(val.view(np.float64), ref.val)
+        x = np.random.default_rng(k).normal(size=orb.n_orb)
+        if cplx:
+            x = x + 1j * np.random.default_rng(k + 100).normal(size=orb.n_orb)
+        assert np.array_equal(eref.spmv(x), ref.spmv(x))
+        del eref, bref
         st, e0, it, _, _ = ref.lanczos_converge(600, 2212456, tol=1e-13, check_every=5)
         assert st == 0
         got = ev[[12, 14, 16].index(n_up) * 24 + k][0]
