@@ -108,3 +108,34 @@ def test_gpu_honeycomb_complex_hopping(ctx):
         ec = q.Ell(ctx, bc, t, value_type=q.COMPLEX, max_terms=ref.W)
         ev, rep, _, _ = q.lanczos(ec, max_iter=300, tol=1e-13)
         assert rep.converged and abs(ev[0] - e0) < 1e-10 * max(1.0, abs(e0))
+
+
+@pytest.mark.gpu
+def test_gpu_honeycomb_3x4_full_sectors(ctx):
+    """A larger two-sublattice case: honeycomb 3x4 (24 sites, C(24,12) states,
+    225,440 orbit rows per irrep), complex hopping, three irreps: the whole
+    reference-layout ELL and the SpMV bit for bit against the oracle, and E0
+    within 1e-10 of the oracle's converged Lanczos."""
+    case, lat, linfo = case_of(3, 4)
+    n_up = 12
+    orb = oracle_orbits(case, n_up)
+    g = q.Group(case["group_el"], case["chars"])
+    t = q.Terms(case["n_sites"], LM.term_table(lat, linfo))
+    for a in (0, 1, 7):
+        say(f"honeycomb 3x4 irrep={a}")
+        ref = oracle_ell(case, orb, a, is_complex=True)
+        b = q.Basis(ctx, g, n_up, a, q.LAYOUT_REFERENCE)
+        e = q.Ell(ctx, b, t, value_type=q.COMPLEX, max_terms=ref.W)
+        ind, val, track = e.export_reference()
+        assert np.array_equal(track, ref.track)
+        assert np.array_equal(ind, ref.ind)
+        assert np.array_equal(val.view(np.float64), ref.val)
+        rng = np.random.default_rng(a + 40)
+        x = rng.normal(size=orb.n_orb) + 1j * rng.normal(size=orb.n_orb)
+        assert np.array_equal(e.spmv(x), ref.spmv(x))
+        st, e0, _, _, _ = ref.lanczos_converge(600, 2212456, tol=1e-13, check_every=5)
+        assert st == 0
+        bc = q.Basis(ctx, g, n_up, a, q.LAYOUT_COMPACT)
+        ec = q.Ell(ctx, bc, t, value_type=q.COMPLEX, max_terms=ref.W)
+        ev, rep, _, _ = q.lanczos(ec, max_iter=600, tol=1e-13)
+        assert rep.converged and abs(ev[0] - e0) < 1e-10 * max(1.0, abs(e0))
