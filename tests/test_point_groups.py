@@ -203,3 +203,56 @@ def test_gpu_affine_ring_cosets(ctx):
             ec = q.Ell(ctx, bc, t, value_type=q.REAL, max_terms=ref.W)
             ev, rep, _, _ = q.lanczos(ec, max_iter=300, tol=1e-13)
             assert rep.converged and abs(ev[0] - dense[0]) < 1e-10 * max(1.0, abs(dense[0]))
+
+
+@pytest.mark.gpu
+def test_gpu_chain24_reflection_full_sectors(ctx):
+    """A full-size product group: J1-J2 chain of 24 sites, translations x spin
+    flip x site reversal (96 elements), n_up = 12, all eight irreps of the two
+    reflection-fixed momenta (about 28,000 orbit rows each). Per irrep: the
+    basis, the whole reference-layout ELL and one SpMV bit for bit against the
+    oracle, and E0 within 1e-10 of the oracle's converged Lanczos. Per
+    translation sector: its GPU E0 equals the lower of its two reflection
+    sectors' GPU E0s."""
+    case = chain_reflection_case(24, 0.4)
+    n, n_up = 24, 12
+    orb = oracle.Orbits(n, n_up, case["group_el"], case["chars"], flip=case["flip"])
+    g = q.Group(case["group_el"], case["chars"], case["flip"])
+    t = q.Terms(n, case["table"])
+    W = OM.max_terms_rule(len(case["table"][3]), n_up, n) + 4
+    e_point = {}
+    for a, (i, j) in enumerate(case["pairs"]):
+        say(f"chain24 reflection irrep={a} (translation irrep {i}, reflection {j})")
+        b = q.Basis(ctx, g, n_up, a, q.LAYOUT_REFERENCE)
+        reps, stab, nul = b.export()
+        assert np.array_equal(reps, orb.reps)
+        assert np.array_equal(stab.astype(np.int32), orb.stab)
+        assert np.array_equal(nul.astype(bool), orb.null_mask(a))
+        ref = oracle_ell({"table": case["table"], "chars": case["chars"]}, orb, a, max_terms=W)
+        vt = q.COMPLEX if ref.is_complex else q.REAL
+        e = q.Ell(ctx, b, t, value_type=vt, max_terms=ref.W)
+        ind, val, track = e.export_reference()
+        assert np.array_equal(track, ref.track)
+        assert np.array_equal(ind, ref.ind)
+        assert np.array_equal(val.view(np.float64), ref.val)
+        x = np.random.default_rng(a + 70).normal(size=orb.n_orb)
+        if ref.is_complex:
+            x = x + 1j * np.random.default_rng(a + 170).normal(size=orb.n_orb)
+        assert np.array_equal(e.spmv(x), ref.spmv(x))
+        st, e0, _, _, _ = ref.lanczos_converge(600, 2212456, tol=1e-13, check_every=5)
+        assert st == 0
+        bc = q.Basis(ctx, g, n_up, a, q.LAYOUT_COMPACT)
+        ec = q.Ell(ctx, bc, t, value_type=vt, max_terms=ref.W)
+        ev, rep, _, _ = q.lanczos(ec, max_iter=600, tol=1e-13)
+        assert rep.converged and abs(ev[0] - e0) < 1e-10 * max(1.0, abs(e0))
+        e_point.setdefault(i, []).append(ev[0])
+    geT, flT, chT = case["base"]
+    gT = q.Group(geT, chT, flT)
+    assert len(e_point) == 4
+    for i, parts in e_point.items():
+        say(f"chain24 translation irrep={i}")
+        bt = q.Basis(ctx, gT, n_up, i, q.LAYOUT_COMPACT)
+        real = bool(np.all(np.asarray(chT)[i].imag == 0))
+        et = q.Ell(ctx, bt, t, value_type=q.REAL if real else q.COMPLEX, max_terms=W)
+        ev, rep, _, _ = q.lanczos(et, max_iter=600, tol=1e-13)
+        assert rep.converged and abs(ev[0] - min(parts)) < 1e-10 * max(1.0, abs(ev[0]))
