@@ -11,7 +11,10 @@
 // Storage is SELL-64: rows in chunks of one wavefront, each chunk stored
 // slot-major and padded to its own longest row, so an SpMV wave reads every
 // slot with one coalesced load. Two passes: count -> chunk offsets -> fill.
+// A packed build's fill writes 16-bit value codes instead of FP64 values; the
+// codes in use are collected and turned into value indexes afterwards.
 #include <cmath>
+#include <cstdlib>
 #include <memory>
 
 #include "device_common.hpp"
@@ -49,9 +52,29 @@ __device__ inline bool set_insert(unsigned long long *set, int slots, unsigned l
     return false;
 }
 
+// Coded values (the build of a packed ELL). An off-diagonal value is
+// (coef * chi(gm)) * (n(m_t) / n(m_s)): a term coefficient with the sign of its
+// Sz factors, a character and two stabiliser sizes, each from a short list. The
+// fill writes, instead of the FP64 value, the 16-bit code
+//   (((coef class * 2 + sign) * n_chi + chi class) * n_m + m_s class) * n_m + m_t class
+// and the diagonal straight into the FP64 diagonal array; the distinct values
+// are formed afterwards from the codes in use, with the same arithmetic, so
+// they are bit-identical to the FP64 form's.
+constexpr int kMaxCodes = 1 << 14;
+
+struct CodeView {
+    const uint16_t *xcls = nullptr;  // [G] character class of each element
+    const uint8_t *mcls = nullptr;   // [kMaxGroup + 1] class of each stabiliser size (0xFF: none)
+    int n_chi = 0, n_m = 0;
+    uint16_t *code = nullptr;        // [stored]
+    double *diag = nullptr;          // [n_rows] (x 2 complex)
+    int *bad = nullptr;              // set when a stabiliser size has no class
+};
+
 // Rank tables in dynamic LDS: the nibble table (rank_tab, n_nib > 0) when it
-// fits, else the binomial table (binom).
-template <class S, bool FILL, bool CPLX>
+// fits, else the binomial table (binom). CODED (fill only): codes instead of
+// values (CodeView).
+template <class S, bool FILL, bool CPLX, bool CODED = false>
 __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint64_t *binom,
                                                         const uint64_t *rank_tab, int n_nib, int n_c, int layout,
                                                         int64_t n_rows, const uint64_t *reps, const uint8_t *stab,
@@ -59,12 +82,14 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                                                         TermsView tv, const double2 *chi,
                                                         const double *nrm, int W, uint16_t *row_len, int *err,
                                                         int all_rows, const int64_t *chunk_ptr, int32_t *col,
-                                                        double *val) {
+                                                        double *val, CodeView cv) {
     extern __shared__ uint64_t rank_lds[];
     __shared__ uint8_t perm_lds[kMaxGroup * 64];
     __shared__ double2 s_chi[kMaxGroup];
     __shared__ double s_nrm[kMaxGroup + 1];
     __shared__ GroupLds gt;
+    __shared__ uint16_t s_xcls[CODED ? kMaxGroup : 1];
+    __shared__ uint8_t s_mcls[CODED ? kMaxGroup + 1 : 1];
     // the count pass over a sector without null orbits needs no orbit lookups
     // (row length = 1 + the acting off-diagonal terms): no tables to stage
     const bool lookups = FILL || !all_rows;
@@ -78,6 +103,10 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
         }
         for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_chi[i] = chi[i];
         for (int i = threadIdx.x; i <= gv.G; i += blockDim.x) s_nrm[i] = nrm[i];
+        if (CODED) {
+            for (int i = threadIdx.x; i < gv.G; i += blockDim.x) s_xcls[i] = cv.xcls[i];
+            for (int i = threadIdx.x; i <= kMaxGroup; i += blockDim.x) s_mcls[i] = cv.mcls[i];
+        }
         __syncthreads();
     }
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -137,7 +166,21 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
                     n_nib ? colex_rank_nib<S>(rep, rank_lds, n_nib, n_c) : colex_rank<S>(rep, rank_lds), dict);
                 if (r2 < 0) continue;
                 if (layout == QMB_LAYOUT_REFERENCE && nullf[r2]) continue;
-                if (FILL) {
+                if (FILL && CODED) {
+                    if (slot < L) {
+                        const TermC &tm = tv.t[h];
+                        const int neg = popc_s((S)(~src & (S)tm.sz)) & 1;
+                        const int cs = s_mcls[m_src], ct = s_mcls[mult];
+                        const bool ok = cs != 0xFF && ct != 0xFF;
+                        if (!ok) *cv.bad = 1;
+                        const int code =
+                            ok ? ((((int)tm.vclass * 2 + neg) * cv.n_chi + s_xcls[gm]) * cv.n_m + cs) * cv.n_m + ct
+                               : 0;
+                        const int64_t pos = base + (int64_t)slot * kChunk + lane;
+                        col[pos] = (int32_t)r2;
+                        cv.code[pos] = (uint16_t)code;
+                    }
+                } else if (FILL) {
                     if (slot < L) {
                         // mult = the target row's stabiliser size (no read of stab[r2]);
                         // equal sizes give the ratio 1 exactly
@@ -163,6 +206,18 @@ __global__ __launch_bounds__(kEllThreads) void ell_rows(GroupView gv, const uint
     if (!FILL) {
         if (len > W) atomicMax(err, 1);
         row_len[r] = (uint16_t)(len < 65535 ? len : 65535);
+        return;
+    }
+    if (CODED) {
+        // the diagonal into its own array; slot 0 and the padding slots hold the row
+        if (CPLX) {
+            cv.diag[2 * r] = null_row ? 0.0 : dr;
+            cv.diag[2 * r + 1] = null_row ? 0.0 : di;
+        } else {
+            cv.diag[r] = null_row ? 0.0 : dr;
+        }
+        if (L > 0) col[base + lane] = (int32_t)r;
+        for (int s = (len > 1 ? len : 1); s < L; ++s) col[base + (int64_t)s * kChunk + lane] = (int32_t)r;
         return;
     }
     // diagonal slot (also written for empty rows, like the reference's ind[id*W] = id)
@@ -200,6 +255,7 @@ __global__ void ell_pad_tail(int64_t n_rows, int64_t n_chunks, const int64_t *ch
     for (int s = 0; s < L; ++s) {
         const int64_t pos = base + (int64_t)s * kChunk + lane;
         col[pos] = 0;
+        if (!val) continue;  // coded build: no FP64 values
         if (cplx) {
             val[2 * pos] = 0.0;
             val[2 * pos + 1] = 0.0;
@@ -401,6 +457,158 @@ __global__ __launch_bounds__(kEllThreads) void ell_pack_values_cplx(int64_t n_ro
     }
 }
 
+// ---- coded values -------------------------------------------------------------
+// Stabiliser sizes present among the rows: bit m of mask[0..4] (m <= kMaxGroup).
+__global__ __launch_bounds__(kEllThreads) void ell_stab_sizes(int64_t n_rows, const uint8_t *stab, unsigned *mask) {
+    __shared__ unsigned s[5];
+    if (threadIdx.x < 5) s[threadIdx.x] = 0u;
+    __syncthreads();
+    uint64_t lo = 0, mid = 0;
+    unsigned hi = 0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int m = stab[r];
+        if (m < 64)
+            lo |= 1ull << m;
+        else if (m < 128)
+            mid |= 1ull << (m - 64);
+        else
+            hi = 1u;
+    }
+    const unsigned w[5] = {(unsigned)lo, (unsigned)(lo >> 32), (unsigned)mid, (unsigned)(mid >> 32), hi};
+    for (int i = 0; i < 5; ++i)
+        if (w[i]) atomicOr(&s[i], w[i]);
+    __syncthreads();
+    if (threadIdx.x < 5 && s[threadIdx.x]) atomicOr(mask + threadIdx.x, s[threadIdx.x]);
+}
+
+// Codes in use (slots 1 .. len-1 of every row): bit c of used, through a
+// per-block bitmap in LDS.
+__global__ __launch_bounds__(kEllThreads) void ell_collect_codes(int64_t n_rows, const int64_t *chunk_ptr,
+                                                                 const uint16_t *row_len, const uint16_t *code,
+                                                                 int n_codes, unsigned *used) {
+    __shared__ unsigned bits[kMaxCodes / 32];
+    for (int i = threadIdx.x; i < kMaxCodes / 32; i += blockDim.x) bits[i] = 0u;
+    __syncthreads();
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t base = chunk_ptr[r / kChunk] + r % kChunk;
+        const int len = row_len[r];
+        int last = -1;
+        // eight slots' codes loaded at once, then marked (6x6 sector: 2.0 ms, 5.4 ms
+        // one slot at a time)
+        for (int k = 1; k < len; k += 8) {
+            int c[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c[u] = k + u < len ? (int)code[base + (int64_t)(k + u) * kChunk] : -1;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (c[u] < 0 || c[u] == last || c[u] >= n_codes) continue;
+                last = c[u];
+                const unsigned b = 1u << (c[u] & 31);
+                if (!(bits[c[u] >> 5] & b)) atomicOr(&bits[c[u] >> 5], b);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMaxCodes / 32; i += blockDim.x)
+        if (bits[i]) atomicOr(used + i, bits[i]);
+}
+
+// The factors a code is made of (host tables, one device blob).
+struct CodeTables {
+    const double2 *coef = nullptr;  // [n_tc] term coefficient classes
+    const double2 *chi = nullptr;   // [n_chi] character classes
+    const int *m = nullptr;         // [n_m] stabiliser size of each class
+    const double *nrm = nullptr;    // [G + 1] orbit norms
+    int n_chi = 0, n_m = 0;
+};
+
+// One block: the value of every code in use, formed as the fill forms it
+// (same operands, same operations, no contraction), into cval and the sets of
+// distinct values (real parts in vset, imaginary parts in vset + kSetSlots).
+__global__ __launch_bounds__(1024) void ell_code_values(const unsigned *used, int n_codes, CodeTables ct, int cplx,
+                                                        double2 *cval, unsigned long long *vset, int *vflag) {
+    for (int c = threadIdx.x; c < n_codes; c += blockDim.x) {
+        if (!((used[c >> 5] >> (c & 31)) & 1u)) continue;
+        int t = c;
+        const int mt = t % ct.n_m;
+        t /= ct.n_m;
+        const int ms = t % ct.n_m;
+        t /= ct.n_m;
+        const int xc = t % ct.n_chi;
+        t /= ct.n_chi;
+        const double2 co = ct.coef[t >> 1];
+        const bool neg = (t & 1) != 0;
+        const double cr = neg ? -co.x : co.x;
+        const double ci = neg ? -co.y : co.y;
+        const int m_src = ct.m[ms], mult = ct.m[mt];
+        const double n_src = ct.nrm[m_src];
+        const double ratio = mult == m_src ? 1.0 : ct.nrm[mult] / n_src;
+        const double2 x = ct.chi[xc];
+        const double pr = cr * x.x - ci * x.y;
+        const double pi = cr * x.y + ci * x.x;
+        const double vr = pr * ratio, vi = pi * ratio;
+        cval[c] = make_double2(vr, cplx ? vi : 0.0);
+        if (!set_insert(vset, kSetSlots, (unsigned long long)__double_as_longlong(vr))) vflag[0] = 1;
+        if (cplx && !set_insert(vset + kSetSlots, kSetSlots, (unsigned long long)__double_as_longlong(vi)))
+            vflag[0] = 1;
+    }
+}
+
+// One block, when the tables are complete (f: the packing flags): map[c] = the
+// value index of code c (complex: re index | im index << 16).
+__global__ __launch_bounds__(1024) void ell_code_map(const unsigned *used, int n_codes, const double2 *cval,
+                                                     const unsigned long long *keys, int *f, int cplx,
+                                                     uint32_t *map) {
+    if (!f[2] || (cplx && !f[5])) return;
+    for (int c = threadIdx.x; c < n_codes; c += blockDim.x) {
+        if (!((used[c >> 5] >> (c & 31)) & 1u)) continue;
+        const double2 v = cval[c];
+        uint32_t m = key_index(keys, f[1], (unsigned long long)__double_as_longlong(v.x), f + 3);
+        if (cplx) m |= key_index(keys + kMaxValues, f[4], (unsigned long long)__double_as_longlong(v.y), f + 3) << 16;
+        map[c] = m;
+    }
+}
+
+// In place, when the tables are complete and the indexes fit: every
+// off-diagonal slot k < len gets its value index(es) beside the column (slot 0
+// and the padding slots keep index 0, the value +0.0). Four slots at a time,
+// each slot's map read issued with its word load (6x6 sector: 12.2 ms; eight
+// slots, or the map staged in LDS and read after the loads, 14.4-16.8 ms).
+__global__ __launch_bounds__(kEllThreads) void ell_pack_codes(int64_t n_rows, const int64_t *chunk_ptr,
+                                                              const uint16_t *row_len, uint32_t *colw,
+                                                              const uint16_t *code, const uint32_t *map,
+                                                              const int *f, int cplx, int cb) {
+    if (!f[2] || (cplx && !f[5])) return;
+    int sh2 = 0;
+    if (cplx) {
+        sh2 = cb + index_bits(f[1]);
+        if (sh2 + index_bits(f[4]) > 32) return;
+    }
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t base = chunk_ptr[r / kChunk] + r % kChunk;
+        const int len = row_len[r];
+        for (int k = 1; k < len; k += 4) {
+            uint32_t m[4], w[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t p = base + (int64_t)(k + u) * kChunk;
+                m[u] = k + u < len ? map[code[p]] : 0u;
+                w[u] = k + u < len ? colw[p] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k + u >= len) continue;
+                const uint64_t ix = cplx ? (((uint64_t)(m[u] & 0xFFFFu) << cb) | ((uint64_t)(m[u] >> 16) << sh2))
+                                         : (uint64_t)m[u] << cb;
+                colw[base + (int64_t)(k + u) * kChunk] = w[u] | (uint32_t)ix;
+            }
+        }
+    }
+}
+
 // Packing state of one build: the distinct values collected after the fill,
 // then sort + pack on the device; the host reads the flags once at the end.
 // flags: {overflow, n, ok, missing} (real) or {overflow, n_re, ok_re, missing,
@@ -409,6 +617,12 @@ struct ValuePacking {
     bool on = false, cplx = false;
     int cb = 0, cap = 0;
     DevBuf vset, flags;
+    // coded build (plan_codes, before the fill): flags[6] = a stabiliser size without a class
+    bool coded = false;
+    int n_codes = 0;
+    DevBuf codes, used, cval, map, ctab;
+    CodeView cv;
+    CodeTables ct;
     bool prepare(qmb_ctx *ctx, const qmb_ell &e) {
         const int64_t n = e.n_rows;
         if (n < 2) return false;
@@ -424,6 +638,116 @@ struct ValuePacking {
         QMB_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, ctx->stream));
         on = true;
         return true;
+    }
+    // Whether the fill writes codes: the coefficient classes of the terms, the
+    // character classes of the irrep row and the stabiliser sizes present (one
+    // small kernel and a read-back) must give at most kMaxCodes codes. Sets up
+    // the code tables, the code array and the diagonal.
+    // QMB_VALUE_CODES=0 in the environment selects the FP64 fill and the value
+    // collection pass instead (the route of tables with too many codes), so
+    // tests can compare the two routes.
+    bool plan_codes(qmb_ctx *ctx, qmb_ell &e, const qmb_basis &b, const DeviceTerms &dt, const double *ch, int G) {
+        const char *env = std::getenv("QMB_VALUE_CODES");
+        if (env && env[0] == '0') return false;
+        if (!on || dt.classes.size() > DeviceTerms::kMaxClasses || G > kMaxGroup) return false;
+        DevBuf dmask(5 * sizeof(unsigned));
+        QMB_HIP(hipMemsetAsync(dmask.p, 0, dmask.bytes, ctx->stream));
+        const unsigned blocks = (unsigned)std::min<int64_t>(1024, (e.n_rows + kEllThreads - 1) / kEllThreads);
+        hipLaunchKernelGGL(ell_stab_sizes, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, e.n_rows,
+                           b.stab.as<uint8_t>(), dmask.as<unsigned>());
+        QMB_HIP(hipGetLastError());
+        unsigned *hm = ctx->host_stage<unsigned>(5 * sizeof(unsigned));
+        QMB_HIP(hipMemcpyAsync(hm, dmask.p, 5 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+        QMB_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<int> m_of;
+        std::vector<uint8_t> mcls(kMaxGroup + 1, 0xFF);
+        for (int m = 0; m <= kMaxGroup; ++m)
+            if ((hm[m >> 5] >> (m & 31)) & 1u) {
+                if (m < 1 || m > G) return false;  // not a stabiliser size of this group
+                mcls[m] = (uint8_t)m_of.size();
+                m_of.push_back(m);
+            }
+        std::vector<double2> xc;
+        std::vector<uint16_t> xcls(G);
+        for (int g = 0; g < G; ++g) {
+            const double2 v = make_double2(ch[2 * g], ch[2 * g + 1]);
+            size_t k = 0;
+            while (k < xc.size() && std::memcmp(&xc[k], &v, sizeof(v)) != 0) ++k;
+            if (k == xc.size()) xc.push_back(v);
+            xcls[g] = (uint16_t)k;
+        }
+        std::vector<double2> tc = dt.classes;
+        if (tc.empty()) tc.push_back(make_double2(0.0, 0.0));
+        const int64_t nc = (int64_t)tc.size() * 2 * (int64_t)xc.size() * (int64_t)m_of.size() * (int64_t)m_of.size();
+        if (m_of.empty() || nc > kMaxCodes) return false;
+        n_codes = (int)nc;
+        // one blob: xcls, mcls, coefficient classes, character classes, sizes
+        auto up16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+        const size_t o_m = up16((size_t)G * 2), o_tc = up16(o_m + kMaxGroup + 1), o_xc = up16(o_tc + tc.size() * 16),
+                     o_mo = up16(o_xc + xc.size() * 16), total = up16(o_mo + m_of.size() * sizeof(int));
+        std::vector<unsigned char> h(total, 0);
+        std::memcpy(h.data(), xcls.data(), (size_t)G * 2);
+        std::memcpy(h.data() + o_m, mcls.data(), mcls.size());
+        std::memcpy(h.data() + o_tc, tc.data(), tc.size() * 16);
+        std::memcpy(h.data() + o_xc, xc.data(), xc.size() * 16);
+        std::memcpy(h.data() + o_mo, m_of.data(), m_of.size() * sizeof(int));
+        ctab.alloc(total);
+        QMB_HIP(hipMemcpy(ctab.p, h.data(), total, hipMemcpyHostToDevice));
+        const char *d = static_cast<const char *>(ctab.p);
+        codes.alloc((size_t)std::max<int64_t>(e.stored, 1) * sizeof(uint16_t));
+        used.alloc(kMaxCodes / 8);
+        QMB_HIP(hipMemsetAsync(used.p, 0, used.bytes, ctx->stream));
+        cval.alloc((size_t)n_codes * sizeof(double2));
+        map.alloc((size_t)n_codes * sizeof(uint32_t));
+        e.diag.alloc((size_t)e.n_rows * (cplx ? 2 : 1) * sizeof(double));
+        cv.xcls = reinterpret_cast<const uint16_t *>(d);
+        cv.mcls = reinterpret_cast<const uint8_t *>(d + o_m);
+        cv.n_chi = (int)xc.size();
+        cv.n_m = (int)m_of.size();
+        cv.code = codes.as<uint16_t>();
+        cv.diag = e.diag.as<double>();
+        cv.bad = flags.as<int>() + 6;
+        ct.coef = reinterpret_cast<const double2 *>(d + o_tc);
+        ct.chi = reinterpret_cast<const double2 *>(d + o_xc);
+        ct.m = reinterpret_cast<const int *>(d + o_mo);
+        ct.nrm = dt.nrm;
+        ct.n_chi = cv.n_chi;
+        ct.n_m = cv.n_m;
+        coded = true;
+        return true;
+    }
+    // after a coded fill: codes in use -> their values -> sorted tables -> code
+    // to index map -> packed words
+    void launch_coded(qmb_ctx *ctx, qmb_ell &e) {
+        const int parts = cplx ? 2 : 1;
+        e.vtab.alloc((size_t)parts * kMaxValues * 8);
+        int *f = flags.as<int>();
+        unsigned long long *keys = e.vtab.as<unsigned long long>();
+        const unsigned blocks = (unsigned)std::min<int64_t>(4096, (e.n_rows + kEllThreads - 1) / kEllThreads);
+        hipLaunchKernelGGL(ell_collect_codes, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, e.n_rows,
+                           e.chunk_ptr.as<int64_t>(), e.row_len.as<uint16_t>(), codes.as<uint16_t>(), n_codes,
+                           used.as<unsigned>());
+        QMB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ell_code_values, dim3(1), dim3(1024), 0, ctx->stream, used.as<unsigned>(), n_codes, ct,
+                           cplx ? 1 : 0, cval.as<double2>(), vset.as<unsigned long long>(), f);
+        QMB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ell_sort_values, dim3(1), dim3(1024), 0, ctx->stream, vset.as<unsigned long long>(), f,
+                           keys, f + 1, cap);
+        QMB_HIP(hipGetLastError());
+        if (cplx) {
+            hipLaunchKernelGGL(ell_sort_values, dim3(1), dim3(1024), 0, ctx->stream,
+                               vset.as<unsigned long long>() + kSetSlots, f, keys + kMaxValues, f + 4, cap);
+            QMB_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(ell_code_map, dim3(1), dim3(1024), 0, ctx->stream, used.as<unsigned>(), n_codes,
+                           cval.as<double2>(), keys, f, cplx ? 1 : 0, map.as<uint32_t>());
+        QMB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ell_pack_codes, dim3(blocks), dim3(kEllThreads), 0, ctx->stream, e.n_rows,
+                           e.chunk_ptr.as<int64_t>(), e.row_len.as<uint16_t>(), e.col.as<uint32_t>(),
+                           codes.as<uint16_t>(), map.as<uint32_t>(), f, cplx ? 1 : 0, cb);
+        QMB_HIP(hipGetLastError());
+        QMB_HIP(hipMemcpyAsync(ctx->host_stage<int>(8 * sizeof(int)), flags.p, 8 * sizeof(int),
+                               hipMemcpyDeviceToHost, ctx->stream));
     }
     void launch(qmb_ctx *ctx, qmb_ell &e) {
         const int parts = cplx ? 2 : 1;
@@ -460,11 +784,15 @@ struct ValuePacking {
     }
     // after the stream has drained (the flags were copied into the context's
     // staging buffer at the end of launch)
-    void finish(qmb_ctx *ctx, qmb_ell &e) {
+    // Returns whether the matrix is packed; a coded build that is not must be
+    // filled again with FP64 values.
+    bool finish(qmb_ctx *ctx, qmb_ell &e) {
         const int *h = ctx->host_stage<int>(8 * sizeof(int));
         bool ok = h[2] != 0;
         if (cplx) ok = ok && h[5] != 0 && cb + index_bits(h[1]) + index_bits(h[4]) <= 32;
         QMB_REQUIRE(!ok || h[3] == 0, "qmb_ell_build: internal error while packing the values");
+        QMB_REQUIRE(!coded || h[6] == 0, "qmb_ell_build: internal error (a stabiliser size without a value code)");
+        codes.release();
         if (ok) {
             e.val.release();
             e.packed = 1;
@@ -478,6 +806,7 @@ struct ValuePacking {
             e.diag.release();
             e.vtab.release();
         }
+        return ok;
     }
 };
 
@@ -585,7 +914,7 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->dict.as<uint64_t>(),
                                tv, dt.chi, dt.nrm, W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), b->all_rows ? 1 : 0, nullptr, nullptr,
-                               nullptr);
+                               nullptr, CodeView{});
             QMB_HIP(hipGetLastError());
         }
         // the offsets are scanned before the overflow flag is read (one synchronisation)
@@ -596,31 +925,49 @@ qmb_status qmb_ell_build(qmb_ctx *ctx, const qmb_basis *b, const qmb_terms *t, i
         QMB_REQUIRE(herr == 0, "qmb_ell_build: a row needs more than max_terms = " + std::to_string(W) + " slots");
         e->max_row_len = max_len;
         e->col.alloc((size_t)std::max<int64_t>(e->stored, 1) * sizeof(int32_t));
-        e->val.alloc((size_t)std::max<int64_t>(e->stored, 1) * (cplx ? 16 : 8));
         ValuePacking vp;
         if (n > 0 && !keep_fp64) vp.prepare(ctx, *e);
-        if (n > 0) {
+        // a packed build writes value codes (2 B per entry) instead of FP64 values
+        // when the factors of the values give few enough codes
+        const bool coded = vp.on && vp.plan_codes(ctx, *e, *b, dt, ch, g->G);
+        // the fill (coded, or with FP64 values) and the padding of the last chunk
+        auto fill = [&](bool with_codes) {
             const bool narrow = b->n_sites <= 32;
-            auto kfill = narrow ? (cplx ? ell_rows<uint32_t, true, true> : ell_rows<uint32_t, true, false>)
-                                : (cplx ? ell_rows<uint64_t, true, true> : ell_rows<uint64_t, true, false>);
+            auto kfill = with_codes
+                             ? (narrow ? (cplx ? ell_rows<uint32_t, true, true, true> : ell_rows<uint32_t, true, false, true>)
+                                       : (cplx ? ell_rows<uint64_t, true, true, true> : ell_rows<uint64_t, true, false, true>))
+                             : (narrow ? (cplx ? ell_rows<uint32_t, true, true> : ell_rows<uint32_t, true, false>)
+                                       : (cplx ? ell_rows<uint64_t, true, true> : ell_rows<uint64_t, true, false>));
+            double *val = with_codes ? nullptr : e->val.as<double>();
             hipLaunchKernelGGL(kfill, dim3(blocks), dim3(kEllThreads), rank.lds_bytes, ctx->stream, gv,
                                ctx->binom.as<uint64_t>(), rank.tab(), rank.n_nib, rank.n_c, b->layout, n,
                                b->reps.as<uint64_t>(),
                                b->stab.as<uint8_t>(), b->nullf.as<uint8_t>(), b->dict.as<uint64_t>(),
                                tv, dt.chi, dt.nrm, W,
                                e->row_len.as<uint16_t>(), d_err.as<int>(), 0, e->chunk_ptr.as<int64_t>(),
-                               e->col.as<int32_t>(), e->val.as<double>());
+                               e->col.as<int32_t>(), val, with_codes ? vp.cv : CodeView{});
             QMB_HIP(hipGetLastError());
             if (n % kChunk) {
                 hipLaunchKernelGGL(ell_pad_tail, dim3(1), dim3(kChunk), 0, ctx->stream, n, e->n_chunks,
-                                   e->chunk_ptr.as<int64_t>(), e->col.as<int32_t>(), e->val.as<double>(),
-                                   cplx ? 1 : 0);
+                                   e->chunk_ptr.as<int64_t>(), e->col.as<int32_t>(), val, cplx ? 1 : 0);
                 QMB_HIP(hipGetLastError());
             }
-            if (vp.on) vp.launch(ctx, *e);
+        };
+        if (!coded) e->val.alloc((size_t)std::max<int64_t>(e->stored, 1) * (cplx ? 16 : 8));
+        if (n > 0) {
+            fill(coded);
+            if (coded)
+                vp.launch_coded(ctx, *e);
+            else if (vp.on)
+                vp.launch(ctx, *e);
         }
         QMB_HIP(hipStreamSynchronize(ctx->stream));
-        if (vp.on) vp.finish(ctx, *e);
+        if (vp.on && !vp.finish(ctx, *e) && coded) {
+            // more distinct values than the index bits hold: the FP64 form
+            e->val.alloc((size_t)std::max<int64_t>(e->stored, 1) * (cplx ? 16 : 8));
+            fill(false);
+            QMB_HIP(hipStreamSynchronize(ctx->stream));
+        }
         *out = e.release();
     });
 }

@@ -20,11 +20,12 @@ namespace qmb {
 // still carries its original bit and has it 0 (the parity of ~x & sz). The
 // factors are 1, +-0.5, so the products are exact in any order: the values
 // equal the sequential ones bit for bit. A term that can never act has
-// req = 1, req_val = 2.
+// req = 1, req_val = 2. vclass: the term's coefficient class (terms whose coef
+// has the same bit pattern share it), used to code ELL values.
 struct TermC {
     uint64_t req, req_val, set, set_val, sz;
     double2 coef;
-    uint64_t pad;
+    uint64_t vclass;
 };
 
 struct TermsView {
@@ -133,6 +134,8 @@ struct DeviceTerms {
     TermsView view{};
     const double2 *chi = nullptr;  // [G]
     const double *nrm = nullptr;   // [G + 1]
+    static constexpr size_t kMaxClasses = 1 << 14;
+    std::vector<double2> classes;  // distinct term coefficients, in order of first use (TermC::vclass)
     DeviceTerms(const qmb_terms &t, const double *chi_row, int G, const std::vector<int> *order = nullptr) {
         const int nt = t.n_terms;
         auto up16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
@@ -141,6 +144,18 @@ struct DeviceTerms {
         std::vector<unsigned char> h(total, 0);
         TermC *tc = reinterpret_cast<TermC *>(h.data() + o_terms);
         for (int k = 0; k < nt; ++k) tc[k] = compile_term(t, order ? (*order)[k] : k);
+        // classes of the terms that can leave the diagonal (the others never give
+        // an off-diagonal entry: no operator changes a spin, or none acts); beyond
+        // kMaxClasses the classes are not used and no longer counted
+        for (int k = 0; k < nt; ++k) {
+            tc[k].vclass = 0;
+            if (tc[k].set == 0 || tc[k].req_val > tc[k].req || classes.size() > kMaxClasses) continue;
+            for (const double2 &c : classes) {
+                if (std::memcmp(&c, &tc[k].coef, sizeof(double2)) == 0) break;
+                ++tc[k].vclass;
+            }
+            if (tc[k].vclass == classes.size()) classes.push_back(tc[k].coef);
+        }
         if (chi_row) std::memcpy(h.data() + o_chi, chi_row, (size_t)G * 16);
         const std::vector<double> nt_tab = orbit_norm_table(G);
         std::memcpy(h.data() + o_nrm, nt_tab.data(), nt_tab.size() * 8);

@@ -6,6 +6,8 @@ because both sides use the same FP64 operation sequence (no FMA contraction).
 Eigenvalues must agree within 1e-10 relative with the oracle's converged
 Lanczos and with dense diagonalisation where it fits.
 """
+import contextlib
+import os
 import sys
 
 import numpy as np
@@ -18,6 +20,8 @@ from tests.helpers import (DeviceBuffer, chain_case, ell_dense_nonnull, oracle_e
                            square_case)
 
 pytestmark = pytest.mark.gpu
+
+SZ, SP, SM = OM.OP_SZ, OM.OP_SP, OM.OP_SM
 
 
 def say(msg):
@@ -96,39 +100,110 @@ def test_ell_reference_layout_bit_exact(ctx, label, mk, n_up, alphas):
         assert np.array_equal(e.spmv(x), ref.spmv(x))
 
 
+@contextlib.contextmanager
+def value_route(route):
+    """'codes': the fill writes value codes (the default); 'collect': the FP64
+    fill and the value collection pass (QMB_VALUE_CODES=0)."""
+    old = os.environ.get("QMB_VALUE_CODES")
+    if route == "collect":
+        os.environ["QMB_VALUE_CODES"] = "0"
+    else:
+        os.environ.pop("QMB_VALUE_CODES", None)
+    try:
+        yield
+    finally:
+        if old is None:
+            os.environ.pop("QMB_VALUE_CODES", None)
+        else:
+            os.environ["QMB_VALUE_CODES"] = old
+
+
 @pytest.mark.parametrize("label,mk,n_up,alphas", CASES)
 def test_packed_values_equal_fp64_values(ctx, label, mk, n_up, alphas):
     """A real or complex ELL is stored packed (column | value index words, FP64
     diagonal): its export, SpMV results and eigenvalue are bit-identical to the
-    FP64 form."""
+    FP64 form, by either packing route (value codes written by the fill, or FP64
+    values collected after it), and both routes give the same value tables."""
     case = mk()
-    orb = oracle_orbits(case, n_up)
     for a in alphas:
         real = bool(np.all(np.asarray(case["chars"])[a].imag == 0))
         for layout in (q.LAYOUT_REFERENCE, q.LAYOUT_COMPACT):
-            say(f"packed {label} irrep={a} layout={layout}")
             g = q.Group(case["group_el"], case["chars"], case["flip"])
             t = q.Terms(case["n_sites"], case["table"])
             b = q.Basis(ctx, g, n_up, a, layout)
-            ep = q.Ell(ctx, b, t)
             ef = q.Ell(ctx, b, t, keep_fp64_values=True)
-            ip, jf = ep.info(), ef.info()
+            jf = ef.info()
             assert jf.packed == 0
-            assert ip.packed == 1
-            assert 1 <= ip.n_values <= (1024 if real else 2048)
-            assert ip.device_bytes < jf.device_bytes
-            assert ip.format_bytes < jf.format_bytes == jf.spmv_bytes
-            assert ip.spmv_bytes == jf.spmv_bytes and ip.nnz == jf.nnz and ip.stored == jf.stored
-            for u, v in zip(ep.export_reference(), ef.export_reference()):
-                assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
             rng = np.random.default_rng(a + 7)
             x = rng.normal(size=b.n_rows)
             if not real:
                 x = x + 1j * rng.normal(size=b.n_rows)
-            assert np.array_equal(ep.spmv(x), ef.spmv(x))
-            e1, r1, _, _ = q.lanczos(ep, max_iter=300, tol=1e-13)
             e2, r2, _, _ = q.lanczos(ef, max_iter=300, tol=1e-13)
-            assert e1[0] == e2[0] and r1.iters == r2.iters
+            tables = []
+            for route in ("codes", "collect"):
+                say(f"packed {label} irrep={a} layout={layout} route={route}")
+                with value_route(route):
+                    ep = q.Ell(ctx, b, t)
+                ip = ep.info()
+                assert ip.packed == 1
+                assert 1 <= ip.n_values <= (1024 if real else 2048)
+                assert ip.device_bytes < jf.device_bytes
+                assert ip.format_bytes < jf.format_bytes == jf.spmv_bytes
+                assert ip.spmv_bytes == jf.spmv_bytes and ip.nnz == jf.nnz and ip.stored == jf.stored
+                for u, v in zip(ep.export_reference(), ef.export_reference()):
+                    assert np.array_equal(np.asarray(u).view(np.uint8), np.asarray(v).view(np.uint8))
+                assert np.array_equal(ep.spmv(x), ef.spmv(x))
+                e1, r1, _, _ = q.lanczos(ep, max_iter=300, tol=1e-13)
+                assert e1[0] == e2[0] and r1.iters == r2.iters
+                tables.append((ip.n_values, ip.format_bytes, ip.device_bytes))
+            assert tables[0] == tables[1]
+
+
+def three_body_case(n=12, seed=5):
+    """No symmetry (one group element), terms S+_i S-_j Sz_k with a distinct
+    random coefficient each: thousands of distinct matrix values, more than a
+    packed value table holds."""
+    rng = np.random.default_rng(seed)
+    ops, sites, scal = [], [], []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                if len({i, j, k}) < 3:
+                    continue
+                ops += [SP, SM, SZ]
+                sites += [i, j, k]
+                scal.append(rng.uniform(0.5, 1.5))
+    table = (np.array(ops, np.int32), np.array(sites, np.int32), np.array(scal, np.complex128),
+             np.full(len(scal), 3, np.int32), 3)
+    return dict(n_sites=n, table=table, group_el=np.arange(n, dtype=np.uint32)[None, :],
+                chars=np.ones((1, 1), np.complex128), flip=None)
+
+
+def test_too_many_values_stay_fp64(ctx):
+    """A matrix with more distinct values than a value table holds stays in the
+    FP64 form by either packing route (the coded fill is redone with FP64
+    values), bit-exact against the oracle's ELL."""
+    case = three_body_case()
+    n_up = 6
+    orb = oracle_orbits(case, n_up)
+    W = 400
+    ref = oracle_ell(case, orb, 0, max_terms=W, is_complex=False)
+    assert len(np.unique(ref.val[ref.val != 0.0])) > 1024
+    g = q.Group(case["group_el"], case["chars"], None)
+    t = q.Terms(case["n_sites"], case["table"])
+    b = q.Basis(ctx, g, n_up, 0, q.LAYOUT_REFERENCE)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=orb.n_orb)
+    for route in ("codes", "collect"):
+        say(f"three-body n=12 route={route}")
+        with value_route(route):
+            e = q.Ell(ctx, b, t, value_type=q.REAL, max_terms=W)
+        assert e.info().packed == 0
+        ind, val, track = e.export_reference()
+        assert np.array_equal(track, ref.track)
+        assert np.array_equal(ind, ref.ind)
+        assert np.array_equal(val.view(np.float64), ref.val)
+        assert np.array_equal(e.spmv(x), ref.spmv(x))
 
 
 def chain12_with_constant():
