@@ -164,7 +164,12 @@ typedef struct {
  * beside the column: col | (re index << col_bits) | (im index << (col_bits +
  * re bits)). Values and products are unchanged (SpMV results are bit-identical
  * to the unpacked form); the matrix stream is 4 instead of 12 (20 complex)
- * bytes per entry. value_type | QMB_KEEP_FP64_VALUES keeps the unpacked form. */
+ * bytes per entry. value_type | QMB_KEEP_FP64_VALUES keeps the unpacked form.
+ * While a packed matrix is built it takes 6 bytes of device memory per stored
+ * entry (column word + 16-bit value code) when its term coefficients, character
+ * classes and stabiliser sizes give at most 2^14 value codes, else the FP64 form's
+ * 12 (20) bytes; QMB_VALUE_CODES=0 in the environment selects the second route
+ * (same result). */
 enum { QMB_KEEP_FP64_VALUES = 16 };
 /* value_type | QMB_LOCALITY_SLOTS: the slots of each row follow the terms
  * sorted on (lowest site, then highest site) descending, stable (a term with no
