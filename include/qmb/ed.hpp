@@ -265,6 +265,7 @@ class HeisenbergXXZAbelianSymms {
                 o.seed = (uint64_t)seed;
                 o.max_iter = opts_.max_iter;
                 o.tol = opts_.tol;
+                o.check_every = 5;  // iterations per graph replay
                 std::vector<double> E(batch.size());
                 std::vector<qmb_lanczos_report> reps(batch.size());
                 check(qmb_lanczos_batched(ctx, (int)batch.size(), batch.data(), &o, E.data(), reps.data()),

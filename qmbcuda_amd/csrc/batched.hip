@@ -60,7 +60,9 @@ struct BTask {
     int64_t lo, hi;
 };
 
-constexpr int kVecPerBlock = kThreads * 8;
+// vector-phase elements (doubles) per block: 4 per thread (72 N=24 sectors:
+// 8 per thread 3 % slower, 16 per thread 17 % slower, 2 per thread equal)
+constexpr int kVecPerBlock = kThreads * 4;
 
 // NT: matrix stream past the caches (nontemporal) or plain loads; U: slots per
 // group of the packed row products. Both are picked by the autotuner.
@@ -643,8 +645,14 @@ static void run_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const q
         for (auto &f : fam)
             for (size_t q = 0; q < f.members.size(); ++q) hin[f.first_inst + q].vec_first -= f.vec_first_block;
 
-        // autotune the SpMV shape per family: chunks per block x (stream policy,
-        // slots per group), each candidate timed over 3 launches after a warm one
+        // autotune the SpMV shape per family: the four kernel variants (stream
+        // policy x slots per group) at 4 chunks per block, then chunks per block
+        // halved or doubled from there while that helps, under the best variant;
+        // each candidate timed over 2 launches, every variant launched once
+        // untimed first. Measured over all 24 shapes on five batches (6-66
+        // sectors, real and complex), this search found the fastest shape each
+        // time with 6-8 candidates; timing all 24, with a warm launch and 3 timed
+        // ones each, took as long as the 72-sector solve itself.
         for (auto &f : fam) {
             if (f.members.empty()) continue;
             int64_t rows = 0;
@@ -662,31 +670,52 @@ static void run_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const q
             QMB_HIP(hipEventCreate(&ea));
             QMB_HIP(hipEventCreate(&eb));
             const auto &vars = spmv_variants();
-            for (int cpb : {1, 2, 4, 8, 16, 32}) {
-                int fb = 0;
-                build_spmv_tasks(f, cpb, fb);
-                upload_inst();
-                DevBuf dt(f.spmv_tasks.size() * sizeof(BTask));
-                QMB_HIP(hipMemcpy(dt.p, f.spmv_tasks.data(), f.spmv_tasks.size() * sizeof(BTask),
-                                  hipMemcpyHostToDevice));
-                for (int vi = 0; vi < (int)vars.size(); ++vi) {
-                    const SpmvKernel k = f.cplx ? vars[vi].cplx : vars[vi].real;
-                    hipLaunchKernelGGL(k, dim3((unsigned)f.spmv_tasks.size()), dim3(kThreads), 0, ctx->stream,
-                                       d_inst.as<BInst>(), dt.as<BTask>(), d_active.as<int>(), d_it.as<int>(),
-                                       part.as<double>());
-                    QMB_HIP(hipEventRecord(ea, ctx->stream));
-                    for (int rep = 0; rep < 3; ++rep)
-                        hipLaunchKernelGGL(k, dim3((unsigned)f.spmv_tasks.size()), dim3(kThreads), 0, ctx->stream,
-                                           d_inst.as<BInst>(), dt.as<BTask>(), d_active.as<int>(), d_it.as<int>(),
-                                           part.as<double>());
-                    QMB_HIP(hipEventRecord(eb, ctx->stream));
-                    QMB_HIP(hipEventSynchronize(eb));
-                    float t = 0.f;
-                    QMB_HIP(hipEventElapsedTime(&t, ea, eb));
-                    if (t < best) {
-                        best = t;
-                        best_c = TileChoice{cpb, vi};
-                    }
+            std::vector<char> warm(vars.size(), 0);
+            std::map<std::pair<int, int>, float> tried;
+            int cur_cpb = -1;
+            DevBuf dt;
+            auto measure = [&](int cpb, int vi) -> float {
+                const auto it = tried.find({cpb, vi});
+                if (it != tried.end()) return it->second;
+                if (cpb != cur_cpb) {
+                    int fb = 0;
+                    build_spmv_tasks(f, cpb, fb);
+                    upload_inst();
+                    dt.alloc(f.spmv_tasks.size() * sizeof(BTask));
+                    QMB_HIP(hipMemcpy(dt.p, f.spmv_tasks.data(), f.spmv_tasks.size() * sizeof(BTask),
+                                      hipMemcpyHostToDevice));
+                    cur_cpb = cpb;
+                }
+                const SpmvKernel k = f.cplx ? vars[vi].cplx : vars[vi].real;
+                const dim3 grid((unsigned)f.spmv_tasks.size());
+                if (!warm[vi]) {
+                    hipLaunchKernelGGL(k, grid, dim3(kThreads), 0, ctx->stream, d_inst.as<BInst>(), dt.as<BTask>(),
+                                       d_active.as<int>(), d_it.as<int>(), part.as<double>());
+                    warm[vi] = 1;
+                }
+                QMB_HIP(hipEventRecord(ea, ctx->stream));
+                for (int rep = 0; rep < 2; ++rep)
+                    hipLaunchKernelGGL(k, grid, dim3(kThreads), 0, ctx->stream, d_inst.as<BInst>(), dt.as<BTask>(),
+                                       d_active.as<int>(), d_it.as<int>(), part.as<double>());
+                QMB_HIP(hipEventRecord(eb, ctx->stream));
+                QMB_HIP(hipEventSynchronize(eb));
+                float t = 0.f;
+                QMB_HIP(hipEventElapsedTime(&t, ea, eb));
+                tried[{cpb, vi}] = t;
+                if (t < best) {
+                    best = t;
+                    best_c = TileChoice{cpb, vi};
+                }
+                return t;
+            };
+            for (int vi = 0; vi < (int)vars.size(); ++vi) measure(4, vi);
+            const int v = best_c.variant;
+            for (const int grow : {0, 1}) {
+                int c = 4;
+                while (true) {
+                    const int nc = grow ? c * 2 : c / 2;
+                    if (nc < 1 || nc > 32 || measure(nc, v) >= tried[{c, v}]) break;
+                    c = nc;
                 }
             }
             (void)hipEventDestroy(ea);
