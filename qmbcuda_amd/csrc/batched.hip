@@ -666,7 +666,14 @@ static void run_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const q
             }
             float best = 1e30f;
             TileChoice best_c{4, 0};
-            hipEvent_t ea, eb;
+            hipEvent_t ea = nullptr, eb = nullptr;
+            struct TuneEvents {
+                hipEvent_t *a, *b;
+                ~TuneEvents() {
+                    if (*a) (void)hipEventDestroy(*a);
+                    if (*b) (void)hipEventDestroy(*b);
+                }
+            } tune_events{&ea, &eb};
             QMB_HIP(hipEventCreate(&ea));
             QMB_HIP(hipEventCreate(&eb));
             const auto &vars = spmv_variants();
@@ -718,8 +725,6 @@ static void run_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const q
                     c = nc;
                 }
             }
-            (void)hipEventDestroy(ea);
-            (void)hipEventDestroy(eb);
             f.cpb = best_c.cpb;
             f.variant = best_c.variant;
             tile_cache_put(key, best_c);
@@ -820,10 +825,10 @@ static void run_batched(qmb_ctx *ctx, int n, const qmb_ell *const *ells, const q
         std::vector<hipEvent_t> rev;
         auto launch_chunk = [&](int slot) {
             if (o.profile) {
-                hipEvent_t e0, e1;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
                 QMB_HIP(hipEventCreate(&e0));
+                rev.push_back(e0);  // owned by rev_guard from here on
                 QMB_HIP(hipEventCreate(&e1));
-                rev.push_back(e0);
                 rev.push_back(e1);
                 QMB_HIP(hipEventRecord(e0, ctx->stream));
             }

@@ -755,8 +755,15 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     // profiling events: one pair per SpMV, two check windows' worth (the first
     // step of a window may be queued while the previous window is evaluated)
     std::vector<hipEvent_t> ev;
+    struct EventVec {
+        std::vector<hipEvent_t> &v;
+        ~EventVec() {
+            for (auto e : v)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } ev_guard{ev};
     if (o.profile) {
-        ev.resize(4 * (size_t)check);
+        ev.resize(4 * (size_t)check, nullptr);
         for (auto &x : ev) QMB_HIP(hipEventCreate(&x));
     }
     struct EventHolder {
@@ -1022,7 +1029,6 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         QMB_HIP(hipStreamSynchronize(ctx->stream));
         n_spmv += m + 1;
     }
-    for (auto &x : ev) (void)hipEventDestroy(x);
     if (rep) {
         std::memset(rep, 0, sizeof(*rep));
         rep->iters = m;
@@ -1262,18 +1268,23 @@ qmb_status qmb_spmv_benchmark(qmb_ctx *ctx, const qmb_ell *e, int reps, double *
                            (uint64_t)7, dx.as<double>());
         QMB_HIP(hipGetLastError());
         launch_spmv(ctx, e, dx.as<double>(), dy.as<double>(), false, nullptr, nullptr, nullptr, nullptr);
-        hipEvent_t a, b;
-        QMB_HIP(hipEventCreate(&a));
-        QMB_HIP(hipEventCreate(&b));
-        QMB_HIP(hipEventRecord(a, ctx->stream));
+        std::vector<hipEvent_t> ev(2, nullptr);
+        struct EventVec {
+            std::vector<hipEvent_t> &v;
+            ~EventVec() {
+                for (auto x : v)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } ev_guard{ev};
+        QMB_HIP(hipEventCreate(&ev[0]));
+        QMB_HIP(hipEventCreate(&ev[1]));
+        QMB_HIP(hipEventRecord(ev[0], ctx->stream));
         for (int i = 0; i < reps; ++i)
             launch_spmv(ctx, e, dx.as<double>(), dy.as<double>(), false, nullptr, nullptr, nullptr, nullptr);
-        QMB_HIP(hipEventRecord(b, ctx->stream));
-        QMB_HIP(hipEventSynchronize(b));
+        QMB_HIP(hipEventRecord(ev[1], ctx->stream));
+        QMB_HIP(hipEventSynchronize(ev[1]));
         float t = 0.f;
-        QMB_HIP(hipEventElapsedTime(&t, a, b));
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
+        QMB_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
         *ms_per_spmv = (double)t / reps;
     });
 }
