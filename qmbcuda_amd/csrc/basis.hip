@@ -139,9 +139,14 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
     if (gv.lut == 1) {
         // Full torus product, two passes. Pass 1 keeps the states whose top block
         // equals the smallest in-block rotation of any block (complemented too
-        // under flip): a necessary condition, cheap and the same work for every
-        // lane. Pass 2 takes each lane's survivors in turn (stepping its state
-        // forward) and decides them from the candidate images alone.
+        // under flip): a necessary condition. The top block t is at most its own
+        // rotation minimum only when it is one (lut[t] <= t), so the condition
+        // is t == lut[t], t <= lut[t ^ mB] under flip, and t <= lut[v], lut[v ^ mB]
+        // for the other blocks; the first part rejects most states with two
+        // lookups, and since the top block changes rarely in colex order, whole
+        // waves take that exit together. Pass 2 takes each lane's survivors in
+        // turn (stepping its state forward) and decides them from the candidate
+        // images alone.
         const int B = gv.block, nb = gv.n_sites / B;
         const unsigned mB = (1u << B) - 1u;
         const int topsh = (nb - 1) * B;
@@ -149,13 +154,15 @@ __global__ __launch_bounds__(kBasisThreads) void basis_mark(GroupView gv, const 
         uint64_t surv = 0;
         S y = x;
         for (int i = 0; i < cnt; ++i) {
-            unsigned tmin = 0xFFu;
-            for (int j = 0; j < nb; ++j) {
-                const unsigned v = (unsigned)(y >> (j * B)) & mB;
-                tmin = min(tmin, (unsigned)gt.lut[v]);
-                if (hf) tmin = min(tmin, (unsigned)gt.lut[v ^ mB]);
+            const unsigned t = (unsigned)(y >> topsh) & mB;
+            bool ok = gt.lut[t] == t && (!hf || gt.lut[t ^ mB] >= t);
+            if (ok) {
+                for (int j = 0; j < nb - 1; ++j) {
+                    const unsigned v = (unsigned)(y >> (j * B)) & mB;
+                    ok = ok && gt.lut[v] >= t && (!hf || gt.lut[v ^ mB] >= t);
+                }
             }
-            if (((unsigned)(y >> topsh) & mB) == tmin) surv |= 1ULL << i;
+            if (ok) surv |= 1ULL << i;
             if (i + 1 < cnt) y = next_comb<S>(y);
         }
         y = x;
