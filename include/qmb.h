@@ -217,7 +217,10 @@ typedef struct {
     int32_t check_every; /* iterations between convergence checks (0 -> 10) */
     int32_t reorth;      /* 0 plain recurrence, 1 full reorthogonalisation every step,
                           * 2 partial: only when the estimated loss of orthogonality
-                          * passes sqrt(eps) (Simon 1984); 1 and 2 keep max_iter vectors */
+                          * passes sqrt(eps) (Simon 1984), 3 selective: against the Ritz
+                          * vectors whose residual is below sqrt(eps) ||T|| (Parlett and
+                          * Scott 1979; tested after every step, not with fixed_iter or
+                          * checkpoints; single-sector solves only); 1-3 keep max_iter vectors */
     int32_t fixed_iter;  /* 1: run exactly max_iter steps, no test (reference semantics) */
     int32_t start;       /* QMB_START_RANDOM, QMB_START_USER or QMB_START_DEVICE */
     int32_t want_eigvec; /* 1: rebuild the ground-state vector (second pass) */

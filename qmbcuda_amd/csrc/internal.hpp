@@ -443,5 +443,10 @@ void ensure_binom(qmb_ctx *ctx);
 // eigenvector of the lowest eigenvalue with z0[0] >= 0.
 void tridiag_lowest(int m, const double *a, const double *b, int k, double *w, double *zlast,
                     double *z0);
+// The k highest eigenvalues of T, largest first, and the last component of each
+// unit eigenvector (up to sign).
+void tridiag_highest(int m, const double *a, const double *b, int k, double *w, double *zlast);
+// Unit eigenvector z[0..m) of T for its eigenvalue lam (inverse iteration).
+void tridiag_vector(int m, const double *a, const double *b, double lam, double *z);
 
 }  // namespace qmb

@@ -560,18 +560,23 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
     const int M = o.max_iter;
     const int k_eigs = o.n_eigs;
     // reorth 0: plain recurrence; 1: full (every step); 2: partial, driven by
-    // the estimated loss of orthogonality. 1 and 2 keep every phi_n (VB).
+    // the estimated loss of orthogonality; 3: selective, against the Ritz
+    // vectors that have converged. 1-3 keep every phi_n (VB).
     const int rmode = o.reorth;
     const bool reorth = rmode != 0;
-    const int check = o.fixed_iter ? M : (o.check_every > 0 ? o.check_every : 10);
+    const bool selective = rmode == 3;
+    // the selective scheme looks for converged Ritz pairs after every step
+    const int check = o.fixed_iter ? M : selective ? 1 : (o.check_every > 0 ? o.check_every : 10);
+    constexpr int kMaxGood = 8;  // Ritz vectors kept by the selective scheme (full reorthogonalisation beyond)
     const double tol = o.tol > 0 ? o.tol : 1e-13;
     QMB_HIP(hipSetDevice(ctx->device));
     const unsigned rblocks = grid_for(n, kThreads * 4);
     {
         // the plan's vectors, refused up front when they cannot fit in device memory:
         // w, H phi, zero, then phi_{n-1}, phi_n or every kept phi_n with its coefficients
-        const double need = (double)cnt * sizeof(double) * (3.0 + (reorth ? (double)M : 2.0)) +
-                            (reorth ? (double)M * rblocks * 2 * sizeof(double) : 0.0);
+        const double need =
+            (double)cnt * sizeof(double) * (3.0 + (reorth ? (double)M : 2.0) + (selective ? kMaxGood : 0.0)) +
+            (reorth ? (double)M * rblocks * 2 * sizeof(double) : 0.0);
         size_t free_b = 0, total_b = 0;
         QMB_HIP(hipMemGetInfo(&free_b, &total_b));
         const double avail = (double)free_b + (double)ctx->pool.cached_bytes();
@@ -700,6 +705,41 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
         std::swap(om_cur, om_next);
         return run;
     };
+    // w (in P) -= sum_j <V_j, w> V_j over nv vectors of stride cnt, classical
+    // Gram-Schmidt; pass 1 only where pass 0 cancelled most of the norm (the flag
+    // in misc[3] is set by pass 0's finish on the device); the new norm goes to
+    // b[bslot], 1/b[bslot]
+    auto gram_schmidt = [&](const double *V, int nv, int bslot) {
+        double *gate = wk.misc() + 3;
+        for (int pass = 0; pass < 2; ++pass) {
+            const double *g = pass ? gate : nullptr;
+            auto kd = cplx ? reorth_dots<true> : reorth_dots<false>;
+            hipLaunchKernelGGL(kd, dim3(rblocks, (nv + kDotGroup - 1) / kDotGroup), dim3(kThreads), 0, ctx->stream,
+                               n, (int64_t)cnt, nv, V, P.as<double>(), rpart.as<double>(), (int)rblocks, g);
+            QMB_HIP(hipGetLastError());
+            hipLaunchKernelGGL(reorth_reduce, dim3(nv), dim3(kThreads), 0, ctx->stream, rpart.as<double>(),
+                               (int)rblocks, rcoef.as<double>(), g);
+            QMB_HIP(hipGetLastError());
+            auto ka = cplx ? reorth_apply<true> : reorth_apply<false>;
+            hipLaunchKernelGGL(ka, dim3(rblocks), dim3(kThreads), 0, ctx->stream, n, (int64_t)cnt, nv, V,
+                               rcoef.as<double>(), P.as<double>(), wk.partials.as<double>(), g);
+            QMB_HIP(hipGetLastError());
+            launch_finish(ctx, wk.partials.as<double>(), rblocks, wk.beta() + bslot, wk.invb() + bslot, pass ? 1 : 2,
+                          gate);
+        }
+    };
+    // Selective orthogonalisation (B. N. Parlett and D. S. Scott, Math. Comp. 33
+    // (1979) 217-238): orthogonality is lost only towards converged Ritz vectors
+    // (C. C. Paige), so once a Ritz pair of T is good, i.e. its residual
+    // b[m] |s_m| is at most sqrt(eps) ||T||, its Ritz vector y = sum_k s_k phi_k
+    // is formed once from the kept vectors and every later Lanczos vector is
+    // orthogonalised against the good vectors. The lowest max(4, n_eigs + 1) and
+    // the four highest Ritz pairs are watched (the extremes converge first);
+    // beyond kMaxGood good vectors every vector is orthogonalised against all
+    // kept ones.
+    DevBuf GV, dzs;
+    std::vector<double> good_theta, zs(selective ? (size_t)kMaxGood * M : 0);
+    bool so_full = false;
     // finish: write b[it+1], 1/b[it+1] now (needed by a convergence check)
     auto k2 = [&](int it, bool finish) {
         if (fuse) {
@@ -718,30 +758,14 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
                            nullptr, 0);
         QMB_HIP(hipGetLastError());
         launch_finish(ctx, wk.partials.as<double>(), ublocks, wk.beta() + it + 1, wk.invb() + it + 1, 1);
-        const bool run = rmode == 1 || (rmode == 2 && pro_decide(it));
+        const bool run = rmode == 1 || (rmode == 2 && pro_decide(it)) ||
+                         (selective && (so_full || !good_theta.empty()));
         if (run) {
             ++n_reorth;
-            const int nv = it + 1;
-            // pass 0 always; pass 1 only where pass 0 cancelled most of the norm
-            // (the flag in misc[3] is set by pass 0's finish on the device)
-            double *gate = wk.misc() + 3;
-            for (int pass = 0; pass < 2; ++pass) {
-                const double *g = pass ? gate : nullptr;
-                auto kd = cplx ? reorth_dots<true> : reorth_dots<false>;
-                hipLaunchKernelGGL(kd, dim3(rblocks, (nv + kDotGroup - 1) / kDotGroup), dim3(kThreads), 0,
-                                   ctx->stream, n, (int64_t)cnt, nv, VB.as<double>(), P.as<double>(),
-                                   rpart.as<double>(), (int)rblocks, g);
-                QMB_HIP(hipGetLastError());
-                hipLaunchKernelGGL(reorth_reduce, dim3(nv), dim3(kThreads), 0, ctx->stream, rpart.as<double>(),
-                                   (int)rblocks, rcoef.as<double>(), g);
-                QMB_HIP(hipGetLastError());
-                auto ka = cplx ? reorth_apply<true> : reorth_apply<false>;
-                hipLaunchKernelGGL(ka, dim3(rblocks), dim3(kThreads), 0, ctx->stream, n, (int64_t)cnt, nv,
-                                   VB.as<double>(), rcoef.as<double>(), P.as<double>(), wk.partials.as<double>(), g);
-                QMB_HIP(hipGetLastError());
-                launch_finish(ctx, wk.partials.as<double>(), rblocks, wk.beta() + it + 1, wk.invb() + it + 1,
-                              pass ? 1 : 2, gate);
-            }
+            if (selective && !so_full)
+                gram_schmidt(GV.as<double>(), (int)good_theta.size(), it + 1);
+            else
+                gram_schmidt(VB.as<double>(), it + 1, it + 1);
             if (rmode == 2) {
                 // the orthogonalised vector has a new norm b[it+1]
                 double *st = ctx->host_stage<double>(sizeof(double));
@@ -974,6 +998,55 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             predicted = next < check;
             q_prev = q;
             it_prev = it;
+        }
+        if (selective && it < M) {
+            // good Ritz pairs of T_m among the lowest max(4, n_eigs + 1) and the
+            // four highest
+            Timer ts;
+            const int m = m_eff;
+            const int kl = std::min(std::max(4, k_eigs + 1), m), kh = std::min(4, m);
+            std::vector<double> wl(kl), zlo(kl), wh(kh), zhi(kh);
+            tridiag_lowest(m, ha.data(), hb.data(), kl, wl.data(), zlo.data(), nullptr);
+            tridiag_highest(m, ha.data(), hb.data(), kh, wh.data(), zhi.data());
+            const double tnorm = std::max(std::fabs(wl[0]), std::fabs(wh[0]));
+            const double thr = 1.4901161193847656e-08 * tnorm;  // sqrt(eps) ||T||
+            int added = 0;
+            auto consider = [&](double theta, double zlast) {
+                if (so_full || !(std::fabs(hb[m] * zlast) <= thr)) return;
+                for (const double g : good_theta)
+                    if (std::fabs(theta - g) <= 1e-9 * tnorm) return;
+                if ((int)good_theta.size() == kMaxGood) {
+                    so_full = true;
+                    return;
+                }
+                if (!GV.p) {
+                    GV.alloc((size_t)kMaxGood * cnt * sizeof(double));
+                    dzs.alloc((size_t)kMaxGood * M * sizeof(double));
+                }
+                const int slot = (int)good_theta.size();
+                tridiag_vector(m, ha.data(), hb.data(), theta, zs.data() + (size_t)slot * M);
+                double *dz = dzs.as<double>() + (size_t)slot * M;
+                QMB_HIP(hipMemcpyAsync(dz, zs.data() + (size_t)slot * M, (size_t)m * sizeof(double),
+                                       hipMemcpyHostToDevice, ctx->stream));
+                hipLaunchKernelGGL(basis_combine, dim3(grid_for(cnt, kThreads * 4)), dim3(kThreads), 0, ctx->stream,
+                                   cnt, (int64_t)cnt, m, VB.as<double>(), dz, GV.as<double>() + (size_t)slot * cnt);
+                QMB_HIP(hipGetLastError());
+                good_theta.push_back(theta);
+                ++added;
+            };
+            for (int j = 0; j < kl; ++j) consider(wl[j], zlo[j]);
+            for (int j = 0; j < kh; ++j) consider(wh[j], zhi[j]);
+            // the current w becomes phi_m at the next step: orthogonalise it now
+            if (so_full || added) {
+                ++n_reorth;
+                if (so_full)
+                    gram_schmidt(VB.as<double>(), m, m);
+                else
+                    gram_schmidt(GV.as<double>(), (int)good_theta.size(), m);
+            }
+            // the host coefficients (zs) are read by the transfers until here
+            QMB_HIP(hipStreamSynchronize(ctx->stream));
+            ms_host += ts.ms();
         }
         if (ck && ck->path && it < M) {
             const bool stop = ck->stop_after > 0 && it - it_start >= ck->stop_after;
@@ -1212,8 +1285,10 @@ qmb_ctx *default_ctx() {
 void check_opts(const qmb_lanczos_opts &o) {
     QMB_REQUIRE(o.max_iter >= 1 && o.max_iter <= 100000, "qmb_lanczos: max_iter must be in [1, 100000]");
     QMB_REQUIRE(o.n_eigs >= 1 && o.n_eigs <= 8, "qmb_lanczos: n_eigs must be in [1, 8]");
-    QMB_REQUIRE(o.reorth >= 0 && o.reorth <= 2, "qmb_lanczos: reorth must be 0, 1 or 2");
-    QMB_REQUIRE(o.n_eigs == 1 || o.reorth, "qmb_lanczos: n_eigs > 1 needs reorth = 1 or 2");
+    QMB_REQUIRE(o.reorth >= 0 && o.reorth <= 3, "qmb_lanczos: reorth must be 0, 1, 2 or 3");
+    QMB_REQUIRE(o.n_eigs == 1 || o.reorth, "qmb_lanczos: n_eigs > 1 needs reorth = 1, 2 or 3");
+    QMB_REQUIRE(o.reorth != 3 || !o.fixed_iter,
+                "qmb_lanczos: the selective scheme (reorth = 3) tests every step, not with fixed_iter");
     QMB_REQUIRE(o.start == QMB_START_RANDOM || o.start == QMB_START_USER || o.start == QMB_START_DEVICE,
                 "qmb_lanczos: bad start mode");
 }
@@ -1319,6 +1394,8 @@ qmb_status qmb_lanczos_checkpointed(qmb_ctx *ctx, const qmb_ell *e, const qmb_la
         QMB_REQUIRE(every >= 0 && stop_after >= 0 && (resume == 0 || resume == 1),
                     "qmb_lanczos_checkpointed: bad checkpoint arguments");
         check_opts(*opts);
+        QMB_REQUIRE(opts->reorth != 3,
+                    "qmb_lanczos_checkpointed: the selective scheme (reorth = 3) keeps no checkpoint state");
         qmb_lanczos_opts o = *opts;
         if (o.max_iter > e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1) + 1 && !o.fixed_iter)
             o.max_iter = (int)std::max<int64_t>(1, e->n_rows * (e->value_type == QMB_COMPLEX ? 2 : 1) + 1);

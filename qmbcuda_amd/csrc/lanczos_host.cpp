@@ -96,7 +96,51 @@ void unit(std::vector<double> &x) {
         for (double &v : x) v /= s;
 }
 
+
+// Unit eigenvector of T for the eigenvalue lam (three inverse-iteration steps
+// from a fixed, slightly varied start; seed varies the start between
+// eigenvalues), with its first non-negligible component positive.
+void eigvec_at(int m, const double *a, const double *b, double lam, double tiny, int seed, std::vector<double> &x) {
+    x.assign(m, 0.0);
+    for (int i = 0; i < m; ++i) x[i] = 1.0 + 1e-3 * (double)((i * 7919 + seed * 104729) % 97) / 97.0;
+    for (int rep = 0; rep < 3; ++rep) {
+        shifted_solve(m, a, b, lam, tiny, x);
+        unit(x);
+    }
+    int lead = 0;
+    while (lead + 1 < m && std::fabs(x[lead]) < 1e-300) ++lead;
+    if (x[lead] < 0)
+        for (double &v : x) v = -v;
+}
+
+double tridiag_scale(int m, const double *a, const double *b) {
+    double lo = a[0], hi = a[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i > 0 ? std::fabs(b[i]) : 0.0) + (i + 1 < m ? std::fabs(b[i + 1]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    return std::max({std::fabs(lo), std::fabs(hi), 1e-300});
+}
+
 }  // namespace
+
+void tridiag_vector(int m, const double *a, const double *b, double lam, double *z) {
+    if (m <= 0) return;
+    std::vector<double> x;
+    eigvec_at(m, a, b, lam, DBL_EPSILON * tridiag_scale(m, a, b), 0, x);
+    for (int i = 0; i < m; ++i) z[i] = x[i];
+}
+
+void tridiag_highest(int m, const double *a, const double *b, int k, double *w, double *zlast) {
+    if (m <= 0 || k <= 0) return;
+    // the lowest of [-a, b], which has the spectrum of -T (and the eigenvectors of
+    // T up to alternating signs, so the same |last component|)
+    std::vector<double> na(m);
+    for (int i = 0; i < m; ++i) na[i] = -a[i];
+    tridiag_lowest(m, na.data(), b, k, w, zlast, nullptr);
+    for (int j = 0; j < std::min(k, m); ++j) w[j] = -w[j];
+}
 
 void tridiag_lowest(int m, const double *a, const double *b, int k, double *w, double *zlast, double *z0) {
     if (m <= 0 || k <= 0) return;
@@ -114,18 +158,9 @@ void tridiag_lowest(int m, const double *a, const double *b, int k, double *w, d
     hi += 2.0 * DBL_EPSILON * scale + pivmin;
     for (int j = 0; j < k; ++j) w[j] = bisect(m, a, b, j, lo, hi, pivmin);
     const double tiny = DBL_EPSILON * scale;
+    std::vector<double> x;
     for (int j = 0; j < k; ++j) {
-        std::vector<double> x(m);
-        for (int i = 0; i < m; ++i) x[i] = 1.0 + 1e-3 * (double)((i * 7919 + j * 104729) % 97) / 97.0;
-        for (int rep = 0; rep < 3; ++rep) {
-            shifted_solve(m, a, b, w[j], tiny, x);
-            unit(x);
-        }
-        // keep a deterministic sign: first non-negligible component positive
-        int lead = 0;
-        while (lead + 1 < m && std::fabs(x[lead]) < 1e-300) ++lead;
-        if (x[lead] < 0)
-            for (double &v : x) v = -v;
+        eigvec_at(m, a, b, w[j], tiny, j, x);
         zlast[j] = x[m - 1];
         if (j == 0 && z0)
             for (int i = 0; i < m; ++i) z0[i] = x[i];
