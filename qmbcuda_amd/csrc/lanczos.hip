@@ -1010,6 +1010,7 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             tridiag_highest(m, ha.data(), hb.data(), kh, wh.data(), zhi.data());
             const double tnorm = std::max(std::fabs(wl[0]), std::fabs(wh[0]));
             const double thr = 1.4901161193847656e-08 * tnorm;  // sqrt(eps) ||T||
+            const bool was_full = so_full;
             int added = 0;
             auto consider = [&](double theta, double zlast) {
                 if (so_full || !(std::fabs(hb[m] * zlast) <= thr)) return;
@@ -1037,16 +1038,17 @@ qmb_status run_lanczos(qmb_ctx *ctx, const qmb_ell *e, const qmb_lanczos_opts &o
             for (int j = 0; j < kl; ++j) consider(wl[j], zlo[j]);
             for (int j = 0; j < kh; ++j) consider(wh[j], zhi[j]);
             // the current w becomes phi_m at the next step: orthogonalise it now
-            if (so_full || added) {
+            // against what changed (every later step does so in k2)
+            if ((so_full && !was_full) || (!so_full && added)) {
                 ++n_reorth;
                 if (so_full)
                     gram_schmidt(VB.as<double>(), m, m);
                 else
                     gram_schmidt(GV.as<double>(), (int)good_theta.size(), m);
             }
+            ms_host += ts.ms();
             // the host coefficients (zs) are read by the transfers until here
             QMB_HIP(hipStreamSynchronize(ctx->stream));
-            ms_host += ts.ms();
         }
         if (ck && ck->path && it < M) {
             const bool stop = ck->stop_after > 0 && it - it_start >= ck->stop_after;
