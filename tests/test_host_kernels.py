@@ -2,8 +2,10 @@
 the term compiler (compiled masks and scalars equal the operator-by-operator
 application bit for bit, tests/cpp/test_terms), the bit-sliced ring window
 scan of the ring orbit minimum (equal to a plain scan of every rotation's
-window, tests/cpp/test_ring_windows) and the nibble-table colex rank of the ELL
-kernels (equal to the rank's definition, tests/cpp/test_rank)."""
+window, tests/cpp/test_ring_windows), the nibble-table colex rank of the ELL
+kernels (equal to the rank's definition, tests/cpp/test_rank) and the host
+tridiagonal solver of the Lanczos driver (closed-form constant tridiagonals and
+a dense Jacobi solve, tests/cpp/test_tridiag)."""
 import os
 import subprocess
 
@@ -31,3 +33,7 @@ def test_bit_sliced_ring_windows_equal_plain_scan():
 
 def test_nibble_table_rank_equals_colex_rank():
     run("test_rank")
+
+
+def test_tridiagonal_solver_equals_closed_forms_and_dense_solve():
+    run("test_tridiag")
