@@ -111,10 +111,17 @@ def test_selective_eigenvector(ctx):
     assert np.linalg.norm(E @ x - ev[0] * x) <= bound
 
 
-def test_selective_refuses_fixed_iter_and_checkpoints(ctx, tmp_path):
+def test_selective_refuses_fixed_iter_checkpoints_and_batches(ctx, tmp_path):
+    """Selective orthogonalisation tests every step on the host: it is refused
+    with fixed_iter, with checkpoints and by the batched solver (whose steps run
+    inside a replayed graph); the batched solver still takes reorth 2 after it."""
     case = chain_case(12)
     *_, e = objects(ctx, case, 6, 0)
     with pytest.raises(q.QMBError):
         q.lanczos(e, max_iter=20, reorth=3, fixed_iter=True)
     with pytest.raises(q.QMBError):
         q.lanczos_checkpointed(e, str(tmp_path / "lz.qlz"), max_iter=50, reorth=3, every=5)
+    with pytest.raises(q.QMBError, match="reorth"):
+        q.lanczos_batched([e, e], reorth=3)
+    ev, reps = q.lanczos_batched([e, e], reorth=2)
+    assert all(r.converged for r in reps) and abs(ev[0, 0] - ev[1, 0]) <= 1e-10 * abs(ev[0, 0])
